@@ -1,0 +1,267 @@
+/* marx_gpu.h -- C-ABI of libmarx_gpu, the MI355X photon ray-trace library.
+ *
+ * The library is a drop-in for the libmarx entry points that the marx
+ * driver calls on its per-photon hot path (marx/src/marx.c:240-273 and
+ * :641-656).  Every entry point below keeps the reference's name, argument
+ * meaning and 0 / -1 / module-id return convention.  The photon record
+ * keeps the reference's in-memory layout byte for byte, so an unchanged
+ * marx.o can hand its Marx_Photon_Type to this library.
+ *
+ * Functions whose names start with mxg_ have no counterpart in the
+ * reference: they expose the device-resident pipeline (photons created,
+ * traced and binned without leaving HBM), the batched multi-source launch
+ * and the survival-weighted / adjoint mode.
+ *
+ * All signatures use plain C types and pointers.  Device pointers are
+ * passed as void * or as typed pointers documented as device memory.
+ */
+#ifndef MARX_GPU_H
+#define MARX_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Photon list.  Layout identical to marx/libsrc/marx.h:32-151 (136 bytes per
+ * attribute record, offsets energy@0 x@8 p@32 arrival_time@56 flags@64
+ * y_pixel@68 dither_state@84 pi@108 pulse_height@112 mirror_shell@116
+ * ccd_num@120 order@122 support_orders@123 tag@128).                      */
+
+typedef struct
+{
+   double x, y, z;
+}
+JDMVector_Type;
+
+typedef struct
+{
+   float ra, dec, roll;
+   float dy, dz, dtheta;
+}
+Marx_Dither_Type;
+
+typedef struct
+{
+   double energy;
+   JDMVector_Type x;
+   JDMVector_Type p;
+   double arrival_time;
+   unsigned int flags;
+   float y_pixel;
+   float z_pixel;
+   float u_pixel;
+   float v_pixel;
+   Marx_Dither_Type dither_state;
+   float pi;
+   short pulse_height;
+   unsigned int mirror_shell;
+   signed char ccd_num;
+   signed char detector_region;
+   signed char order;
+   signed char support_orders[4];
+   unsigned int tag;
+}
+Marx_Photon_Attr_Type;
+
+/* Absorption / history bits of Marx_Photon_Attr_Type.flags (marx.h:65-75) */
+#define PHOTON_UNDETECTED        0x01
+#define PHOTON_UNREFLECTED       0x02
+#define PHOTON_UNDIFFRACTED      0x04
+#define PHOTON_MISSED_DETECTOR   0x08
+#define PHOTON_MIRROR_VBLOCKED   0x10
+#define PHOTON_DRAKE_BLOCKED     0x20
+#define PHOTON_GRATING_VBLOCKED  0x40
+#define BAD_PHOTON_MASK          0xFF
+#define PHOTON_DRAKE_REFLECTED   0x100
+#define PHOTON_ACIS_STREAKED     0x200
+
+typedef struct
+{
+   double source_distance;
+   Marx_Photon_Attr_Type *attributes;
+   unsigned int n_photons;
+   unsigned int max_n_photons;
+   double total_time;
+   double start_time;
+   unsigned int *sorted_index;
+   double *sorted_energies;
+   unsigned int num_sorted;
+   unsigned long history;
+   unsigned int tag_start;
+}
+Marx_Photon_Type;
+
+/* Marx_Photon_Type.history bits (marx.h:126-147) */
+#define MARX_ENERGY_OK        0x00000001
+#define MARX_TIME_OK          0x00000002
+#define MARX_X_VECTOR_OK      0x00000004
+#define MARX_P_VECTOR_OK      0x00000008
+#define MARX_TAG_OK           0x00000010
+#define MARX_PULSEHEIGHT_OK   0x00000020
+#define MARX_PI_OK            0x00000040
+#define MARX_DET_PIXEL_OK     0x00000080
+#define MARX_DET_NUM_OK       0x00000100
+#define MARX_DET_REGION_OK    0x00000200
+#define MARX_DET_UV_PIXEL_OK  0x00000400
+#define MARX_MIRROR_SHELL_OK  0x00000800
+#define MARX_SKY_DITHER_OK    0x00001000
+#define MARX_DET_DITHER_OK    0x00002000
+#define MARX_ORDER_OK         0x00100000
+#define MARX_ORDER1_OK        0x00200000
+#define MARX_ORDER2_OK        0x00400000
+#define MARX_ORDER3_OK        0x00800000
+#define MARX_ORDER4_OK        0x01000000
+
+/* Module ids returned by the init entry points (marx.h:364-376). */
+#define MARX_DETECTOR_HRC_S   1
+#define MARX_DETECTOR_HRC_I   2
+#define MARX_DETECTOR_ACIS_S  3
+#define MARX_DETECTOR_ACIS_I  4
+#define MARX_MIRROR_HRMA      2
+#define MARX_GRATING_HETG     1
+#define MARX_GRATING_LETG     2
+
+/* Photon list management.  Replaces marx/libsrc/photon.c:40-108
+ * (declared at marx.h:155-157). */
+Marx_Photon_Type *marx_alloc_photon_type (unsigned int num);
+int marx_dealloc_photon_type (Marx_Photon_Type *pt);
+/* Rebuilds sorted_index / sorted_energies from the photons whose
+ * flags & BAD_PHOTON_MASK is zero, keeping their order (photon.c:40-63). */
+void marx_prune_photons (Marx_Photon_Type *pt);
+
+/* ------------------------------------------------------------------------ */
+/* Parameter files.  The subset of the pfile API (pfile/src/pfile.h:43-64)
+ * that the path's init functions use, over IRAF-style .par files
+ * (name,type,mode,value,min,max,prompt).  Getters return 0 on success and
+ * -1 when the parameter is absent or malformed.                            */
+
+typedef struct _Param_File_Type Param_File_Type;
+
+Param_File_Type *pf_open_parameter_file (char *file, char *mode);
+int pf_close_parameter_file (Param_File_Type *pf);
+int pf_get_double (Param_File_Type *pf, char *name, double *v);
+int pf_get_integer (Param_File_Type *pf, char *name, int *v);
+int pf_get_boolean (Param_File_Type *pf, char *name, int *v);
+int pf_get_string (Param_File_Type *pf, char *name, char *buf, unsigned int buflen);
+int pf_get_file (Param_File_Type *pf, char *name, char *buf, unsigned int buflen);
+/* Replaces (or adds) the value of a parameter; the equivalent of a
+ * name=value command-line override (marx/libsrc/util.c:238). */
+int pf_set_value (Param_File_Type *pf, char *name, char *value);
+
+/* Directory that relative calibration-file names are resolved against
+ * (marx/libsrc/data.c:83-126).  "$VAR" takes the value of an environment
+ * variable.  When never set, the data directory bundled with the library
+ * is used. */
+int marx_set_data_directory (char *dir);
+
+/* ------------------------------------------------------------------------ */
+/* Hot-path entry points (marx/libsrc/marx.h:354-362), called from
+ * marx/src/marx.c:240-273 (run) and :641-656 (init).  The init functions
+ * return the module id (0 for NONE) or -1 on error.  The run functions
+ * return 0 or -1.  Each run function sets its history bit and returns at
+ * once when that bit is already set, as the reference stages do.           */
+
+/* Replaces marx/libsrc/mirror.c:48-91 (HRMA: hrma.c:1097-1157). */
+int marx_mirror_init (Param_File_Type *pf);
+/* Replaces marx/libsrc/mirror.c:93-108 -> hrma.c:1161-1341. */
+int marx_mirror_reflect (Marx_Photon_Type *pt, int verbose);
+/* Replaces marx/libsrc/grating.c:44-86 (diffract.c:1687-1863). */
+int marx_grating_init (Param_File_Type *pf);
+/* Replaces marx/libsrc/grating.c:88-114 -> diffract.c:974-1130. */
+int marx_grating_diffract (Marx_Photon_Type *pt, int verbose);
+/* Replaces marx/libsrc/detector.c:298-357. */
+int marx_detector_init (Param_File_Type *pf);
+/* Replaces marx/libsrc/detector.c:361-379 (acis-s.c:177-248,
+ * acis-i.c:96-167, hrc-s.c:236-312, hrc-i.c:118-185). */
+int marx_detect (Marx_Photon_Type *pt, int verbose);
+
+/* Geometric collecting area of the open HRMA shells in cm^2, set by
+ * marx_mirror_init (hrma.c:766-770). */
+extern double Marx_Mirror_Geometric_Area;
+
+/* ------------------------------------------------------------------------ */
+/* Source side (marx/libsrc/marx.h:287-293; source.c:268-417).  POINT and
+ * DISK sources with FLAT or FILE spectra and the INTERNAL / NONE dither
+ * models.  Photons are created on the GPU and copied into pt.              */
+
+typedef struct _Marx_Source_Type Marx_Source_Type;
+
+Marx_Source_Type *marx_create_source (Param_File_Type *pf);
+int marx_open_source (Marx_Source_Type *st);
+int marx_close_source (Marx_Source_Type *st);
+int marx_create_photons (Marx_Source_Type *st, Marx_Photon_Type *pt,
+                         unsigned int num, unsigned int *num_collected,
+                         double *exposure_time);
+
+/* ------------------------------------------------------------------------ */
+/* mxg_ extensions (no counterpart in the reference).                       */
+
+/* Last error message of this library (also printed on stderr). */
+const char *mxg_last_error (void);
+
+/* Selects the HIP device, creates the library's stream and checks that the
+ * device is a gfx950.  Returns 0, or -1 with mxg_last_error() set when no
+ * usable GPU is present.  Every entry point that launches work calls it. */
+int mxg_init (void);
+
+/* Random-number seed of the per-photon counter streams (Philox4x32-10 keyed
+ * by seed; counter = (photon tag, stage, block, source id)).  The
+ * reference's RandomSeed parameter maps here (marx/src/marx.c:846-849). */
+void mxg_set_seed (unsigned long long seed);
+unsigned long long mxg_get_seed (void);
+
+/* When nonzero, marx_mirror_reflect and marx_grating_diffract leave the
+ * photon state resident in HBM and only marx_detect copies it back into
+ * the AoS list; when zero (the default) every stage leaves pt->attributes
+ * valid on return, like the reference. */
+void mxg_set_lazy_sync (int on);
+
+/* Device-resident pipeline: creates num photons of the initialised source
+ * on the GPU and runs dither, mirror, grating and detector stages without
+ * leaving HBM.  tag_start / start_time play the role of the fields of
+ * Marx_Photon_Type.  *num_collected receives the number of photons created
+ * (fewer than num when exposure_time truncates the chunk), *num_detected
+ * the number that survive every stage.  stage_ms, when non-NULL, receives
+ * 6 per-stage GPU times in ms measured with HIP events on the library's
+ * stream: source, dither, mirror, grating, detector, total. */
+int mxg_trace_device (unsigned int num, unsigned int tag_start,
+                      double start_time, double exposure_time,
+                      unsigned int *num_collected, unsigned int *num_detected,
+                      float *stage_ms);
+
+/* Copies the device-resident photons of the last mxg_trace_device call into
+ * pt (AoS), setting n_photons, history and the pruned sorted_index. */
+int mxg_download_photons (Marx_Photon_Type *pt);
+
+/* Device pointers of the resident structure-of-arrays columns.  Returned
+ * pointers stay valid until the next call that grows the store. */
+typedef struct
+{
+   unsigned int n;             /* photons in the store */
+   unsigned int capacity;
+   double *energy;             /* [n] */
+   double *x[3];               /* [n] each */
+   double *p[3];
+   double *arrival_time;
+   unsigned int *flags;
+   float *y_pixel, *z_pixel, *u_pixel, *v_pixel;
+   float *dither[6];           /* ra dec roll dy dz dtheta */
+   float *pi;
+   short *pulse_height;
+   unsigned char *mirror_shell;
+   signed char *ccd_num, *detector_region, *order;
+   signed char *support_orders[4];
+   unsigned int *tag;
+}
+mxg_soa_view;
+int mxg_get_soa (mxg_soa_view *view);
+
+/* Name of the device the library runs on, and its compute-unit count. */
+int mxg_device_info (char *name, unsigned int namelen, int *num_cus);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MARX_GPU_H */

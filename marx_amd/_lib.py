@@ -1,0 +1,147 @@
+"""ctypes binding of libmarx_gpu (include/marx_gpu.h).
+
+The library is loaded from ``marx_amd/lib/libmarx_gpu.so`` (built in-tree by
+``__graft_entry__.build()`` / ``make -C marx_amd/csrc``).  There is no
+fallback: a missing library raises ``ImportError`` and a missing GPU makes
+every launching entry point return -1 with ``mxg_last_error()`` set.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libmarx_gpu.so")
+DATA_DIR = os.path.join(PKG_DIR, "data")
+PAR_FILE = os.path.join(DATA_DIR, "marx.par")
+
+# Marx_Photon_Attr_Type (marx/libsrc/marx.h:51-103), 136 bytes.
+ATTR_DTYPE = np.dtype({
+    "names": ["energy", "x", "p", "arrival_time", "flags", "y_pixel", "z_pixel",
+              "u_pixel", "v_pixel", "dither", "pi", "pulse_height", "mirror_shell",
+              "ccd_num", "detector_region", "order", "support_orders", "tag"],
+    "formats": ["<f8", ("<f8", 3), ("<f8", 3), "<f8", "<u4", "<f4", "<f4", "<f4", "<f4",
+                ("<f4", 6), "<f4", "<i2", "<u4", "i1", "i1", "i1", ("i1", 4), "<u4"],
+    "offsets": [0, 8, 32, 56, 64, 68, 72, 76, 80, 84, 108, 112, 116, 120, 121, 122, 123, 128],
+    "itemsize": 136,
+})
+
+PHOTON_UNDETECTED = 0x01
+PHOTON_UNREFLECTED = 0x02
+PHOTON_UNDIFFRACTED = 0x04
+PHOTON_MISSED_DETECTOR = 0x08
+PHOTON_MIRROR_VBLOCKED = 0x10
+PHOTON_DRAKE_BLOCKED = 0x20
+PHOTON_GRATING_VBLOCKED = 0x40
+BAD_PHOTON_MASK = 0xFF
+PHOTON_DRAKE_REFLECTED = 0x100
+PHOTON_ACIS_STREAKED = 0x200
+
+MARX_MIRROR_SHELL_OK = 0x00000800
+MARX_ORDER_OK = 0x00100000
+MARX_DET_NUM_OK = 0x00000100
+
+
+class PhotonList(ctypes.Structure):
+    """Marx_Photon_Type (marx/libsrc/marx.h:108-151)."""
+    _fields_ = [
+        ("source_distance", ctypes.c_double),
+        ("attributes", ctypes.c_void_p),
+        ("n_photons", ctypes.c_uint),
+        ("max_n_photons", ctypes.c_uint),
+        ("total_time", ctypes.c_double),
+        ("start_time", ctypes.c_double),
+        ("sorted_index", ctypes.POINTER(ctypes.c_uint)),
+        ("sorted_energies", ctypes.POINTER(ctypes.c_double)),
+        ("num_sorted", ctypes.c_uint),
+        ("history", ctypes.c_ulong),
+        ("tag_start", ctypes.c_uint),
+    ]
+
+
+class SoaView(ctypes.Structure):
+    """mxg_soa_view: device pointers of the resident columns."""
+    _fields_ = [
+        ("n", ctypes.c_uint), ("capacity", ctypes.c_uint),
+        ("energy", ctypes.c_void_p), ("x", ctypes.c_void_p * 3), ("p", ctypes.c_void_p * 3),
+        ("arrival_time", ctypes.c_void_p), ("flags", ctypes.c_void_p),
+        ("y_pixel", ctypes.c_void_p), ("z_pixel", ctypes.c_void_p),
+        ("u_pixel", ctypes.c_void_p), ("v_pixel", ctypes.c_void_p),
+        ("dither", ctypes.c_void_p * 6), ("pi", ctypes.c_void_p),
+        ("pulse_height", ctypes.c_void_p), ("mirror_shell", ctypes.c_void_p),
+        ("ccd_num", ctypes.c_void_p), ("detector_region", ctypes.c_void_p),
+        ("order", ctypes.c_void_p), ("support_orders", ctypes.c_void_p * 4),
+        ("tag", ctypes.c_void_p),
+    ]
+
+
+PPhotonList = ctypes.POINTER(PhotonList)
+
+# (name, restype, argtypes) of every exported C-ABI symbol of include/marx_gpu.h
+_SIGNATURES = [
+    ("marx_alloc_photon_type", PPhotonList, [ctypes.c_uint]),
+    ("marx_dealloc_photon_type", ctypes.c_int, [PPhotonList]),
+    ("marx_prune_photons", None, [PPhotonList]),
+    ("pf_open_parameter_file", ctypes.c_void_p, [ctypes.c_char_p, ctypes.c_char_p]),
+    ("pf_close_parameter_file", ctypes.c_int, [ctypes.c_void_p]),
+    ("pf_get_double", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]),
+    ("pf_get_integer", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
+    ("pf_get_boolean", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
+    ("pf_get_string", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint]),
+    ("pf_get_file", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint]),
+    ("pf_set_value", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p]),
+    ("marx_set_data_directory", ctypes.c_int, [ctypes.c_char_p]),
+    ("marx_mirror_init", ctypes.c_int, [ctypes.c_void_p]),
+    ("marx_mirror_reflect", ctypes.c_int, [PPhotonList, ctypes.c_int]),
+    ("marx_grating_init", ctypes.c_int, [ctypes.c_void_p]),
+    ("marx_grating_diffract", ctypes.c_int, [PPhotonList, ctypes.c_int]),
+    ("marx_detector_init", ctypes.c_int, [ctypes.c_void_p]),
+    ("marx_detect", ctypes.c_int, [PPhotonList, ctypes.c_int]),
+    ("mxg_last_error", ctypes.c_char_p, []),
+    ("mxg_init", ctypes.c_int, []),
+    ("mxg_set_seed", None, [ctypes.c_ulonglong]),
+    ("mxg_get_seed", ctypes.c_ulonglong, []),
+    ("mxg_set_lazy_sync", None, [ctypes.c_int]),
+    ("mxg_get_soa", ctypes.c_int, [ctypes.POINTER(SoaView)]),
+    ("mxg_device_info", ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint, ctypes.POINTER(ctypes.c_int)]),
+    ("mxg_get_config", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
+                                      ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_void_p),
+                                      ctypes.POINTER(ctypes.c_uint)]),
+]
+
+_lib = None
+
+
+def load():
+    """Loads libmarx_gpu.so once; raises ImportError when it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} not found: build it with `make -C marx_amd/csrc` "
+            "(or __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+    for name, res, args in _SIGNATURES:
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return [s[0] for s in _SIGNATURES]
+
+
+def last_error():
+    msg = load().mxg_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(status, what):
+    if status == -1:
+        raise RuntimeError(f"{what} failed: {last_error()}")
+    return status

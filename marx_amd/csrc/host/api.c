@@ -1,0 +1,282 @@
+/* api.c -- the drop-in entry points of libmarx_gpu.
+ *
+ * marx_mirror_* / marx_grating_* / marx_detect / photon-list management keep
+ * the reference's signatures and conventions (marx/libsrc/marx.h:155-157,
+ * 354-362; mirror.c:48-108; grating.c:44-114; detector.c:298-379;
+ * photon.c:40-108).  The photon state of a chunk lives in HBM as columns;
+ * the AoS list the caller owns is filled from it.  By default every stage
+ * call leaves pt->attributes valid on return, as the reference does; with
+ * mxg_set_lazy_sync(1) the state stays resident between the mirror, grating
+ * and detector calls of one chunk and only marx_detect copies it back.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "mxg_host.h"
+
+static int Mirror_Type = -1;
+static int Grating_Type = -1;
+static int Detector_Type = -1;
+static unsigned long long Seed = 1;
+static int Lazy_Sync = 0;
+
+/* the list whose photons are currently resident in HBM */
+static const Marx_Photon_Type *Resident = NULL;
+static unsigned int Resident_Tag_Start = 0;
+
+void mxg_set_seed (unsigned long long seed) { Seed = seed; }
+unsigned long long mxg_get_seed (void) { return Seed; }
+void mxg_set_lazy_sync (int on) { Lazy_Sync = on; }
+
+int mxg_init (void)
+{
+   if (-1 == mxgd_init ())
+     {
+        mxg_error ("GPU initialisation failed: %s", mxgd_last_error ());
+        return -1;
+     }
+   return 0;
+}
+
+int mxg_device_info (char *name, unsigned int namelen, int *num_cus)
+{
+   return mxgd_device_info (name, namelen, num_cus);
+}
+
+static int sync_config (void)
+{
+   if (-1 == mxg_init ())
+     return -1;
+   if (-1 == mxgd_sync_config (&Mxg_Cfg, mxg_farena_base (), mxg_farena_size (),
+                               mxg_darena_base (), mxg_darena_size (), Mxg_Cfg_Generation))
+     {
+        mxg_error ("%s", mxgd_last_error ());
+        return -1;
+     }
+   return 0;
+}
+
+/* ---- photon lists (photon.c) ------------------------------------------------ */
+
+Marx_Photon_Type *marx_alloc_photon_type (unsigned int num)
+{
+   Marx_Photon_Type *pt = (Marx_Photon_Type *) calloc (1, sizeof (Marx_Photon_Type));
+   if (pt == NULL) return NULL;
+   pt->attributes = (Marx_Photon_Attr_Type *) calloc (num ? num : 1, sizeof (Marx_Photon_Attr_Type));
+   pt->sorted_energies = (double *) calloc (num ? num : 1, sizeof (double));
+   pt->sorted_index = (unsigned int *) calloc (num ? num : 1, sizeof (unsigned int));
+   if ((pt->attributes == NULL) || (pt->sorted_energies == NULL) || (pt->sorted_index == NULL))
+     {
+        marx_dealloc_photon_type (pt);
+        return NULL;
+     }
+   pt->max_n_photons = num;
+   return pt;
+}
+
+int marx_dealloc_photon_type (Marx_Photon_Type *pt)
+{
+   if (pt == NULL) return -1;
+   if (Resident == pt) Resident = NULL;
+   free (pt->attributes);
+   free (pt->sorted_index);
+   free (pt->sorted_energies);
+   free (pt);
+   return 0;
+}
+
+void marx_prune_photons (Marx_Photon_Type *pt)
+{
+   unsigned int i, n = 0;
+   if ((pt == NULL) || (pt->sorted_index == NULL)) return;
+   for (i = 0; i < pt->num_sorted; i++)
+     {
+        unsigned int indx = pt->sorted_index[i];
+        Marx_Photon_Attr_Type *at = pt->attributes + indx;
+        if (at->flags & BAD_PHOTON_MASK) continue;
+        if (pt->sorted_energies != NULL) pt->sorted_energies[n] = at->energy;
+        pt->sorted_index[n] = indx;
+        n++;
+     }
+   pt->num_sorted = n;
+}
+
+/* ---- residency ---------------------------------------------------------------- */
+
+static int make_resident (Marx_Photon_Type *pt)
+{
+   if ((Resident == pt) && (Resident_Tag_Start == pt->tag_start) && Lazy_Sync)
+     return 0;
+   if (pt->num_sorted > pt->n_photons)
+     {
+        mxg_error ("photon list has num_sorted > n_photons");
+        return -1;
+     }
+   if (-1 == mxgd_upload_aos (pt->attributes, pt->n_photons, pt->sorted_index, pt->num_sorted))
+     {
+        mxg_error ("%s", mxgd_last_error ());
+        return -1;
+     }
+   Resident = pt;
+   Resident_Tag_Start = pt->tag_start;
+   return 0;
+}
+
+static int copy_back (Marx_Photon_Type *pt)
+{
+   if (-1 == mxgd_download_aos (pt->attributes, pt->n_photons))
+     {
+        mxg_error ("%s", mxgd_last_error ());
+        return -1;
+     }
+   marx_prune_photons (pt);
+   return 0;
+}
+
+static int finish_stage (Marx_Photon_Type *pt, int force_copy)
+{
+   if (Lazy_Sync && !force_copy)
+     return 0;
+   return copy_back (pt);
+}
+
+/* ---- mirror (mirror.c:48-108) ------------------------------------------------- */
+
+int marx_mirror_init (Param_File_Type *pf)
+{
+   char buf[256];
+   if (-1 == pf_get_string (pf, "MirrorType", buf, sizeof (buf)))
+     return -1;
+   if (strcmp (buf, "HRMA"))
+     {
+        mxg_error ("MirrorType %s not supported.", buf);
+        Mirror_Type = -1;
+        return -1;
+     }
+   if (-1 == mxg_hrma_init (pf))
+     {
+        mxg_error ("Error initializing MirrorType %s.", buf);
+        Mirror_Type = -1;
+        return -1;
+     }
+   Mirror_Type = MARX_MIRROR_HRMA;
+   return Mirror_Type;
+}
+
+int marx_mirror_reflect (Marx_Photon_Type *pt, int verbose)
+{
+   if (Mirror_Type != MARX_MIRROR_HRMA)
+     {
+        mxg_error ("Mirror not initialized.");
+        return -1;
+     }
+   if (verbose > 0) mxg_message (0, "Reflecting from HRMA\n");
+   if (pt->history & MARX_MIRROR_SHELL_OK)
+     return 0;
+   pt->history |= MARX_MIRROR_SHELL_OK;
+   if ((-1 == sync_config ()) || (-1 == make_resident (pt)))
+     return -1;
+   if (-1 == mxgd_run_mirror (pt->source_distance, Seed, NULL))
+     {
+        mxg_error ("%s", mxgd_last_error ());
+        return -1;
+     }
+   return finish_stage (pt, 0);
+}
+
+/* ---- grating (grating.c:44-114) ------------------------------------------------ */
+
+int marx_grating_init (Param_File_Type *pf)
+{
+   char buf[256];
+   int type;
+   if (-1 == pf_get_string (pf, "GratingType", buf, sizeof (buf)))
+     return -1;
+   if (!strcmp (buf, "NONE"))
+     {
+        memset (&Mxg_Cfg.grating, 0, sizeof (Mxg_Cfg.grating));
+        Mxg_Cfg_Generation++;
+        Grating_Type = 0;
+        return 0;
+     }
+   if (strcmp (buf, "HETG") && strcmp (buf, "LETG"))
+     {
+        mxg_error ("GratingType %s not supported.", buf);
+        Grating_Type = -1;
+        return -1;
+     }
+   type = mxg_grating_init (pf);
+   if (type == -1)
+     mxg_error ("Error initializing GratingType %s.", buf);
+   Grating_Type = type;
+   return type;
+}
+
+int marx_grating_diffract (Marx_Photon_Type *pt, int verbose)
+{
+   if (Grating_Type == 0)
+     return 0;
+   if ((Grating_Type != MARX_GRATING_HETG) && (Grating_Type != MARX_GRATING_LETG))
+     {
+        mxg_error ("Grating not initialized.");
+        return -1;
+     }
+   if (verbose > 0)
+     mxg_message (0, "Diffracting from %s.\n", (Grating_Type == MARX_GRATING_LETG) ? "LETG" : "HETG");
+   if (pt->history & MARX_ORDER_OK)
+     return 0;
+   pt->history |= MARX_ORDER_OK;
+   if (Grating_Type == MARX_GRATING_LETG)
+     pt->history |= (MARX_ORDER1_OK | MARX_ORDER2_OK | MARX_ORDER3_OK | MARX_ORDER4_OK);
+   if ((-1 == sync_config ()) || (-1 == make_resident (pt)))
+     return -1;
+   if (-1 == mxgd_run_grating (Seed, NULL))
+     {
+        mxg_error ("%s", mxgd_last_error ());
+        return -1;
+     }
+   return finish_stage (pt, 0);
+}
+
+/* ---- detector (detector.c:298-379) ------------------------------------------- */
+
+int marx_detector_init (Param_File_Type *pf)
+{
+   int type = mxg_detector_init (pf);
+   Detector_Type = type;
+   return type;
+}
+
+int marx_detect (Marx_Photon_Type *pt, int verbose)
+{
+   if (Detector_Type == -1)
+     {
+        mxg_error ("Detector not initialized.");
+        return -1;
+     }
+   if (Detector_Type == 0)
+     {
+        /* NONE: nothing to detect, but a lazily synchronised chunk ends here */
+        if (Lazy_Sync && (Resident == pt))
+          return copy_back (pt);
+        return 0;
+     }
+   if (verbose > 0) mxg_message (0, "Detecting\n");
+   if (pt->history & MARX_DET_NUM_OK)
+     return 0;
+   if ((Detector_Type == MARX_DETECTOR_ACIS_S) || (Detector_Type == MARX_DETECTOR_ACIS_I))
+     pt->history |= (MARX_DET_PIXEL_OK | MARX_DET_NUM_OK | MARX_PULSEHEIGHT_OK | MARX_PI_OK);
+   else
+     pt->history |= (MARX_DET_REGION_OK | MARX_PULSEHEIGHT_OK | MARX_DET_PIXEL_OK | MARX_DET_NUM_OK);
+   if (Detector_Type == MARX_DETECTOR_HRC_S)
+     pt->history |= MARX_DET_UV_PIXEL_OK;
+   if ((-1 == sync_config ()) || (-1 == make_resident (pt)))
+     return -1;
+   if (-1 == mxgd_run_detect (pt->start_time, Seed, NULL))
+     {
+        mxg_error ("%s", mxgd_last_error ());
+        return -1;
+     }
+   return finish_stage (pt, 1);
+}

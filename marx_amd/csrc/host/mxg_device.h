@@ -1,0 +1,60 @@
+/* mxg_device.h -- extern "C" interface of the HIP layer of libmarx_gpu.
+ *
+ * The host C code owns configuration and the AoS photon list; the HIP layer
+ * owns HBM: the configuration copy, the structure-of-arrays photon store,
+ * the active-photon index lists and the kernels.  All functions return 0 on
+ * success and -1 on error (message in mxgd_last_error()).
+ */
+#ifndef MXG_DEVICE_H
+#define MXG_DEVICE_H
+
+#include "marx_gpu.h"
+#include "marx_gpu_cfg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char *mxgd_last_error (void);
+int mxgd_init (void);
+int mxgd_device_info (char *name, unsigned int namelen, int *num_cus);
+
+/* Copies the configuration and arenas to HBM when generation changed. */
+int mxgd_sync_config (const mxg_config *cfg,
+                      const float *fa, unsigned int nf,
+                      const double *da, unsigned int nd,
+                      unsigned int generation);
+
+/* AoS list <-> resident SoA store.  The active list becomes the given
+ * index list (the reference's sorted_index). */
+int mxgd_upload_aos (const Marx_Photon_Attr_Type *at, unsigned int n,
+                     const unsigned int *active, unsigned int nactive);
+int mxgd_download_aos (Marx_Photon_Attr_Type *at, unsigned int n);
+unsigned int mxgd_store_size (void);
+int mxgd_get_soa (mxg_soa_view *view);
+
+/* Stages on the resident store.  Each one processes the current active
+ * list and replaces it by the photons still unabsorbed. */
+int mxgd_run_mirror (double source_distance, unsigned long long seed, float *ms);
+int mxgd_run_grating (unsigned long long seed, float *ms);
+int mxgd_run_detect (double tstart, unsigned long long seed, float *ms);
+
+/* Source: creates n photons (tags tag_start..), times and energies; when
+ * exposure_time > 0 the list is cut after the first photon whose arrival
+ * time reaches it (source.c:313-342).  *n_out receives the photon count. */
+int mxgd_run_source (unsigned int n, unsigned int tag_start,
+                     double exposure_time, unsigned long long seed,
+                     unsigned int *n_out, double *last_time, float *ms);
+int mxgd_run_dither (double start_time, unsigned long long seed, float *ms);
+
+/* Number of photons in the active list (device -> host copy). */
+int mxgd_active_count (unsigned int *n);
+/* Active list indices (host copy, n = mxgd_active_count). */
+int mxgd_active_list (unsigned int *idx, unsigned int n);
+
+int mxgd_synchronize (void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,434 @@
+// dev_common.h -- per-photon device primitives for gfx950.
+//
+// Everything here runs in one lane per photon.  Arithmetic that feeds
+// positions, directions and threshold tests is written in the operation
+// order of the reference's jdmath routines (jdmath/src/vector.c,
+// finterpo.c, dinterpo.c, qroot.c, complex.c) and the file is compiled with
+// -ffp-contract=off, so a lane rounds exactly like the reference's CPU code
+// apart from the last-bit differences of the transcendental functions.
+//
+// Random numbers: one Philox4x32-10 counter stream per (photon tag, stage):
+// key = seed, counter = (tag, stage, block, source id); 4 uniforms per
+// block, u = u32 / (2^32 - 1) (the jdmath/src/random.c:151-154 mapping);
+// polar Gaussians keep their second variate inside the stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "marx_gpu_cfg.h"
+
+#define MXG_PI 3.14159265358979323846264338327950288
+#define MXG_HBAR_C 1.973269631e-4
+
+namespace mxg {
+
+struct vec3 { double x, y, z; };
+
+__device__ __forceinline__ vec3 v3 (double x, double y, double z) { vec3 a; a.x = x; a.y = y; a.z = z; return a; }
+
+// ---- Philox4x32-10 ----------------------------------------------------------
+
+struct Stream
+{
+   uint32_t k0, k1;
+   uint32_t c0, c1, c2, c3;
+   uint32_t b0, b1, b2, b3;
+   int pos;
+   int have_gauss;
+   double g2;
+};
+
+__device__ __forceinline__ void philox (uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                        uint32_t k0, uint32_t k1,
+                                        uint32_t &o0, uint32_t &o1, uint32_t &o2, uint32_t &o3)
+{
+#pragma unroll
+   for (int r = 0; r < 10; r++)
+     {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint32_t lo0 = 0xD2511F53u * c0;
+        uint32_t hi0 = __umulhi (0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2;
+        uint32_t hi1 = __umulhi (0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+     }
+   o0 = c0; o1 = c1; o2 = c2; o3 = c3;
+}
+
+__device__ __forceinline__ void stream_init (Stream &s, unsigned long long seed, uint32_t tag,
+                                             uint32_t stage, uint32_t source_id)
+{
+   s.k0 = (uint32_t) seed;
+   s.k1 = (uint32_t) (seed >> 32);
+   s.c0 = tag; s.c1 = stage; s.c2 = 0; s.c3 = source_id;
+   s.pos = 4;
+   s.have_gauss = 0;
+   s.g2 = 0.0;
+}
+
+__device__ __forceinline__ double uniform (Stream &s)
+{
+   if (s.pos == 4)
+     {
+        philox (s.c0, s.c1, s.c2, s.c3, s.k0, s.k1, s.b0, s.b1, s.b2, s.b3);
+        s.c2++;
+        s.pos = 0;
+     }
+   uint32_t u;
+   switch (s.pos)
+     {
+      case 0: u = s.b0; break;
+      case 1: u = s.b1; break;
+      case 2: u = s.b2; break;
+      default: u = s.b3; break;
+     }
+   s.pos++;
+   return (double) u * (1.0 / ((double) 0xFFFFFFFFu));
+}
+
+__device__ __forceinline__ double gaussian (Stream &s)
+{
+   if (s.have_gauss)
+     {
+        s.have_gauss = 0;
+        return s.g2;
+     }
+   s.have_gauss = 1;
+   double g1, g2, g;
+   do
+     {
+        g1 = 2.0 * uniform (s) - 1.0;
+        g2 = 2.0 * uniform (s) - 1.0;
+        g = g1 * g1 + g2 * g2;
+     }
+   while ((g >= 1.0) || (g == 0.0));
+   double f = sqrt (-2.0 * log (g) / g);
+   s.g2 = g2 * f;
+   return g1 * f;
+}
+
+__device__ __forceinline__ double expn (Stream &s)
+{
+   double r;
+   do r = uniform (s); while (r == 0.0);
+   return -log (r);
+}
+
+// ---- searches and interpolation ----------------------------------------------
+
+__device__ __forceinline__ unsigned int search_f (float x, const float *__restrict__ xp, unsigned int n)
+{
+   unsigned int n0 = 0, n1 = n;
+   while (n1 > n0 + 1)
+     {
+        unsigned int n2 = (n0 + n1) >> 1;
+        float v = xp[n2];
+        if (v >= x)
+          {
+             if (v == x) return n2;
+             n1 = n2;
+          }
+        else n0 = n2;
+     }
+   if (x >= xp[n0]) return n1;
+   return n0;
+}
+
+// float table, float abscissa, double internal arithmetic, float result
+__device__ __forceinline__ float interp_f (float x, const float *__restrict__ xp,
+                                           const float *__restrict__ yp, unsigned int n)
+{
+   if (n == 1) return yp[0];
+   unsigned int n1 = search_f (x, xp, n);
+   unsigned int n0 = n1 - 1;
+   if ((n1 < n) && (x == xp[n1])) return yp[n1];
+   if (n1 == n) { n1--; n0--; }
+   if (n1 == 0) n0 = 1;
+   double x0 = xp[n0], x1 = xp[n1];
+   if (x1 == x0) return yp[n1];
+   float y0 = yp[n0];
+   float dy = yp[n1] - y0;
+   return (float) ((double) y0 + (double) dy / (x1 - x0) * ((double) x - x0));
+}
+
+__device__ __forceinline__ unsigned int search_d (double x, const double *__restrict__ xp, unsigned int n)
+{
+   unsigned int n0 = 0, n1 = n;
+   while (n1 > n0 + 1)
+     {
+        unsigned int n2 = (n0 + n1) >> 1;
+        double v = xp[n2];
+        if (v >= x)
+          {
+             if (v == x) return n2;
+             n1 = n2;
+          }
+        else n0 = n2;
+     }
+   if (x >= xp[n0]) return n1;
+   return n0;
+}
+
+__device__ __forceinline__ double interp_d (double x, const double *__restrict__ xp,
+                                            const double *__restrict__ yp, unsigned int n)
+{
+   if (n == 1) return yp[0];
+   unsigned int n1 = search_d (x, xp, n);
+   unsigned int n0 = n1 - 1;
+   if ((n1 < n) && (x == xp[n1])) return yp[n1];
+   if (n1 == n) { n1--; n0--; }
+   if (n1 == 0) n0 = 1;
+   double x0 = xp[n0], x1 = xp[n1];
+   if (x1 == x0) return yp[n1];
+   return yp[n0] + (yp[n1] - yp[n0]) / (x1 - x0) * (x - x0);
+}
+
+// numerically stable roots; returns 1 real, 0 complex, -1 if a == 0
+__device__ __forceinline__ int quadratic_root (double a, double b, double c, double &rplus, double &rminus)
+{
+   if (a == 0.0) return -1;
+   double bsqr = b * b;
+   double ac4 = a * c * 4;
+   double neg_b_over_2a = -b / (2.0 * a);
+   if (bsqr > ac4)
+     {
+        double factor = 1.0 + sqrt (1.0 - ac4 / bsqr);
+        rplus = -2.0 * c / (b * factor);
+        rminus = neg_b_over_2a * factor;
+        return 1;
+     }
+   if (bsqr == ac4)
+     {
+        rplus = rminus = neg_b_over_2a;
+        return 1;
+     }
+   rplus = neg_b_over_2a;
+   rminus = sqrt (c / a) * sqrt (1.0 - bsqr / ac4);
+   return 0;
+}
+
+// ---- vectors -------------------------------------------------------------------
+
+__device__ __forceinline__ double length (vec3 a)
+{
+   double x = fabs (a.x), y = fabs (a.y), z = fabs (a.z), t;
+   if (z < x) { t = z; z = x; x = t; }
+   if (z < y) { t = z; z = y; y = t; }
+   if (z == 0.0) return 0.0;
+   x = x / z;
+   y = y / z;
+   return z * sqrt (1.0 + x * x + y * y);
+}
+
+__device__ __forceinline__ void normalize (vec3 &a)
+{
+   double len = length (a);
+   if (len != 0.0)
+     {
+        a.x = a.x / len;
+        a.y = a.y / len;
+        a.z = a.z / len;
+     }
+}
+
+__device__ __forceinline__ vec3 unit (vec3 a) { normalize (a); return a; }
+
+__device__ __forceinline__ double dot (vec3 a, vec3 b)
+{
+   return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+
+__device__ __forceinline__ vec3 cross (vec3 a, vec3 b)
+{
+   vec3 c;
+   c.z = a.x * b.y - a.y * b.x;
+   c.x = a.y * b.z - a.z * b.y;
+   c.y = a.z * b.x - a.x * b.z;
+   return c;
+}
+
+__device__ __forceinline__ vec3 ax1_bx2 (double a, vec3 x1, double b, vec3 x2)
+{
+   return v3 (a * x1.x + b * x2.x, a * x1.y + b * x2.y, a * x1.z + b * x2.z);
+}
+
+__device__ __forceinline__ vec3 ax1_bx2_cx3 (double a, vec3 x1, double b, vec3 x2, double c, vec3 x3)
+{
+   return v3 (a * x1.x + b * x2.x + c * x3.x,
+              a * x1.y + b * x2.y + c * x3.y,
+              a * x1.z + b * x2.z + c * x3.z);
+}
+
+__device__ __forceinline__ vec3 add (vec3 a, vec3 b) { return v3 (a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ vec3 sub (vec3 a, vec3 b) { return v3 (a.x - b.x, a.y - b.y, a.z - b.z); }
+
+__device__ __forceinline__ vec3 rotate_vector1 (vec3 p, vec3 n, double c, double s)
+{
+   double pn = dot (p, n);
+   return ax1_bx2_cx3 (c, p, pn * (1.0 - c), n, s, cross (n, p));
+}
+
+__device__ __forceinline__ vec3 rotate_unit_vector1 (vec3 p, vec3 n, double c, double s)
+{
+   vec3 u = rotate_vector1 (p, n, c, s);
+   normalize (u);
+   return u;
+}
+
+__device__ __forceinline__ vec3 rotate_unit_vector (vec3 p, vec3 n, double theta)
+{
+   double s, c;
+   sincos (theta, &s, &c);
+   return rotate_unit_vector1 (p, n, c, s);
+}
+
+__device__ __forceinline__ vec3 m3_vmul (const double *__restrict__ a, vec3 v)
+{
+   return v3 (a[0] * v.x + a[1] * v.y + a[2] * v.z,
+              a[3] * v.x + a[4] * v.y + a[5] * v.z,
+              a[6] * v.x + a[7] * v.y + a[8] * v.z);
+}
+
+__device__ __forceinline__ vec3 ld3 (const double *a) { return v3 (a[0], a[1], a[2]); }
+
+// ---- Fresnel reflectivity (Jackson 1975, eq 7.39 / 7.41) ----------------------
+
+struct cplx { double r, i; };
+
+__device__ __forceinline__ cplx c_mul (cplx a, cplx b)
+{
+   cplx z; z.r = a.r * b.r - a.i * b.i; z.i = a.r * b.i + a.i * b.r; return z;
+}
+
+__device__ __forceinline__ cplx c_div (cplx z1, cplx z2)
+{
+   cplx z;
+   double r1 = z1.r, i1 = z1.i, r2 = z2.r, i2 = z2.i, ratio, denom;
+   if (fabs (r2) > fabs (i2))
+     {
+        ratio = i2 / r2;
+        denom = r2 + i2 * ratio;
+        z.r = (r1 + ratio * i1) / denom;
+        z.i = (i1 - r1 * ratio) / denom;
+     }
+   else
+     {
+        ratio = r2 / i2;
+        denom = r2 * ratio + i2;
+        z.r = (r1 * ratio + i1) / denom;
+        z.i = (i1 * ratio - r1) / denom;
+     }
+   return z;
+}
+
+__device__ __forceinline__ cplx c_a_bz (double a, double b, cplx z1)
+{
+   cplx z; z.r = a + b * z1.r; z.i = b * z1.i; return z;
+}
+
+__device__ __forceinline__ cplx c_az1_bz2 (double a, cplx z1, double b, cplx z2)
+{
+   cplx z; z.r = a * z1.r + b * z2.r; z.i = a * z1.i + b * z2.i; return z;
+}
+
+__device__ __forceinline__ double c_abs (cplx z)
+{
+   double fr = fabs (z.r), fi = fabs (z.i), ratio;
+   if (fr > fi)
+     {
+        ratio = z.i / z.r;
+        return fr * sqrt (1.0 + ratio * ratio);
+     }
+   if (fi == 0.0) return 0.0;
+   ratio = z.r / z.i;
+   return fi * sqrt (1.0 + ratio * ratio);
+}
+
+__device__ __forceinline__ cplx c_sqrt (cplx a)
+{
+   double r = c_abs (a);
+   if (r == 0.0) return a;
+   if (a.r >= 0.0)
+     {
+        a.r = sqrt (0.5 * (r + a.r));
+        a.i = 0.5 * a.i / a.r;
+     }
+   else
+     {
+        r = sqrt (0.5 * (r - a.r));
+        a.r = 0.5 * a.i / r;
+        a.i = r;
+        if (a.r < 0.0) { a.r = -a.r; a.i = -a.i; }
+     }
+   return a;
+}
+
+__device__ __forceinline__ double reflectivity (double cos_theta, double beta, double delta)
+{
+   cplx n; n.r = 1.0 - delta; n.i = beta;
+   double sin_theta = sqrt (1.0 - cos_theta * cos_theta);
+   cplx nsqr = c_mul (n, n);
+   cplx root = c_sqrt (c_a_bz (-sin_theta * sin_theta, 1.0, nsqr));
+   cplx e_perp = c_div (c_a_bz (cos_theta, -1.0, root), c_a_bz (cos_theta, 1.0, root));
+   cplx e_par = c_div (c_az1_bz2 (cos_theta, nsqr, -1.0, root), c_az1_bz2 (cos_theta, nsqr, 1.0, root));
+   return 0.5 * (e_par.r * e_par.r + e_par.i * e_par.i + e_perp.r * e_perp.r + e_perp.i * e_perp.i);
+}
+
+// ---- WFold scattering-angle tables -------------------------------------------------
+
+__device__ __forceinline__ double fold_theta (const double *__restrict__ fa, const float *__restrict__ farena, double p)
+{
+   double p_min = fa[1];
+   if (p < p_min) return 0.0;
+   double p_max = fa[3];
+   if (p > p_max) return pow (fa[4] * (1.0 - p), fa[5]);
+   unsigned int num = (unsigned int) fa[6];
+   const float *t = farena + (unsigned int) fa[7];
+   double delta_i = (p - p_min) / fa[2];
+   unsigned int i = (unsigned int) delta_i;
+   if (i + 1 >= num) return (double) t[num - 1];
+   delta_i -= (double) i;
+   return (1.0 - delta_i) * (double) t[i] + delta_i * (double) t[i + 1];
+}
+
+__device__ __forceinline__ double wfold_interp (const mxg_wfold &w, const float *__restrict__ fa,
+                                                const double *__restrict__ da,
+                                                double energy, double sin_alpha, double r)
+{
+   if (w.n_arrays == 0) return 0.0;
+   const double *arrays = da + w.arrays_off;
+   if (w.n_arrays == 1) return fold_theta (arrays, fa, r);
+   const double *e_alphas = da + w.e_alphas_off;
+   double e_alpha = energy * sin_alpha;
+   unsigned int i = search_d (e_alpha, e_alphas, w.n_arrays);
+   if (i == w.n_arrays) i--;
+   if (i == 0) i++;
+   double theta_0 = fold_theta (arrays + (i - 1) * MXG_FOLD_STRIDE, fa, r);
+   double theta_1 = fold_theta (arrays + i * MXG_FOLD_STRIDE, fa, r);
+   double e0 = e_alphas[i - 1], e1 = e_alphas[i];
+   return theta_0 + (theta_1 - theta_0) * (e_alpha - e0) / (e1 - e0);
+}
+
+// ---- wave-aggregated append of survivors to an index list ---------------------------
+
+__device__ __forceinline__ void append_survivor (bool keep, uint32_t idx, uint32_t *__restrict__ out,
+                                                 uint32_t *__restrict__ out_count)
+{
+   unsigned long long mask = __ballot (keep);
+   if (mask == 0) return;
+   int lane = __lane_id ();
+   int leader = __ffsll ((long long) mask) - 1;
+   uint32_t base = 0;
+   if (lane == leader)
+     base = atomicAdd (out_count, (uint32_t) __popcll (mask));
+   base = __shfl (base, leader);
+   if (keep)
+     {
+        unsigned long long below = mask & ((1ull << lane) - 1ull);
+        out[base + __popcll (below)] = idx;
+     }
+}
+
+} // namespace mxg

@@ -1,0 +1,405 @@
+// device.hip -- HBM side of libmarx_gpu: device selection, the configuration
+// copy, the SoA photon store, AoS <-> SoA conversion, active-list ping-pong
+// and the stage runners called by the host C layer (mxg_device.h).
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+#include <stdarg.h>
+#include <stdint.h>
+
+#include "dev_store.h"
+#include "mxg_device.h"
+
+using namespace mxg;
+
+namespace {
+
+char g_err[512];
+int g_inited = 0;
+hipStream_t g_stream = nullptr;
+hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
+
+// configuration in HBM
+mxg_config *d_cfg = nullptr;
+float *d_fa = nullptr;
+double *d_da = nullptr;
+size_t d_fa_cap = 0, d_da_cap = 0;
+unsigned int g_cfg_gen = 0;
+
+// photon store
+void *d_store = nullptr;
+size_t g_capacity = 0;
+Soa g_soa;
+unsigned int g_n = 0;              // photons in the store
+uint32_t *d_idx[2] = {nullptr, nullptr};
+uint32_t *d_counts = nullptr;      // [0],[1] list lengths, [2..] scratch
+int g_cur = 0;                     // list holding the active photons
+unsigned int g_upper = 0;          // host-side upper bound of the active count
+Marx_Photon_Attr_Type *d_aos = nullptr;
+size_t g_aos_cap = 0;
+
+int fail (const char *fmt, ...)
+{
+   va_list ap;
+   va_start (ap, fmt);
+   vsnprintf (g_err, sizeof (g_err), fmt, ap);
+   va_end (ap);
+   fprintf (stderr, "***marx_gpu (device): %s\n", g_err);
+   return -1;
+}
+
+#define HIPCHECK(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) \
+   return fail ("%s failed: %s", #call, hipGetErrorString (e_)); } while (0)
+
+size_t align256 (size_t n) { return (n + 255) & ~(size_t) 255; }
+
+__global__ void k_set_u32 (uint32_t *p, uint32_t v) { *p = v; }
+
+__global__ void k_iota (uint32_t *idx, uint32_t n)
+{
+   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) idx[i] = i;
+}
+
+__global__ __launch_bounds__ (kBlock)
+void k_aos_to_soa (const Marx_Photon_Attr_Type *__restrict__ a, Soa s, uint32_t n)
+{
+   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= n) return;
+   const Marx_Photon_Attr_Type &at = a[i];
+   s.energy[i] = at.energy;
+   s.x0[i] = at.x.x; s.x1[i] = at.x.y; s.x2[i] = at.x.z;
+   s.p0[i] = at.p.x; s.p1[i] = at.p.y; s.p2[i] = at.p.z;
+   s.t[i] = at.arrival_time;
+   s.flags[i] = at.flags;
+   s.ypix[i] = at.y_pixel; s.zpix[i] = at.z_pixel;
+   s.upix[i] = at.u_pixel; s.vpix[i] = at.v_pixel;
+   s.dith[0][i] = at.dither_state.ra; s.dith[1][i] = at.dither_state.dec;
+   s.dith[2][i] = at.dither_state.roll; s.dith[3][i] = at.dither_state.dy;
+   s.dith[4][i] = at.dither_state.dz; s.dith[5][i] = at.dither_state.dtheta;
+   s.pi[i] = at.pi;
+   s.pha[i] = at.pulse_height;
+   s.shell[i] = (uint8_t) at.mirror_shell;
+   s.ccd[i] = at.ccd_num; s.region[i] = at.detector_region; s.order[i] = at.order;
+   for (int k = 0; k < 4; k++) s.sup[k][i] = at.support_orders[k];
+   s.tag[i] = at.tag;
+}
+
+__global__ __launch_bounds__ (kBlock)
+void k_soa_to_aos (Marx_Photon_Attr_Type *__restrict__ a, Soa s, uint32_t n)
+{
+   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= n) return;
+   Marx_Photon_Attr_Type at;
+   memset (&at, 0, sizeof (at));
+   at.energy = s.energy[i];
+   at.x.x = s.x0[i]; at.x.y = s.x1[i]; at.x.z = s.x2[i];
+   at.p.x = s.p0[i]; at.p.y = s.p1[i]; at.p.z = s.p2[i];
+   at.arrival_time = s.t[i];
+   at.flags = s.flags[i];
+   at.y_pixel = s.ypix[i]; at.z_pixel = s.zpix[i];
+   at.u_pixel = s.upix[i]; at.v_pixel = s.vpix[i];
+   at.dither_state.ra = s.dith[0][i]; at.dither_state.dec = s.dith[1][i];
+   at.dither_state.roll = s.dith[2][i]; at.dither_state.dy = s.dith[3][i];
+   at.dither_state.dz = s.dith[4][i]; at.dither_state.dtheta = s.dith[5][i];
+   at.pi = s.pi[i];
+   at.pulse_height = s.pha[i];
+   at.mirror_shell = s.shell[i];
+   at.ccd_num = s.ccd[i]; at.detector_region = s.region[i]; at.order = s.order[i];
+   for (int k = 0; k < 4; k++) at.support_orders[k] = s.sup[k][i];
+   at.tag = s.tag[i];
+   a[i] = at;
+}
+
+int ensure_capacity (size_t n)
+{
+   if (n <= g_capacity) return 0;
+   size_t cap = g_capacity ? g_capacity : 1024;
+   while (cap < n) cap *= 2;
+   // bytes per photon of all columns, each column 256-byte aligned
+   size_t sz[] = {8, 8, 8, 8, 8, 8, 8, 8, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 2, 1, 1, 1, 1, 1, 1, 1, 1, 4};
+   size_t total = 0;
+   for (size_t k = 0; k < sizeof (sz) / sizeof (sz[0]); k++) total += align256 (sz[k] * cap);
+   void *mem = nullptr;
+   if (d_store != nullptr) { (void) hipStreamSynchronize (g_stream); (void) hipFree (d_store); d_store = nullptr; }
+   for (int k = 0; k < 2; k++) if (d_idx[k]) { (void) hipFree (d_idx[k]); d_idx[k] = nullptr; }
+   g_capacity = 0;
+   HIPCHECK (hipMalloc (&mem, total));
+   HIPCHECK (hipMalloc (&d_idx[0], cap * sizeof (uint32_t)));
+   HIPCHECK (hipMalloc (&d_idx[1], cap * sizeof (uint32_t)));
+   d_store = mem;
+   char *p = (char *) mem;
+   size_t k = 0;
+   auto take = [&] (void) -> void * { void *r = p; p += align256 (sz[k++] * cap); return r; };
+   g_soa.energy = (double *) take ();
+   g_soa.x0 = (double *) take (); g_soa.x1 = (double *) take (); g_soa.x2 = (double *) take ();
+   g_soa.p0 = (double *) take (); g_soa.p1 = (double *) take (); g_soa.p2 = (double *) take ();
+   g_soa.t = (double *) take ();
+   g_soa.flags = (uint32_t *) take ();
+   g_soa.ypix = (float *) take (); g_soa.zpix = (float *) take ();
+   g_soa.upix = (float *) take (); g_soa.vpix = (float *) take ();
+   for (int j = 0; j < 6; j++) g_soa.dith[j] = (float *) take ();
+   g_soa.pi = (float *) take ();
+   g_soa.pha = (int16_t *) take ();
+   g_soa.shell = (uint8_t *) take ();
+   g_soa.ccd = (int8_t *) take (); g_soa.region = (int8_t *) take (); g_soa.order = (int8_t *) take ();
+   for (int j = 0; j < 4; j++) g_soa.sup[j] = (int8_t *) take ();
+   g_soa.tag = (uint32_t *) take ();
+   g_capacity = cap;
+   return 0;
+}
+
+int ensure_aos (size_t n)
+{
+   if (n <= g_aos_cap) return 0;
+   if (d_aos) { (void) hipStreamSynchronize (g_stream); (void) hipFree (d_aos); d_aos = nullptr; }
+   g_aos_cap = 0;
+   HIPCHECK (hipMalloc (&d_aos, n * sizeof (Marx_Photon_Attr_Type)));
+   g_aos_cap = n;
+   return 0;
+}
+
+ListIO list_io ()
+{
+   ListIO io;
+   io.idx_in = d_idx[g_cur];
+   io.n_in = d_counts + g_cur;
+   io.idx_out = d_idx[1 - g_cur];
+   io.n_out = d_counts + (1 - g_cur);
+   return io;
+}
+
+int begin_stage ()
+{
+   hipLaunchKernelGGL (k_set_u32, dim3 (1), dim3 (1), 0, g_stream, d_counts + (1 - g_cur), 0u);
+   HIPCHECK (hipGetLastError ());
+   return 0;
+}
+
+int end_stage (float *ms)
+{
+   g_cur = 1 - g_cur;
+   if (ms != nullptr)
+     {
+        HIPCHECK (hipEventRecord (g_ev1, g_stream));
+        HIPCHECK (hipEventSynchronize (g_ev1));
+        HIPCHECK (hipEventElapsedTime (ms, g_ev0, g_ev1));
+     }
+   return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *mxgd_last_error (void) { return g_err; }
+
+int mxgd_init (void)
+{
+   if (g_inited) return 0;
+   int n = 0;
+   hipError_t e = hipGetDeviceCount (&n);
+   if ((e != hipSuccess) || (n == 0))
+     return fail ("no HIP device available (%s)", hipGetErrorString (e));
+   int dev = 0;
+   HIPCHECK (hipGetDevice (&dev));
+   hipDeviceProp_t prop;
+   HIPCHECK (hipGetDeviceProperties (&prop, dev));
+   if (strncmp (prop.gcnArchName, "gfx950", 6) != 0)
+     return fail ("device %d is %s; this library is built for gfx950", dev, prop.gcnArchName);
+   HIPCHECK (hipStreamCreateWithFlags (&g_stream, hipStreamNonBlocking));
+   HIPCHECK (hipEventCreate (&g_ev0));
+   HIPCHECK (hipEventCreate (&g_ev1));
+   HIPCHECK (hipMalloc (&d_counts, 64 * sizeof (uint32_t)));
+   HIPCHECK (hipMemset (d_counts, 0, 64 * sizeof (uint32_t)));
+   g_inited = 1;
+   return 0;
+}
+
+int mxgd_device_info (char *name, unsigned int namelen, int *num_cus)
+{
+   if (mxgd_init ()) return -1;
+   int dev = 0;
+   hipDeviceProp_t prop;
+   HIPCHECK (hipGetDevice (&dev));
+   HIPCHECK (hipGetDeviceProperties (&prop, dev));
+   if (name && namelen) { strncpy (name, prop.gcnArchName, namelen - 1); name[namelen - 1] = 0; }
+   if (num_cus) *num_cus = prop.multiProcessorCount;
+   return 0;
+}
+
+int mxgd_sync_config (const mxg_config *cfg, const float *fa, unsigned int nf,
+                      const double *da, unsigned int nd, unsigned int generation)
+{
+   if (mxgd_init ()) return -1;
+   if ((generation == g_cfg_gen) && d_cfg) return 0;
+   HIPCHECK (hipStreamSynchronize (g_stream));
+   if (!d_cfg) HIPCHECK (hipMalloc (&d_cfg, sizeof (mxg_config)));
+   if (nf + 1 > d_fa_cap)
+     {
+        if (d_fa) HIPCHECK (hipFree (d_fa));
+        d_fa = nullptr;
+        d_fa_cap = 0;
+        HIPCHECK (hipMalloc (&d_fa, (nf + 1) * sizeof (float)));
+        d_fa_cap = nf + 1;
+     }
+   if (nd + 1 > d_da_cap)
+     {
+        if (d_da) HIPCHECK (hipFree (d_da));
+        d_da = nullptr;
+        d_da_cap = 0;
+        HIPCHECK (hipMalloc (&d_da, (nd + 1) * sizeof (double)));
+        d_da_cap = nd + 1;
+     }
+   HIPCHECK (hipMemcpy (d_cfg, cfg, sizeof (mxg_config), hipMemcpyHostToDevice));
+   if (nf) HIPCHECK (hipMemcpy (d_fa, fa, nf * sizeof (float), hipMemcpyHostToDevice));
+   if (nd) HIPCHECK (hipMemcpy (d_da, da, nd * sizeof (double), hipMemcpyHostToDevice));
+   g_cfg_gen = generation;
+   return 0;
+}
+
+int mxgd_upload_aos (const Marx_Photon_Attr_Type *at, unsigned int n,
+                     const unsigned int *active, unsigned int nactive)
+{
+   if (mxgd_init ()) return -1;
+   if (nactive > n) return fail ("active list longer than the photon list");
+   if (ensure_capacity (n ? n : 1) || ensure_aos (n ? n : 1)) return -1;
+   if (n)
+     {
+        HIPCHECK (hipMemcpyAsync (d_aos, at, (size_t) n * sizeof (Marx_Photon_Attr_Type),
+                                  hipMemcpyHostToDevice, g_stream));
+        hipLaunchKernelGGL (k_aos_to_soa, dim3 ((n + kBlock - 1) / kBlock), dim3 (kBlock), 0,
+                            g_stream, d_aos, g_soa, (uint32_t) n);
+        HIPCHECK (hipGetLastError ());
+     }
+   g_cur = 0;
+   if (nactive)
+     HIPCHECK (hipMemcpyAsync (d_idx[0], active, (size_t) nactive * sizeof (uint32_t),
+                               hipMemcpyHostToDevice, g_stream));
+   hipLaunchKernelGGL (k_set_u32, dim3 (1), dim3 (1), 0, g_stream, d_counts, (uint32_t) nactive);
+   HIPCHECK (hipGetLastError ());
+   g_n = n;
+   g_upper = nactive;
+   HIPCHECK (hipStreamSynchronize (g_stream));
+   return 0;
+}
+
+int mxgd_download_aos (Marx_Photon_Attr_Type *at, unsigned int n)
+{
+   if (mxgd_init ()) return -1;
+   if (n > g_n) return fail ("download of %u photons from a store of %u", n, g_n);
+   if (n == 0) return 0;
+   if (ensure_aos (n)) return -1;
+   hipLaunchKernelGGL (k_soa_to_aos, dim3 ((n + kBlock - 1) / kBlock), dim3 (kBlock), 0,
+                       g_stream, d_aos, g_soa, (uint32_t) n);
+   HIPCHECK (hipGetLastError ());
+   HIPCHECK (hipMemcpyAsync (at, d_aos, (size_t) n * sizeof (Marx_Photon_Attr_Type),
+                             hipMemcpyDeviceToHost, g_stream));
+   HIPCHECK (hipStreamSynchronize (g_stream));
+   return 0;
+}
+
+unsigned int mxgd_store_size (void) { return g_n; }
+
+int mxgd_get_soa (mxg_soa_view *v)
+{
+   if (mxgd_init ()) return -1;
+   memset (v, 0, sizeof (*v));
+   v->n = g_n;
+   v->capacity = (unsigned int) g_capacity;
+   if (g_capacity == 0) return 0;
+   v->energy = g_soa.energy;
+   v->x[0] = g_soa.x0; v->x[1] = g_soa.x1; v->x[2] = g_soa.x2;
+   v->p[0] = g_soa.p0; v->p[1] = g_soa.p1; v->p[2] = g_soa.p2;
+   v->arrival_time = g_soa.t;
+   v->flags = g_soa.flags;
+   v->y_pixel = g_soa.ypix; v->z_pixel = g_soa.zpix; v->u_pixel = g_soa.upix; v->v_pixel = g_soa.vpix;
+   for (int k = 0; k < 6; k++) v->dither[k] = g_soa.dith[k];
+   v->pi = g_soa.pi;
+   v->pulse_height = g_soa.pha;
+   v->mirror_shell = g_soa.shell;
+   v->ccd_num = g_soa.ccd; v->detector_region = g_soa.region; v->order = g_soa.order;
+   for (int k = 0; k < 4; k++) v->support_orders[k] = g_soa.sup[k];
+   v->tag = g_soa.tag;
+   return 0;
+}
+
+int mxgd_run_mirror (double source_distance, unsigned long long seed, float *ms)
+{
+   if (!d_cfg) return fail ("configuration not on the device");
+   if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
+   if (begin_stage ()) return -1;
+   HIPCHECK (launch_hrma (d_cfg, d_fa, d_da, g_soa, list_io (), g_upper, source_distance, seed, 0,
+                          g_stream));
+   return end_stage (ms);
+}
+
+int mxgd_run_grating (unsigned long long seed, float *ms)
+{
+   if (!d_cfg) return fail ("configuration not on the device");
+   if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
+   if (begin_stage ()) return -1;
+   HIPCHECK (launch_grating (d_cfg, d_fa, d_da, g_soa, list_io (), g_upper, seed, 0, g_stream));
+   return end_stage (ms);
+}
+
+int mxgd_run_detect (double tstart, unsigned long long seed, float *ms)
+{
+   if (!d_cfg) return fail ("configuration not on the device");
+   if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
+   if (begin_stage ()) return -1;
+   HIPCHECK (launch_detect (d_cfg, d_fa, d_da, g_soa, list_io (), g_upper, tstart, seed, 0,
+                            g_stream));
+   return end_stage (ms);
+}
+
+int mxgd_active_count (unsigned int *n)
+{
+   uint32_t v = 0;
+   HIPCHECK (hipMemcpyAsync (&v, d_counts + g_cur, sizeof (uint32_t), hipMemcpyDeviceToHost, g_stream));
+   HIPCHECK (hipStreamSynchronize (g_stream));
+   *n = v;
+   return 0;
+}
+
+int mxgd_active_list (unsigned int *idx, unsigned int n)
+{
+   if (n == 0) return 0;
+   HIPCHECK (hipMemcpyAsync (idx, d_idx[g_cur], (size_t) n * sizeof (uint32_t),
+                             hipMemcpyDeviceToHost, g_stream));
+   HIPCHECK (hipStreamSynchronize (g_stream));
+   return 0;
+}
+
+int mxgd_synchronize (void)
+{
+   if (!g_inited) return 0;
+   HIPCHECK (hipStreamSynchronize (g_stream));
+   return 0;
+}
+
+int mxgd_run_source (unsigned int, unsigned int, double, unsigned long long,
+                     unsigned int *, double *, float *)
+{
+   return fail ("device source stage not built yet");
+}
+
+int mxgd_run_dither (double, unsigned long long, float *)
+{
+   return fail ("device dither stage not built yet");
+}
+
+} // extern "C"
+
+namespace mxg {
+hipError_t launch_grating (const mxg_config *, const float *, const double *, Soa, ListIO,
+                           uint32_t, unsigned long long, uint32_t, hipStream_t)
+{
+   return hipErrorNotSupported;
+}
+hipError_t launch_detect (const mxg_config *, const float *, const double *, Soa, ListIO,
+                          uint32_t, double, unsigned long long, uint32_t, hipStream_t)
+{
+   return hipErrorNotSupported;
+}
+}

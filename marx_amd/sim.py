@@ -1,0 +1,156 @@
+"""Python host API over the C-ABI of libmarx_gpu.
+
+Mirrors the reference's operator interface: a parameter file drives
+``marx_mirror_init`` / ``marx_grating_init`` / ``marx_detector_init`` /
+``marx_create_source``, and a ``Marx_Photon_Type`` list flows through
+``marx_mirror_reflect`` -> ``marx_grating_diffract`` -> ``marx_detect``
+(marx/src/marx.c:240-273, :641-656).  Return codes follow the reference:
+-1 raises ``RuntimeError`` carrying the library's error message.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import ATTR_DTYPE, PAR_FILE, PhotonList, check
+
+
+class ParamFile:
+    """An IRAF-style parameter file (pfile API subset) with overrides."""
+
+    def __init__(self, path=PAR_FILE, **overrides):
+        self.lib = _lib.load()
+        self.handle = self.lib.pf_open_parameter_file(path.encode(), b"rH")
+        if not self.handle:
+            raise RuntimeError(f"cannot open {path}: {_lib.last_error()}")
+        for k, v in overrides.items():
+            self.set(k, v)
+
+    def set(self, name, value):
+        if isinstance(value, bool):
+            value = "yes" if value else "no"
+        name = name.replace("__", "-")
+        check(self.lib.pf_set_value(self.handle, name.encode(), str(value).encode()),
+              f"pf_set_value({name})")
+
+    def get_string(self, name):
+        buf = ctypes.create_string_buffer(1024)
+        check(self.lib.pf_get_string(self.handle, name.encode(), buf, 1024), f"pf_get_string({name})")
+        return buf.value.decode()
+
+    def get_double(self, name):
+        v = ctypes.c_double()
+        check(self.lib.pf_get_double(self.handle, name.encode(), ctypes.byref(v)), f"pf_get_double({name})")
+        return v.value
+
+    def close(self):
+        if self.handle:
+            self.lib.pf_close_parameter_file(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Photons:
+    """A Marx_Photon_Type owned by the library, with a numpy view of its
+    136-byte attribute records."""
+
+    def __init__(self, capacity):
+        self.lib = _lib.load()
+        self.ptr = self.lib.marx_alloc_photon_type(int(capacity))
+        if not self.ptr:
+            raise MemoryError("marx_alloc_photon_type failed")
+        self.capacity = int(capacity)
+
+    @property
+    def pt(self):
+        return self.ptr.contents
+
+    @property
+    def attrs(self):
+        """All capacity records (structured array view, no copy)."""
+        buf = (ctypes.c_char * (self.capacity * ATTR_DTYPE.itemsize)).from_address(self.pt.attributes)
+        return np.frombuffer(buf, dtype=ATTR_DTYPE)
+
+    @property
+    def active(self):
+        """The first n_photons records."""
+        return self.attrs[: self.pt.n_photons]
+
+    def sorted_index(self):
+        n = self.pt.num_sorted
+        return np.ctypeslib.as_array(self.pt.sorted_index, shape=(max(n, 1),))[:n].copy()
+
+    def load(self, records, sort=True):
+        """Fills the list with records (ATTR_DTYPE) and builds sorted_index in
+        energy order with index tie-break (marx/libsrc/source.c:361-371)."""
+        n = len(records)
+        if n > self.capacity:
+            raise ValueError("too many photons for this list")
+        a = self.attrs
+        a[:n] = records
+        p = self.pt
+        p.n_photons = n
+        p.num_sorted = n
+        p.history = 0x1F
+        order = np.lexsort((np.arange(n), records["energy"])) if sort else np.arange(n)
+        idx = np.ctypeslib.as_array(p.sorted_index, shape=(self.capacity,))
+        idx[:n] = order.astype(np.uint32)
+        en = np.ctypeslib.as_array(p.sorted_energies, shape=(self.capacity,))
+        en[:n] = records["energy"][order]
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                self.lib.marx_dealloc_photon_type(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class Tracer:
+    """Initialises the path's modules from a parameter file and runs them."""
+
+    def __init__(self, par=None, seed=1, **overrides):
+        self.lib = _lib.load()
+        self.par = par if par is not None else ParamFile(**overrides)
+        self.lib.mxg_set_seed(int(seed))
+        self.mirror = check(self.lib.marx_mirror_init(self.par.handle), "marx_mirror_init")
+        self.grating = check(self.lib.marx_grating_init(self.par.handle), "marx_grating_init")
+        self.detector = check(self.lib.marx_detector_init(self.par.handle), "marx_detector_init")
+
+    def set_seed(self, seed):
+        self.lib.mxg_set_seed(int(seed))
+
+    def reflect(self, photons, verbose=0):
+        check(self.lib.marx_mirror_reflect(photons.ptr, verbose), "marx_mirror_reflect")
+
+    def diffract(self, photons, verbose=0):
+        check(self.lib.marx_grating_diffract(photons.ptr, verbose), "marx_grating_diffract")
+
+    def detect(self, photons, verbose=0):
+        check(self.lib.marx_detect(photons.ptr, verbose), "marx_detect")
+
+    def config_pointers(self):
+        """(cfg, farena, nf, darena, nd) raw pointers of the initialised tables."""
+        cfg, fa, da = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        nf, nd = ctypes.c_uint(), ctypes.c_uint()
+        self.lib.mxg_get_config(ctypes.byref(cfg), ctypes.byref(fa), ctypes.byref(nf),
+                                ctypes.byref(da), ctypes.byref(nd))
+        return cfg.value, fa.value, nf.value, da.value, nd.value
+
+
+def on_axis_photons(n, energy, tag_start=0, rng=None, time_span=1.0e4):
+    """n records of an on-axis point source: p = (-1, 0, 0), given energy
+    (scalar or array), arrival times uniform over time_span."""
+    rec = np.zeros(n, dtype=ATTR_DTYPE)
+    rec["energy"] = energy
+    rec["p"] = (-1.0, 0.0, 0.0)
+    rec["tag"] = np.arange(tag_start, tag_start + n, dtype=np.uint32)
+    rng = rng if rng is not None else np.random.default_rng(0)
+    rec["arrival_time"] = np.sort(rng.uniform(0.0, time_span, n))
+    return rec

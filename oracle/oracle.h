@@ -1,0 +1,92 @@
+/* oracle.h -- FP64 CPU oracle of the MARX photon ray-trace.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the
+ * reference's per-photon algorithm (marx/libsrc/{hrma,reflect,wfold,
+ * diffract,detector,acis-*,hrc-*,hrcblur,drake,source,s-point,s-disk,
+ * spectrum,dither}.c and the jdmath/src sources), written to follow the reference's
+ * own loop structure: stage-major passes over an energy-sorted index list,
+ * pruning between passes, shell-major grating passes, the HESF pass before
+ * HRC detection.  It is used by the tests, by the smoke check and as the
+ * CPU baseline of the benchmark, as the checker -- never by the library.
+ *
+ * Random numbers: where the reference draws from its single serial stream
+ * (jdmath/src/random.c:151-154), the oracle draws from the per-photon
+ * counter streams that the GPU library uses: Philox4x32-10 with
+ * key = (seed low, seed high) and counter = (photon tag, stage, block,
+ * source id), 4 draws per block, uniform = u32 / (2^32 - 1) as in
+ * random.c:151-154.  Each photon-stage stream keeps its own cached second
+ * polar-Gaussian variate (the reference keeps one global cache,
+ * gaussrnd.c:30-55).  With that stream definition the GPU and the oracle
+ * must agree photon for photon.
+ *
+ * Tables and geometry come from the library's initialised mxg_config
+ * (include/marx_gpu_cfg.h), so both sides read identical inputs.
+ */
+#ifndef MARX_ORACLE_H
+#define MARX_ORACLE_H
+
+#include "marx_gpu.h"
+#include "marx_gpu_cfg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct
+{
+   const mxg_config *cfg;
+   const float *fa;
+   const double *da;
+   unsigned long long seed;
+   unsigned int source_id;
+}
+orc_ctx;
+
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), one block. */
+void orc_philox4x32_10 (const unsigned int ctr[4], const unsigned int key[2],
+                        unsigned int out[4]);
+
+/* First n uniforms / Gaussians of one photon-stage stream. */
+void orc_stream_uniforms (unsigned long long seed, unsigned int tag,
+                          unsigned int stage, unsigned int source_id,
+                          double *u, unsigned int n);
+void orc_stream_gaussians (unsigned long long seed, unsigned int tag,
+                           unsigned int stage, unsigned int source_id,
+                           double *g, unsigned int n);
+
+/* jdmath restatements exposed for the known-answer tests. */
+unsigned int orc_binary_search_f (float x, const float *xp, unsigned int n);
+float orc_interpolate_f (float x, const float *xp, const float *yp, unsigned int n);
+unsigned int orc_binary_search_d (double x, const double *xp, unsigned int n);
+double orc_interpolate_d (double x, const double *xp, const double *yp, unsigned int n);
+int orc_quadratic_root (double a, double b, double c, double *rplus, double *rminus);
+double orc_reflectivity (double cos_theta, double beta, double delta);
+double orc_wfold_interp (const orc_ctx *c, const mxg_wfold *w, double energy,
+                         double sin_alpha, double r);
+/* JDMinterpolate_n_fvector over one column set: xp sorted, out[k][i]. */
+int orc_interpolate_n_fvector (const float *xp, float **yp, unsigned int n,
+                               const float *oldxp, float **oldyp,
+                               unsigned int oldn, unsigned int n_yp);
+
+/* Stage-major passes, in the order marx/src/marx.c:240-273 calls them. */
+int orc_create_photons (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int num,
+                        unsigned int *num_collected, double *exposure_time);
+int orc_mirror_reflect (const orc_ctx *c, Marx_Photon_Type *pt);
+int orc_grating_diffract (const orc_ctx *c, Marx_Photon_Type *pt);
+int orc_detect (const orc_ctx *c, Marx_Photon_Type *pt);
+
+/* Sorts the photons by energy (index tie-break) into sorted_index /
+ * sorted_energies as source.c:361-371 does, allocating sorted_index. */
+int orc_sort_photons (Marx_Photon_Type *pt);
+void orc_prune_photons (Marx_Photon_Type *pt);
+
+/* Whole pipeline over num photons, OpenMP-parallel over photons (chunks of
+ * chunk photons, each chunk traced stage-major).  Used as the CPU baseline.
+ * Returns the number of photons detected, -1 on error. */
+long orc_trace_parallel (const orc_ctx *c, unsigned int num, unsigned int chunk,
+                         int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

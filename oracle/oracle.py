@@ -1,0 +1,175 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's CPU-baseline leg, always as the checker.  The oracle reads the
+tables of an initialised libmarx_gpu configuration (mxg_get_config) and runs
+the reference's stage-major passes on the CPU over a Marx_Photon_Type list.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class OrcCtx(ctypes.Structure):
+    _fields_ = [("cfg", ctypes.c_void_p), ("fa", ctypes.c_void_p), ("da", ctypes.c_void_p),
+                ("seed", ctypes.c_ulonglong), ("source_id", ctypes.c_uint)]
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not built: run `make -C oracle`")
+    lib = ctypes.CDLL(LIB_PATH)
+    u32p = ctypes.POINTER(ctypes.c_uint)
+    f32p = ctypes.POINTER(ctypes.c_float)
+    f64p = ctypes.POINTER(ctypes.c_double)
+    lib.orc_philox4x32_10.argtypes = [u32p, u32p, u32p]
+    lib.orc_philox4x32_10.restype = None
+    for name in ("orc_stream_uniforms", "orc_stream_gaussians"):
+        fn = getattr(lib, name)
+        fn.argtypes = [ctypes.c_ulonglong, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, f64p, ctypes.c_uint]
+        fn.restype = None
+    lib.orc_binary_search_f.argtypes = [ctypes.c_float, f32p, ctypes.c_uint]
+    lib.orc_binary_search_f.restype = ctypes.c_uint
+    lib.orc_interpolate_f.argtypes = [ctypes.c_float, f32p, f32p, ctypes.c_uint]
+    lib.orc_interpolate_f.restype = ctypes.c_float
+    lib.orc_binary_search_d.argtypes = [ctypes.c_double, f64p, ctypes.c_uint]
+    lib.orc_binary_search_d.restype = ctypes.c_uint
+    lib.orc_interpolate_d.argtypes = [ctypes.c_double, f64p, f64p, ctypes.c_uint]
+    lib.orc_interpolate_d.restype = ctypes.c_double
+    lib.orc_quadratic_root.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, f64p, f64p]
+    lib.orc_quadratic_root.restype = ctypes.c_int
+    lib.orc_reflectivity.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    lib.orc_reflectivity.restype = ctypes.c_double
+    lib.orc_wfold_interp.argtypes = [ctypes.POINTER(OrcCtx), ctypes.c_void_p, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double]
+    lib.orc_wfold_interp.restype = ctypes.c_double
+    lib.orc_interpolate_n_fvector.argtypes = [f32p, ctypes.POINTER(f32p), ctypes.c_uint, f32p,
+                                              ctypes.POINTER(f32p), ctypes.c_uint, ctypes.c_uint]
+    lib.orc_interpolate_n_fvector.restype = ctypes.c_int
+    for name in ("orc_mirror_reflect", "orc_grating_diffract", "orc_detect"):
+        fn = getattr(lib, name)
+        fn.argtypes = [ctypes.POINTER(OrcCtx), ctypes.c_void_p]
+        fn.restype = ctypes.c_int
+    lib.orc_create_photons.argtypes = [ctypes.POINTER(OrcCtx), ctypes.c_void_p, ctypes.c_uint,
+                                       u32p, f64p]
+    lib.orc_create_photons.restype = ctypes.c_int
+    lib.orc_sort_photons.argtypes = [ctypes.c_void_p]
+    lib.orc_sort_photons.restype = ctypes.c_int
+    lib.orc_trace_parallel.argtypes = [ctypes.POINTER(OrcCtx), ctypes.c_uint, ctypes.c_uint, ctypes.c_int]
+    lib.orc_trace_parallel.restype = ctypes.c_long
+    _lib = lib
+    return lib
+
+
+def _arr(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def philox(ctr, key):
+    lib = load()
+    c = np.asarray(ctr, dtype=np.uint32)
+    k = np.asarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib.orc_philox4x32_10(_arr(c, ctypes.c_uint), _arr(k, ctypes.c_uint), _arr(out, ctypes.c_uint))
+    return out
+
+
+def stream_uniforms(seed, tag, stage, n, source_id=0):
+    out = np.zeros(n, dtype=np.float64)
+    load().orc_stream_uniforms(seed, tag, stage, source_id, _arr(out, ctypes.c_double), n)
+    return out
+
+
+def stream_gaussians(seed, tag, stage, n, source_id=0):
+    out = np.zeros(n, dtype=np.float64)
+    load().orc_stream_gaussians(seed, tag, stage, source_id, _arr(out, ctypes.c_double), n)
+    return out
+
+
+def interpolate_f(x, xp, yp):
+    xp = np.ascontiguousarray(xp, dtype=np.float32)
+    yp = np.ascontiguousarray(yp, dtype=np.float32)
+    return load().orc_interpolate_f(float(x), _arr(xp, ctypes.c_float), _arr(yp, ctypes.c_float), len(xp))
+
+
+def binary_search_f(x, xp):
+    xp = np.ascontiguousarray(xp, dtype=np.float32)
+    return load().orc_binary_search_f(float(x), _arr(xp, ctypes.c_float), len(xp))
+
+
+def interpolate_d(x, xp, yp):
+    xp = np.ascontiguousarray(xp, dtype=np.float64)
+    yp = np.ascontiguousarray(yp, dtype=np.float64)
+    return load().orc_interpolate_d(float(x), _arr(xp, ctypes.c_double), _arr(yp, ctypes.c_double), len(xp))
+
+
+def quadratic_root(a, b, c):
+    rp, rm = ctypes.c_double(), ctypes.c_double()
+    st = load().orc_quadratic_root(a, b, c, ctypes.byref(rp), ctypes.byref(rm))
+    return st, rp.value, rm.value
+
+
+def reflectivity(cos_theta, beta, delta):
+    return load().orc_reflectivity(cos_theta, beta, delta)
+
+
+def interpolate_n_fvector(x, oldx, oldy):
+    """x (n,), oldx (m,), oldy (k, m) -> (k, n) float32."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    oldx = np.ascontiguousarray(oldx, dtype=np.float32)
+    oldy = [np.ascontiguousarray(r, dtype=np.float32) for r in oldy]
+    out = [np.zeros(len(x), dtype=np.float32) for _ in oldy]
+    f32p = ctypes.POINTER(ctypes.c_float)
+    yp = (f32p * len(out))(*[_arr(o, ctypes.c_float) for o in out])
+    oyp = (f32p * len(oldy))(*[_arr(o, ctypes.c_float) for o in oldy])
+    st = load().orc_interpolate_n_fvector(_arr(x, ctypes.c_float), yp, len(x), _arr(oldx, ctypes.c_float),
+                                          oyp, len(oldx), len(oldy))
+    if st != 0:
+        raise ValueError("interpolate_n_fvector needs at least 2 table rows")
+    return np.array(out)
+
+
+class Oracle:
+    """Runs the oracle passes with the tables of an initialised Tracer."""
+
+    def __init__(self, tracer, seed=None, source_id=0):
+        self.lib = load()
+        cfg, fa, _, da, _ = tracer.config_pointers()
+        s = tracer.lib.mxg_get_seed() if seed is None else seed
+        self.ctx = OrcCtx(cfg, fa, da, int(s), int(source_id))
+
+    def _run(self, fn, photons):
+        st = fn(ctypes.byref(self.ctx), ctypes.cast(photons.ptr, ctypes.c_void_p))
+        if st != 0:
+            raise RuntimeError(f"oracle pass {fn.__name__} failed ({st})")
+
+    def reflect(self, photons):
+        self._run(self.lib.orc_mirror_reflect, photons)
+
+    def diffract(self, photons):
+        self._run(self.lib.orc_grating_diffract, photons)
+
+    def detect(self, photons):
+        self._run(self.lib.orc_detect, photons)
+
+    def create_photons(self, photons, num, exposure_time=None):
+        n = ctypes.c_uint()
+        exp = ctypes.c_double(exposure_time) if exposure_time is not None else None
+        st = self.lib.orc_create_photons(ctypes.byref(self.ctx), ctypes.cast(photons.ptr, ctypes.c_void_p),
+                                         num, ctypes.byref(n), ctypes.byref(exp) if exp is not None else None)
+        if st != 0:
+            raise RuntimeError("oracle create_photons failed")
+        return n.value
+
+    def trace_parallel(self, num, chunk=100000, nthreads=0):
+        return self.lib.orc_trace_parallel(ctypes.byref(self.ctx), num, chunk, nthreads)

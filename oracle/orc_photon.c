@@ -1,0 +1,87 @@
+/* orc_photon.c -- oracle photon-list bookkeeping: energy sort, prune and
+ * the per-photon stream tables.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *   sort:  marx/libsrc/source.c:361-371 with jdmath/src/dsort.c:28-61
+ *          (qsort of indices by energy, index tie-break)
+ *   prune: marx/libsrc/photon.c:40-63
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle_int.h"
+
+static const double *Sort_Energies;
+
+static int energy_cmp (const void *a, const void *b)
+{
+   unsigned int ia = *(const unsigned int *) a, ib = *(const unsigned int *) b;
+   double x = Sort_Energies[ia], y = Sort_Energies[ib];
+   if (x > y) return 1;
+   if (x < y) return -1;
+   if (ia > ib) return 1;
+   if (ia == ib) return 0;
+   return -1;
+}
+
+int orc_sort_photons (Marx_Photon_Type *pt)
+{
+   unsigned int n = pt->n_photons, i;
+   double *e;
+   unsigned int *idx;
+
+   e = (double *) malloc ((n ? n : 1) * sizeof (double));
+   idx = (unsigned int *) malloc ((n ? n : 1) * sizeof (unsigned int));
+   if ((e == NULL) || (idx == NULL))
+     {
+        free (e);
+        free (idx);
+        return -1;
+     }
+   for (i = 0; i < n; i++)
+     {
+        e[i] = pt->attributes[i].energy;
+        idx[i] = i;
+     }
+   Sort_Energies = e;
+   qsort (idx, n, sizeof (unsigned int), energy_cmp);
+   free (pt->sorted_index);
+   pt->sorted_index = idx;
+   if (pt->sorted_energies == NULL)
+     pt->sorted_energies = (double *) malloc ((pt->max_n_photons ? pt->max_n_photons : 1) * sizeof (double));
+   for (i = 0; i < n; i++)
+     pt->sorted_energies[i] = e[idx[i]];
+   pt->num_sorted = n;
+   free (e);
+   return 0;
+}
+
+void orc_prune_photons (Marx_Photon_Type *pt)
+{
+   unsigned int i, new_num = 0;
+   for (i = 0; i < pt->num_sorted; i++)
+     {
+        unsigned int indx = pt->sorted_index[i];
+        Marx_Photon_Attr_Type *at = pt->attributes + indx;
+        if (at->flags & BAD_PHOTON_MASK) continue;
+        pt->sorted_energies[new_num] = at->energy;
+        pt->sorted_index[new_num] = indx;
+        new_num++;
+     }
+   pt->num_sorted = new_num;
+}
+
+orc_stream *orc_streams_alloc (unsigned int n)
+{
+   return (orc_stream *) calloc (n ? n : 1, sizeof (orc_stream));
+}
+
+orc_stream *orc_photon_stream (const orc_ctx *c, orc_stream *streams,
+                               const Marx_Photon_Type *pt,
+                               const Marx_Photon_Attr_Type *at, unsigned int stage)
+{
+   orc_stream *s = streams + (at - pt->attributes);
+   if (!s->inited)
+     orc_stream_init (s, c->seed, at->tag, stage, c->source_id);
+   return s;
+}
