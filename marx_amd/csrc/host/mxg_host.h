@@ -95,6 +95,8 @@ int mxg_fits_get_float (mxg_fits_table *t, const char *name, unsigned int row,
 int mxg_fits_get_string (mxg_fits_table *t, const char *name, unsigned int row,
                          char *buf, unsigned int buflen);
 
+float mxg_interpolate_f (float x, const float *xp, const float *yp, unsigned int n);
+
 /* ---- vector helpers restating jdmath's arithmetic (vector.c, rotate.c) --- */
 JDMVector_Type mxg_v (double x, double y, double z);
 double mxg_v_length (JDMVector_Type a);

@@ -2,12 +2,6 @@
 #include <string.h>
 #include "mxg_host.h"
 
-int mxg_grating_init (Param_File_Type *pf)
-{
-   (void) pf;
-   mxg_error ("grating module not built yet");
-   return -1;
-}
 
 int mxg_detector_init (Param_File_Type *pf)
 {

@@ -128,6 +128,42 @@ int mxg_get_config (const mxg_config **cfg, const float **farena, unsigned int *
    return 0;
 }
 
+/* ---- float-table interpolation (jdmath/src/finterpo.c:37-85) --------------- */
+
+static unsigned int search_f (float x, const float *xp, unsigned int n)
+{
+   unsigned int n0 = 0, n1 = n, n2;
+   while (n1 > n0 + 1)
+     {
+        n2 = (n0 + n1) / 2;
+        if (xp[n2] >= x)
+          {
+             if (xp[n2] == x) return n2;
+             n1 = n2;
+          }
+        else n0 = n2;
+     }
+   if (x >= xp[n0]) return n1;
+   return n0;
+}
+
+/* float abscissa and table, double arithmetic, linear extrapolation */
+float mxg_interpolate_f (float x, const float *xp, const float *yp, unsigned int n)
+{
+   unsigned int n0, n1;
+   double x0, x1;
+   if (n == 1) return yp[0];
+   n1 = search_f (x, xp, n);
+   n0 = n1 - 1;
+   if ((n1 < n) && (x == xp[n1])) return yp[n1];
+   if (n1 == n) { n1--; n0--; }
+   if (n1 == 0) n0 = 1;
+   x0 = xp[n0];
+   x1 = xp[n1];
+   if (x1 == x0) return yp[n1];
+   return (float) (yp[n0] + (yp[n1] - yp[n0]) / (x1 - x0) * (x - x0));
+}
+
 /* ---- vectors (jdmath/src/vector.c) -------------------------------------- */
 
 JDMVector_Type mxg_v (double x, double y, double z)
