@@ -391,10 +391,3 @@ int mxgd_run_dither (double, unsigned long long, float *)
 
 } // extern "C"
 
-namespace mxg {
-hipError_t launch_detect (const mxg_config *, const float *, const double *, Soa, ListIO,
-                          uint32_t, double, unsigned long long, uint32_t, hipStream_t)
-{
-   return hipErrorNotSupported;
-}
-}

@@ -1,7 +1,6 @@
 /* orc_pending.c -- oracle passes not restated yet (test infrastructure). */
 #include "oracle_int.h"
 
-int orc_detect (const orc_ctx *c, Marx_Photon_Type *pt) { (void) c; (void) pt; return -1; }
 int orc_create_photons (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int num,
                         unsigned int *num_collected, double *exposure_time)
 { (void) c; (void) pt; (void) num; (void) num_collected; (void) exposure_time; return -1; }
