@@ -217,6 +217,11 @@ unsigned long long mxg_get_seed (void);
  * valid on return, like the reference. */
 void mxg_set_lazy_sync (int on);
 
+/* Arrival-time model of the initialised source: tmax > 0 draws times
+ * uniformly over [0, tmax); tmax <= 0 restores the reference's Poisson
+ * arrivals at the source's flux (source.c:313-342). */
+void mxg_set_uniform_times (double tmax);
+
 /* Device-resident pipeline: creates num photons of the initialised source
  * on the GPU and runs dither, mirror, grating and detector stages without
  * leaving HBM.  tag_start / start_time play the role of the fields of

@@ -103,6 +103,16 @@ _SIGNATURES = [
     ("mxg_get_seed", ctypes.c_ulonglong, []),
     ("mxg_set_lazy_sync", None, [ctypes.c_int]),
     ("mxg_get_soa", ctypes.c_int, [ctypes.POINTER(SoaView)]),
+    ("marx_create_source", ctypes.c_void_p, [ctypes.c_void_p]),
+    ("marx_open_source", ctypes.c_int, [ctypes.c_void_p]),
+    ("marx_close_source", ctypes.c_int, [ctypes.c_void_p]),
+    ("marx_create_photons", ctypes.c_int, [ctypes.c_void_p, PPhotonList, ctypes.c_uint,
+                                           ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_double)]),
+    ("mxg_set_uniform_times", None, [ctypes.c_double]),
+    ("mxg_trace_device", ctypes.c_int, [ctypes.c_uint, ctypes.c_uint, ctypes.c_double, ctypes.c_double,
+                                        ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_uint),
+                                        ctypes.POINTER(ctypes.c_float)]),
+    ("mxg_download_photons", ctypes.c_int, [PPhotonList]),
     ("mxg_device_info", ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint, ctypes.POINTER(ctypes.c_int)]),
     ("mxg_get_config", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
                                       ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_void_p),

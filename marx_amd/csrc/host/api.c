@@ -280,3 +280,183 @@ int marx_detect (Marx_Photon_Type *pt, int verbose)
      }
    return finish_stage (pt, 1);
 }
+
+/* ---- source (source.c:268-384) ------------------------------------------------- */
+
+static const double *Sort_Keys;
+
+static int energy_cmp (const void *a, const void *b)
+{
+   unsigned int ia = *(const unsigned int *) a, ib = *(const unsigned int *) b;
+   double x = Sort_Keys[ia], y = Sort_Keys[ib];
+   if (x > y) return 1;
+   if (x < y) return -1;
+   return (ia > ib) - (ia < ib);
+}
+
+/* sorted_index / sorted_energies in energy order with index tie-break
+ * (source.c:361-371, dsort.c:28-61) */
+static int sort_by_energy (Marx_Photon_Type *pt)
+{
+   unsigned int n = pt->n_photons, i;
+   double *e = (double *) malloc ((n ? n : 1) * sizeof (double));
+   if (e == NULL) return -1;
+   for (i = 0; i < n; i++)
+     {
+        e[i] = pt->attributes[i].energy;
+        pt->sorted_index[i] = i;
+     }
+   Sort_Keys = e;
+   qsort (pt->sorted_index, n, sizeof (unsigned int), energy_cmp);
+   for (i = 0; i < n; i++)
+     pt->sorted_energies[i] = e[pt->sorted_index[i]];
+   pt->num_sorted = n;
+   free (e);
+   return 0;
+}
+
+int marx_create_photons (Marx_Source_Type *st, Marx_Photon_Type *pt, unsigned int num,
+                         unsigned int *num_collected, double *exposure_time)
+{
+   const mxg_source_cfg *s = &Mxg_Cfg.src;
+   unsigned int n = 0;
+   double last_t = 0.0;
+
+   *num_collected = 0;
+   if ((st == NULL) || (pt == NULL))
+     return -1;
+   if (num > pt->max_n_photons)
+     {
+        mxg_error ("marx_create_photons: %u photons requested, list holds %u", num, pt->max_n_photons);
+        return -1;
+     }
+   pt->source_distance = s->distance;
+   pt->history = 0;
+   pt->start_time += pt->total_time;
+
+   if (-1 == sync_config ())
+     return -1;
+   if (-1 == mxgd_run_source (num, pt->tag_start, pt->start_time,
+                              (exposure_time != NULL) ? *exposure_time : -1.0,
+                              Seed, 0, &n, &last_t, NULL))
+     {
+        mxg_error ("%s", mxgd_last_error ());
+        return -1;
+     }
+   if ((n > 0) && (-1 == mxgd_download_aos (pt->attributes, n)))
+     {
+        mxg_error ("%s", mxgd_last_error ());
+        return -1;
+     }
+   Resident = pt;
+   pt->n_photons = n;
+   pt->history |= (MARX_ENERGY_OK | MARX_TIME_OK | MARX_X_VECTOR_OK | MARX_P_VECTOR_OK | MARX_TAG_OK);
+   pt->tag_start += n;
+   Resident_Tag_Start = pt->tag_start;
+   if (-1 == sort_by_energy (pt))
+     return -1;
+   if (n == 0) pt->total_time = 0.0;
+   else pt->total_time = (s->time_mode == MXG_TIME_UNIFORM) ? s->uniform_tmax : last_t;
+   *num_collected = n;
+   /* the device list is in tag order; the next stage uploads sorted_index */
+   Resident = NULL;
+   return 0;
+}
+
+/* ---- device-resident pipeline ------------------------------------------------------ */
+
+static unsigned int Last_Collected = 0;
+static unsigned int Last_Tag_Start = 0;
+static unsigned long Last_History = 0;
+
+int mxg_trace_device (unsigned int num, unsigned int tag_start, double start_time,
+                      double exposure_time, unsigned int *num_collected,
+                      unsigned int *num_detected, float *stage_ms)
+{
+   float ms[6] = {0, 0, 0, 0, 0, 0};
+   unsigned int n = 0, ndet = 0;
+   double last_t;
+   unsigned long hist = MARX_ENERGY_OK | MARX_TIME_OK | MARX_X_VECTOR_OK | MARX_P_VECTOR_OK | MARX_TAG_OK;
+
+   if (-1 == sync_config ())
+     return -1;
+   if (Mirror_Type != MARX_MIRROR_HRMA)
+     {
+        mxg_error ("Mirror not initialized.");
+        return -1;
+     }
+   if ((-1 == mxgd_run_source (num, tag_start, start_time, exposure_time, Seed, 0, &n, &last_t,
+                               stage_ms ? &ms[0] : NULL))
+       || (-1 == mxgd_run_mirror (Mxg_Cfg.src.distance, Seed, stage_ms ? &ms[2] : NULL)))
+     {
+        mxg_error ("%s", mxgd_last_error ());
+        return -1;
+     }
+   hist |= MARX_MIRROR_SHELL_OK;
+   if ((Grating_Type == MARX_GRATING_HETG) || (Grating_Type == MARX_GRATING_LETG))
+     {
+        if (-1 == mxgd_run_grating (Seed, stage_ms ? &ms[3] : NULL))
+          {
+             mxg_error ("%s", mxgd_last_error ());
+             return -1;
+          }
+        hist |= MARX_ORDER_OK;
+     }
+   if (Detector_Type > 0)
+     {
+        if (-1 == mxgd_run_detect (start_time, Seed, stage_ms ? &ms[4] : NULL))
+          {
+             mxg_error ("%s", mxgd_last_error ());
+             return -1;
+          }
+        hist |= MARX_DET_NUM_OK | MARX_DET_PIXEL_OK;
+     }
+   if (-1 == mxgd_active_count (&ndet))
+     {
+        mxg_error ("%s", mxgd_last_error ());
+        return -1;
+     }
+   Resident = NULL;
+   Last_Collected = n;
+   Last_Tag_Start = tag_start;
+   Last_History = hist;
+   if (num_collected) *num_collected = n;
+   if (num_detected) *num_detected = ndet;
+   if (stage_ms)
+     {
+        ms[5] = ms[0] + ms[1] + ms[2] + ms[3] + ms[4];
+        memcpy (stage_ms, ms, sizeof (ms));
+     }
+   return 0;
+}
+
+int mxg_download_photons (Marx_Photon_Type *pt)
+{
+   unsigned int n = Last_Collected;
+   if (n > pt->max_n_photons)
+     {
+        mxg_error ("photon list too small: %u > %u", n, pt->max_n_photons);
+        return -1;
+     }
+   if ((n > 0) && (-1 == mxgd_download_aos (pt->attributes, n)))
+     {
+        mxg_error ("%s", mxgd_last_error ());
+        return -1;
+     }
+   pt->n_photons = n;
+   pt->tag_start = Last_Tag_Start + n;
+   pt->history = Last_History;
+   if (-1 == sort_by_energy (pt)) return -1;
+   marx_prune_photons (pt);
+   return 0;
+}
+
+int mxg_get_soa (mxg_soa_view *view)
+{
+   if (-1 == mxgd_get_soa (view))
+     {
+        mxg_error ("%s", mxgd_last_error ());
+        return -1;
+     }
+   return 0;
+}

@@ -39,13 +39,14 @@ int mxgd_run_mirror (double source_distance, unsigned long long seed, float *ms)
 int mxgd_run_grating (unsigned long long seed, float *ms);
 int mxgd_run_detect (double tstart, unsigned long long seed, float *ms);
 
-/* Source: creates n photons (tags tag_start..), times and energies; when
- * exposure_time > 0 the list is cut after the first photon whose arrival
- * time reaches it (source.c:313-342).  *n_out receives the photon count. */
-int mxgd_run_source (unsigned int n, unsigned int tag_start,
-                     double exposure_time, unsigned long long seed,
+/* Source: creates n photons (tags tag_start..) with energies, directions,
+ * arrival times and dither; when exposure_time > 0 the list ends at the
+ * first photon whose arrival time reaches it (source.c:313-342).  The
+ * active list becomes those photons; *n_out receives their number and
+ * *last_time the arrival time of the last one. */
+int mxgd_run_source (unsigned int n, unsigned int tag_start, double start_time,
+                     double exposure_time, unsigned long long seed, unsigned int source_id,
                      unsigned int *n_out, double *last_time, float *ms);
-int mxgd_run_dither (double start_time, unsigned long long seed, float *ms);
 
 /* Number of photons in the active list (device -> host copy). */
 int mxgd_active_count (unsigned int *n);

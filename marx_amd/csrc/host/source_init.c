@@ -1,0 +1,367 @@
+/* source_init.c -- source, spectrum and dither initialisation, and the
+ * marx_create_source / marx_create_photons entry points.
+ *
+ * Follows marx/libsrc/source.c:44-417 (select_source, marx_compute_elaz,
+ * marx_create_photons), s-point.c:59-97, s-disk.c:63-127,
+ * spectrum.c:48-300 (FLAT and FILE spectra, rectangle-rule cumulative),
+ * dither.c:111-155 (setup_pointing_ra_dec), :350-450 (_marx_init_dither,
+ * INTERNAL model) and pixlib.c:531-577 (marx_compute_ra_dec_offsets).
+ *
+ * Photons are generated on the GPU from per-photon counter streams; see
+ * the mxg_source_cfg fields for what the kernels read.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+
+#include "mxg_host.h"
+
+struct _Marx_Source_Type
+{
+   int open;
+};
+
+static double Nominal_Ra, Nominal_Dec, Nominal_Roll;
+static double Pointing_Ra, Pointing_Dec, Pointing_Roll;
+
+void mxg_get_nominal_pointing (double *ra, double *dec, double *roll)
+{
+   *ra = Nominal_Ra; *dec = Nominal_Dec; *roll = Nominal_Roll;
+}
+
+void mxg_get_pointing (double *ra, double *dec, double *roll)
+{
+   *ra = Pointing_Ra; *dec = Pointing_Dec; *roll = Pointing_Roll;
+}
+
+/* x to [-PI, PI) */
+static void normalize_angle (double *xp)
+{
+   double x = fmod (*xp, 2 * PI);
+   if (x >= PI) x -= (2 * PI);
+   if (x < -PI) x += (2 * PI);
+   *xp = x;
+}
+
+static void spherical_to_triad (double theta, double phi, JDMVector_Type *r_hat,
+                                JDMVector_Type *theta_hat, JDMVector_Type *phi_hat)
+{
+   double ct = cos (theta), st = sin (theta), cp = cos (phi), sp = sin (phi);
+   *r_hat = mxg_v (st * cp, st * sp, ct);
+   *phi_hat = mxg_v (-sp, cp, 0.0);
+   *theta_hat = mxg_v (ct * cp, ct * sp, -st);
+}
+
+static void setup_pointing_ra_dec (double offset_y, double offset_z)
+{
+   JDMVector_Type p, nom, a, d;
+   double cos_delta, alpha = offset_y, delta = offset_z;
+
+   spherical_to_triad (PI / 2.0 - Nominal_Dec, Nominal_Ra, &nom, &d, &a);
+   d.x = -d.x; d.y = -d.y; d.z = -d.z;
+   cos_delta = cos (delta);
+   {
+      double a1 = cos_delta * cos (alpha), b1 = cos_delta * sin (alpha), c1 = sin (delta);
+      p = mxg_v (a1 * nom.x + b1 * a.x + c1 * d.x,
+                 a1 * nom.y + b1 * a.y + c1 * d.y,
+                 a1 * nom.z + b1 * a.z + c1 * d.z);
+   }
+   p = mxg_v_rotate_unit_vector (p, nom, Nominal_Roll);
+   mxg_v_unit_vector_to_spherical (p, &Pointing_Dec, &Pointing_Ra);
+   Pointing_Dec = PI * 0.5 - Pointing_Dec;
+   Pointing_Roll = Nominal_Roll;
+}
+
+int mxg_dither_init (Param_File_Type *pf, double *yoff, double *zoff)
+{
+   mxg_source_cfg *s = &Mxg_Cfg.src;
+   char model[64];
+   double offset_y, offset_z;
+
+   if ((-1 == pf_get_string (pf, "DitherModel", model, sizeof (model)))
+       || (-1 == pf_get_double (pf, "DitherAmp_RA", &s->ra_amp))
+       || (-1 == pf_get_double (pf, "DitherAmp_Dec", &s->dec_amp))
+       || (-1 == pf_get_double (pf, "DitherAmp_Roll", &s->roll_amp))
+       || (-1 == pf_get_double (pf, "DitherPeriod_RA", &s->ra_period))
+       || (-1 == pf_get_double (pf, "DitherPeriod_Dec", &s->dec_period))
+       || (-1 == pf_get_double (pf, "DitherPeriod_Roll", &s->roll_period))
+       || (-1 == pf_get_double (pf, "DitherPhase_RA", &s->ra_phase))
+       || (-1 == pf_get_double (pf, "DitherPhase_Dec", &s->dec_phase))
+       || (-1 == pf_get_double (pf, "DitherPhase_Roll", &s->roll_phase))
+       || (-1 == pf_get_double (pf, "AspectBlur", &s->aspect_blur))
+       || (-1 == pf_get_double (pf, "RA_Nom", &Nominal_Ra))
+       || (-1 == pf_get_double (pf, "Dec_Nom", &Nominal_Dec))
+       || (-1 == pf_get_double (pf, "Roll_Nom", &Nominal_Roll))
+       || (-1 == pf_get_double (pf, "PointingOffsetY", &offset_y))
+       || (-1 == pf_get_double (pf, "PointingOffsetZ", &offset_z)))
+     {
+        mxg_error ("error getting dither parameters");
+        return -1;
+     }
+   s->aspect_blur *= PI / (180.0 * 3600.0);
+   if (s->aspect_blur < 0) s->aspect_blur = 0.0;
+   offset_y *= PI / (180.0 * 3600.0);
+   offset_z *= PI / (180.0 * 3600.0);
+   *yoff = offset_y;
+   *zoff = offset_z;
+   Nominal_Ra *= PI / 180.0;
+   Nominal_Dec *= PI / 180.0;
+   Nominal_Roll *= PI / 180.0;
+   normalize_angle (&Nominal_Roll);
+   s->nominal_roll = Nominal_Roll;
+
+   if (0 == strcmp (model, "NONE"))
+     {
+        s->dither_mode = MXG_DITHER_NONE;
+        Pointing_Roll = 0.0;
+        setup_pointing_ra_dec (offset_y, offset_z);
+        return 0;
+     }
+   if (0 == strcmp (model, "INTERNAL"))
+     {
+        s->dither_mode = MXG_DITHER_INTERNAL;
+        s->ra_amp = s->ra_amp * (PI / (180.0 * 3600));
+        s->dec_amp = s->dec_amp * (PI / (180.0 * 3600));
+        s->roll_amp = s->roll_amp * (PI / (180.0 * 3600));
+        setup_pointing_ra_dec (offset_y, offset_z);
+        return 0;
+     }
+   mxg_error ("DitherModel = %s is not supported (NONE and INTERNAL are)", model);
+   return -1;
+}
+
+static void compute_ra_dec_offsets (double ra_0, double dec_0, double ra, double dec,
+                                    double *delta_ra, double *delta_dec)
+{
+   double d_ra = ra - ra_0, d_dec = dec - dec_0;
+   double c = cos (dec);
+   double factor = c * (1 - cos (d_ra));
+   double sin_delta_dec = sin (d_dec) + factor * sin (dec_0);
+   double num, den;
+   if (fabs (sin_delta_dec) > 1.0)
+     sin_delta_dec = (sin_delta_dec < 0) ? -1 : 1;
+   *delta_dec = asin (sin_delta_dec);
+   num = c * sin (d_ra);
+   den = cos (d_dec) - factor * cos (dec_0);
+   if (den >= 0) *delta_ra = atan (num / den);
+   else if (num >= 0) *delta_ra = atan (num / den) + PI;
+   else *delta_ra = atan (num / den) - PI;
+}
+
+static void compute_elaz (double ra, double dec, double *azp, double *elp)
+{
+   JDMVector_Type src, pnt;
+   double az, el;
+   src = mxg_v_spherical_to_vector (1.0, PI / 2.0 - dec, ra);
+   pnt = mxg_v_spherical_to_vector (1.0, PI / 2.0 - Pointing_Dec, Pointing_Ra);
+   src = mxg_v_rotate_unit_vector (src, pnt, -Pointing_Roll);
+   mxg_v_unit_vector_to_spherical (src, &el, &az);
+   el = PI / 2.0 - el;
+   compute_ra_dec_offsets (Pointing_Ra, Pointing_Dec, az, el, &az, &el);
+   *elp = el;
+   *azp = az;
+}
+
+/* Cumulative spectrum by the rectangle rule, normalised (spectrum.c:48-92) */
+static int file_spectrum (Param_File_Type *pf, mxg_source_cfg *s)
+{
+   char buf[1024], *file;
+   double *cols[2], tot = 0.0, prev_x, flux;
+   unsigned int n, i;
+   double *cum;
+
+   if ((-1 == pf_get_file (pf, "SpectrumFile", buf, sizeof (buf)))
+       || (-1 == pf_get_double (pf, "SourceFlux", &flux)))
+     return -1;
+   file = (buf[0] == '/') ? strdup (buf) : mxg_data_file_name (buf);
+   if ((buf[0] != '/') && (NULL != file))
+     {
+        FILE *fp = fopen (buf, "r");   /* relative to the working directory first */
+        if (fp != NULL)
+          {
+             fclose (fp);
+             free (file);
+             file = strdup (buf);
+          }
+     }
+   if (file == NULL) return -1;
+   if ((-1 == mxg_read_columns (file, 2, cols, &n)) || (n == 0))
+     {
+        mxg_error ("Error reading spectrum from %s", file);
+        free (file);
+        return -1;
+     }
+   free (file);
+   cum = (double *) malloc (n * sizeof (double));
+   if (cum == NULL) return -1;
+   prev_x = cols[0][0];
+   for (i = 0; i < n; i++)
+     {
+        double this_x = cols[0][i];
+        if (this_x - prev_x < 0.0)
+          {
+             mxg_error ("Your file spectrum appears corrupt.  Perhaps you are using a wavelength grid");
+             free (cum); free (cols[0]); free (cols[1]);
+             return -1;
+          }
+        tot += cols[1][i] * (this_x - prev_x);
+        cum[i] = tot;
+        prev_x = this_x;
+     }
+   if (tot == 0.0)
+     {
+        mxg_error ("Your file spectrum seems to have zero flux in it.");
+        free (cum); free (cols[0]); free (cols[1]);
+        return -1;
+     }
+   for (i = 0; i < n; i++) cum[i] = cum[i] / tot;
+   if (flux <= 0.0) flux = tot;
+   if ((flux <= 0.0) || !isfinite (flux))
+     {
+        mxg_error ("The flux must be positive and finite.  Check your spectrum file");
+        free (cum); free (cols[0]); free (cols[1]);
+        return -1;
+     }
+   s->file_n = n;
+   s->file_e_off = mxg_darena_push (cols[0], n);
+   s->file_cum_off = mxg_darena_push (cum, n);
+   s->total_flux = flux;
+   s->spectrum = MXG_SPECTRUM_FILE;
+   free (cum); free (cols[0]); free (cols[1]);
+   return 0;
+}
+
+static int spectrum_init (Param_File_Type *pf, mxg_source_cfg *s, const char *name)
+{
+   char buf[64];
+   if (-1 == pf_get_string (pf, "SpectrumType", buf, sizeof (buf)))
+     return -1;
+   if (!strcmp (buf, "FLAT"))
+     {
+        double emin, emax, flux;
+        if ((-1 == pf_get_double (pf, "MinEnergy", &emin))
+            || (-1 == pf_get_double (pf, "MaxEnergy", &emax))
+            || (-1 == pf_get_double (pf, "SourceFlux", &flux)))
+          return -1;
+        if (flux <= 0.0)
+          {
+             mxg_error ("SourceFlux for FLAT spectrum must be positive, not %e", flux);
+             return -1;
+          }
+        if ((emax - emin < 0.0) || (emax <= 0.0))
+          {
+             mxg_error ("MinEnergy/MaxEnergy out of range");
+             return -1;
+          }
+        s->spectrum = MXG_SPECTRUM_FLAT;
+        s->emin = emin;
+        s->emax = emax;
+        s->total_flux = flux;
+        return 0;
+     }
+   if (!strcmp (buf, "FILE"))
+     return file_spectrum (pf, s);
+   mxg_error ("SpectrumType: \"%s\" not supported by \"%s\" source", buf, name);
+   return -1;
+}
+
+int mxg_source_init (Param_File_Type *pf)
+{
+   mxg_source_cfg *s = &Mxg_Cfg.src;
+   char name[64];
+   double ra, dec, dist, yoff, zoff, az, el;
+   JDMVector_Type p;
+
+   memset (s, 0, sizeof (*s));
+   if ((-1 == pf_get_double (pf, "SourceDistance", &dist))
+       || (-1 == pf_get_double (pf, "SourceRA", &ra))
+       || (-1 == pf_get_double (pf, "SourceDEC", &dec))
+       || (-1 == pf_get_string (pf, "SourceType", name, sizeof (name))))
+     return -1;
+   if (!strcmp (name, "POINT")) s->type = MXG_SOURCE_POINT;
+   else if (!strcmp (name, "DISK")) s->type = MXG_SOURCE_DISK;
+   else
+     {
+        mxg_error ("SourceType %s not supported (POINT and DISK are)", name);
+        return -1;
+     }
+   dist *= 1000.0;
+   ra *= (PI / 180.0);
+   dec *= (PI / 180.0);
+
+   if (-1 == mxg_dither_init (pf, &yoff, &zoff))
+     return -1;
+   compute_elaz (ra, dec, &az, &el);
+
+   p = mxg_v_spherical_to_vector (1.0, 0.5 * PI - zoff, yoff);
+   p = mxg_v_rotate_unit_vector (p, mxg_v (0, -1, 0), el);
+   p = mxg_v_rotate_unit_vector (p, mxg_v (0, 0, 1), az);
+   /* roll so the point is invariant under the (average) dither roll */
+   p = mxg_v_rotate_unit_vector (p, mxg_v (1, 0, 0), Nominal_Roll);
+   s->p[0] = -p.x; s->p[1] = -p.y; s->p[2] = -p.z;
+   if (p.x <= 0.0)
+     {
+        mxg_error ("Source is located at RA=%f, Dec=%f: rays will not hit the telescope",
+                   ra * 180.0 / PI, dec * 180.0 / PI);
+        return -1;
+     }
+   {
+      JDMVector_Type nrm = mxg_v (-p.y / p.x, 1.0, 0.0);
+      mxg_v_normalize (&nrm);
+      s->p_normal[0] = nrm.x; s->p_normal[1] = nrm.y; s->p_normal[2] = nrm.z;
+   }
+   s->distance = dist;
+
+   if (s->type == MXG_SOURCE_DISK)
+     {
+        double th1, th0;
+        if ((-1 == pf_get_double (pf, "S-DiskTheta1", &th1))
+            || (-1 == pf_get_double (pf, "S-DiskTheta0", &th0)))
+          return -1;
+        s->disk_theta = th1 * (1.0 / 3600.0 * PI / 180.0);
+        s->disk_theta_min = th0 * (1.0 / 3600.0 * PI / 180.0);
+     }
+   if (-1 == spectrum_init (pf, s, name))
+     return -1;
+   /* mean time between photons: 1 / (flux * area), source.c:258-264 */
+   s->mean_time = (s->total_flux <= 0.0) ? 0.0
+     : 1.0 / s->total_flux / Marx_Mirror_Geometric_Area;
+   s->time_mode = MXG_TIME_POISSON;
+   Mxg_Cfg_Generation++;
+   return 0;
+}
+
+/* ---- entry points (source.c:268-417) --------------------------------------- */
+
+Marx_Source_Type *marx_create_source (Param_File_Type *pf)
+{
+   Marx_Source_Type *st;
+   if (-1 == mxg_source_init (pf))
+     return NULL;
+   st = (Marx_Source_Type *) calloc (1, sizeof (Marx_Source_Type));
+   return st;
+}
+
+int marx_open_source (Marx_Source_Type *st)
+{
+   if (st == NULL) return -1;
+   st->open = 1;
+   return 0;
+}
+
+int marx_close_source (Marx_Source_Type *st)
+{
+   if (st == NULL) return -1;
+   free (st);
+   return 0;
+}
+
+/* Arrival-time model of the source: Poisson (the reference's) or, with
+ * tmax > 0, uniform over [0, tmax). */
+void mxg_set_uniform_times (double tmax)
+{
+   Mxg_Cfg.src.time_mode = (tmax > 0.0) ? MXG_TIME_UNIFORM : MXG_TIME_POISSON;
+   Mxg_Cfg.src.uniform_tmax = tmax;
+   Mxg_Cfg_Generation++;
+}

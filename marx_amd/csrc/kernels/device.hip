@@ -37,6 +37,8 @@ int g_cur = 0;                     // list holding the active photons
 unsigned int g_upper = 0;          // host-side upper bound of the active count
 Marx_Photon_Attr_Type *d_aos = nullptr;
 size_t g_aos_cap = 0;
+uint64_t *d_ticks = nullptr;       // arrival-time scan (fixed-point gaps)
+uint64_t *d_tile_tot = nullptr;
 
 int fail (const char *fmt, ...)
 {
@@ -123,8 +125,12 @@ int ensure_capacity (size_t n)
    void *mem = nullptr;
    if (d_store != nullptr) { (void) hipStreamSynchronize (g_stream); (void) hipFree (d_store); d_store = nullptr; }
    for (int k = 0; k < 2; k++) if (d_idx[k]) { (void) hipFree (d_idx[k]); d_idx[k] = nullptr; }
+   if (d_ticks) { (void) hipFree (d_ticks); d_ticks = nullptr; }
+   if (d_tile_tot) { (void) hipFree (d_tile_tot); d_tile_tot = nullptr; }
    g_capacity = 0;
    HIPCHECK (hipMalloc (&mem, total));
+   HIPCHECK (hipMalloc (&d_ticks, cap * sizeof (uint64_t)));
+   HIPCHECK (hipMalloc (&d_tile_tot, (cap / 2048 + 2) * sizeof (uint64_t)));
    HIPCHECK (hipMalloc (&d_idx[0], cap * sizeof (uint32_t)));
    HIPCHECK (hipMalloc (&d_idx[1], cap * sizeof (uint32_t)));
    d_store = mem;
@@ -378,15 +384,38 @@ int mxgd_synchronize (void)
    return 0;
 }
 
-int mxgd_run_source (unsigned int, unsigned int, double, unsigned long long,
-                     unsigned int *, double *, float *)
+int mxgd_run_source (unsigned int n, unsigned int tag_start, double start_time,
+                     double exposure_time, unsigned long long seed, unsigned int source_id,
+                     unsigned int *n_out, double *last_time, float *ms)
 {
-   return fail ("device source stage not built yet");
-}
-
-int mxgd_run_dither (double, unsigned long long, float *)
-{
-   return fail ("device dither stage not built yet");
+   if (!d_cfg) return fail ("configuration not on the device");
+   if (n > (1u << 26)) return fail ("chunk of %u photons exceeds the 2^26 limit of one launch", n);
+   if (ensure_capacity (n ? n : 1)) return -1;
+   if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
+   g_cur = 0;
+   HIPCHECK (launch_source (d_cfg, d_da, g_soa, n, tag_start, exposure_time, start_time, seed,
+                            source_id, d_ticks, d_tile_tot, d_counts + 2, d_counts, d_idx[0],
+                            g_stream));
+   if (ms)
+     {
+        HIPCHECK (hipEventRecord (g_ev1, g_stream));
+        HIPCHECK (hipEventSynchronize (g_ev1));
+        HIPCHECK (hipEventElapsedTime (ms, g_ev0, g_ev1));
+     }
+   uint32_t m = 0;
+   HIPCHECK (hipMemcpyAsync (&m, d_counts, sizeof (uint32_t), hipMemcpyDeviceToHost, g_stream));
+   HIPCHECK (hipStreamSynchronize (g_stream));
+   double t = 0.0;
+   if (m > 0)
+     {
+        HIPCHECK (hipMemcpyAsync (&t, g_soa.t + (m - 1), sizeof (double), hipMemcpyDeviceToHost, g_stream));
+        HIPCHECK (hipStreamSynchronize (g_stream));
+     }
+   g_n = m;
+   g_upper = m;
+   if (n_out) *n_out = m;
+   if (last_time) *last_time = t;
+   return 0;
 }
 
 } // extern "C"

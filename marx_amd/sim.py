@@ -115,13 +115,39 @@ class Photons:
 class Tracer:
     """Initialises the path's modules from a parameter file and runs them."""
 
-    def __init__(self, par=None, seed=1, **overrides):
+    def __init__(self, par=None, seed=1, source=False, uniform_times=0.0, **overrides):
         self.lib = _lib.load()
         self.par = par if par is not None else ParamFile(**overrides)
         self.lib.mxg_set_seed(int(seed))
         self.mirror = check(self.lib.marx_mirror_init(self.par.handle), "marx_mirror_init")
         self.grating = check(self.lib.marx_grating_init(self.par.handle), "marx_grating_init")
         self.detector = check(self.lib.marx_detector_init(self.par.handle), "marx_detector_init")
+        self.source = None
+        if source:
+            self.source = self.lib.marx_create_source(self.par.handle)
+            if not self.source:
+                raise RuntimeError("marx_create_source failed: " + _lib.last_error())
+            self.lib.mxg_set_uniform_times(float(uniform_times))
+
+    def create_photons(self, photons, num, exposure_time=None):
+        """marx_create_photons: num photons of the source into photons."""
+        n = ctypes.c_uint()
+        exp = ctypes.byref(ctypes.c_double(exposure_time)) if exposure_time is not None else None
+        check(self.lib.marx_create_photons(self.source, photons.ptr, int(num), ctypes.byref(n), exp),
+              "marx_create_photons")
+        return n.value
+
+    def trace_device(self, num, tag_start=0, start_time=0.0, exposure_time=-1.0, timing=False):
+        """Source -> mirror -> grating -> detector without leaving HBM.
+        Returns (collected, detected, stage_ms or None)."""
+        ncol, ndet = ctypes.c_uint(), ctypes.c_uint()
+        ms = (ctypes.c_float * 6)() if timing else None
+        check(self.lib.mxg_trace_device(int(num), int(tag_start), float(start_time), float(exposure_time),
+                                        ctypes.byref(ncol), ctypes.byref(ndet), ms), "mxg_trace_device")
+        return ncol.value, ndet.value, (list(ms) if timing else None)
+
+    def download(self, photons):
+        check(self.lib.mxg_download_photons(photons.ptr), "mxg_download_photons")
 
     def set_seed(self, seed):
         self.lib.mxg_set_seed(int(seed))

@@ -1,0 +1,141 @@
+/* orc_source.c -- oracle restatement of photon creation and dither.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).  Follows
+ *   marx/libsrc/source.c:268-384  marx_create_photons (source call, arrival
+ *                                 times with the exposure cut, tags, dither,
+ *                                 energy sort)
+ *   marx/libsrc/s-point.c:59-83, s-disk.c:63-110   directions
+ *   marx/libsrc/spectrum.c:138-189, prob.c:46-58  FLAT / FILE energies
+ *   marx/libsrc/dither.c:167-182, 551-581, 609-658  INTERNAL dither
+ * Deviations fixed by the per-photon stream design, shared with the GPU:
+ *  - the disk azimuth of each photon is drawn independently (the reference
+ *    rotates one normal cumulatively from photon to photon, s-disk.c:99);
+ *  - the exponential gaps are summed as 64-bit integers in units of 2^-32
+ *    of the mean gap, so that a parallel scan gives the same times;
+ *  - MXG_TIME_UNIFORM draws times uniformly over [0, tmax).
+ */
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+
+#include "oracle_int.h"
+
+static orc_vec apply_dither (double ra, double dec, double roll, orc_vec p)
+{
+   double cos_dec, sin_dec, cos_ra, sin_ra, cos_theta, sin_theta;
+   orc_vec n;
+
+   p = orc_v_rotate_unit_vector (p, orc_v (1, 0, 0), -roll);
+   cos_ra = cos (ra);
+   sin_ra = sin (ra);
+   cos_dec = cos (dec);
+   sin_dec = sin (dec);
+   cos_theta = cos_dec * cos_ra;
+   n = orc_v (0, sin_dec, -cos_dec * sin_ra);
+   sin_theta = orc_v_length (n);
+   if (sin_theta <= 1e-20) return p;
+   n.x /= sin_theta;
+   n.y /= sin_theta;
+   n.z /= sin_theta;
+   return orc_v_rotate_unit_vector1 (p, n, cos_theta, sin_theta);
+}
+
+int orc_create_photons (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int num,
+                        unsigned int *num_collected, double *exposure_time)
+{
+   const mxg_source_cfg *s = &c->cfg->src;
+   Marx_Photon_Attr_Type *at = pt->attributes;
+   orc_stream *streams;
+   unsigned int n = num, i;
+   unsigned long long S = 0;
+
+   *num_collected = 0;
+   if (num > pt->max_n_photons) return -1;
+   pt->source_distance = s->distance;
+   pt->history = 0;
+   pt->start_time += pt->total_time;
+   memset (at, 0, num * sizeof (Marx_Photon_Attr_Type));
+   if (NULL == (streams = orc_streams_alloc (num))) return -1;
+
+   /* source: energies and directions */
+   for (i = 0; i < num; i++)
+     {
+        orc_stream *st = streams + i;
+        at[i].tag = pt->tag_start + i;
+        orc_stream_init (st, c->seed, at[i].tag, MXG_STAGE_SOURCE, c->source_id);
+        if (s->spectrum == MXG_SPECTRUM_FLAT)
+          {
+             double emin = s->emin, de = s->emax - emin;
+             at[i].energy = emin + de * orc_uniform (st);
+          }
+        else
+          {
+             double r = orc_uniform (st);
+             at[i].energy = orc_interpolate_d (r, c->da + s->file_cum_off, c->da + s->file_e_off, s->file_n);
+          }
+        at[i].p = orc_v (s->p[0], s->p[1], s->p[2]);
+        if (s->type == MXG_SOURCE_DISK)
+          {
+             orc_vec normal = orc_v (s->p_normal[0], s->p_normal[1], s->p_normal[2]);
+             double x0 = s->disk_theta_min / s->disk_theta, x1, rnd;
+             x0 = x0 * x0;
+             x1 = 1.0 - x0;
+             normal = orc_v_rotate_unit_vector (normal, at[i].p, 2.0 * PI * orc_uniform (st));
+             rnd = s->disk_theta * sqrt (x0 + x1 * orc_uniform (st));
+             at[i].p = orc_v_rotate_unit_vector (at[i].p, normal, rnd);
+          }
+     }
+
+   /* arrival times */
+   for (i = 0; i < num; i++)
+     {
+        orc_stream *st = streams + i;
+        double t;
+        if (s->time_mode == MXG_TIME_POISSON)
+          {
+             double e = orc_expn (st);
+             S += (unsigned long long) (e * 4294967296.0 + 0.5);
+             t = s->mean_time * ((double) S * (1.0 / 4294967296.0));
+          }
+        else
+          t = s->uniform_tmax * orc_uniform (st);
+        at[i].arrival_time = t;
+        at[i].flags = 0;
+        if ((s->time_mode == MXG_TIME_POISSON) && (exposure_time != NULL)
+            && (*exposure_time > 0.0) && (*exposure_time <= t))
+          {
+             n = i + 1;
+             break;
+          }
+     }
+   free (streams);
+
+   pt->history |= (MARX_ENERGY_OK | MARX_TIME_OK | MARX_X_VECTOR_OK | MARX_P_VECTOR_OK | MARX_TAG_OK);
+   pt->n_photons = n;
+   pt->tag_start += n;
+
+   /* dither (dither.c:609-658) */
+   if ((n > 0) && (s->dither_mode == MXG_DITHER_INTERNAL))
+     for (i = 0; i < n; i++)
+       {
+          orc_stream st;
+          double t, ra, dec, roll, delta_ra, delta_dec;
+          Marx_Dither_Type *d = &at[i].dither_state;
+          orc_stream_init (&st, c->seed, at[i].tag, MXG_STAGE_DITHER, c->source_id);
+          t = pt->start_time + at[i].arrival_time;
+          t = (2.0 * PI) * t;
+          ra = d->ra = (float) (s->ra_amp * sin (t / s->ra_period + s->ra_phase));
+          dec = d->dec = (float) (s->dec_amp * sin (t / s->dec_period + s->dec_phase));
+          roll = d->roll = (float) (s->nominal_roll + s->roll_amp * sin (t / s->roll_period + s->roll_phase));
+          d->dy = d->dz = d->dtheta = 0;
+          delta_ra = s->aspect_blur * orc_gaussian (&st);
+          delta_dec = s->aspect_blur * orc_gaussian (&st);
+          at[i].p = apply_dither (ra + delta_ra, dec + delta_dec, roll, at[i].p);
+       }
+
+   if (-1 == orc_sort_photons (pt)) return -1;
+   if (n == 0) pt->total_time = 0.0;
+   else pt->total_time = (s->time_mode == MXG_TIME_UNIFORM) ? s->uniform_tmax : at[n - 1].arrival_time;
+   *num_collected = n;
+   return 0;
+}
