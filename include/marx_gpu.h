@@ -228,12 +228,19 @@ void mxg_set_uniform_times (double tmax);
  * Marx_Photon_Type.  *num_collected receives the number of photons created
  * (fewer than num when exposure_time truncates the chunk), *num_detected
  * the number that survive every stage.  stage_ms, when non-NULL, receives
- * 6 per-stage GPU times in ms measured with HIP events on the library's
- * stream: source, dither, mirror, grating, detector, total. */
+ * 6 per-stage GPU times in ms measured with HIP events recorded on the
+ * library's stream around each stage's kernels (read after the call's
+ * final synchronisation, so timing adds no host round trip): source
+ * (including dither), 0 (dither is fused into the source stage), mirror,
+ * grating, detector, and their sum. */
 int mxg_trace_device (unsigned int num, unsigned int tag_start,
                       double start_time, double exposure_time,
                       unsigned int *num_collected, unsigned int *num_detected,
                       float *stage_ms);
+
+/* Photons alive after each stage of the last mxg_trace_device call:
+ * created, after the mirror, after the grating, after the detector. */
+int mxg_last_stage_counts (unsigned int counts[4]);
 
 /* Copies the device-resident photons of the last mxg_trace_device call into
  * pt (AoS), setting n_photons, history and the pruned sorted_index. */

@@ -177,7 +177,7 @@ int marx_mirror_reflect (Marx_Photon_Type *pt, int verbose)
    pt->history |= MARX_MIRROR_SHELL_OK;
    if ((-1 == sync_config ()) || (-1 == make_resident (pt)))
      return -1;
-   if (-1 == mxgd_run_mirror (pt->source_distance, Seed, NULL))
+   if (-1 == mxgd_run_mirror (pt->source_distance, Seed, -1, NULL))
      {
         mxg_error ("%s", mxgd_last_error ());
         return -1;
@@ -231,7 +231,7 @@ int marx_grating_diffract (Marx_Photon_Type *pt, int verbose)
      pt->history |= (MARX_ORDER1_OK | MARX_ORDER2_OK | MARX_ORDER3_OK | MARX_ORDER4_OK);
    if ((-1 == sync_config ()) || (-1 == make_resident (pt)))
      return -1;
-   if (-1 == mxgd_run_grating (Seed, NULL))
+   if (-1 == mxgd_run_grating (Seed, -1, NULL))
      {
         mxg_error ("%s", mxgd_last_error ());
         return -1;
@@ -273,7 +273,7 @@ int marx_detect (Marx_Photon_Type *pt, int verbose)
      pt->history |= MARX_DET_UV_PIXEL_OK;
    if ((-1 == sync_config ()) || (-1 == make_resident (pt)))
      return -1;
-   if (-1 == mxgd_run_detect (pt->start_time, Seed, NULL))
+   if (-1 == mxgd_run_detect (pt->start_time, Seed, -1, NULL))
      {
         mxg_error ("%s", mxgd_last_error ());
         return -1;
@@ -338,7 +338,7 @@ int marx_create_photons (Marx_Source_Type *st, Marx_Photon_Type *pt, unsigned in
      return -1;
    if (-1 == mxgd_run_source (num, pt->tag_start, pt->start_time,
                               (exposure_time != NULL) ? *exposure_time : -1.0,
-                              Seed, 0, &n, &last_t, NULL))
+                              Seed, 0, -1, &n, &last_t, NULL))
      {
         mxg_error ("%s", mxgd_last_error ());
         return -1;
@@ -369,14 +369,18 @@ static unsigned int Last_Collected = 0;
 static unsigned int Last_Tag_Start = 0;
 static unsigned long Last_History = 0;
 
+static unsigned int Stage_Counts[4];
+
 int mxg_trace_device (unsigned int num, unsigned int tag_start, double start_time,
                       double exposure_time, unsigned int *num_collected,
                       unsigned int *num_detected, float *stage_ms)
 {
-   float ms[6] = {0, 0, 0, 0, 0, 0};
-   unsigned int n = 0, ndet = 0;
+   float ms[8];
+   unsigned int counts[8];
+   unsigned int n = 0, ndet;
    double last_t;
    unsigned long hist = MARX_ENERGY_OK | MARX_TIME_OK | MARX_X_VECTOR_OK | MARX_P_VECTOR_OK | MARX_TAG_OK;
+   int with_grating = (Grating_Type == MARX_GRATING_HETG) || (Grating_Type == MARX_GRATING_LETG);
 
    if (-1 == sync_config ())
      return -1;
@@ -385,37 +389,25 @@ int mxg_trace_device (unsigned int num, unsigned int tag_start, double start_tim
         mxg_error ("Mirror not initialized.");
         return -1;
      }
-   if ((-1 == mxgd_run_source (num, tag_start, start_time, exposure_time, Seed, 0, &n, &last_t,
-                               stage_ms ? &ms[0] : NULL))
-       || (-1 == mxgd_run_mirror (Mxg_Cfg.src.distance, Seed, stage_ms ? &ms[2] : NULL)))
+   /* timing slots: 0 source+dither, 1 mirror, 2 grating, 3 detector */
+   if ((-1 == mxgd_run_source (num, tag_start, start_time, exposure_time, Seed, 0, 0, &n, &last_t, NULL))
+       || (-1 == mxgd_run_mirror (Mxg_Cfg.src.distance, Seed, 1, NULL))
+       || (with_grating && (-1 == mxgd_run_grating (Seed, 2, NULL)))
+       || ((Detector_Type > 0) && (-1 == mxgd_run_detect (start_time, Seed, 3, NULL)))
+       || (-1 == mxgd_read_timings (ms, counts, 4)))
      {
         mxg_error ("%s", mxgd_last_error ());
         return -1;
      }
    hist |= MARX_MIRROR_SHELL_OK;
-   if ((Grating_Type == MARX_GRATING_HETG) || (Grating_Type == MARX_GRATING_LETG))
-     {
-        if (-1 == mxgd_run_grating (Seed, stage_ms ? &ms[3] : NULL))
-          {
-             mxg_error ("%s", mxgd_last_error ());
-             return -1;
-          }
-        hist |= MARX_ORDER_OK;
-     }
-   if (Detector_Type > 0)
-     {
-        if (-1 == mxgd_run_detect (start_time, Seed, stage_ms ? &ms[4] : NULL))
-          {
-             mxg_error ("%s", mxgd_last_error ());
-             return -1;
-          }
-        hist |= MARX_DET_NUM_OK | MARX_DET_PIXEL_OK;
-     }
-   if (-1 == mxgd_active_count (&ndet))
-     {
-        mxg_error ("%s", mxgd_last_error ());
-        return -1;
-     }
+   if (with_grating) hist |= MARX_ORDER_OK;
+   if (Detector_Type > 0) hist |= MARX_DET_NUM_OK | MARX_DET_PIXEL_OK;
+   /* photons alive after each stage (a skipped stage passes them through) */
+   Stage_Counts[0] = n;
+   Stage_Counts[1] = counts[1];
+   Stage_Counts[2] = with_grating ? counts[2] : counts[1];
+   Stage_Counts[3] = (Detector_Type > 0) ? counts[3] : Stage_Counts[2];
+   ndet = Stage_Counts[3];
    Resident = NULL;
    Last_Collected = n;
    Last_Tag_Start = tag_start;
@@ -424,9 +416,19 @@ int mxg_trace_device (unsigned int num, unsigned int tag_start, double start_tim
    if (num_detected) *num_detected = ndet;
    if (stage_ms)
      {
-        ms[5] = ms[0] + ms[1] + ms[2] + ms[3] + ms[4];
-        memcpy (stage_ms, ms, sizeof (ms));
+        stage_ms[0] = ms[0];
+        stage_ms[1] = 0.0f;           /* dither runs inside the source stage */
+        stage_ms[2] = ms[1];
+        stage_ms[3] = with_grating ? ms[2] : 0.0f;
+        stage_ms[4] = (Detector_Type > 0) ? ms[3] : 0.0f;
+        stage_ms[5] = stage_ms[0] + stage_ms[2] + stage_ms[3] + stage_ms[4];
      }
+   return 0;
+}
+
+int mxg_last_stage_counts (unsigned int counts[4])
+{
+   memcpy (counts, Stage_Counts, sizeof (Stage_Counts));
    return 0;
 }
 

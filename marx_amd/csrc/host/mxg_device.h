@@ -35,9 +35,15 @@ int mxgd_get_soa (mxg_soa_view *view);
 
 /* Stages on the resident store.  Each one processes the current active
  * list and replaces it by the photons still unabsorbed. */
-int mxgd_run_mirror (double source_distance, unsigned long long seed, float *ms);
-int mxgd_run_grating (unsigned long long seed, float *ms);
-int mxgd_run_detect (double tstart, unsigned long long seed, float *ms);
+/* slot >= 0 records HIP events around the stage's kernels into timing slot
+ * `slot` (read with mxgd_read_timings); ms, when non-NULL, synchronises and
+ * returns the stage time at once. */
+int mxgd_run_mirror (double source_distance, unsigned long long seed, int slot, float *ms);
+int mxgd_run_grating (unsigned long long seed, int slot, float *ms);
+int mxgd_run_detect (double tstart, unsigned long long seed, int slot, float *ms);
+/* Synchronises and returns the times (ms) of the stages recorded in slots
+ * 0..nslots-1 and, for stage slots, the survivor count each one left. */
+int mxgd_read_timings (float *ms, unsigned int *counts, int nslots);
 
 /* Source: creates n photons (tags tag_start..) with energies, directions,
  * arrival times and dither; when exposure_time > 0 the list ends at the
@@ -46,7 +52,7 @@ int mxgd_run_detect (double tstart, unsigned long long seed, float *ms);
  * *last_time the arrival time of the last one. */
 int mxgd_run_source (unsigned int n, unsigned int tag_start, double start_time,
                      double exposure_time, unsigned long long seed, unsigned int source_id,
-                     unsigned int *n_out, double *last_time, float *ms);
+                     int slot, unsigned int *n_out, double *last_time, float *ms);
 
 /* Number of photons in the active list (device -> host copy). */
 int mxgd_active_count (unsigned int *n);

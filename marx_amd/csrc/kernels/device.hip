@@ -18,6 +18,11 @@ char g_err[512];
 int g_inited = 0;
 hipStream_t g_stream = nullptr;
 hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
+// asynchronous stage timing: events recorded around each stage, read after
+// the stream has been synchronised for another reason
+constexpr int kSlots = 8;
+hipEvent_t g_tev[kSlots][2];
+int g_slot_used[kSlots];
 
 // configuration in HBM
 mxg_config *d_cfg = nullptr;
@@ -175,16 +180,32 @@ ListIO list_io ()
    return io;
 }
 
-int begin_stage ()
+__global__ void k_copy_u32 (uint32_t *dst, const uint32_t *src) { *dst = *src; }
+
+int begin_stage (int slot)
 {
    hipLaunchKernelGGL (k_set_u32, dim3 (1), dim3 (1), 0, g_stream, d_counts + (1 - g_cur), 0u);
    HIPCHECK (hipGetLastError ());
+   if ((slot >= 0) && (slot < kSlots))
+     {
+        HIPCHECK (hipEventRecord (g_tev[slot][0], g_stream));
+        g_slot_used[slot] = 1;
+     }
    return 0;
 }
 
-int end_stage (float *ms)
+// count slot 8 + k receives the length of the survivor list of stage k
+int end_stage (int slot, float *ms)
 {
+   if ((slot >= 0) && (slot < kSlots))
+     HIPCHECK (hipEventRecord (g_tev[slot][1], g_stream));
    g_cur = 1 - g_cur;
+   if ((slot >= 0) && (slot < kSlots))
+     {
+        hipLaunchKernelGGL (k_copy_u32, dim3 (1), dim3 (1), 0, g_stream, d_counts + 8 + slot,
+                            d_counts + g_cur);
+        HIPCHECK (hipGetLastError ());
+     }
    if (ms != nullptr)
      {
         HIPCHECK (hipEventRecord (g_ev1, g_stream));
@@ -216,6 +237,12 @@ int mxgd_init (void)
    HIPCHECK (hipStreamCreateWithFlags (&g_stream, hipStreamNonBlocking));
    HIPCHECK (hipEventCreate (&g_ev0));
    HIPCHECK (hipEventCreate (&g_ev1));
+   for (int k = 0; k < kSlots; k++)
+     {
+        HIPCHECK (hipEventCreate (&g_tev[k][0]));
+        HIPCHECK (hipEventCreate (&g_tev[k][1]));
+        g_slot_used[k] = 0;
+     }
    HIPCHECK (hipMalloc (&d_counts, 64 * sizeof (uint32_t)));
    HIPCHECK (hipMemset (d_counts, 0, 64 * sizeof (uint32_t)));
    g_inited = 1;
@@ -330,33 +357,33 @@ int mxgd_get_soa (mxg_soa_view *v)
    return 0;
 }
 
-int mxgd_run_mirror (double source_distance, unsigned long long seed, float *ms)
+int mxgd_run_mirror (double source_distance, unsigned long long seed, int slot, float *ms)
 {
    if (!d_cfg) return fail ("configuration not on the device");
    if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
-   if (begin_stage ()) return -1;
+   if (begin_stage (slot)) return -1;
    HIPCHECK (launch_hrma (d_cfg, d_fa, d_da, g_soa, list_io (), g_upper, source_distance, seed, 0,
                           g_stream));
-   return end_stage (ms);
+   return end_stage (slot, ms);
 }
 
-int mxgd_run_grating (unsigned long long seed, float *ms)
+int mxgd_run_grating (unsigned long long seed, int slot, float *ms)
 {
    if (!d_cfg) return fail ("configuration not on the device");
    if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
-   if (begin_stage ()) return -1;
+   if (begin_stage (slot)) return -1;
    HIPCHECK (launch_grating (d_cfg, d_fa, d_da, g_soa, list_io (), g_upper, seed, 0, g_stream));
-   return end_stage (ms);
+   return end_stage (slot, ms);
 }
 
-int mxgd_run_detect (double tstart, unsigned long long seed, float *ms)
+int mxgd_run_detect (double tstart, unsigned long long seed, int slot, float *ms)
 {
    if (!d_cfg) return fail ("configuration not on the device");
    if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
-   if (begin_stage ()) return -1;
+   if (begin_stage (slot)) return -1;
    HIPCHECK (launch_detect (d_cfg, d_fa, d_da, g_soa, list_io (), g_upper, tstart, seed, 0,
                             g_stream));
-   return end_stage (ms);
+   return end_stage (slot, ms);
 }
 
 int mxgd_active_count (unsigned int *n)
@@ -377,6 +404,22 @@ int mxgd_active_list (unsigned int *idx, unsigned int n)
    return 0;
 }
 
+int mxgd_read_timings (float *ms, unsigned int *counts, int nslots)
+{
+   HIPCHECK (hipStreamSynchronize (g_stream));
+   uint32_t c[kSlots];
+   HIPCHECK (hipMemcpy (c, d_counts + 8, sizeof (c), hipMemcpyDeviceToHost));
+   for (int k = 0; k < nslots && k < kSlots; k++)
+     {
+        ms[k] = 0.0f;
+        if (counts) counts[k] = c[k];
+        if (g_slot_used[k])
+          HIPCHECK (hipEventElapsedTime (ms + k, g_tev[k][0], g_tev[k][1]));
+        g_slot_used[k] = 0;
+     }
+   return 0;
+}
+
 int mxgd_synchronize (void)
 {
    if (!g_inited) return 0;
@@ -386,16 +429,23 @@ int mxgd_synchronize (void)
 
 int mxgd_run_source (unsigned int n, unsigned int tag_start, double start_time,
                      double exposure_time, unsigned long long seed, unsigned int source_id,
-                     unsigned int *n_out, double *last_time, float *ms)
+                     int slot, unsigned int *n_out, double *last_time, float *ms)
 {
    if (!d_cfg) return fail ("configuration not on the device");
    if (n > (1u << 26)) return fail ("chunk of %u photons exceeds the 2^26 limit of one launch", n);
    if (ensure_capacity (n ? n : 1)) return -1;
    if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
    g_cur = 0;
+   if (slot >= 0 && slot < kSlots)
+     {
+        HIPCHECK (hipEventRecord (g_tev[slot][0], g_stream));
+        g_slot_used[slot] = 1;
+     }
    HIPCHECK (launch_source (d_cfg, d_da, g_soa, n, tag_start, exposure_time, start_time, seed,
                             source_id, d_ticks, d_tile_tot, d_counts + 2, d_counts, d_idx[0],
                             g_stream));
+   if (slot >= 0 && slot < kSlots)
+     HIPCHECK (hipEventRecord (g_tev[slot][1], g_stream));
    if (ms)
      {
         HIPCHECK (hipEventRecord (g_ev1, g_stream));

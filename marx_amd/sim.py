@@ -124,10 +124,28 @@ class Tracer:
         self.detector = check(self.lib.marx_detector_init(self.par.handle), "marx_detector_init")
         self.source = None
         if source:
-            self.source = self.lib.marx_create_source(self.par.handle)
-            if not self.source:
-                raise RuntimeError("marx_create_source failed: " + _lib.last_error())
-            self.lib.mxg_set_uniform_times(float(uniform_times))
+            self.init_source(uniform_times=uniform_times)
+
+    @property
+    def geometric_area(self):
+        """Open HRMA aperture in cm^2 (Marx_Mirror_Geometric_Area)."""
+        return ctypes.c_double.in_dll(self.lib, "Marx_Mirror_Geometric_Area").value
+
+    def init_source(self, uniform_times=0.0, **overrides):
+        """marx_create_source with the parameter file plus overrides."""
+        for k, v in overrides.items():
+            self.par.set(k, v)
+        if self.source:
+            self.lib.marx_close_source(self.source)
+        self.source = self.lib.marx_create_source(self.par.handle)
+        if not self.source:
+            raise RuntimeError("marx_create_source failed: " + _lib.last_error())
+        self.lib.mxg_set_uniform_times(float(uniform_times))
+
+    def stage_counts(self):
+        c = (ctypes.c_uint * 4)()
+        self.lib.mxg_last_stage_counts(c)
+        return list(c)
 
     def create_photons(self, photons, num, exposure_time=None):
         """marx_create_photons: num photons of the source into photons."""

@@ -1,0 +1,32 @@
+"""Named workloads (parameter sets) of the benchmark and the tests."""
+import os
+
+from .sim import Tracer
+from .spectra import write_power_law
+
+# on-axis: the source at the nominal pointing of the standard parameter file
+ON_AXIS = dict(SourceRA=250.2134679741175, SourceDEC=-53.75743813458669)
+
+
+def hetg_acis_s(tmpdir, photons=10_000_000, exposure=50_000.0, seed=1):
+    """HETG + ACIS-S: point source on axis, power law of photon index 2 over
+    0.4-8 keV (tabulated FILE spectrum), HEG/MEG orders from the efficiency
+    tables, INTERNAL dither of 16 arcsec amplitude and 1000 / 707 s periods.
+    The flux is set so that `photons` photons arrive in `exposure` seconds."""
+    spec = write_power_law(os.path.join(tmpdir, "powerlaw_g2_0.4_8.dat"), gamma=2.0, emin=0.4, emax=8.0)
+    t = Tracer(seed=seed, GratingType="HETG", DetectorType="ACIS-S", SourceType="POINT",
+               SpectrumType="FILE", SpectrumFile=spec, DitherModel="INTERNAL",
+               DitherAmp_RA=16.0, DitherAmp_Dec=16.0, DitherPeriod_RA=1000.0, DitherPeriod_Dec=707.0,
+               **ON_AXIS)
+    flux = photons / (exposure * t.geometric_area)
+    t.init_source(SourceFlux=flux)
+    return t
+
+
+def plain_acis_s(photons=1_000_000, tmax=10_000.0, seed=1):
+    """Plain: on-axis point source at 1.49 keV, arrival times uniform over
+    10 ks, no dither, no grating, ACIS-S."""
+    t = Tracer(seed=seed, GratingType="NONE", DetectorType="ACIS-S", SourceType="POINT",
+               SpectrumType="FLAT", MinEnergy=1.49, MaxEnergy=1.49, DitherModel="NONE", **ON_AXIS)
+    t.init_source(uniform_times=tmax)
+    return t

@@ -1,0 +1,100 @@
+"""Per-photon FP64 operation counts of the oracle's stages (test
+infrastructure).
+
+Runs the op-counting oracle build (liboracle_opcount.so) over a sample of a
+workload and reports, per stage, the FP64 adds/subs, muls, divs, sqrts and
+transcendental calls per photon entering the stage.  These are the
+algorithmic work figures behind the benchmark's roofline; they count the
+restated reference algorithm and nothing of the GPU code.
+
+    python -m oracle.opcount.opcount [--photons 100000]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+NAMES = ["add", "mul", "div", "sqrt", "sin", "cos", "tan", "atan", "atan2", "asin", "acos",
+         "exp", "expm1", "log", "log1p", "pow", "fmod"]
+
+
+def load_opcount():
+    from oracle import oracle as O
+    lib = ctypes.CDLL(os.path.join(os.path.dirname(HERE), "liboracle_opcount.so"))
+    for name in ("orc_mirror_reflect", "orc_grating_diffract", "orc_detect"):
+        getattr(lib, name).argtypes = [ctypes.POINTER(O.OrcCtx), ctypes.c_void_p]
+        getattr(lib, name).restype = ctypes.c_int
+    lib.orc_create_photons.argtypes = [ctypes.POINTER(O.OrcCtx), ctypes.c_void_p, ctypes.c_uint,
+                                       ctypes.POINTER(ctypes.c_uint), ctypes.c_void_p]
+    lib.orc_create_photons.restype = ctypes.c_int
+    lib.orc_opcount_reset.restype = None
+    lib.orc_opcount_get.argtypes = [ctypes.POINTER(ctypes.c_ulonglong)]
+    return lib
+
+
+def count_stages(tracer, num, seed=1):
+    """Per-stage counts per entering photon for one chunk of num photons."""
+    from oracle import oracle as O
+    from marx_amd import Photons
+    lib = load_opcount()
+    cfg, fa, _, da, _ = tracer.config_pointers()
+    ctx = O.OrcCtx(cfg, fa, da, seed, 0)
+    ph = Photons(num)
+    out = {}
+
+    def run(stage, call, entering):
+        lib.orc_opcount_reset()
+        if call() != 0:
+            raise RuntimeError(f"op-count oracle stage {stage} failed")
+        c = (ctypes.c_ulonglong * 17)()
+        lib.orc_opcount_get(c)
+        per = {k: c[i] / max(entering, 1) for i, k in enumerate(NAMES)}
+        per["flops"] = per["add"] + per["mul"] + per["div"] + per["sqrt"]
+        per["transcendental"] = sum(per[k] for k in NAMES[4:])
+        per["photons_in"] = entering
+        out[stage] = per
+
+    n = ctypes.c_uint()
+    run("source", lambda: lib.orc_create_photons(ctypes.byref(ctx), ctypes.cast(ph.ptr, ctypes.c_void_p),
+                                                 num, ctypes.byref(n), None), num)
+    alive = lambda: int(((ph.active["flags"] & 0xFF) == 0).sum())
+    e = alive()
+    run("mirror", lambda: lib.orc_mirror_reflect(ctypes.byref(ctx), ctypes.cast(ph.ptr, ctypes.c_void_p)), e)
+    e = alive()
+    run("grating", lambda: lib.orc_grating_diffract(ctypes.byref(ctx), ctypes.cast(ph.ptr, ctypes.c_void_p)), e)
+    e = alive()
+    run("detector", lambda: lib.orc_detect(ctypes.byref(ctx), ctypes.cast(ph.ptr, ctypes.c_void_p)), e)
+    out["detected"] = alive()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--photons", type=int, default=100000)
+    ap.add_argument("--json", default=None)
+    a = ap.parse_args()
+    from marx_amd.workloads import hetg_acis_s
+    with tempfile.TemporaryDirectory() as tmp:
+        tracer = hetg_acis_s(tmp, photons=a.photons)
+        res = count_stages(tracer, a.photons)
+    for st in ("source", "mirror", "grating", "detector"):
+        r = res[st]
+        print(f"{st:9s} photons_in={r['photons_in']:8d} flops/photon={r['flops']:8.1f} "
+              f"(add {r['add']:.1f} mul {r['mul']:.1f} div {r['div']:.1f} sqrt {r['sqrt']:.1f}) "
+              f"transcendental/photon={r['transcendental']:.2f}")
+    if a.json:
+        with open(a.json, "w") as fp:
+            json.dump(res, fp, indent=1)
+
+
+if __name__ == "__main__":
+    main()
