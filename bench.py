@@ -70,6 +70,9 @@ def main():
     ap.add_argument("--photons", type=int, default=10_000_000)
     ap.add_argument("--exposure", type=float, default=50_000.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="after the steps, copy a MIB-MiB device buffer three times (the known-size "
+                         "reference that HBM-traffic counters are scaled by)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -157,6 +160,13 @@ def main():
         "stages_ms": {s: stage_ms[s] / K for s in STAGES},
         "stage_photons_in": {s: stage_in[s] / K for s in STAGES},
     }
+    if args.copy_probe > 0:
+        src = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_probe_bytes"] = 2 * src.numel() * 4
     if not args.no_cpu:
         cb = cpu_baseline(tracer)
         out["cpu_baseline"] = cb

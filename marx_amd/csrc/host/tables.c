@@ -228,6 +228,7 @@ typedef struct
 {
    double v[6];            /* e_alpha p_min delta_p p_max norm expon */
    unsigned int n;
+   unsigned int order;     /* position in the file */
    const unsigned char *theta;
 }
 Fold_Array;
@@ -238,7 +239,7 @@ static int fold_cmp (const void *a, const void *b)
    if (fa->v[0] > fb->v[0]) return 1;
    if (fa->v[0] < fb->v[0]) return -1;
    /* keep file order among equal e_alpha so the first one survives */
-   return (fa < fb) ? -1 : (fa > fb);
+   return (fa->order < fb->order) ? -1 : (fa->order > fb->order);
 }
 
 int mxg_read_wfold (const char *file, mxg_wfold *w)
@@ -275,6 +276,7 @@ int mxg_read_wfold (const char *file, mxg_wfold *w)
         for (k = 0; k < 6; k++)
           a->v[k] = (double) be_float (buf + pos + 4 + 4 * k);
         a->n = n;
+        a->order = num;
         a->theta = buf + pos + 28;
         /* delta_p re-derived from the end points (wfold.c:164-173) */
         if (n == 1)

@@ -151,16 +151,17 @@ void k_detect (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
    const mxg_detector_cfg &d = cfg->det;
    const bool is_acis = (d.type == MARX_DETECTOR_ACIS_S) || (d.type == MARX_DETECTOR_ACIS_I);
    const bool is_hrc_s = (d.type == MARX_DETECTOR_HRC_S);
-   const uint32_t n = *io.n_in;
-   const uint32_t stride = gridDim.x * blockDim.x;
+   __shared__ uint32_t seg_wc[2 * (kBlock / 64)];
+   const BlockRange range = block_range (*io.n_in, blockIdx.x, gridDim.x);
+   SegWriter survivors (io, range, seg_wc);
    (void) da;
 
-   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride)
+   for (uint32_t base = range.lo; base < range.hi; base += kBlock)
      {
         uint32_t i = base + threadIdx.x;
         bool keep = false;
         uint32_t id = 0;
-        if (i < n)
+        if (i < range.hi)
           {
              id = io.idx_in[i];
              uint32_t flags = s.flags[id];
@@ -361,8 +362,9 @@ void k_detect (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
                   keep = (flags & BAD_PHOTON_MASK) == 0;
                }
           }
-        append_survivor (keep, id, io.idx_out, io.n_out);
+        survivors.add (keep, id);
      }
+   survivors.finish (io);
 }
 
 hipError_t launch_detect (const mxg_config *cfg, const float *fa, const double *da, Soa s,
@@ -370,10 +372,11 @@ hipError_t launch_detect (const mxg_config *cfg, const float *fa, const double *
                           uint32_t source_id, hipStream_t st)
 {
    if (n_upper == 0) return hipSuccess;
-   uint32_t blocks = (n_upper + kBlock - 1) / kBlock;
-   hipLaunchKernelGGL (k_detect, dim3 (blocks), dim3 (kBlock), 0, st, cfg, fa, da, s, io, tstart,
+   hipLaunchKernelGGL (k_detect, dim3 (io.grid), dim3 (kBlock), 0, st, cfg, fa, da, s, io, tstart,
                        seed, source_id);
-   return hipGetLastError ();
+   hipError_t e = hipGetLastError ();
+   if (e == hipSuccess) e = launch_compact (io, st);
+   return e;
 }
 
 } // namespace mxg

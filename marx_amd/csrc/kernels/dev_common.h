@@ -171,6 +171,27 @@ __device__ __forceinline__ unsigned int search_d (double x, const double *__rest
    return n0;
 }
 
+// search over a table whose elements widen exactly to double (a float table
+// holding float32 file values, or a double table): same result as search_d
+template <class T>
+__device__ __forceinline__ unsigned int search_w (double x, const T *__restrict__ xp, unsigned int n)
+{
+   unsigned int n0 = 0, n1 = n;
+   while (n1 > n0 + 1)
+     {
+        unsigned int n2 = (n0 + n1) >> 1;
+        double v = (double) xp[n2];
+        if (v >= x)
+          {
+             if (v == x) return n2;
+             n1 = n2;
+          }
+        else n0 = n2;
+     }
+   if (x >= (double) xp[n0]) return n1;
+   return n0;
+}
+
 __device__ __forceinline__ double interp_d (double x, const double *__restrict__ xp,
                                             const double *__restrict__ yp, unsigned int n)
 {
@@ -393,42 +414,25 @@ __device__ __forceinline__ double fold_theta (const double *__restrict__ fa, con
    return (1.0 - delta_i) * (double) t[i] + delta_i * (double) t[i + 1];
 }
 
+// e_alphas: the table's e_alpha values (float32 file values, exact in
+// float), possibly staged in LDS; fold arrays stay in the double arena
+template <class T>
 __device__ __forceinline__ double wfold_interp (const mxg_wfold &w, const float *__restrict__ fa,
                                                 const double *__restrict__ da,
+                                                const T *__restrict__ e_alphas,
                                                 double energy, double sin_alpha, double r)
 {
    if (w.n_arrays == 0) return 0.0;
    const double *arrays = da + w.arrays_off;
    if (w.n_arrays == 1) return fold_theta (arrays, fa, r);
-   const double *e_alphas = da + w.e_alphas_off;
    double e_alpha = energy * sin_alpha;
-   unsigned int i = search_d (e_alpha, e_alphas, w.n_arrays);
+   unsigned int i = search_w (e_alpha, e_alphas, w.n_arrays);
    if (i == w.n_arrays) i--;
    if (i == 0) i++;
    double theta_0 = fold_theta (arrays + (i - 1) * MXG_FOLD_STRIDE, fa, r);
    double theta_1 = fold_theta (arrays + i * MXG_FOLD_STRIDE, fa, r);
-   double e0 = e_alphas[i - 1], e1 = e_alphas[i];
+   double e0 = (double) e_alphas[i - 1], e1 = (double) e_alphas[i];
    return theta_0 + (theta_1 - theta_0) * (e_alpha - e0) / (e1 - e0);
-}
-
-// ---- wave-aggregated append of survivors to an index list ---------------------------
-
-__device__ __forceinline__ void append_survivor (bool keep, uint32_t idx, uint32_t *__restrict__ out,
-                                                 uint32_t *__restrict__ out_count)
-{
-   unsigned long long mask = __ballot (keep);
-   if (mask == 0) return;
-   int lane = __lane_id ();
-   int leader = __ffsll ((long long) mask) - 1;
-   uint32_t base = 0;
-   if (lane == leader)
-     base = atomicAdd (out_count, (uint32_t) __popcll (mask));
-   base = __shfl (base, leader);
-   if (keep)
-     {
-        unsigned long long below = mask & ((1ull << lane) - 1ull);
-        out[base + __popcll (below)] = idx;
-     }
 }
 
 } // namespace mxg

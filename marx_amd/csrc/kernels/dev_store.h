@@ -28,20 +28,102 @@ struct Soa
 };
 
 // Per-stage launch arguments: the active index list in, the survivors out.
+//
+// Survivors are collected without global atomics: block b of a stage kernel
+// (grid = `grid` blocks of kBlock lanes) owns the input positions
+// [lo_b, hi_b) of block_range() and writes the ids of its survivors, in input
+// order, to seg[lo_b ...] and their number to seg_count[b].  k_compact then
+// concatenates the segments into idx_out / n_out.  An input list sorted by id
+// therefore yields an output list sorted by id.
 struct ListIO
 {
    const uint32_t *idx_in;
    const uint32_t *n_in;
-   uint32_t *idx_out;
+   uint32_t *idx_out;      // compacted survivors (written by k_compact)
    uint32_t *n_out;
+   uint32_t *seg;          // per-block survivor segments, capacity entries
+   uint32_t *seg_count;    // grid entries
+   uint32_t grid;          // blocks of the stage kernel (<= kMaxSegments)
 };
 
 constexpr int kBlock = 256;
+constexpr uint32_t kMaxSegments = 8192;
+
+struct BlockRange { uint32_t lo, hi; };
+
+// The input positions owned by block b of nblocks: equal chunks rounded up
+// to whole blocks of lanes.
+__device__ __forceinline__ BlockRange block_range (uint32_t n, uint32_t b, uint32_t nblocks)
+{
+   uint64_t per = ((uint64_t) n + nblocks - 1) / nblocks;
+   uint64_t chunk = (per + kBlock - 1) / kBlock * kBlock;
+   uint64_t lo = (uint64_t) b * chunk;
+   uint64_t hi = lo + chunk;
+   BlockRange r;
+   r.lo = (uint32_t) (lo < n ? lo : n);
+   r.hi = (uint32_t) (hi < n ? hi : n);
+   return r;
+}
+
+// Order-preserving survivor output of one block (see ListIO).  add() is
+// called by every lane of the block once per pass of kBlock inputs; it
+// holds one barrier.  wc is a 2 x (kBlock / 64) LDS array.
+struct SegWriter
+{
+   uint32_t *out;
+   uint32_t *wc;
+   uint32_t count;
+   uint32_t parity;
+
+   __device__ __forceinline__ SegWriter (const ListIO &io, const BlockRange &r, uint32_t *lds_wc)
+      : out (io.seg + r.lo), wc (lds_wc), count (0), parity (0) {}
+
+   __device__ __forceinline__ void add (bool keep, uint32_t id)
+   {
+      constexpr int kWaves = kBlock / 64;
+      unsigned long long mask = __ballot (keep);
+      const int wave = threadIdx.x >> 6, lane = __lane_id ();
+      uint32_t *w = wc + parity * kWaves;
+      if (lane == 0) w[wave] = (uint32_t) __popcll (mask);
+      __syncthreads ();
+      uint32_t before = 0, total = 0;
+#pragma unroll
+      for (int k = 0; k < kWaves; k++)
+        {
+           uint32_t c = w[k];
+           before += (k < wave) ? c : 0u;
+           total += c;
+        }
+      if (keep)
+        out[count + before + (uint32_t) __popcll (mask & ((1ull << lane) - 1ull))] = id;
+      count += total;
+      parity ^= 1u;
+   }
+
+   __device__ __forceinline__ void finish (const ListIO &io)
+   {
+      if (threadIdx.x == 0) io.seg_count[blockIdx.x] = count;
+   }
+};
+
+// Launches k_compact over the segments a stage kernel of io.grid blocks wrote.
+hipError_t launch_compact (const ListIO &io, hipStream_t st);
+
+// Device buffers the mirror stage uses between its two kernels.
+struct HrmaScratch
+{
+   uint32_t *idx_mid;    // photons that hit the paraboloid (capacity entries)
+   uint32_t *n_mid;      // its length (written by k_compact)
+   uint32_t *rng_pos;    // per photon: random-stream position at the split
+};
 
 // Stage launchers (defined in the stage .hip files).
-hipError_t launch_hrma (const mxg_config *cfg, const float *fa, const double *da, Soa s,
-                        ListIO io, uint32_t n_upper, double source_distance,
-                        unsigned long long seed, uint32_t source_id, hipStream_t st);
+// cfg: device copy; hcfg: host copy of the same configuration (launch
+// geometry and LDS layout are computed from it)
+hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const float *fa,
+                        const double *da, Soa s, ListIO io, HrmaScratch hs, uint32_t n_upper,
+                        double source_distance, unsigned long long seed, uint32_t source_id,
+                        hipStream_t st);
 hipError_t launch_grating (const mxg_config *cfg, const float *fa, const double *da, Soa s,
                            ListIO io, uint32_t n_upper, unsigned long long seed,
                            uint32_t source_id, hipStream_t st);

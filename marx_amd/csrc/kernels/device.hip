@@ -26,6 +26,8 @@ int g_slot_used[kSlots];
 
 // configuration in HBM
 mxg_config *d_cfg = nullptr;
+mxg_config g_hcfg;                 // host copy of the device configuration
+int g_num_cus = 0;
 float *d_fa = nullptr;
 double *d_da = nullptr;
 size_t d_fa_cap = 0, d_da_cap = 0;
@@ -37,7 +39,12 @@ size_t g_capacity = 0;
 Soa g_soa;
 unsigned int g_n = 0;              // photons in the store
 uint32_t *d_idx[2] = {nullptr, nullptr};
-uint32_t *d_counts = nullptr;      // [0],[1] list lengths, [2..] scratch
+uint32_t *d_idx_mid = nullptr;     // the mirror's middle list
+uint32_t *d_seg = nullptr;         // per-block survivor segments of the running stage
+uint32_t *d_seg_count = nullptr;   // kMaxSegments entries
+uint32_t *d_rng_pos = nullptr;     // the mirror's per-photon stream position
+uint32_t *d_counts = nullptr;      // [0],[1] list lengths, [2] source cut, [4] middle list,
+                                   // [8..15] per-stage survivor counts
 int g_cur = 0;                     // list holding the active photons
 unsigned int g_upper = 0;          // host-side upper bound of the active count
 Marx_Photon_Attr_Type *d_aos = nullptr;
@@ -132,12 +139,18 @@ int ensure_capacity (size_t n)
    for (int k = 0; k < 2; k++) if (d_idx[k]) { (void) hipFree (d_idx[k]); d_idx[k] = nullptr; }
    if (d_ticks) { (void) hipFree (d_ticks); d_ticks = nullptr; }
    if (d_tile_tot) { (void) hipFree (d_tile_tot); d_tile_tot = nullptr; }
+   if (d_idx_mid) { (void) hipFree (d_idx_mid); d_idx_mid = nullptr; }
+   if (d_seg) { (void) hipFree (d_seg); d_seg = nullptr; }
+   if (d_rng_pos) { (void) hipFree (d_rng_pos); d_rng_pos = nullptr; }
    g_capacity = 0;
    HIPCHECK (hipMalloc (&mem, total));
    HIPCHECK (hipMalloc (&d_ticks, cap * sizeof (uint64_t)));
    HIPCHECK (hipMalloc (&d_tile_tot, (cap / 2048 + 2) * sizeof (uint64_t)));
    HIPCHECK (hipMalloc (&d_idx[0], cap * sizeof (uint32_t)));
    HIPCHECK (hipMalloc (&d_idx[1], cap * sizeof (uint32_t)));
+   HIPCHECK (hipMalloc (&d_idx_mid, cap * sizeof (uint32_t)));
+   HIPCHECK (hipMalloc (&d_seg, cap * sizeof (uint32_t)));
+   HIPCHECK (hipMalloc (&d_rng_pos, cap * sizeof (uint32_t)));
    d_store = mem;
    char *p = (char *) mem;
    size_t k = 0;
@@ -170,6 +183,17 @@ int ensure_aos (size_t n)
    return 0;
 }
 
+// Blocks of a stage kernel over at most n_upper active photons: enough to
+// fill every CU several times over, each block owning an equal range.
+uint32_t stage_grid (uint32_t n_upper)
+{
+   uint32_t blocks = (n_upper + kBlock - 1) / kBlock;
+   uint32_t cap = (uint32_t) (g_num_cus > 0 ? g_num_cus : 256) * 8u;
+   if (cap > kMaxSegments) cap = kMaxSegments;
+   if (blocks > cap) blocks = cap;
+   return blocks ? blocks : 1u;
+}
+
 ListIO list_io ()
 {
    ListIO io;
@@ -177,6 +201,9 @@ ListIO list_io ()
    io.n_in = d_counts + g_cur;
    io.idx_out = d_idx[1 - g_cur];
    io.n_out = d_counts + (1 - g_cur);
+   io.seg = d_seg;
+   io.seg_count = d_seg_count;
+   io.grid = stage_grid (g_upper);
    return io;
 }
 
@@ -217,6 +244,40 @@ int end_stage (int slot, float *ms)
 
 } // namespace
 
+namespace mxg {
+
+// Concatenates the survivor segments of a stage kernel of gridDim.x blocks
+// (block b of this kernel copies segment b after the sum of the counts
+// before it) and writes the total to n_out.
+__global__ __launch_bounds__ (kBlock)
+void k_compact (const uint32_t *__restrict__ seg, const uint32_t *__restrict__ seg_count,
+                const uint32_t *__restrict__ n_in, uint32_t *__restrict__ out,
+                uint32_t *__restrict__ n_out)
+{
+   __shared__ uint32_t part[kBlock / 64];
+   const uint32_t b = blockIdx.x;
+   uint32_t sum = 0;
+   for (uint32_t k = threadIdx.x; k < b; k += kBlock) sum += seg_count[k];
+   for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor (sum, off);
+   if (__lane_id () == 0) part[threadIdx.x >> 6] = sum;
+   __syncthreads ();
+   uint32_t prefix = 0;
+   for (int w = 0; w < kBlock / 64; w++) prefix += part[w];
+   const uint32_t cnt = seg_count[b];
+   const BlockRange r = block_range (*n_in, b, gridDim.x);
+   for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) out[prefix + k] = seg[r.lo + k];
+   if ((b == gridDim.x - 1) && (threadIdx.x == 0)) *n_out = prefix + cnt;
+}
+
+hipError_t launch_compact (const ListIO &io, hipStream_t st)
+{
+   hipLaunchKernelGGL (k_compact, dim3 (io.grid), dim3 (kBlock), 0, st, io.seg, io.seg_count,
+                       io.n_in, io.idx_out, io.n_out);
+   return hipGetLastError ();
+}
+
+} // namespace mxg
+
 extern "C" {
 
 const char *mxgd_last_error (void) { return g_err; }
@@ -234,6 +295,7 @@ int mxgd_init (void)
    HIPCHECK (hipGetDeviceProperties (&prop, dev));
    if (strncmp (prop.gcnArchName, "gfx950", 6) != 0)
      return fail ("device %d is %s; this library is built for gfx950", dev, prop.gcnArchName);
+   g_num_cus = prop.multiProcessorCount;
    HIPCHECK (hipStreamCreateWithFlags (&g_stream, hipStreamNonBlocking));
    HIPCHECK (hipEventCreate (&g_ev0));
    HIPCHECK (hipEventCreate (&g_ev1));
@@ -244,6 +306,7 @@ int mxgd_init (void)
         g_slot_used[k] = 0;
      }
    HIPCHECK (hipMalloc (&d_counts, 64 * sizeof (uint32_t)));
+   HIPCHECK (hipMalloc (&d_seg_count, kMaxSegments * sizeof (uint32_t)));
    HIPCHECK (hipMemset (d_counts, 0, 64 * sizeof (uint32_t)));
    g_inited = 1;
    return 0;
@@ -285,6 +348,7 @@ int mxgd_sync_config (const mxg_config *cfg, const float *fa, unsigned int nf,
         d_da_cap = nd + 1;
      }
    HIPCHECK (hipMemcpy (d_cfg, cfg, sizeof (mxg_config), hipMemcpyHostToDevice));
+   memcpy (&g_hcfg, cfg, sizeof (mxg_config));
    if (nf) HIPCHECK (hipMemcpy (d_fa, fa, nf * sizeof (float), hipMemcpyHostToDevice));
    if (nd) HIPCHECK (hipMemcpy (d_da, da, nd * sizeof (double), hipMemcpyHostToDevice));
    g_cfg_gen = generation;
@@ -362,8 +426,9 @@ int mxgd_run_mirror (double source_distance, unsigned long long seed, int slot, 
    if (!d_cfg) return fail ("configuration not on the device");
    if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
    if (begin_stage (slot)) return -1;
-   HIPCHECK (launch_hrma (d_cfg, d_fa, d_da, g_soa, list_io (), g_upper, source_distance, seed, 0,
-                          g_stream));
+   HrmaScratch hs = {d_idx_mid, d_counts + 4, d_rng_pos};
+   HIPCHECK (launch_hrma (d_cfg, &g_hcfg, d_fa, d_da, g_soa, list_io (), hs, g_upper,
+                          source_distance, seed, 0, g_stream));
    return end_stage (slot, ms);
 }
 

@@ -182,16 +182,17 @@ void k_grating (const mxg_config *__restrict__ cfg, const float *__restrict__ fa
                 unsigned long long seed, uint32_t source_id)
 {
    const mxg_grating_cfg &gc = cfg->grating;
-   const uint32_t n = *io.n_in;
-   const uint32_t stride = gridDim.x * blockDim.x;
+   __shared__ uint32_t seg_wc[2 * (kBlock / 64)];
+   const BlockRange range = block_range (*io.n_in, blockIdx.x, gridDim.x);
+   SegWriter survivors (io, range, seg_wc);
    const mxg_sectors no_sectors = {0u, 0u};
 
-   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride)
+   for (uint32_t base = range.lo; base < range.hi; base += kBlock)
      {
         uint32_t i = base + threadIdx.x;
         bool keep = false;
         uint32_t id = 0;
-        if (i < n)
+        if (i < range.hi)
           {
              id = io.idx_in[i];
              uint32_t flags = s.flags[id];
@@ -247,8 +248,9 @@ void k_grating (const mxg_config *__restrict__ cfg, const float *__restrict__ fa
                     }
                }
           }
-        append_survivor (keep, id, io.idx_out, io.n_out);
+        survivors.add (keep, id);
      }
+   survivors.finish (io);
 }
 
 hipError_t launch_grating (const mxg_config *cfg, const float *fa, const double *da, Soa s,
@@ -256,10 +258,11 @@ hipError_t launch_grating (const mxg_config *cfg, const float *fa, const double 
                            uint32_t source_id, hipStream_t st)
 {
    if (n_upper == 0) return hipSuccess;
-   uint32_t blocks = (n_upper + kBlock - 1) / kBlock;
-   hipLaunchKernelGGL (k_grating, dim3 (blocks), dim3 (kBlock), 0, st, cfg, fa, da, s, io, seed,
+   hipLaunchKernelGGL (k_grating, dim3 (io.grid), dim3 (kBlock), 0, st, cfg, fa, da, s, io, seed,
                        source_id);
-   return hipGetLastError ();
+   hipError_t e = hipGetLastError ();
+   if (e == hipSuccess) e = launch_compact (io, st);
+   return e;
 }
 
 } // namespace mxg

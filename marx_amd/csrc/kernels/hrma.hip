@@ -6,8 +6,8 @@
 //   intersection (hrma.c:411-478), normal blur (:1054-1093), Fresnel
 //   reflectivity draw (reflect.c:39-77), specular reflection, WFold scatter
 //   (wfold.c:334-369) -> cap struts -> hyperboloid: the same -> post-collimator
-//   struts.  Geometry is FP64.  Survivors are appended to the next active list
-//   with one atomic per wavefront.
+//   struts.  Geometry is FP64.  Survivors go to the next active list in input
+//   order (SegWriter + k_compact).
 #include "dev_common.h"
 #include "dev_store.h"
 
@@ -38,8 +38,11 @@ __device__ __forceinline__ bool in_strut_shadow (const mxg_hrma_cfg &h, vec3 x, 
    return false;
 }
 
+// The hit of a ray on the quadric x^2+y^2+z^2 = a x^2 + b x + c between
+// xmin <= x < xmax (the larger valid root in x).  On success x0 becomes the
+// hit; on a miss x0 is unchanged and -1 is returned.
 __device__ __forceinline__ int conic_intersection (double a, double b, double c, vec3 &x0, vec3 p,
-                                                   vec3 &normal, double xmin, double xmax)
+                                                   double xmin, double xmax)
 {
    double t_plus = -x0.x / p.x;
    double x_y = x0.y + t_plus * p.y;
@@ -72,11 +75,18 @@ __device__ __forceinline__ int conic_intersection (double a, double b, double c,
    x0.x = p.x * t;
    x0.y = x_y + p.y * t;
    x0.z = x_z + p.z * t;
-   normal.x = (a - 1) * x0.x + 0.5 * b;
-   normal.y = -x0.y;
-   normal.z = -x0.z;
-   normalize (normal);
    return 0;
+}
+
+// inward unit normal of the quadric at a point on it (hrma.c:470-476)
+__device__ __forceinline__ vec3 conic_normal (double a, double b, vec3 x)
+{
+   vec3 n;
+   n.x = (a - 1) * x.x + 0.5 * b;
+   n.y = -x.y;
+   n.z = -x.z;
+   normalize (n);
+   return n;
 }
 
 __device__ __forceinline__ void blur_normal (vec3 &n, double blur, Stream &st)
@@ -91,17 +101,17 @@ __device__ __forceinline__ void blur_normal (vec3 &n, double blur, Stream &st)
    n = rotate_unit_vector (n, perp, phi);
 }
 
-// 0 reflected, -1 absorbed or scattered out, -2 missed the surface
-__device__ __forceinline__ int reflect_surface (const mxg_hrma_cfg &h, const mxg_shell_cfg &S, int k,
-                                                vec3 &x, vec3 &p, double energy, double beta,
-                                                double delta, double corr,
-                                                const float *__restrict__ fa,
-                                                const double *__restrict__ da, Stream &st)
+// Reflection at a surface point x already on conic k of shell S: normal
+// blur, Fresnel draw, specular reflection, WFold scatter (hrma.c:488-545).
+// 0 reflected, -1 absorbed or scattered out.
+__device__ __forceinline__ int reflect_at (const mxg_hrma_cfg &h, const mxg_shell_cfg &S, int k,
+                                           vec3 x, vec3 &p, double energy, double beta,
+                                           double delta, double corr,
+                                           const float *__restrict__ fa,
+                                           const double *__restrict__ da,
+                                           const float *__restrict__ e_alphas, Stream &st)
 {
-   vec3 normal;
-   if (-1 == conic_intersection (S.conic_a[k], S.conic_b[k], S.conic_c[k], x, p, normal,
-                                 S.conic_xmin[k], S.conic_xmax[k]))
-     return -2;
+   vec3 normal = conic_normal (S.conic_a[k], S.conic_b[k], x);
    if (h.use_blur)
      blur_normal (normal, S.blur[k], st);
    double p_dot_n = dot (p, normal);
@@ -116,7 +126,7 @@ __device__ __forceinline__ int reflect_surface (const mxg_hrma_cfg &h, const mxg
    if (h.use_wfold == 0)
      return 0;
    double r = uniform (st);
-   double dg = wfold_interp (S.wfold[k], fa, da, energy, -p_dot_n, r);
+   double dg = wfold_interp (S.wfold[k], fa, da, e_alphas, energy, -p_dot_n, r);
    dg *= S.scat_factor[k];
    if (dg > MXG_PI / 4) return -1;
    if (uniform (st) < 0.5) dg = -dg;
@@ -124,21 +134,51 @@ __device__ __forceinline__ int reflect_surface (const mxg_hrma_cfg &h, const mxg
    return 0;
 }
 
-__global__ __launch_bounds__ (kBlock)
-void k_hrma (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
-             const double *__restrict__ da, Soa s, ListIO io, double source_distance,
-             unsigned long long seed, uint32_t source_id)
+// Copies the mirror configuration (a multiple of 4 bytes) into LDS.
+__device__ __forceinline__ void stage_hrma_cfg (const mxg_config *__restrict__ cfg, unsigned char *lds)
 {
-   const mxg_hrma_cfg &h = cfg->hrma;
-   const uint32_t n = *io.n_in;
-   const uint32_t stride = gridDim.x * blockDim.x;
+   constexpr uint32_t cfg_words = sizeof (mxg_hrma_cfg) / 4;
+   const uint32_t *src = (const uint32_t *) &cfg->hrma;
+   uint32_t *dst = (uint32_t *) lds;
+   for (uint32_t w = threadIdx.x; w < cfg_words; w += blockDim.x) dst[w] = src[w];
+}
 
-   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride)
+// The mirror runs as two kernels so that the reflection work runs on full
+// wavefronts: about half of the photons sampled over the aperture annulus
+// miss the paraboloid, and in a single kernel their lanes would idle through
+// both reflections.
+//
+//   k_hrma_enter:   vignetting, aperture, precollimator struts, frame change
+//                   and the paraboloid intersection, for every active photon.
+//                   Photons that hit go to the middle list, and their
+//                   random-stream position is saved.
+//   k_hrma_reflect: paraboloid reflection, cap struts, hyperboloid
+//                   intersection and reflection, post-collimator struts, for
+//                   the middle list.
+//
+// The stream position is the Philox block counter c2 and the index of the
+// next unused word of the current block, packed as (c2 << 3) | pos.  No
+// Gaussian variate is pending at the split (none is drawn before it), and
+// the buffered block is recomputed from c2 - 1 on resumption.
+__global__ __launch_bounds__ (kBlock)
+void k_hrma_enter (const mxg_config *__restrict__ cfg, Soa s, ListIO io,
+                   uint32_t *__restrict__ rng_pos, double source_distance,
+                   unsigned long long seed, uint32_t source_id)
+{
+   __shared__ __align__ (16) unsigned char lds[sizeof (mxg_hrma_cfg)];
+   stage_hrma_cfg (cfg, lds);
+   __syncthreads ();
+   const mxg_hrma_cfg &h = *(const mxg_hrma_cfg *) lds;
+   __shared__ uint32_t seg_wc[2 * (kBlock / 64)];
+   const BlockRange range = block_range (*io.n_in, blockIdx.x, gridDim.x);
+   SegWriter survivors (io, range, seg_wc);
+
+   for (uint32_t base = range.lo; base < range.hi; base += kBlock)
      {
         uint32_t i = base + threadIdx.x;
         bool keep = false;
         uint32_t id = 0;
-        if (i < n)
+        if (i < range.hi)
           {
              id = io.idx_in[i];
              uint32_t flags = s.flags[id];
@@ -153,7 +193,6 @@ void k_hrma (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
                   else
                     {
                        vec3 x, p = v3 (s.p0[id], s.p1[id], s.p2[id]);
-                       double energy = s.energy[id];
 
                        // aperture: shell by cumulative area, radius, open quadrant
                        int si;
@@ -195,67 +234,196 @@ void k_hrma (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
                        x = m3_vmul (S.fwd[0], x);
                        p = m3_vmul (S.fwd[0], p);
 
-                       double beta = 0.0, delta = 1.0, corr = 1.0;
-                       if (h.opt_n != 0)
-                         {
-                            float ef = (float) energy;
-                            beta = interp_f (ef, fa + h.opt_e_off, fa + h.opt_beta_off, h.opt_n);
-                            delta = interp_f (ef, fa + h.opt_e_off, fa + h.opt_delta_off, h.opt_n);
-                            if (h.use_scale)
-                              {
-                                 corr = interp_f (ef, fa + S.corr_e_off, fa + S.corr_f_off, S.corr_n);
-                                 corr = sqrt (corr);
-                              }
-                         }
-
-                       if (0 != reflect_surface (h, S, 0, x, p, energy, beta, delta, corr, fa, da, st))
+                       if (-1 == conic_intersection (S.conic_a[0], S.conic_b[0], S.conic_c[0], x, p,
+                                                     S.conic_xmin[0], S.conic_xmax[0]))
                          flags |= PHOTON_UNREFLECTED;
                        else
                          {
-                            p = m3_vmul (S.bwd[0], p);
-                            x = m3_vmul (S.bwd[0], x);
-                            x = sub (x, ld3 (S.to_osac[0]));
-                            if (h.use_struts && in_strut_shadow (h, x, p, h.cap_struts))
-                              flags |= PHOTON_MIRROR_VBLOCKED;
-                            else
-                              {
-                                 x = add (x, ld3 (S.to_osac[1]));
-                                 p = m3_vmul (S.fwd[1], p);
-                                 x = m3_vmul (S.fwd[1], x);
-                                 if (0 != reflect_surface (h, S, 1, x, p, energy, beta, delta, corr,
-                                                           fa, da, st))
-                                   flags |= PHOTON_UNREFLECTED;
-                                 else
-                                   {
-                                      p = m3_vmul (S.bwd[1], p);
-                                      x = m3_vmul (S.bwd[1], x);
-                                      x = sub (x, ld3 (S.to_osac[1]));
-                                      if (h.use_struts && in_strut_shadow (h, x, p, h.post_struts))
-                                        flags |= PHOTON_MIRROR_VBLOCKED;
-                                   }
-                              }
+                            keep = true;
+                            rng_pos[id] = (st.c2 << 3) | (uint32_t) st.pos;
                          }
                        s.x0[id] = x.x; s.x1[id] = x.y; s.x2[id] = x.z;
                        s.p0[id] = p.x; s.p1[id] = p.y; s.p2[id] = p.z;
                        s.shell[id] = (uint8_t) si;
                        s.flags[id] = flags;
-                       keep = (flags & BAD_PHOTON_MASK) == 0;
                     }
                }
           }
-        append_survivor (keep, id, io.idx_out, io.n_out);
+        survivors.add (keep, id);
      }
+   survivors.finish (io);
 }
 
-hipError_t launch_hrma (const mxg_config *cfg, const float *fa, const double *da, Soa s,
-                        ListIO io, uint32_t n_upper, double source_distance,
-                        unsigned long long seed, uint32_t source_id, hipStream_t st)
+// Offsets (in floats) of the tables staged in LDS by k_hrma_reflect.
+struct HrmaLds
+{
+   uint32_t opt;                  // energies, betas, deltas: 3 * opt_n
+   uint32_t corr_e[MXG_NUM_SHELLS], corr_f[MXG_NUM_SHELLS];
+   uint32_t ea[MXG_NUM_SHELLS][2];  // e_alpha of each WFold table (float32 on disk: exact)
+   uint32_t floats;               // staged floats after the configuration copy
+};
+
+// The block stages the mirror configuration and its small, hot tables (the
+// binary-search targets of every photon) in LDS once, then loops over its
+// range of the middle list.
+__global__ __launch_bounds__ (kBlock)
+void k_hrma_reflect (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
+                     const double *__restrict__ da, Soa s, ListIO io,
+                     const uint32_t *__restrict__ rng_pos, unsigned long long seed,
+                     uint32_t source_id, HrmaLds lay)
+{
+   extern __shared__ __align__ (16) unsigned char lds[];
+   float *ltab = (float *) (lds + sizeof (mxg_hrma_cfg));
+   stage_hrma_cfg (cfg, lds);
+   {
+      const mxg_hrma_cfg &gh = cfg->hrma;
+      for (uint32_t w = threadIdx.x; w < gh.opt_n; w += blockDim.x)
+        {
+           ltab[lay.opt + w] = fa[gh.opt_e_off + w];
+           ltab[lay.opt + gh.opt_n + w] = fa[gh.opt_beta_off + w];
+           ltab[lay.opt + 2 * gh.opt_n + w] = fa[gh.opt_delta_off + w];
+        }
+      for (int sh = 0; sh < MXG_NUM_SHELLS; sh++)
+        {
+           const mxg_shell_cfg &S = gh.shell[sh];
+           for (uint32_t w = threadIdx.x; w < S.corr_n; w += blockDim.x)
+             {
+                ltab[lay.corr_e[sh] + w] = fa[S.corr_e_off + w];
+                ltab[lay.corr_f[sh] + w] = fa[S.corr_f_off + w];
+             }
+           for (int k = 0; k < 2; k++)
+             for (uint32_t w = threadIdx.x; w < S.wfold[k].n_arrays; w += blockDim.x)
+               ltab[lay.ea[sh][k] + w] = (float) da[S.wfold[k].e_alphas_off + w];
+        }
+   }
+   __syncthreads ();
+
+   const mxg_hrma_cfg &h = *(const mxg_hrma_cfg *) lds;
+   const float *opt_e = ltab + lay.opt;
+   const float *opt_b = opt_e + h.opt_n;
+   const float *opt_d = opt_b + h.opt_n;
+   __shared__ uint32_t seg_wc[2 * (kBlock / 64)];
+   const BlockRange range = block_range (*io.n_in, blockIdx.x, gridDim.x);
+   SegWriter survivors (io, range, seg_wc);
+
+   for (uint32_t base = range.lo; base < range.hi; base += kBlock)
+     {
+        uint32_t i = base + threadIdx.x;
+        bool keep = false;
+        uint32_t id = 0;
+        if (i < range.hi)
+          {
+             id = io.idx_in[i];
+             uint32_t flags = s.flags[id];
+             const int si = s.shell[id];
+             const mxg_shell_cfg &S = h.shell[si];
+             Stream st;
+             stream_init (st, seed, s.tag[id], MXG_STAGE_MIRROR, source_id);
+             {
+                uint32_t w = rng_pos[id];
+                st.c2 = w >> 3;
+                st.pos = (int) (w & 7u);
+                if (st.pos < 4)
+                  philox (st.c0, st.c1, st.c2 - 1u, st.c3, st.k0, st.k1, st.b0, st.b1, st.b2, st.b3);
+             }
+             vec3 x = v3 (s.x0[id], s.x1[id], s.x2[id]);
+             vec3 p = v3 (s.p0[id], s.p1[id], s.p2[id]);
+             double energy = s.energy[id];
+
+             double beta = 0.0, delta = 1.0, corr = 1.0;
+             if (h.opt_n != 0)
+               {
+                  float ef = (float) energy;
+                  beta = interp_f (ef, opt_e, opt_b, h.opt_n);
+                  delta = interp_f (ef, opt_e, opt_d, h.opt_n);
+                  if (h.use_scale)
+                    {
+                       corr = interp_f (ef, ltab + lay.corr_e[si], ltab + lay.corr_f[si], S.corr_n);
+                       corr = sqrt (corr);
+                    }
+               }
+
+             if (0 != reflect_at (h, S, 0, x, p, energy, beta, delta, corr, fa, da,
+                                  ltab + lay.ea[si][0], st))
+               flags |= PHOTON_UNREFLECTED;
+             else
+               {
+                  p = m3_vmul (S.bwd[0], p);
+                  x = m3_vmul (S.bwd[0], x);
+                  x = sub (x, ld3 (S.to_osac[0]));
+                  if (h.use_struts && in_strut_shadow (h, x, p, h.cap_struts))
+                    flags |= PHOTON_MIRROR_VBLOCKED;
+                  else
+                    {
+                       x = add (x, ld3 (S.to_osac[1]));
+                       p = m3_vmul (S.fwd[1], p);
+                       x = m3_vmul (S.fwd[1], x);
+                       if ((-1 == conic_intersection (S.conic_a[1], S.conic_b[1], S.conic_c[1], x, p,
+                                                      S.conic_xmin[1], S.conic_xmax[1]))
+                           || (0 != reflect_at (h, S, 1, x, p, energy, beta, delta, corr, fa, da,
+                                                ltab + lay.ea[si][1], st)))
+                         flags |= PHOTON_UNREFLECTED;
+                       else
+                         {
+                            p = m3_vmul (S.bwd[1], p);
+                            x = m3_vmul (S.bwd[1], x);
+                            x = sub (x, ld3 (S.to_osac[1]));
+                            if (h.use_struts && in_strut_shadow (h, x, p, h.post_struts))
+                              flags |= PHOTON_MIRROR_VBLOCKED;
+                         }
+                    }
+               }
+             s.x0[id] = x.x; s.x1[id] = x.y; s.x2[id] = x.z;
+             s.p0[id] = p.x; s.p1[id] = p.y; s.p2[id] = p.z;
+             s.flags[id] = flags;
+             keep = (flags & BAD_PHOTON_MASK) == 0;
+          }
+        survivors.add (keep, id);
+     }
+   survivors.finish (io);
+}
+
+hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const float *fa,
+                        const double *da, Soa s, ListIO io, HrmaScratch hs, uint32_t n_upper,
+                        double source_distance, unsigned long long seed, uint32_t source_id,
+                        hipStream_t st)
 {
    if (n_upper == 0) return hipSuccess;
-   uint32_t blocks = (n_upper + kBlock - 1) / kBlock;
-   hipLaunchKernelGGL (k_hrma, dim3 (blocks), dim3 (kBlock), 0, st, cfg, fa, da, s, io,
+   const mxg_hrma_cfg &h = hcfg->hrma;
+   HrmaLds lay;
+   uint32_t f = 0;
+   lay.opt = f; f += 3 * h.opt_n;
+   for (int sh = 0; sh < MXG_NUM_SHELLS; sh++)
+     {
+        lay.corr_e[sh] = f; f += h.shell[sh].corr_n;
+        lay.corr_f[sh] = f; f += h.shell[sh].corr_n;
+        for (int k = 0; k < 2; k++)
+          {
+             lay.ea[sh][k] = f;
+             f += h.shell[sh].wfold[k].n_arrays;
+          }
+     }
+   lay.floats = f;
+   size_t lds_bytes = sizeof (mxg_hrma_cfg) + (size_t) f * sizeof (float);
+   if (lds_bytes > 64 * 1024)
+     return hipErrorInvalidConfiguration;
+
+   // io.idx_in -> k_hrma_enter -> middle list -> k_hrma_reflect -> io.idx_out
+   ListIO a = io, b = io;
+   a.idx_out = hs.idx_mid;
+   a.n_out = hs.n_mid;
+   b.idx_in = hs.idx_mid;
+   b.n_in = hs.n_mid;
+   hipLaunchKernelGGL (k_hrma_enter, dim3 (a.grid), dim3 (kBlock), 0, st, cfg, s, a, hs.rng_pos,
                        source_distance, seed, source_id);
-   return hipGetLastError ();
+   hipError_t e = hipGetLastError ();
+   if (e == hipSuccess) e = launch_compact (a, st);
+   if (e != hipSuccess) return e;
+   hipLaunchKernelGGL (k_hrma_reflect, dim3 (b.grid), dim3 (kBlock), lds_bytes, st, cfg, fa, da, s,
+                       b, hs.rng_pos, seed, source_id, lay);
+   e = hipGetLastError ();
+   if (e == hipSuccess) e = launch_compact (b, st);
+   return e;
 }
 
 } // namespace mxg
