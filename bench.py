@@ -21,48 +21,84 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-# Algorithmic FP64 work per photon entering each stage (adds/subs, muls,
-# divs and sqrts of the restated reference algorithm, counted by the
-# op-counting oracle build on this workload: `python -m oracle.opcount.opcount
-# --photons 100000`).  Transcendental calls per photon are listed beside.
-FLOPS_PER_PHOTON = {"source": 146.0, "mirror": 438.7, "grating": 249.3, "detector": 200.1}
-TRANSCENDENTALS_PER_PHOTON = {"source": 11.0, "mirror": 10.84, "grating": 8.81, "detector": 0.69}
-# Algorithmic HBM bytes per photon entering each stage: structure-of-arrays
-# columns the stage must read and write, plus the active-list index in and out.
+# Algorithmic FP64 work per photon entering each stage, and each of the
+# mirror's two kernels (adds/subs, muls, divs and sqrts of the restated
+# reference algorithm, counted by the op-counting oracle build on this
+# workload: `python -m oracle.opcount.opcount --photons 100000`).
+# Transcendental calls per photon are listed beside.
+FLOPS_PER_PHOTON = {"source": 146.0, "mirror": 438.7, "grating": 249.3, "detector": 200.1,
+                    "k_hrma_enter": 130.3, "k_hrma_reflect": 598.6}
+TRANSCENDENTALS_PER_PHOTON = {"source": 11.0, "mirror": 10.84, "grating": 8.81, "detector": 0.69,
+                              "k_hrma_enter": 4.00, "k_hrma_reflect": 13.62}
+# Algorithmic HBM bytes per photon entering each kernel: the structure-of-
+# arrays columns it must read and write, plus the active-list index in and
+# the survivor index out.
 BYTES_PER_PHOTON = {
     "source": 8 + 24 + 8 + 4 + 4 + 24 + 8,     # energy, p, t, flags, tag, dither, scan word
-    "mirror": (4 + 4 + 4 + 8 + 24) + (24 + 24 + 1 + 4 + 4),
+    # in: index, flags, tag, p; out: x, p, shell, flags, stream position, index
+    "k_hrma_enter": (4 + 4 + 4 + 24) + (24 + 24 + 1 + 4 + 4 + 4),
+    # in: index, flags, shell, tag, stream position, x, p, energy; out: x, p, flags, index
+    "k_hrma_reflect": (4 + 4 + 1 + 4 + 4 + 24 + 24 + 8) + (24 + 24 + 4 + 4),
     "grating": (4 + 4 + 4 + 1 + 8 + 24 + 24 + 1) + (24 + 24 + 4 + 1 + 4),
     "detector": (4 + 4 + 4 + 8 + 8 + 24 + 24) + (24 + 24 + 4 + 1 + 4 + 4 + 4),
 }
+# HBM bytes per photon entering k_hrma_reflect, measured with rocprofv3
+# counters (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per dispatch, the factors
+# that make a 1 GiB device copy read back 1 GiB each way); the command and
+# the figures are in profiles/pmc_hetg_acis_s_1e7.md.  None when not measured
+# for the current kernels.
+TRAFFIC_BYTES_PER_PHOTON = {"k_hrma_reflect": 317.3}
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
 STAGES = ["source", "mirror", "grating", "detector"]
 
 
-def cpu_baseline(tracer, target_seconds=12.0):
+def cpu_baseline(tracer, target_seconds=20.0):
     """The FP64 oracle over a bounded sample of the same workload, OpenMP
-    over the host cores (one thread per chunk of 50k photons)."""
+    over the host cores (one thread per chunk of 50k photons).  Two probe
+    runs size the timed sample to about target_seconds."""
     from oracle.oracle import Oracle
     threads = max(1, min(16, os.cpu_count() or 1))
     o = Oracle(tracer)
-    probe = 200_000
-    t0 = time.perf_counter()
-    if o.trace_parallel(probe, 50_000, threads) < 0:
-        raise RuntimeError("oracle failed")
-    rate = probe / max(time.perf_counter() - t0, 1e-6)
-    sample = int(min(max(rate * target_seconds, 400_000), 2e8))
-    sample -= sample % 50_000
-    t0 = time.perf_counter()
-    if o.trace_parallel(sample, 50_000, threads) < 0:
-        raise RuntimeError("oracle failed")
-    dt = time.perf_counter() - t0
+    chunk = 50_000
+
+    def timed(n):
+        t0 = time.perf_counter()
+        if o.trace_parallel(n, chunk, threads) < 0:
+            raise RuntimeError("oracle failed")
+        return time.perf_counter() - t0
+
+    n = chunk * threads
+    for target in (0.5, 2.0):
+        dt = timed(n)
+        n = int(min(max(n * target / max(dt, 1e-3), chunk * threads), 4e8))
+    sample = int(min(max(n * target_seconds / 2.0, chunk * threads), 4e8))
+    sample -= sample % chunk
+    dt = timed(sample)
     return {"value": sample / dt, "unit": "photons/s", "cores": threads, "kind": "port",
-            "sample": f"{sample} photons of the same workload (FP64 CPU oracle, chunks of 50000, "
+            "sample": f"{sample} photons of the same workload (FP64 CPU oracle, chunks of {chunk}, "
                       f"{threads} OpenMP threads), {dt:.1f} s"}
 
 
+def run_quietly(fn):
+    """Runs fn with file descriptor 1 sent to stderr, so that the library's
+    progress messages do not mix with the one JSON line on stdout."""
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        return fn()
+    finally:
+        sys.stdout.flush()
+        os.dup2(saved, 1)
+        os.close(saved)
+
+
 def main():
+    print(json.dumps(run_quietly(measure)))
+
+
+def measure():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -102,11 +138,17 @@ def main():
     torch.cuda.synchronize()
     stage_ms = {s: 0.0 for s in STAGES}
     stage_in = {s: 0 for s in STAGES}
+    kern_ms = {"k_hrma_enter": 0.0, "k_hrma_reflect": 0.0}
+    kern_in = {"k_hrma_enter": 0, "k_hrma_reflect": 0}
     detected = 0
     t0 = time.perf_counter()
     for k in range(args.steps):
         ncol, ndet, ms = step(args.warmup + k, timing=True)
         c = tracer.stage_counts()
+        kms, kin = tracer.kernel_stats()
+        for j, name in enumerate(("k_hrma_enter", "k_hrma_reflect")):
+            kern_ms[name] += kms[j]
+            kern_in[name] += kin[j]
         stage_ms["source"] += ms[0]
         stage_ms["mirror"] += ms[2]
         stage_ms["grating"] += ms[3]
@@ -121,13 +163,27 @@ def main():
     K = args.steps
     value = K * N / elapsed
 
-    dom = max(STAGES, key=lambda s: stage_ms[s])
-    t_launch = stage_ms[dom] / K * 1e-3
-    units = stage_in[dom] / K
+    # the dominant kernel, timed alone with HIP events on the library's stream
+    dom = "k_hrma_reflect"
+    t_launch = kern_ms[dom] / K * 1e-3
+    units = kern_in[dom] / K
     flops = FLOPS_PER_PHOTON[dom] * units
     nbytes = BYTES_PER_PHOTON[dom] * units
     achieved = flops / t_launch / 1e12
     achieved_bw = nbytes / t_launch / 1e9
+    traffic = (TRAFFIC_BYTES_PER_PHOTON[dom] * units
+               if TRAFFIC_BYTES_PER_PHOTON.get(dom) is not None else None)
+
+    kernels = {}
+    for name, ms_k, n_k in (("k_hrma_enter", kern_ms["k_hrma_enter"], kern_in["k_hrma_enter"]),
+                         ("k_hrma_reflect", kern_ms["k_hrma_reflect"], kern_in["k_hrma_reflect"]),
+                         ("grating stage", stage_ms["grating"], stage_in["grating"]),
+                         ("detector stage", stage_ms["detector"], stage_in["detector"])):
+        key = name.split()[0]
+        t = ms_k / K * 1e-3
+        kernels[name] = {"ms": ms_k / K, "photons_in": n_k / K,
+                      "tflops": FLOPS_PER_PHOTON[key] * n_k / K / t / 1e12 if t > 0 else None,
+                      "gbs": BYTES_PER_PHOTON[key] * n_k / K / t / 1e9 if t > 0 else None}
 
     out = {
         "metric": "photons/s traced (HRMA+grating+detector) 1 GPU vs host CPU; gradient evals/s",
@@ -150,7 +206,7 @@ def main():
         "roofline": {"bound": "mfma", "pipe": "fp64 vector ALU (no matrix work on this path; 78.6 TFLOP/s "
                                               "is MI355X's FP64 peak)",
                      "kernel": dom, "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "flops_per_photon": FLOPS_PER_PHOTON[dom],
                      "transcendentals_per_photon": TRANSCENDENTALS_PER_PHOTON[dom],
                      "photons_per_launch": units, "launch_ms": t_launch * 1e3,
@@ -159,6 +215,7 @@ def main():
                              "bytes_per_photon": BYTES_PER_PHOTON[dom]}},
         "stages_ms": {s: stage_ms[s] / K for s in STAGES},
         "stage_photons_in": {s: stage_in[s] / K for s in STAGES},
+        "kernels": kernels,
     }
     if args.copy_probe > 0:
         src = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
@@ -171,7 +228,7 @@ def main():
         cb = cpu_baseline(tracer)
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = value / cb["value"]
-    print(json.dumps(out))
+    return out
 
 
 if __name__ == "__main__":

@@ -242,6 +242,14 @@ int mxg_trace_device (unsigned int num, unsigned int tag_start,
  * created, after the mirror, after the grating, after the detector. */
 int mxg_last_stage_counts (unsigned int counts[4]);
 
+/* The two mirror kernels of the last mxg_trace_device call, each timed
+ * alone with HIP events on the library's stream: [0] aperture + paraboloid
+ * intersection, [1] the two reflections.  ms receives their device times in
+ * milliseconds, photons_in the photons each one processed (the photons
+ * entering the mirror, and those that hit the paraboloid).  Either pointer
+ * may be NULL. */
+int mxg_last_kernel_stats (float ms[2], unsigned int photons_in[2]);
+
 /* Copies the device-resident photons of the last mxg_trace_device call into
  * pt (AoS), setting n_photons, history and the pruned sorted_index. */
 int mxg_download_photons (Marx_Photon_Type *pt);

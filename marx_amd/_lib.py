@@ -114,6 +114,7 @@ _SIGNATURES = [
                                         ctypes.POINTER(ctypes.c_float)]),
     ("mxg_download_photons", ctypes.c_int, [PPhotonList]),
     ("mxg_last_stage_counts", ctypes.c_int, [ctypes.POINTER(ctypes.c_uint)]),
+    ("mxg_last_kernel_stats", ctypes.c_int, [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint)]),
     ("mxg_device_info", ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint, ctypes.POINTER(ctypes.c_int)]),
     ("mxg_get_config", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
                                       ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_void_p),

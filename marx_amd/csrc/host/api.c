@@ -370,6 +370,8 @@ static unsigned int Last_Tag_Start = 0;
 static unsigned long Last_History = 0;
 
 static unsigned int Stage_Counts[4];
+static float Kernel_Ms[2];
+static unsigned int Kernel_In[2];
 
 int mxg_trace_device (unsigned int num, unsigned int tag_start, double start_time,
                       double exposure_time, unsigned int *num_collected,
@@ -389,12 +391,13 @@ int mxg_trace_device (unsigned int num, unsigned int tag_start, double start_tim
         mxg_error ("Mirror not initialized.");
         return -1;
      }
-   /* timing slots: 0 source+dither, 1 mirror, 2 grating, 3 detector */
+   /* timing slots: 0 source+dither, 1 mirror, 2 grating, 3 detector,
+    * 4 and 5 the two mirror kernels alone */
    if ((-1 == mxgd_run_source (num, tag_start, start_time, exposure_time, Seed, 0, 0, &n, &last_t, NULL))
        || (-1 == mxgd_run_mirror (Mxg_Cfg.src.distance, Seed, 1, NULL))
        || (with_grating && (-1 == mxgd_run_grating (Seed, 2, NULL)))
        || ((Detector_Type > 0) && (-1 == mxgd_run_detect (start_time, Seed, 3, NULL)))
-       || (-1 == mxgd_read_timings (ms, counts, 4)))
+       || (-1 == mxgd_read_timings (ms, counts, 6)))
      {
         mxg_error ("%s", mxgd_last_error ());
         return -1;
@@ -408,6 +411,10 @@ int mxg_trace_device (unsigned int num, unsigned int tag_start, double start_tim
    Stage_Counts[2] = with_grating ? counts[2] : counts[1];
    Stage_Counts[3] = (Detector_Type > 0) ? counts[3] : Stage_Counts[2];
    ndet = Stage_Counts[3];
+   Kernel_Ms[0] = ms[MXGD_SLOT_HRMA_ENTER];
+   Kernel_Ms[1] = ms[MXGD_SLOT_HRMA_REFLECT];
+   Kernel_In[0] = counts[MXGD_SLOT_HRMA_ENTER];
+   Kernel_In[1] = counts[MXGD_SLOT_HRMA_REFLECT];
    Resident = NULL;
    Last_Collected = n;
    Last_Tag_Start = tag_start;
@@ -429,6 +436,13 @@ int mxg_trace_device (unsigned int num, unsigned int tag_start, double start_tim
 int mxg_last_stage_counts (unsigned int counts[4])
 {
    memcpy (counts, Stage_Counts, sizeof (Stage_Counts));
+   return 0;
+}
+
+int mxg_last_kernel_stats (float ms[2], unsigned int photons_in[2])
+{
+   if (ms) memcpy (ms, Kernel_Ms, sizeof (Kernel_Ms));
+   if (photons_in) memcpy (photons_in, Kernel_In, sizeof (Kernel_In));
    return 0;
 }
 

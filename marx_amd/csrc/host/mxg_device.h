@@ -39,6 +39,10 @@ int mxgd_get_soa (mxg_soa_view *view);
  * `slot` (read with mxgd_read_timings); ms, when non-NULL, synchronises and
  * returns the stage time at once. */
 int mxgd_run_mirror (double source_distance, unsigned long long seed, int slot, float *ms);
+/* Timing slots that mxgd_run_mirror, given a slot, also fills with its two
+ * kernels alone. */
+#define MXGD_SLOT_HRMA_ENTER 4
+#define MXGD_SLOT_HRMA_REFLECT 5
 int mxgd_run_grating (unsigned long long seed, int slot, float *ms);
 int mxgd_run_detect (double tstart, unsigned long long seed, int slot, float *ms);
 /* Synchronises and returns the times (ms) of the stages recorded in slots

@@ -38,6 +38,9 @@ struct Stream
    double g2;
 };
 
+// Philox4x32-10 (Salmon et al., SC'11).  Each round's two 32x32->64-bit
+// products are formed as one 64-bit multiply (v_mad_u64_u32), half the
+// multiply issue of separate low and high products.
 __device__ __forceinline__ void philox (uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                         uint32_t k0, uint32_t k1,
                                         uint32_t &o0, uint32_t &o1, uint32_t &o2, uint32_t &o3)
@@ -46,13 +49,11 @@ __device__ __forceinline__ void philox (uint32_t c0, uint32_t c1, uint32_t c2, u
    for (int r = 0; r < 10; r++)
      {
         if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
-        uint32_t lo0 = 0xD2511F53u * c0;
-        uint32_t hi0 = __umulhi (0xD2511F53u, c0);
-        uint32_t lo1 = 0xCD9E8D57u * c2;
-        uint32_t hi1 = __umulhi (0xCD9E8D57u, c2);
-        uint32_t n0 = hi1 ^ c1 ^ k0;
-        uint32_t n2 = hi0 ^ c3 ^ k1;
-        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        uint64_t p0 = (uint64_t) 0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t) 0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t) (p1 >> 32) ^ c1 ^ k0;
+        uint32_t n2 = (uint32_t) (p0 >> 32) ^ c3 ^ k1;
+        c0 = n0; c1 = (uint32_t) p1; c2 = n2; c3 = (uint32_t) p0;
      }
    o0 = c0; o1 = c1; o2 = c2; o3 = c3;
 }

@@ -49,6 +49,7 @@ struct ListIO
 constexpr int kBlock = 256;
 constexpr uint32_t kMaxSegments = 8192;
 
+
 struct BlockRange { uint32_t lo, hi; };
 
 // The input positions owned by block b of nblocks: equal chunks rounded up
@@ -119,11 +120,12 @@ struct HrmaScratch
 
 // Stage launchers (defined in the stage .hip files).
 // cfg: device copy; hcfg: host copy of the same configuration (launch
-// geometry and LDS layout are computed from it)
+// geometry and LDS layout are computed from it).  ev, when non-null, holds
+// four events recorded around k_hrma_enter (0, 1) and k_hrma_reflect (2, 3).
 hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const float *fa,
                         const double *da, Soa s, ListIO io, HrmaScratch hs, uint32_t n_upper,
                         double source_distance, unsigned long long seed, uint32_t source_id,
-                        hipStream_t st);
+                        hipStream_t st, hipEvent_t *ev);
 hipError_t launch_grating (const mxg_config *cfg, const float *fa, const double *da, Soa s,
                            ListIO io, uint32_t n_upper, unsigned long long seed,
                            uint32_t source_id, hipStream_t st);

@@ -427,8 +427,21 @@ int mxgd_run_mirror (double source_distance, unsigned long long seed, int slot, 
    if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
    if (begin_stage (slot)) return -1;
    HrmaScratch hs = {d_idx_mid, d_counts + 4, d_rng_pos};
+   hipEvent_t kev[4] = {g_tev[MXGD_SLOT_HRMA_ENTER][0], g_tev[MXGD_SLOT_HRMA_ENTER][1],
+                        g_tev[MXGD_SLOT_HRMA_REFLECT][0], g_tev[MXGD_SLOT_HRMA_REFLECT][1]};
+   bool timed = (slot >= 0) && (g_upper > 0);
    HIPCHECK (launch_hrma (d_cfg, &g_hcfg, d_fa, d_da, g_soa, list_io (), hs, g_upper,
-                          source_distance, seed, 0, g_stream));
+                          source_distance, seed, 0, g_stream, timed ? kev : nullptr));
+   if (timed)
+     {
+        // photons entering each kernel: the active list, the middle list
+        g_slot_used[MXGD_SLOT_HRMA_ENTER] = g_slot_used[MXGD_SLOT_HRMA_REFLECT] = 1;
+        hipLaunchKernelGGL (k_copy_u32, dim3 (1), dim3 (1), 0, g_stream,
+                            d_counts + 8 + MXGD_SLOT_HRMA_ENTER, d_counts + g_cur);
+        hipLaunchKernelGGL (k_copy_u32, dim3 (1), dim3 (1), 0, g_stream,
+                            d_counts + 8 + MXGD_SLOT_HRMA_REFLECT, hs.n_mid);
+        HIPCHECK (hipGetLastError ());
+     }
    return end_stage (slot, ms);
 }
 

@@ -254,6 +254,14 @@ void k_hrma_enter (const mxg_config *__restrict__ cfg, Soa s, ListIO io,
    survivors.finish (io);
 }
 
+// Minimum waves per SIMD for k_hrma_reflect (256 lanes per block = one wave
+// per SIMD per block).  Unbounded, the compiler gives it ~200 VGPRs, i.e.
+// two waves per SIMD, too few to hide its LDS and table-load latency; four
+// waves (128 VGPRs, a few spills) measured fastest.
+#ifndef MXG_REFLECT_WAVES
+#define MXG_REFLECT_WAVES 4
+#endif
+
 // Offsets (in floats) of the tables staged in LDS by k_hrma_reflect.
 struct HrmaLds
 {
@@ -266,7 +274,7 @@ struct HrmaLds
 // The block stages the mirror configuration and its small, hot tables (the
 // binary-search targets of every photon) in LDS once, then loops over its
 // range of the middle list.
-__global__ __launch_bounds__ (kBlock)
+__global__ __launch_bounds__ (kBlock, MXG_REFLECT_WAVES)
 void k_hrma_reflect (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
                      const double *__restrict__ da, Soa s, ListIO io,
                      const uint32_t *__restrict__ rng_pos, unsigned long long seed,
@@ -386,7 +394,7 @@ void k_hrma_reflect (const mxg_config *__restrict__ cfg, const float *__restrict
 hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const float *fa,
                         const double *da, Soa s, ListIO io, HrmaScratch hs, uint32_t n_upper,
                         double source_distance, unsigned long long seed, uint32_t source_id,
-                        hipStream_t st)
+                        hipStream_t st, hipEvent_t *ev)
 {
    if (n_upper == 0) return hipSuccess;
    const mxg_hrma_cfg &h = hcfg->hrma;
@@ -414,14 +422,20 @@ hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const flo
    a.n_out = hs.n_mid;
    b.idx_in = hs.idx_mid;
    b.n_in = hs.n_mid;
+   hipError_t e = hipSuccess;
+   if (ev) e = hipEventRecord (ev[0], st);
+   if (e != hipSuccess) return e;
    hipLaunchKernelGGL (k_hrma_enter, dim3 (a.grid), dim3 (kBlock), 0, st, cfg, s, a, hs.rng_pos,
                        source_distance, seed, source_id);
-   hipError_t e = hipGetLastError ();
+   e = hipGetLastError ();
+   if ((e == hipSuccess) && ev) e = hipEventRecord (ev[1], st);
    if (e == hipSuccess) e = launch_compact (a, st);
+   if ((e == hipSuccess) && ev) e = hipEventRecord (ev[2], st);
    if (e != hipSuccess) return e;
    hipLaunchKernelGGL (k_hrma_reflect, dim3 (b.grid), dim3 (kBlock), lds_bytes, st, cfg, fa, da, s,
                        b, hs.rng_pos, seed, source_id, lay);
    e = hipGetLastError ();
+   if ((e == hipSuccess) && ev) e = hipEventRecord (ev[3], st);
    if (e == hipSuccess) e = launch_compact (b, st);
    return e;
 }

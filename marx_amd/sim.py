@@ -147,6 +147,14 @@ class Tracer:
         self.lib.mxg_last_stage_counts(c)
         return list(c)
 
+    def kernel_stats(self):
+        """(ms, photons_in) of the mirror's two kernels [aperture + paraboloid,
+        reflections] in the last timed trace_device call."""
+        m = (ctypes.c_float * 2)()
+        c = (ctypes.c_uint * 2)()
+        self.lib.mxg_last_kernel_stats(m, c)
+        return list(m), list(c)
+
     def create_photons(self, photons, num, exposure_time=None):
         """marx_create_photons: num photons of the source into photons."""
         n = ctypes.c_uint()

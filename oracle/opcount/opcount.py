@@ -38,7 +38,18 @@ def load_opcount():
     lib.orc_create_photons.restype = ctypes.c_int
     lib.orc_opcount_reset.restype = None
     lib.orc_opcount_get.argtypes = [ctypes.POINTER(ctypes.c_ulonglong)]
+    lib.orc_opcount_get_phase.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_ulonglong)]
+    lib.orc_opcount_marks.argtypes = [ctypes.c_int]
+    lib.orc_opcount_marks.restype = ctypes.c_ulonglong
     return lib
+
+
+def _per_photon(c, entering):
+    per = {k: c[i] / max(entering, 1) for i, k in enumerate(NAMES)}
+    per["flops"] = per["add"] + per["mul"] + per["div"] + per["sqrt"]
+    per["transcendental"] = sum(per[k] for k in NAMES[4:])
+    per["photons_in"] = entering
+    return per
 
 
 def count_stages(tracer, num, seed=1):
@@ -57,11 +68,21 @@ def count_stages(tracer, num, seed=1):
             raise RuntimeError(f"op-count oracle stage {stage} failed")
         c = (ctypes.c_ulonglong * 17)()
         lib.orc_opcount_get(c)
-        per = {k: c[i] / max(entering, 1) for i, k in enumerate(NAMES)}
-        per["flops"] = per["add"] + per["mul"] + per["div"] + per["sqrt"]
-        per["transcendental"] = sum(per[k] for k in NAMES[4:])
-        per["photons_in"] = entering
-        out[stage] = per
+        out[stage] = _per_photon(c, entering)
+        if stage == "mirror":
+            # the mirror's two kernels: up to the paraboloid hit (every photon
+            # entering the stage), and after it (photons that hit) including
+            # the energy-only optical-constant lookups
+            ph = [(ctypes.c_ulonglong * 17)() for _ in range(3)]
+            for k in range(3):
+                lib.orc_opcount_get_phase(k, ph[k])
+            hits = lib.orc_opcount_marks(1)
+            looked_up = lib.orc_opcount_marks(2)
+            out["mirror_enter"] = _per_photon(ph[0], entering)
+            a = _per_photon(ph[1], hits)
+            b = _per_photon(ph[2], looked_up)
+            out["mirror_reflect"] = {k: a[k] + b[k] for k in a}
+            out["mirror_reflect"]["photons_in"] = hits
 
     n = ctypes.c_uint()
     run("source", lambda: lib.orc_create_photons(ctypes.byref(ctx), ctypes.cast(ph.ptr, ctypes.c_void_p),
@@ -86,9 +107,9 @@ def main():
     with tempfile.TemporaryDirectory() as tmp:
         tracer = hetg_acis_s(tmp, photons=a.photons)
         res = count_stages(tracer, a.photons)
-    for st in ("source", "mirror", "grating", "detector"):
+    for st in ("source", "mirror", "mirror_enter", "mirror_reflect", "grating", "detector"):
         r = res[st]
-        print(f"{st:9s} photons_in={r['photons_in']:8d} flops/photon={r['flops']:8.1f} "
+        print(f"{st:14s} photons_in={r['photons_in']:8d} flops/photon={r['flops']:8.1f} "
               f"(add {r['add']:.1f} mul {r['mul']:.1f} div {r['div']:.1f} sqrt {r['sqrt']:.1f}) "
               f"transcendental/photon={r['transcendental']:.2f}")
     if a.json:

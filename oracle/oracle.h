@@ -32,6 +32,19 @@
 extern "C" {
 #endif
 
+/* Work-accounting marks, active only in the op-counting build (which
+ * force-includes opcount/opcount.hpp): ORC_PHASE selects the bucket that
+ * the following FP64 operations are counted in, ORC_MARK counts a photon
+ * entering a bucket.  Buckets of the mirror pass: 0 up to and including the
+ * paraboloid hit, 1 after it, 2 the energy-only optical-constant lookups. */
+#ifdef ORC_OPCOUNT_HPP
+# define ORC_PHASE(k) (Orc_Opcount_Phase = (k))
+# define ORC_MARK(k) (Orc_Opcount_Marks[k]++)
+#else
+# define ORC_PHASE(k) ((void) 0)
+# define ORC_MARK(k) ((void) 0)
+#endif
+
 typedef struct
 {
    const mxg_config *cfg;

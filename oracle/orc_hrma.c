@@ -174,6 +174,11 @@ static int reflect_from_conic (const orc_ctx *c, const mxg_shell_cfg *s, unsigne
    if (-1 == compute_conic_intersection (s->conic_a[k], s->conic_b[k], s->conic_c[k],
                                          x, *p, &normal, s->conic_xmin[k], s->conic_xmax[k]))
      return -2;
+   if (k == 0)
+     {
+        ORC_PHASE (1);
+        ORC_MARK (1);
+     }
 
    if (h->use_blur)
      blur_normal (&normal, s->blur[k], st);
@@ -244,6 +249,7 @@ int orc_mirror_reflect (const orc_ctx *c, Marx_Photon_Type *pt)
         double energy, beta, delta, correction_factor = 1.0;
         int status;
 
+        ORC_PHASE (0);
         (void) project_photon_to_hrma (h, at, source_distance, st);
         s = h->shell + at->mirror_shell;
 
@@ -252,6 +258,8 @@ int orc_mirror_reflect (const orc_ctx *c, Marx_Photon_Type *pt)
         at->p = orc_m3_vmul (s->fwd[0], at->p);
 
         energy = at->energy;
+        ORC_PHASE (2);
+        ORC_MARK (2);
         if (h->opt_n == 0)
           {
              beta = 0.0;
@@ -271,6 +279,7 @@ int orc_mirror_reflect (const orc_ctx *c, Marx_Photon_Type *pt)
                   correction_factor = sqrt (correction_factor);
                }
           }
+        ORC_PHASE (0);
 
         status = reflect_from_conic (c, s, 0, &at->x, &at->p, energy, beta, delta,
                                      correction_factor, st);
@@ -306,6 +315,7 @@ int orc_mirror_reflect (const orc_ctx *c, Marx_Photon_Type *pt)
           (void) intersects_struts (h, at, h->post_struts, 2);
      }
 
+   ORC_PHASE (0);
    free (streams);
    return 0;
 }
