@@ -168,9 +168,16 @@ int pf_close_parameter_file (Param_File_Type *pf)
    return 0;
 }
 
+/* Like pfile's pf_set_value (pfile/src/pfcmd.c:107-128), the parameter must
+ * exist in the file. */
 int pf_set_value (Param_File_Type *pf, char *name, char *value)
 {
    if ((pf == NULL) || (name == NULL) || (value == NULL)) return -1;
+   if (find_entry (pf, name) == NULL)
+     {
+        mxg_error ("Unable to find a parameter named %s.", name);
+        return -1;
+     }
    return add_entry (pf, name, 0, value);
 }
 
