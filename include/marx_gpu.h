@@ -206,8 +206,10 @@ const char *mxg_last_error (void);
 int mxg_init (void);
 
 /* Random-number seed of the per-photon counter streams (Philox4x32-10 keyed
- * by seed; counter = (photon tag, stage, block, source id)).  The
- * reference's RandomSeed parameter maps here (marx/src/marx.c:846-849). */
+ * by seed; counter = (photon tag, stage, block, source id)).  Until this is
+ * called, marx_mirror_init takes the seed from the RandomSeed parameter as
+ * the reference's driver does (marx/src/marx.c:846-849; -1 seeds from the
+ * clock); once called, RandomSeed is ignored. */
 void mxg_set_seed (unsigned long long seed);
 unsigned long long mxg_get_seed (void);
 

@@ -13,19 +13,22 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <time.h>
+
 #include "mxg_host.h"
 
 static int Mirror_Type = -1;
 static int Grating_Type = -1;
 static int Detector_Type = -1;
 static unsigned long long Seed = 1;
+static int Seed_Set = 0;             /* mxg_set_seed called: RandomSeed is ignored */
 static int Lazy_Sync = 0;
 
 /* the list whose photons are currently resident in HBM */
 static const Marx_Photon_Type *Resident = NULL;
 static unsigned int Resident_Tag_Start = 0;
 
-void mxg_set_seed (unsigned long long seed) { Seed = seed; }
+void mxg_set_seed (unsigned long long seed) { Seed = seed; Seed_Set = 1; }
 unsigned long long mxg_get_seed (void) { return Seed; }
 void mxg_set_lazy_sync (int on) { Lazy_Sync = on; }
 
@@ -159,6 +162,14 @@ int marx_mirror_init (Param_File_Type *pf)
         mxg_error ("Error initializing MirrorType %s.", buf);
         Mirror_Type = -1;
         return -1;
+     }
+   /* The seed of the driver's RandomSeed parameter (marx/src/marx.c:846-849:
+    * -1 seeds from the clock), unless the caller set one with mxg_set_seed. */
+   if (Seed_Set == 0)
+     {
+        int rs;
+        if (0 == pf_get_integer (pf, "RandomSeed", &rs))
+          Seed = (rs == -1) ? (unsigned long long) time (NULL) : (unsigned long long) (unsigned int) rs;
      }
    Mirror_Type = MARX_MIRROR_HRMA;
    return Mirror_Type;
