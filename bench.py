@@ -1,14 +1,16 @@
-"""Benchmark: photons/s traced (source + dither + HRMA + HETG + ACIS-S) on one
-MI355X, with the CPU oracle timed on the host cores beside it.
+"""Benchmark: photons/s traced (source + dither + HRMA + grating + detector)
+on one MI355X, with the CPU oracle timed on the host cores beside it.
 
-One step = one device-resident chunk of the HETG + ACIS-S workload: 1e7
-photons of an on-axis point source with a photon-index-2 power law over
-0.4-8 keV, INTERNAL dither (16 arcsec, 1000 / 707 s), 50 ks of arrivals,
-created on the GPU and traced through mirror, grating and detector without
-leaving HBM.  `value` is photons created and traced per second of wall
+One step = one device-resident chunk of a workload, created on the GPU and
+traced through mirror, grating and detector without leaving HBM.  The
+default workload (--config hetg-acis-s) is 1e7 photons of an on-axis point
+source with a photon-index-2 power law over 0.4-8 keV, INTERNAL dither
+(16 arcsec, 1000 / 707 s), HETG and ACIS-S, 50 ks of arrivals;
+--config letg-hrc-s is 5e7 photons of a 2 arcmin disk, flat 0.07-10 keV,
+LETG and HRC-S.  `value` is photons created and traced per second of wall
 time over the timed steps.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--photons N]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME] [--photons N]
 """
 import argparse
 import json
@@ -21,15 +23,24 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-# Algorithmic FP64 work per photon entering each stage, and each of the
-# mirror's two kernels (adds/subs, muls, divs and sqrts of the restated
-# reference algorithm, counted by the op-counting oracle build on this
-# workload: `python -m oracle.opcount.opcount --photons 100000`).
-# Transcendental calls per photon are listed beside.
-FLOPS_PER_PHOTON = {"source": 146.0, "mirror": 438.7, "grating": 249.3, "detector": 200.1,
-                    "k_hrma_enter": 130.3, "k_hrma_reflect": 598.6}
-TRANSCENDENTALS_PER_PHOTON = {"source": 11.0, "mirror": 10.84, "grating": 8.81, "detector": 0.69,
-                              "k_hrma_enter": 4.00, "k_hrma_reflect": 13.62}
+# Algorithmic FP64 work per photon entering each kernel (adds/subs, muls,
+# divs and sqrts of the restated reference algorithm, counted by the
+# op-counting oracle build on each workload:
+# `python -m oracle.opcount.opcount --photons 100000 --workload <name>`).
+# "grating" and "detector" are k_grating and k_detect.  Transcendental calls
+# per photon are listed beside.
+FLOPS_PER_PHOTON = {
+    "hetg-acis-s": {"source": 146.0, "k_hrma_enter": 130.3, "k_hrma_reflect": 598.6,
+                    "grating": 249.3, "detector": 200.1},
+    "letg-hrc-s": {"source": 239.0, "k_hrma_enter": 130.3, "k_hrma_reflect": 442.0,
+                   "grating": 1422.2, "detector": 278.0},
+}
+TRANSCENDENTALS_PER_PHOTON = {
+    "hetg-acis-s": {"source": 11.0, "k_hrma_enter": 4.00, "k_hrma_reflect": 13.62,
+                    "grating": 8.81, "detector": 0.69},
+    "letg-hrc-s": {"source": 15.0, "k_hrma_enter": 4.00, "k_hrma_reflect": 10.09,
+                   "grating": 13.69, "detector": 0.64},
+}
 # Algorithmic HBM bytes per photon entering each kernel: the structure-of-
 # arrays columns it must read and write, plus the active-list index in and
 # the survivor index out.
@@ -42,12 +53,21 @@ BYTES_PER_PHOTON = {
     "grating": (4 + 4 + 4 + 1 + 8 + 24 + 24 + 1) + (24 + 24 + 4 + 1 + 4),
     "detector": (4 + 4 + 4 + 8 + 8 + 24 + 24) + (24 + 24 + 4 + 1 + 4 + 4 + 4),
 }
-# HBM bytes per photon entering k_hrma_reflect, measured with rocprofv3
-# counters (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per dispatch, the factors
-# that make a 1 GiB device copy read back 1 GiB each way); the command and
-# the figures are in profiles/pmc_hetg_acis_s_1e7.md.  None when not measured
-# for the current kernels.
-TRAFFIC_BYTES_PER_PHOTON = {"k_hrma_reflect": 317.3}
+# HBM bytes per photon entering a kernel, measured with rocprofv3 counters
+# (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per dispatch, the factors that make a
+# 1 GiB device copy read back 1 GiB each way); commands and figures in
+# profiles/pmc_hetg_acis_s_1e7.md.  Absent when not measured for the current
+# kernels.
+TRAFFIC_BYTES_PER_PHOTON = {("hetg-acis-s", "k_hrma_reflect"): 317.3}
+DESCRIPTION = {
+    "hetg-acis-s": "HETG + ACIS-S: on-axis point source, power law index 2 over 0.4-8 keV (FILE table), "
+                   "HEG/MEG orders from the CALDB efficiency tables, INTERNAL dither 16 arcsec "
+                   "1000/707 s, 50 ks",
+    "letg-hrc-s": "LETG + HRC-S: uniform disk of 2 arcmin radius, flat 0.07-10 keV, LETG orders from the "
+                  "CALDB efficiency tables with fine and coarse support gratings, HRC-S with HESF, MCP "
+                  "blur and pixel map, INTERNAL dither, 50 ks",
+}
+DEFAULT_PHOTONS = {"hetg-acis-s": 10_000_000, "letg-hrc-s": 50_000_000}
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
 STAGES = ["source", "mirror", "grating", "detector"]
@@ -103,7 +123,8 @@ def measure():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--photons", type=int, default=10_000_000)
+    ap.add_argument("--config", default="hetg-acis-s", choices=sorted(DEFAULT_PHOTONS))
+    ap.add_argument("--photons", type=int, default=0, help="photons per step (default: the config's)")
     ap.add_argument("--exposure", type=float, default=50_000.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -120,15 +141,16 @@ def measure():
 
     import torch
     import marx_amd
-    from marx_amd.workloads import hetg_acis_s
+    from marx_amd.workloads import WORKLOADS
 
     lib = marx_amd.load()
     if lib.mxg_init() != 0:
         raise RuntimeError("GPU init failed: " + marx_amd._lib.last_error())
 
     tmp = tempfile.mkdtemp(prefix="marx_bench_")
-    tracer = hetg_acis_s(tmp, photons=args.photons, exposure=args.exposure)
-    N, T = args.photons, args.exposure
+    cfg = args.config
+    N, T = (args.photons or DEFAULT_PHOTONS[cfg]), args.exposure
+    tracer = WORKLOADS[cfg](tmp, photons=N, exposure=T)
 
     def step(k, timing=False):
         return tracer.trace_device(N, tag_start=(k * N) % (1 << 31), start_time=k * T, timing=timing)
@@ -163,27 +185,26 @@ def measure():
     K = args.steps
     value = K * N / elapsed
 
-    # the dominant kernel, timed alone with HIP events on the library's stream
-    dom = "k_hrma_reflect"
-    t_launch = kern_ms[dom] / K * 1e-3
-    units = kern_in[dom] / K
-    flops = FLOPS_PER_PHOTON[dom] * units
-    nbytes = BYTES_PER_PHOTON[dom] * units
-    achieved = flops / t_launch / 1e12
-    achieved_bw = nbytes / t_launch / 1e9
-    traffic = (TRAFFIC_BYTES_PER_PHOTON[dom] * units
-               if TRAFFIC_BYTES_PER_PHOTON.get(dom) is not None else None)
-
+    # per-kernel figures; the mirror's kernels are timed alone with HIP events
+    # on the library's stream, grating and detector are their stage (one
+    # kernel plus the list compaction)
     kernels = {}
     for name, ms_k, n_k in (("k_hrma_enter", kern_ms["k_hrma_enter"], kern_in["k_hrma_enter"]),
-                         ("k_hrma_reflect", kern_ms["k_hrma_reflect"], kern_in["k_hrma_reflect"]),
-                         ("grating stage", stage_ms["grating"], stage_in["grating"]),
-                         ("detector stage", stage_ms["detector"], stage_in["detector"])):
-        key = name.split()[0]
+                            ("k_hrma_reflect", kern_ms["k_hrma_reflect"], kern_in["k_hrma_reflect"]),
+                            ("grating", stage_ms["grating"], stage_in["grating"]),
+                            ("detector", stage_ms["detector"], stage_in["detector"])):
         t = ms_k / K * 1e-3
         kernels[name] = {"ms": ms_k / K, "photons_in": n_k / K,
-                      "tflops": FLOPS_PER_PHOTON[key] * n_k / K / t / 1e12 if t > 0 else None,
-                      "gbs": BYTES_PER_PHOTON[key] * n_k / K / t / 1e9 if t > 0 else None}
+                         "tflops": FLOPS_PER_PHOTON[cfg][name] * n_k / K / t / 1e12 if t > 0 else None,
+                         "gbs": BYTES_PER_PHOTON[name] * n_k / K / t / 1e9 if t > 0 else None}
+
+    dom = max(kernels, key=lambda k: kernels[k]["ms"])
+    t_launch = kernels[dom]["ms"] * 1e-3
+    units = kernels[dom]["photons_in"]
+    achieved = kernels[dom]["tflops"]
+    achieved_bw = kernels[dom]["gbs"]
+    tb = TRAFFIC_BYTES_PER_PHOTON.get((cfg, dom))
+    traffic = tb * units if tb is not None else None
 
     out = {
         "metric": "photons/s traced (HRMA+grating+detector) 1 GPU vs host CPU; gradient evals/s",
@@ -198,17 +219,17 @@ def measure():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "HETG + ACIS-S: on-axis point source, power law index 2 over 0.4-8 keV "
-                               "(FILE table), HEG/MEG orders from the CALDB efficiency tables, INTERNAL "
-                               "dither 16 arcsec 1000/707 s, 50 ks",
+        "config": {"workload": DESCRIPTION[cfg], "name": cfg,
                    "photons_per_step": N, "exposure_s": T,
                    "detected_per_step": detected / K},
         "roofline": {"bound": "mfma", "pipe": "fp64 vector ALU (no matrix work on this path; 78.6 TFLOP/s "
                                               "is MI355X's FP64 peak)",
-                     "kernel": dom, "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "kernel": "k_grating (stage)" if dom == "grating" else
+                               "k_detect (stage)" if dom == "detector" else dom,
+                     "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
-                     "flops_per_photon": FLOPS_PER_PHOTON[dom],
-                     "transcendentals_per_photon": TRANSCENDENTALS_PER_PHOTON[dom],
+                     "flops_per_photon": FLOPS_PER_PHOTON[cfg][dom],
+                     "transcendentals_per_photon": TRANSCENDENTALS_PER_PHOTON[cfg][dom],
                      "photons_per_launch": units, "launch_ms": t_launch * 1e3,
                      "hbm": {"achieved": achieved_bw, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                              "frac": achieved_bw / PEAK_HBM_GBS,

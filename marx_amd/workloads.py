@@ -23,6 +23,23 @@ def hetg_acis_s(tmpdir, photons=10_000_000, exposure=50_000.0, seed=1):
     return t
 
 
+def letg_hrc_s(tmpdir=None, photons=50_000_000, exposure=50_000.0, seed=1):
+    """LETG + HRC-S: uniform disk of 2 arcmin radius on axis, flat spectrum
+    over 0.07-10 keV, LETG orders from the efficiency tables with the fine
+    and coarse support gratings, HRC-S with the HESF, MCP blur and pixel map,
+    INTERNAL dither with the parameter file's amplitudes and periods.  The
+    flux is set so that `photons` photons arrive in `exposure` seconds."""
+    t = Tracer(seed=seed, GratingType="LETG", DetectorType="HRC-S", SourceType="DISK",
+               SpectrumType="FLAT", MinEnergy=0.07, MaxEnergy=10.0, DitherModel="INTERNAL",
+               **{"S-DiskTheta0": 0.0, "S-DiskTheta1": 120.0}, **ON_AXIS)
+    flux = photons / (exposure * t.geometric_area)
+    t.init_source(SourceFlux=flux)
+    return t
+
+
+WORKLOADS = {"hetg-acis-s": hetg_acis_s, "letg-hrc-s": letg_hrc_s}
+
+
 def plain_acis_s(photons=1_000_000, tmax=10_000.0, seed=1):
     """Plain: on-axis point source at 1.49 keV, arrival times uniform over
     10 ks, no dither, no grating, ACIS-S."""

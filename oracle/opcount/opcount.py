@@ -7,7 +7,7 @@ transcendental calls per photon entering the stage.  These are the
 algorithmic work figures behind the benchmark's roofline; they count the
 restated reference algorithm and nothing of the GPU code.
 
-    python -m oracle.opcount.opcount [--photons 100000]
+    python -m oracle.opcount.opcount [--photons 100000] [--workload hetg-acis-s|letg-hrc-s]
 """
 import argparse
 import ctypes
@@ -102,10 +102,11 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--photons", type=int, default=100000)
     ap.add_argument("--json", default=None)
+    ap.add_argument("--workload", default="hetg-acis-s")
     a = ap.parse_args()
-    from marx_amd.workloads import hetg_acis_s
+    from marx_amd.workloads import WORKLOADS
     with tempfile.TemporaryDirectory() as tmp:
-        tracer = hetg_acis_s(tmp, photons=a.photons)
+        tracer = WORKLOADS[a.workload](tmp, photons=a.photons)
         res = count_stages(tracer, a.photons)
     for st in ("source", "mirror", "mirror_enter", "mirror_reflect", "grating", "detector"):
         r = res[st]

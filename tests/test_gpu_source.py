@@ -70,12 +70,20 @@ def test_source_exposure_cut_and_chunks(gpu):
 
 
 @pytest.mark.gpu
-def test_full_chain_aos_and_device(gpu, tmp_path):
-    # HETG + ACIS-S with a power-law FILE spectrum and INTERNAL dither
-    f = write_power_law(str(tmp_path / "pl.dat"))
-    t = Tracer(seed=33, source=True, GratingType="HETG", DetectorType="ACIS-S",
-               SpectrumType="FILE", SpectrumFile=f, DitherModel="INTERNAL",
-               DitherAmp_RA=16, DitherAmp_Dec=16, DitherPeriod_RA=1000, DitherPeriod_Dec=707, **ON_AXIS)
+@pytest.mark.parametrize("workload", ["hetg-acis-s", "letg-hrc-s"])
+def test_full_chain_aos_and_device(gpu, tmp_path, workload):
+    if workload == "hetg-acis-s":
+        # HETG + ACIS-S with a power-law FILE spectrum and INTERNAL dither
+        f = write_power_law(str(tmp_path / "pl.dat"))
+        t = Tracer(seed=33, source=True, GratingType="HETG", DetectorType="ACIS-S",
+                   SpectrumType="FILE", SpectrumFile=f, DitherModel="INTERNAL",
+                   DitherAmp_RA=16, DitherAmp_Dec=16, DitherPeriod_RA=1000, DitherPeriod_Dec=707,
+                   **ON_AXIS)
+    else:
+        # LETG + HRC-S, 2 arcmin uniform disk, flat 0.07-10 keV, INTERNAL dither
+        t = Tracer(seed=34, source=True, GratingType="LETG", DetectorType="HRC-S", SourceType="DISK",
+                   SpectrumType="FLAT", MinEnergy=0.07, MaxEnergy=10.0, DitherModel="INTERNAL",
+                   **{"S-DiskTheta0": 0.0, "S-DiskTheta1": 120.0}, **ON_AXIS)
     num = 200000
     pg, po = Photons(num), Photons(num)
     t.create_photons(pg, num)
@@ -87,13 +95,13 @@ def test_full_chain_aos_and_device(gpu, tmp_path):
     o.reflect(po)
     o.diffract(po)
     o.detect(po)
-    compare(pg.active, po.active, max_flips=2, label="chain AoS")
+    compare(pg.active, po.active, max_flips=2, label=f"chain AoS {workload}")
 
     # the same chunk traced without leaving HBM
     ncol, ndet, ms = t.trace_device(num, tag_start=0, start_time=0.0, timing=True)
     assert ncol == num
     pd = Photons(num)
     t.download(pd)
-    compare(pd.active, po.active, max_flips=2, label="chain device")
+    compare(pd.active, po.active, max_flips=2, label=f"chain device {workload}")
     assert ndet == int(((po.active["flags"] & 0xFF) == 0).sum())
     assert all(m >= 0 for m in ms)
