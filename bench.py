@@ -14,6 +14,8 @@ time over the timed steps.
 """
 import argparse
 import json
+
+import numpy as np
 import os
 import sys
 import tempfile
@@ -34,12 +36,16 @@ FLOPS_PER_PHOTON = {
                     "grating": 249.3, "detector": 200.1},
     "letg-hrc-s": {"source": 239.0, "k_hrma_enter": 130.3, "k_hrma_reflect": 442.0,
                    "grating": 1422.2, "detector": 278.0},
+    "batched-acis-i": {"source": 11.0, "k_hrma_enter": 130.2, "k_hrma_reflect": 576.9,
+                       "grating": 0.0, "detector": 172.4},
 }
 TRANSCENDENTALS_PER_PHOTON = {
     "hetg-acis-s": {"source": 11.0, "k_hrma_enter": 4.00, "k_hrma_reflect": 13.62,
                     "grating": 8.81, "detector": 0.69},
     "letg-hrc-s": {"source": 15.0, "k_hrma_enter": 4.00, "k_hrma_reflect": 10.09,
                    "grating": 13.69, "detector": 0.64},
+    "batched-acis-i": {"source": 0.0, "k_hrma_enter": 4.00, "k_hrma_reflect": 13.11,
+                       "grating": 0.0, "detector": 0.17},
 }
 # Algorithmic HBM bytes per photon entering each kernel: the structure-of-
 # arrays columns it must read and write, plus the active-list index in and
@@ -67,7 +73,12 @@ DESCRIPTION = {
                   "CALDB efficiency tables with fine and coarse support gratings, HRC-S with HESF, MCP "
                   "blur and pixel map, INTERNAL dither, 50 ks",
 }
-DEFAULT_PHOTONS = {"hetg-acis-s": 10_000_000, "letg-hrc-s": 50_000_000}
+DESCRIPTION["batched-acis-i"] = (
+    "Batched: 4096 point sources at off-axis angles uniform over 0-15 arcmin with random azimuths, "
+    "photon counts log-uniform over 1e2-1e5 per source, power law index 2 over 0.3-10 keV, HRMA + "
+    "ACIS-I, arrival times uniform over the exposure, all sources in one HIP-graph-launched pass "
+    "with the autotuned schedule")
+DEFAULT_PHOTONS = {"hetg-acis-s": 10_000_000, "letg-hrc-s": 50_000_000, "batched-acis-i": 0}
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
 STAGES = ["source", "mirror", "grating", "detector"]
@@ -98,6 +109,131 @@ def cpu_baseline(tracer, target_seconds=20.0):
     return {"value": sample / dt, "unit": "photons/s", "cores": threads, "kind": "port",
             "sample": f"{sample} photons of the same workload (FP64 CPU oracle, chunks of {chunk}, "
                       f"{threads} OpenMP threads), {dt:.1f} s"}
+
+
+def kernel_roofline(cfg, kernels):
+    """The roofline object for the slowest of the timed kernels."""
+    dom = max(kernels, key=lambda k: kernels[k]["ms"])
+    units = kernels[dom]["photons_in"]
+    tb = TRAFFIC_BYTES_PER_PHOTON.get((cfg, dom))
+    return {"bound": "mfma", "pipe": "fp64 vector ALU (no matrix work on this path; 78.6 TFLOP/s "
+                                     "is MI355X's FP64 peak)",
+            "kernel": "k_grating (stage)" if dom == "grating" else
+                      "k_detect (stage)" if dom == "detector" else dom,
+            "achieved": kernels[dom]["tflops"], "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": kernels[dom]["tflops"] / PEAK_FP64_TFLOPS,
+            "traffic": tb * units if tb is not None else None,
+            "flops_per_photon": FLOPS_PER_PHOTON[cfg][dom],
+            "transcendentals_per_photon": TRANSCENDENTALS_PER_PHOTON[cfg][dom],
+            "photons_per_launch": units, "launch_ms": kernels[dom]["ms"],
+            "hbm": {"achieved": kernels[dom]["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": kernels[dom]["gbs"] / PEAK_HBM_GBS,
+                    "bytes_per_photon": BYTES_PER_PHOTON[dom]}}
+
+
+def measure_batched(args, tmp, torch):
+    """Batched point sources: one step is one HIP-graph-launched pass over
+    every source (mxg_trace_batch) with the schedule the autotuner picked;
+    beside it the plain per-source GPU loop (one uncaptured pass per source)
+    over the same photons, and the CPU oracle per source on the host cores."""
+    from marx_amd.workloads import batched_acis_i
+    cfg = "batched-acis-i"
+    tracer, dirs, counts = batched_acis_i(tmp)
+    N, T, K = int(counts.sum()), args.exposure, args.steps
+    progress = lambda msg: print(f"bench batched: {msg}", file=sys.stderr, flush=True)
+    progress(f"{len(counts)} sources, {N} photons; autotuning")
+    sched, tuned_ms = tracer.autotune_batch(dirs, counts, exposure=T, repeats=2)
+    progress(f"schedule {sched}, graph pass {tuned_ms:.2f} ms")
+
+    def step(k):
+        return tracer.trace_batch(dirs, counts, tag_start=(k * N) % (1 << 31), exposure=T, use_graph=True)
+
+    for k in range(args.warmup):
+        step(k)
+    torch.cuda.synchronize()
+    detected = 0
+    t0 = time.perf_counter()
+    for k in range(K):
+        detected += step(args.warmup + k)[0]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = K * N / elapsed
+    progress(f"{K} graph-launched steps: {elapsed / K * 1e3:.2f} ms per step")
+
+    # per-kernel device times from one uncaptured pass of the same batch
+    tracer.trace_batch(dirs, counts, exposure=T, use_graph=False)
+    sms = tracer.stage_ms()
+    kms, kin = tracer.kernel_stats()
+    c = tracer.stage_counts()
+    kernels = {}
+    for name, ms_k, n_k in (("k_hrma_enter", kms[0], kin[0]), ("k_hrma_reflect", kms[1], kin[1]),
+                            ("detector", sms[3], c[1])):
+        t = ms_k * 1e-3
+        kernels[name] = {"ms": ms_k, "photons_in": n_k,
+                         "tflops": FLOPS_PER_PHOTON[cfg][name] * n_k / t / 1e12 if t > 0 else None,
+                         "gbs": BYTES_PER_PHOTON[name] * n_k / t / 1e9 if t > 0 else None}
+
+    # the plain per-source GPU loop over the same photons
+    offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(len(counts)):
+        tracer.trace_batch(dirs[k:k + 1], counts[k:k + 1], tag_start=int(offs[k]), exposure=T, use_graph=False)
+    torch.cuda.synchronize()
+    loop_s = time.perf_counter() - t0
+    progress(f"per-source loop: {loop_s:.2f} s")
+
+    out = {
+        "metric": "photons/s traced (HRMA+grating+detector) 1 GPU vs host CPU; gradient evals/s",
+        "value": value, "unit": "photons/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": elapsed / K * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": DESCRIPTION[cfg], "name": cfg, "sources": len(counts),
+                   "photons_per_step": N, "exposure_s": T, "detected_per_step": detected / K},
+        "schedule": sched, "autotuned_pass_ms": tuned_ms,
+        "per_source_loop": {"value": N / loop_s, "unit": "photons/s", "seconds": loop_s,
+                            "sources": len(counts)},
+        "batched_over_loop": value / (N / loop_s),
+        "roofline": kernel_roofline(cfg, kernels),
+        "stages_ms": {"source": sms[0], "mirror": sms[1], "detector": sms[3]},
+        "kernels": kernels,
+    }
+    if not args.no_cpu:
+        progress("CPU baseline")
+        out["cpu_baseline"] = cpu_baseline_batch(tracer, dirs, counts, T)
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    return out
+
+
+def cpu_baseline_batch(tracer, dirs, counts, exposure, target_seconds=20.0):
+    """The FP64 oracle source by source (one OpenMP task per source) over the
+    first sources of the batch, as many as take about target_seconds."""
+    from oracle.oracle import Oracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    o = Oracle(tracer)
+
+    def timed(m):
+        t0 = time.perf_counter()
+        if o.trace_batch_parallel(dirs[:m], counts[:m], 0, exposure, threads) < 0:
+            raise RuntimeError("oracle failed")
+        return time.perf_counter() - t0
+
+    cum = np.cumsum(counts.astype(np.int64))
+
+    def sources_for(photons):
+        return int(min(len(counts), max(threads, np.searchsorted(cum, photons) + 1)))
+
+    m = sources_for(cum[-1] * 0.001)
+    for target in (0.5, 2.0):
+        dt = timed(m)
+        rate = cum[m - 1] / max(dt, 1e-3)
+        m = sources_for(rate * target)
+    m = sources_for(rate * target_seconds)
+    dt = timed(m)
+    n = int(counts[:m].sum())
+    return {"value": n / dt, "unit": "photons/s", "cores": threads, "kind": "port",
+            "sample": f"the first {m} sources of the batch, {n} photons (FP64 CPU oracle, one OpenMP "
+                      f"task per source, {threads} threads), {dt:.1f} s"}
 
 
 def run_quietly(fn):
@@ -148,6 +284,8 @@ def measure():
         raise RuntimeError("GPU init failed: " + marx_amd._lib.last_error())
 
     tmp = tempfile.mkdtemp(prefix="marx_bench_")
+    if args.config == "batched-acis-i":
+        return measure_batched(args, tmp, torch)
     cfg = args.config
     N, T = (args.photons or DEFAULT_PHOTONS[cfg]), args.exposure
     tracer = WORKLOADS[cfg](tmp, photons=N, exposure=T)

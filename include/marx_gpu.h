@@ -240,9 +240,61 @@ int mxg_trace_device (unsigned int num, unsigned int tag_start,
                       unsigned int *num_collected, unsigned int *num_detected,
                       float *stage_ms);
 
-/* Photons alive after each stage of the last mxg_trace_device call:
- * created, after the mirror, after the grating, after the detector. */
+/* Photons alive after each stage of the last mxg_trace_device or
+ * mxg_trace_batch call: created, after the mirror, after the grating, after
+ * the detector. */
 int mxg_last_stage_counts (unsigned int counts[4]);
+
+/* Device time in ms of the stages of the last timed mxg_trace_device or
+ * uncaptured mxg_trace_batch pass: source (with dither), mirror, grating,
+ * detector (0 for a stage not run or a graph-launched pass). */
+int mxg_last_stage_ms (float ms[4]);
+
+/* ---- batched point sources (no counterpart in the reference, which runs
+ * one source per process) ---- */
+
+/* Traces num_sources point sources in one device pass through the
+ * initialised mirror, grating and detector.  Source k emits counts[k]
+ * photons travelling along the unit vector dirs[3k..3k+2] (MARX frame; an
+ * on-axis source is (-1, 0, 0)), with energies from the spectrum of the
+ * source set up by marx_create_source (FLAT or FILE), arrival times
+ * uniform over [0, exposure) after start_time, and the configured dither.
+ * Photon i of the batch (sources in order) gets tag tag_start + i, and its
+ * random streams depend on the seed and its tag alone: tracing source k by
+ * itself with tag_start + (counts[0] + ... + counts[k-1]) gives the same
+ * photons.  The photons stay resident (mxg_download_photons, mxg_get_soa);
+ * a photon's source follows from its tag.  With use_graph != 0 the launch
+ * sequence is captured into a HIP graph on first use and replayed while the
+ * number of sources and photons, the configuration, seed, exposure,
+ * start_time and schedule stay the same (counts, directions and tag_start
+ * may change).  *num_detected receives the photons that survive every
+ * stage; ms, when non-NULL, the device time of the whole pass.  Returns 0,
+ * or -1 with mxg_last_error() set. */
+int mxg_trace_batch (unsigned int num_sources, const double *dirs, const unsigned int *counts,
+                     unsigned int tag_start, double start_time, double exposure, int use_graph,
+                     unsigned int *num_detected, float *ms);
+
+/* Launch schedule of the stage kernels: blocks per CU of the source,
+ * paraboloid, reflection, grating and detector kernels (1..32), and the
+ * minimum waves per SIMD the reflection kernel is compiled for (2, 3, 4). */
+typedef struct
+{
+   unsigned int blocks_per_cu[5];
+   unsigned int reflect_waves;
+}
+mxg_schedule;
+int mxg_set_schedule (const mxg_schedule *s);
+int mxg_get_schedule (mxg_schedule *s);
+
+/* Autotuner: times the batch of mxg_trace_batch (same arguments) under
+ * candidate schedules -- blocks per CU in {2, 4, 8, 16} for every stage
+ * kernel, then 2, 3 and 4 waves for the reflection kernel -- keeps for each
+ * kernel the fastest (HIP-event time, best of `repeats` runs), installs the
+ * result and returns it in *best with the graph-launched pass time of the
+ * batch under it in *best_ms. */
+int mxg_autotune_batch (unsigned int num_sources, const double *dirs, const unsigned int *counts,
+                        unsigned int tag_start, double start_time, double exposure,
+                        unsigned int repeats, mxg_schedule *best, float *best_ms);
 
 /* The two mirror kernels of the last mxg_trace_device call, each timed
  * alone with HIP events on the library's stream: [0] aperture + paraboloid

@@ -75,6 +75,19 @@ class SoaView(ctypes.Structure):
     ]
 
 
+class Schedule(ctypes.Structure):
+    """mxg_schedule: blocks per CU of the source, paraboloid, reflection,
+    grating and detector kernels, and the reflection kernel's waves per SIMD."""
+    _fields_ = [("blocks_per_cu", ctypes.c_uint * 5), ("reflect_waves", ctypes.c_uint)]
+
+    KERNELS = ("source", "enter", "reflect", "grating", "detect")
+
+    def as_dict(self):
+        d = {k: int(v) for k, v in zip(self.KERNELS, self.blocks_per_cu)}
+        d["reflect_waves"] = int(self.reflect_waves)
+        return d
+
+
 PPhotonList = ctypes.POINTER(PhotonList)
 
 # (name, restype, argtypes) of every exported C-ABI symbol of include/marx_gpu.h
@@ -113,7 +126,17 @@ _SIGNATURES = [
                                         ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_uint),
                                         ctypes.POINTER(ctypes.c_float)]),
     ("mxg_download_photons", ctypes.c_int, [PPhotonList]),
+    ("mxg_trace_batch", ctypes.c_int, [ctypes.c_uint, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint),
+                                       ctypes.c_uint, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_float)]),
+    ("mxg_set_schedule", ctypes.c_int, [ctypes.POINTER(Schedule)]),
+    ("mxg_get_schedule", ctypes.c_int, [ctypes.POINTER(Schedule)]),
+    ("mxg_autotune_batch", ctypes.c_int, [ctypes.c_uint, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_uint), ctypes.c_uint, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_uint, ctypes.POINTER(Schedule),
+                                          ctypes.POINTER(ctypes.c_float)]),
     ("mxg_last_stage_counts", ctypes.c_int, [ctypes.POINTER(ctypes.c_uint)]),
+    ("mxg_last_stage_ms", ctypes.c_int, [ctypes.POINTER(ctypes.c_float)]),
     ("mxg_last_kernel_stats", ctypes.c_int, [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint)]),
     ("mxg_device_info", ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint, ctypes.POINTER(ctypes.c_int)]),
     ("mxg_get_config", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),

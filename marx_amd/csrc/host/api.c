@@ -9,6 +9,9 @@
  * mxg_set_lazy_sync(1) the state stays resident between the mirror, grating
  * and detector calls of one chunk and only marx_detect copies it back.
  */
+#ifndef _GNU_SOURCE
+# define _GNU_SOURCE   /* qsort_r */
+#endif
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -294,12 +297,13 @@ int marx_detect (Marx_Photon_Type *pt, int verbose)
 
 /* ---- source (source.c:268-384) ------------------------------------------------- */
 
-static const double *Sort_Keys;
-
-static int energy_cmp (const void *a, const void *b)
+/* the energies travel with the sort (qsort_r): no state shared between
+ * concurrent sorts */
+static int energy_cmp (const void *a, const void *b, void *energies)
 {
+   const double *en = (const double *) energies;
    unsigned int ia = *(const unsigned int *) a, ib = *(const unsigned int *) b;
-   double x = Sort_Keys[ia], y = Sort_Keys[ib];
+   double x = en[ia], y = en[ib];
    if (x > y) return 1;
    if (x < y) return -1;
    return (ia > ib) - (ia < ib);
@@ -317,8 +321,7 @@ static int sort_by_energy (Marx_Photon_Type *pt)
         e[i] = pt->attributes[i].energy;
         pt->sorted_index[i] = i;
      }
-   Sort_Keys = e;
-   qsort (pt->sorted_index, n, sizeof (unsigned int), energy_cmp);
+   qsort_r (pt->sorted_index, n, sizeof (unsigned int), energy_cmp, e);
    for (i = 0; i < n; i++)
      pt->sorted_energies[i] = e[pt->sorted_index[i]];
    pt->num_sorted = n;
@@ -383,6 +386,9 @@ static unsigned long Last_History = 0;
 static unsigned int Stage_Counts[4];
 static float Kernel_Ms[2];
 static unsigned int Kernel_In[2];
+/* stage times (ms) of the last pass by timing slot; zero for a
+ * graph-launched pass */
+static float Last_Slot_Ms[8];
 
 int mxg_trace_device (unsigned int num, unsigned int tag_start, double start_time,
                       double exposure_time, unsigned int *num_collected,
@@ -426,6 +432,7 @@ int mxg_trace_device (unsigned int num, unsigned int tag_start, double start_tim
    Kernel_Ms[1] = ms[MXGD_SLOT_HRMA_REFLECT];
    Kernel_In[0] = counts[MXGD_SLOT_HRMA_ENTER];
    Kernel_In[1] = counts[MXGD_SLOT_HRMA_REFLECT];
+   memcpy (Last_Slot_Ms, ms, sizeof (Last_Slot_Ms));
    Resident = NULL;
    Last_Collected = n;
    Last_Tag_Start = tag_start;
@@ -441,6 +448,278 @@ int mxg_trace_device (unsigned int num, unsigned int tag_start, double start_tim
         stage_ms[4] = (Detector_Type > 0) ? ms[3] : 0.0f;
         stage_ms[5] = stage_ms[0] + stage_ms[2] + stage_ms[3] + stage_ms[4];
      }
+   return 0;
+}
+
+/* ---- batched point sources -------------------------------------------------- */
+
+typedef struct
+{
+   unsigned int nsrc, n, generation, grating, detector;
+   unsigned long long seed;
+   double exposure, start_time, distance;
+   mxgd_schedule sched;
+}
+Batch_Key;
+
+static mxgd_graph *Batch_Graph = NULL;
+static Batch_Key Batch_Graph_Key;
+
+
+/* one pass over the uploaded batch (n photons): source, mirror, grating,
+ * detector, in timing slots 0-3 (during graph capture the stages record no
+ * events but still copy their survivor counts) */
+static int batch_pass (unsigned int n, double start_time, double exposure, int with_grating)
+{
+   if ((-1 == mxgd_run_batch_source (n, exposure, start_time, Seed, 0))
+       || (-1 == mxgd_run_mirror (Mxg_Cfg.src.distance, Seed, 1, NULL))
+       || (with_grating && (-1 == mxgd_run_grating (Seed, 2, NULL)))
+       || ((Detector_Type > 0) && (-1 == mxgd_run_detect (start_time, Seed, 3, NULL))))
+     return -1;
+   return 0;
+}
+
+static int batch_finish (unsigned int n, unsigned int tag_start, int with_grating,
+                         unsigned int *num_detected)
+{
+   float ms[8];
+   unsigned int counts[8];
+   unsigned long hist = MARX_ENERGY_OK | MARX_TIME_OK | MARX_X_VECTOR_OK | MARX_P_VECTOR_OK
+     | MARX_TAG_OK | MARX_MIRROR_SHELL_OK;
+   if (-1 == mxgd_read_timings (ms, counts, 8))
+     return -1;
+   Stage_Counts[0] = n;
+   Stage_Counts[1] = counts[1];
+   Stage_Counts[2] = with_grating ? counts[2] : Stage_Counts[1];
+   Stage_Counts[3] = (Detector_Type > 0) ? counts[3] : Stage_Counts[2];
+   if (with_grating) hist |= MARX_ORDER_OK;
+   if (Detector_Type > 0) hist |= MARX_DET_NUM_OK | MARX_DET_PIXEL_OK;
+   Kernel_Ms[0] = ms[MXGD_SLOT_HRMA_ENTER];
+   Kernel_Ms[1] = ms[MXGD_SLOT_HRMA_REFLECT];
+   Kernel_In[0] = counts[MXGD_SLOT_HRMA_ENTER];
+   Kernel_In[1] = counts[MXGD_SLOT_HRMA_REFLECT];
+   Resident = NULL;
+   Last_Collected = n;
+   Last_Tag_Start = tag_start;
+   Last_History = hist;
+   memcpy (Last_Slot_Ms, ms, sizeof (ms));
+   if (num_detected) *num_detected = Stage_Counts[3];
+   return 0;
+}
+
+static int batch_setup (unsigned int num_sources, const double *dirs, const unsigned int *counts,
+                        unsigned int *n)
+{
+   if (-1 == sync_config ())
+     return -1;
+   if (Mirror_Type != MARX_MIRROR_HRMA)
+     {
+        mxg_error ("Mirror not initialized.");
+        return -1;
+     }
+   if ((Mxg_Cfg.src.spectrum != MXG_SPECTRUM_FLAT) && (Mxg_Cfg.src.spectrum != MXG_SPECTRUM_FILE))
+     {
+        mxg_error ("Source not initialized (marx_create_source).");
+        return -1;
+     }
+   if ((num_sources == 0) || (dirs == NULL) || (counts == NULL))
+     {
+        mxg_error ("mxg_trace_batch: no sources");
+        return -1;
+     }
+   if (-1 == mxgd_batch_set_sources (num_sources, dirs, counts, n))
+     {
+        mxg_error ("%s", mxgd_last_error ());
+        return -1;
+     }
+   return 0;
+}
+
+int mxg_trace_batch (unsigned int num_sources, const double *dirs, const unsigned int *counts,
+                     unsigned int tag_start, double start_time, double exposure, int use_graph,
+                     unsigned int *num_detected, float *ms)
+{
+   int with_grating = (Grating_Type == MARX_GRATING_HETG) || (Grating_Type == MARX_GRATING_LETG);
+   unsigned int n;
+   Batch_Key key;
+
+   if (-1 == batch_setup (num_sources, dirs, counts, &n))
+     return -1;
+   memset (&key, 0, sizeof (key));
+   key.nsrc = num_sources;
+   key.n = n;
+   key.generation = Mxg_Cfg_Generation;
+   key.grating = (unsigned int) Grating_Type;
+   key.detector = (unsigned int) Detector_Type;
+   key.seed = Seed;
+   key.exposure = exposure;
+   key.start_time = start_time;
+   key.distance = Mxg_Cfg.src.distance;
+   mxgd_get_schedule (&key.sched);
+
+   if (use_graph)
+     {
+        if ((Batch_Graph == NULL) || memcmp (&key, &Batch_Graph_Key, sizeof (key)))
+          {
+             mxgd_graph_destroy (Batch_Graph);
+             Batch_Graph = NULL;
+             if ((-1 == mxgd_batch_prepare (n))
+                 || (-1 == mxgd_graph_begin ()))
+               goto device_error;
+             if (-1 == batch_pass (n, start_time, exposure, with_grating))
+               {
+                  mxgd_graph *junk = NULL;
+                  (void) mxgd_graph_end (&junk);
+                  mxgd_graph_destroy (junk);
+                  goto device_error;
+               }
+             if (-1 == mxgd_graph_end (&Batch_Graph))
+               goto device_error;
+             Batch_Graph_Key = key;
+          }
+        if ((-1 == mxgd_batch_set_tag_start (tag_start))
+            || (-1 == mxgd_graph_launch (Batch_Graph, ms)))
+          goto device_error;
+     }
+   else
+     {
+        if ((-1 == mxgd_batch_prepare (n))
+            || (-1 == mxgd_batch_set_tag_start (tag_start))
+            || (-1 == batch_pass (n, start_time, exposure, with_grating))
+            || (-1 == batch_finish (n, tag_start, with_grating, num_detected)))
+          goto device_error;
+        if (ms) *ms = Last_Slot_Ms[0] + Last_Slot_Ms[1] + Last_Slot_Ms[2] + Last_Slot_Ms[3];
+        return 0;
+     }
+   if (-1 == batch_finish (n, tag_start, with_grating, num_detected))
+     goto device_error;
+   return 0;
+
+device_error:
+   mxg_error ("%s", mxgd_last_error ());
+   return -1;
+}
+
+int mxg_set_schedule (const mxg_schedule *s)
+{
+   mxgd_schedule d;
+   if (s == NULL) return -1;
+   memcpy (d.blocks_per_cu, s->blocks_per_cu, sizeof (d.blocks_per_cu));
+   d.reflect_waves = s->reflect_waves;
+   if (-1 == mxgd_set_schedule (&d))
+     {
+        mxg_error ("%s", mxgd_last_error ());
+        return -1;
+     }
+   return 0;
+}
+
+int mxg_get_schedule (mxg_schedule *s)
+{
+   mxgd_schedule d;
+   if (s == NULL) return -1;
+   mxgd_get_schedule (&d);
+   memcpy (s->blocks_per_cu, d.blocks_per_cu, sizeof (d.blocks_per_cu));
+   s->reflect_waves = d.reflect_waves;
+   return 0;
+}
+
+/* per-kernel times of one timed (uncaptured) batch pass, best of repeats:
+ * source, paraboloid, reflection, grating stage, detector stage */
+static int time_kernels (unsigned int num_sources, const double *dirs, const unsigned int *counts,
+                         unsigned int tag_start, double start_time, double exposure,
+                         unsigned int repeats, float best[5])
+{
+   unsigned int r;
+   for (int k = 0; k < 5; k++) best[k] = 1e30f;
+   for (r = 0; r < repeats; r++)
+     {
+        float t[5];
+        if (-1 == mxg_trace_batch (num_sources, dirs, counts, tag_start, start_time, exposure, 0,
+                                   NULL, NULL))
+          return -1;
+        t[0] = Last_Slot_Ms[0];
+        t[1] = Kernel_Ms[0];
+        t[2] = Kernel_Ms[1];
+        t[3] = Last_Slot_Ms[2];
+        t[4] = Last_Slot_Ms[3];
+        for (int k = 0; k < 5; k++)
+          if (t[k] < best[k]) best[k] = t[k];
+     }
+   return 0;
+}
+
+int mxg_autotune_batch (unsigned int num_sources, const double *dirs, const unsigned int *counts,
+                        unsigned int tag_start, double start_time, double exposure,
+                        unsigned int repeats, mxg_schedule *best, float *best_ms)
+{
+   static const unsigned int bpc[4] = {2, 4, 8, 16};
+   mxg_schedule cur, sel;
+   float best_t[5], t[5];
+   int k, c;
+
+   if (repeats == 0) repeats = 1;
+   if (-1 == mxg_get_schedule (&cur))
+     return -1;
+   sel = cur;
+   for (k = 0; k < 5; k++) best_t[k] = 1e30f;
+   /* blocks per CU, every kernel at once (their times are independent) */
+   for (c = 0; c < 4; c++)
+     {
+        mxg_schedule s = cur;
+        for (k = 0; k < 5; k++) s.blocks_per_cu[k] = bpc[c];
+        if ((-1 == mxg_set_schedule (&s))
+            || (-1 == time_kernels (num_sources, dirs, counts, tag_start, start_time, exposure,
+                                    repeats, t)))
+          goto fail;
+        for (k = 0; k < 5; k++)
+          if (t[k] < best_t[k])
+            {
+               best_t[k] = t[k];
+               sel.blocks_per_cu[k] = bpc[c];
+            }
+     }
+   /* the reflection kernel's register budget */
+   best_t[2] = 1e30f;
+   for (c = 2; c <= 4; c++)
+     {
+        mxg_schedule s = sel;
+        s.reflect_waves = (unsigned int) c;
+        if ((-1 == mxg_set_schedule (&s))
+            || (-1 == time_kernels (num_sources, dirs, counts, tag_start, start_time, exposure,
+                                    repeats, t)))
+          goto fail;
+        if (t[2] < best_t[2])
+          {
+             best_t[2] = t[2];
+             sel.reflect_waves = (unsigned int) c;
+          }
+     }
+   if (-1 == mxg_set_schedule (&sel))
+     goto fail;
+   if (best) *best = sel;
+   if (best_ms)
+     {
+        float ms = 0.0f;
+        *best_ms = 1e30f;
+        for (unsigned int r = 0; r < repeats + 1; r++)
+          {
+             if (-1 == mxg_trace_batch (num_sources, dirs, counts, tag_start, start_time, exposure, 1,
+                                        NULL, &ms))
+               goto fail;
+             if ((r > 0) && (ms < *best_ms)) *best_ms = ms;
+          }
+     }
+   return 0;
+
+fail:
+   (void) mxg_set_schedule (&cur);
+   return -1;
+}
+
+int mxg_last_stage_ms (float ms[4])
+{
+   memcpy (ms, Last_Slot_Ms, 4 * sizeof (float));
    return 0;
 }
 

@@ -58,6 +58,46 @@ int mxgd_run_source (unsigned int n, unsigned int tag_start, double start_time,
                      double exposure_time, unsigned long long seed, unsigned int source_id,
                      int slot, unsigned int *n_out, double *last_time, float *ms);
 
+/* ---- batched point sources ---- */
+
+/* Uploads the directions (3 per source) and photon counts of a batch;
+ * *n_total receives the number of photons. */
+int mxgd_batch_set_sources (unsigned int nsrc, const double *dirs, const unsigned int *counts,
+                            unsigned int *n_total);
+/* Sizes the store for n photons and resets the list state; call before a
+ * batch is launched or captured. */
+int mxgd_batch_prepare (unsigned int n);
+/* Tag of the first photon of the next launch (read by the kernel from
+ * device memory, so a captured launch sequence picks it up). */
+int mxgd_batch_set_tag_start (unsigned int tag_start);
+/* Launches the batch source kernel over the n photons of the uploaded
+ * sources: arrival times uniform over [0, exposure). */
+int mxgd_run_batch_source (unsigned int n, double exposure, double start_time,
+                           unsigned long long seed, int slot);
+
+/* Launch schedule: blocks per CU of the stage kernels and the minimum waves
+ * per SIMD of the reflection kernel (2, 3 or 4). */
+enum { MXGD_K_SOURCE, MXGD_K_ENTER, MXGD_K_REFLECT, MXGD_K_GRATING, MXGD_K_DETECT, MXGD_NUM_K };
+typedef struct
+{
+   unsigned int blocks_per_cu[MXGD_NUM_K];
+   unsigned int reflect_waves;
+}
+mxgd_schedule;
+int mxgd_set_schedule (const mxgd_schedule *s);
+void mxgd_get_schedule (mxgd_schedule *s);
+
+/* HIP graph of a launch sequence: mxgd_graph_begin starts capturing the
+ * library's stream (stage runners then record no timing events and must be
+ * given ms = NULL), mxgd_graph_end instantiates what was captured.
+ * mxgd_graph_launch replays it (ms: its device time, or NULL) and restores
+ * the list state the sequence left at capture. */
+typedef struct mxgd_graph mxgd_graph;
+int mxgd_graph_begin (void);
+int mxgd_graph_end (mxgd_graph **g);
+int mxgd_graph_launch (mxgd_graph *g, float *ms);
+void mxgd_graph_destroy (mxgd_graph *g);
+
 /* Number of photons in the active list (device -> host copy). */
 int mxgd_active_count (unsigned int *n);
 /* Active list indices (host copy, n = mxgd_active_count). */

@@ -722,7 +722,7 @@ mxg_fits_table *mxg_fits_find_table (const char *file, mxg_fits_match_fn match, 
              datasize = (size_t) ((bitpix < 0) ? -bitpix : bitpix) / 8;
              for (k = 1; k <= naxis; k++)
                {
-                  char key[16];
+                  char key[32];
                   snprintf (key, sizeof (key), "NAXIS%ld", k);
                   datasize *= (size_t) header_long (hdr, ncards, key, 0);
                }

@@ -110,12 +110,21 @@ struct SegWriter
 // Launches k_compact over the segments a stage kernel of io.grid blocks wrote.
 hipError_t launch_compact (const ListIO &io, hipStream_t st);
 
-// Device buffers the mirror stage uses between its two kernels.
+// Minimum waves per SIMD of the mirror's reflection kernel unless the
+// launch schedule says otherwise (256 lanes per block = one wave per SIMD
+// per block).  Unbounded, the compiler gives it ~200 VGPRs, i.e. two waves
+// per SIMD, too few to hide its LDS and table-load latency; four waves
+// (128 VGPRs, a few spills) measured fastest on the HETG workload.
+#define MXG_REFLECT_WAVES_DEFAULT 4
+
+// Device buffers and launch shape of the mirror stage's second kernel.
 struct HrmaScratch
 {
-   uint32_t *idx_mid;    // photons that hit the paraboloid (capacity entries)
-   uint32_t *n_mid;      // its length (written by k_compact)
-   uint32_t *rng_pos;    // per photon: random-stream position at the split
+   uint32_t *idx_mid;      // photons that hit the paraboloid (capacity entries)
+   uint32_t *n_mid;        // its length (written by k_compact)
+   uint32_t *rng_pos;      // per photon: random-stream position at the split
+   uint32_t reflect_grid;  // blocks of k_hrma_reflect
+   int reflect_waves;      // 2, 3 or 4: the instance of k_hrma_reflect launched
 };
 
 // Stage launchers (defined in the stage .hip files).
@@ -138,5 +147,13 @@ hipError_t launch_source (const mxg_config *cfg, const double *da, Soa s, uint32
                           unsigned long long seed, uint32_t source_id, uint64_t *ticks,
                           uint64_t *tile_tot, uint32_t *first_over, uint32_t *count,
                           uint32_t *idx, hipStream_t st);
+
+// Batched point sources (see source.hip): n photons over nsrc sources with
+// prefix offsets off[nsrc + 1] and unit directions dir[3 nsrc].
+hipError_t launch_batch_source (const mxg_config *cfg, const double *da, Soa s, uint32_t n,
+                                const uint32_t *off, uint32_t nsrc, const double *dir,
+                                const uint32_t *tag_start, double exposure, double start_time,
+                                unsigned long long seed, uint32_t *idx, uint32_t *count,
+                                uint32_t grid, hipStream_t st);
 
 } // namespace mxg

@@ -3,6 +3,7 @@
 // and the stage runners called by the host C layer (mxg_device.h).
 #include <hip/hip_runtime.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 #include <stdarg.h>
 #include <stdint.h>
@@ -50,6 +51,14 @@ unsigned int g_upper = 0;          // host-side upper bound of the active count
 Marx_Photon_Attr_Type *d_aos = nullptr;
 size_t g_aos_cap = 0;
 uint64_t *d_ticks = nullptr;       // arrival-time scan (fixed-point gaps)
+// batched point sources
+uint32_t *d_batch_off = nullptr;   // nsrc + 1 prefix offsets
+double *d_batch_dir = nullptr;     // 3 nsrc directions
+uint32_t *d_batch_tag = nullptr;   // tag of the first photon of the launch
+unsigned int g_batch_cap = 0, g_batch_nsrc = 0;
+// launch schedule and graph capture
+mxgd_schedule g_sched = {{8, 8, 8, 8, 8}, MXG_REFLECT_WAVES_DEFAULT};
+int g_capturing = 0;
 uint64_t *d_tile_tot = nullptr;
 
 int fail (const char *fmt, ...)
@@ -102,10 +111,15 @@ void k_aos_to_soa (const Marx_Photon_Attr_Type *__restrict__ a, Soa s, uint32_t 
 __global__ __launch_bounds__ (kBlock)
 void k_soa_to_aos (Marx_Photon_Attr_Type *__restrict__ a, Soa s, uint32_t n)
 {
+   static_assert (sizeof (Marx_Photon_Attr_Type) % 8 == 0, "record is whole 8-byte words");
+   constexpr int kWords = sizeof (Marx_Photon_Attr_Type) / 8;
    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
    if (i >= n) return;
-   Marx_Photon_Attr_Type at;
-   memset (&at, 0, sizeof (at));
+   // built in zeroed words and stored word by word, so that the record's
+   // padding bytes are zero (a struct copy may leave them indeterminate)
+   union { Marx_Photon_Attr_Type at; uint64_t w[kWords]; } u;
+   for (int k = 0; k < kWords; k++) u.w[k] = 0;
+   Marx_Photon_Attr_Type &at = u.at;
    at.energy = s.energy[i];
    at.x.x = s.x0[i]; at.x.y = s.x1[i]; at.x.z = s.x2[i];
    at.p.x = s.p0[i]; at.p.y = s.p1[i]; at.p.z = s.p2[i];
@@ -122,7 +136,8 @@ void k_soa_to_aos (Marx_Photon_Attr_Type *__restrict__ a, Soa s, uint32_t n)
    at.ccd_num = s.ccd[i]; at.detector_region = s.region[i]; at.order = s.order[i];
    for (int k = 0; k < 4; k++) at.support_orders[k] = s.sup[k][i];
    at.tag = s.tag[i];
-   a[i] = at;
+   uint64_t *dst = (uint64_t *) (a + i);
+   for (int k = 0; k < kWords; k++) dst[k] = u.w[k];
 }
 
 int ensure_capacity (size_t n)
@@ -183,18 +198,19 @@ int ensure_aos (size_t n)
    return 0;
 }
 
-// Blocks of a stage kernel over at most n_upper active photons: enough to
-// fill every CU several times over, each block owning an equal range.
-uint32_t stage_grid (uint32_t n_upper)
+// Blocks of stage kernel k over at most n_upper active photons: the
+// schedule's blocks per CU times the CUs (each block owning an equal range),
+// fewer when there are fewer photons than that.
+uint32_t stage_grid (uint32_t n_upper, int k)
 {
    uint32_t blocks = (n_upper + kBlock - 1) / kBlock;
-   uint32_t cap = (uint32_t) (g_num_cus > 0 ? g_num_cus : 256) * 8u;
+   uint32_t cap = (uint32_t) (g_num_cus > 0 ? g_num_cus : 256) * g_sched.blocks_per_cu[k];
    if (cap > kMaxSegments) cap = kMaxSegments;
    if (blocks > cap) blocks = cap;
    return blocks ? blocks : 1u;
 }
 
-ListIO list_io ()
+ListIO list_io (int k)
 {
    ListIO io;
    io.idx_in = d_idx[g_cur];
@@ -203,7 +219,7 @@ ListIO list_io ()
    io.n_out = d_counts + (1 - g_cur);
    io.seg = d_seg;
    io.seg_count = d_seg_count;
-   io.grid = stage_grid (g_upper);
+   io.grid = stage_grid (g_upper, k);
    return io;
 }
 
@@ -213,7 +229,7 @@ int begin_stage (int slot)
 {
    hipLaunchKernelGGL (k_set_u32, dim3 (1), dim3 (1), 0, g_stream, d_counts + (1 - g_cur), 0u);
    HIPCHECK (hipGetLastError ());
-   if ((slot >= 0) && (slot < kSlots))
+   if ((slot >= 0) && (slot < kSlots) && !g_capturing)
      {
         HIPCHECK (hipEventRecord (g_tev[slot][0], g_stream));
         g_slot_used[slot] = 1;
@@ -224,7 +240,7 @@ int begin_stage (int slot)
 // count slot 8 + k receives the length of the survivor list of stage k
 int end_stage (int slot, float *ms)
 {
-   if ((slot >= 0) && (slot < kSlots))
+   if ((slot >= 0) && (slot < kSlots) && !g_capturing)
      HIPCHECK (hipEventRecord (g_tev[slot][1], g_stream));
    g_cur = 1 - g_cur;
    if ((slot >= 0) && (slot < kSlots))
@@ -426,16 +442,19 @@ int mxgd_run_mirror (double source_distance, unsigned long long seed, int slot, 
    if (!d_cfg) return fail ("configuration not on the device");
    if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
    if (begin_stage (slot)) return -1;
-   HrmaScratch hs = {d_idx_mid, d_counts + 4, d_rng_pos};
+   HrmaScratch hs = {d_idx_mid, d_counts + 4, d_rng_pos, stage_grid (g_upper, MXGD_K_REFLECT),
+                     (int) g_sched.reflect_waves};
    hipEvent_t kev[4] = {g_tev[MXGD_SLOT_HRMA_ENTER][0], g_tev[MXGD_SLOT_HRMA_ENTER][1],
                         g_tev[MXGD_SLOT_HRMA_REFLECT][0], g_tev[MXGD_SLOT_HRMA_REFLECT][1]};
    bool timed = (slot >= 0) && (g_upper > 0);
-   HIPCHECK (launch_hrma (d_cfg, &g_hcfg, d_fa, d_da, g_soa, list_io (), hs, g_upper,
-                          source_distance, seed, 0, g_stream, timed ? kev : nullptr));
+   HIPCHECK (launch_hrma (d_cfg, &g_hcfg, d_fa, d_da, g_soa, list_io (MXGD_K_ENTER), hs, g_upper,
+                          source_distance, seed, 0, g_stream,
+                          (timed && !g_capturing) ? kev : nullptr));
    if (timed)
      {
         // photons entering each kernel: the active list, the middle list
-        g_slot_used[MXGD_SLOT_HRMA_ENTER] = g_slot_used[MXGD_SLOT_HRMA_REFLECT] = 1;
+        if (!g_capturing)
+          g_slot_used[MXGD_SLOT_HRMA_ENTER] = g_slot_used[MXGD_SLOT_HRMA_REFLECT] = 1;
         hipLaunchKernelGGL (k_copy_u32, dim3 (1), dim3 (1), 0, g_stream,
                             d_counts + 8 + MXGD_SLOT_HRMA_ENTER, d_counts + g_cur);
         hipLaunchKernelGGL (k_copy_u32, dim3 (1), dim3 (1), 0, g_stream,
@@ -450,7 +469,8 @@ int mxgd_run_grating (unsigned long long seed, int slot, float *ms)
    if (!d_cfg) return fail ("configuration not on the device");
    if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
    if (begin_stage (slot)) return -1;
-   HIPCHECK (launch_grating (d_cfg, d_fa, d_da, g_soa, list_io (), g_upper, seed, 0, g_stream));
+   HIPCHECK (launch_grating (d_cfg, d_fa, d_da, g_soa, list_io (MXGD_K_GRATING), g_upper, seed, 0,
+                             g_stream));
    return end_stage (slot, ms);
 }
 
@@ -459,8 +479,8 @@ int mxgd_run_detect (double tstart, unsigned long long seed, int slot, float *ms
    if (!d_cfg) return fail ("configuration not on the device");
    if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
    if (begin_stage (slot)) return -1;
-   HIPCHECK (launch_detect (d_cfg, d_fa, d_da, g_soa, list_io (), g_upper, tstart, seed, 0,
-                            g_stream));
+   HIPCHECK (launch_detect (d_cfg, d_fa, d_da, g_soa, list_io (MXGD_K_DETECT), g_upper, tstart,
+                            seed, 0, g_stream));
    return end_stage (slot, ms);
 }
 
@@ -544,6 +564,164 @@ int mxgd_run_source (unsigned int n, unsigned int tag_start, double start_time,
    if (n_out) *n_out = m;
    if (last_time) *last_time = t;
    return 0;
+}
+
+/* ---- batched point sources ---------------------------------------------------- */
+
+int mxgd_batch_set_sources (unsigned int nsrc, const double *dirs, const unsigned int *counts,
+                            unsigned int *n_total)
+{
+   if (mxgd_init ()) return -1;
+   if (g_capturing) return fail ("sources cannot change during graph capture");
+   uint64_t tot = 0;
+   for (unsigned int k = 0; k < nsrc; k++) tot += counts[k];
+   if (tot > (1ull << 28)) return fail ("batch of %llu photons exceeds 2^28", (unsigned long long) tot);
+   if (nsrc > g_batch_cap)
+     {
+        HIPCHECK (hipStreamSynchronize (g_stream));
+        if (d_batch_off) (void) hipFree (d_batch_off);
+        if (d_batch_dir) (void) hipFree (d_batch_dir);
+        d_batch_off = nullptr;
+        d_batch_dir = nullptr;
+        g_batch_cap = 0;
+        HIPCHECK (hipMalloc (&d_batch_off, (nsrc + 1) * sizeof (uint32_t)));
+        HIPCHECK (hipMalloc (&d_batch_dir, 3 * (size_t) nsrc * sizeof (double)));
+        g_batch_cap = nsrc;
+     }
+   if (!d_batch_tag) HIPCHECK (hipMalloc (&d_batch_tag, sizeof (uint32_t)));
+   uint32_t *off = (uint32_t *) malloc ((nsrc + 1) * sizeof (uint32_t));
+   if (off == nullptr) return fail ("out of memory");
+   off[0] = 0;
+   for (unsigned int k = 0; k < nsrc; k++) off[k + 1] = off[k] + counts[k];
+   hipError_t e = hipMemcpy (d_batch_off, off, (nsrc + 1) * sizeof (uint32_t), hipMemcpyHostToDevice);
+   free (off);
+   HIPCHECK (e);
+   if (nsrc)
+     HIPCHECK (hipMemcpy (d_batch_dir, dirs, 3 * (size_t) nsrc * sizeof (double), hipMemcpyHostToDevice));
+   g_batch_nsrc = nsrc;
+   *n_total = (unsigned int) tot;
+   return 0;
+}
+
+int mxgd_batch_prepare (unsigned int n)
+{
+   if (mxgd_init ()) return -1;
+   if (ensure_capacity (n ? n : 1)) return -1;
+   g_cur = 0;
+   g_n = n;
+   g_upper = n;
+   return 0;
+}
+
+int mxgd_batch_set_tag_start (unsigned int tag_start)
+{
+   if (!d_batch_tag) return fail ("no batch sources uploaded");
+   hipLaunchKernelGGL (k_set_u32, dim3 (1), dim3 (1), 0, g_stream, d_batch_tag, (uint32_t) tag_start);
+   HIPCHECK (hipGetLastError ());
+   return 0;
+}
+
+int mxgd_run_batch_source (unsigned int n, double exposure, double start_time,
+                           unsigned long long seed, int slot)
+{
+   if (!d_cfg) return fail ("configuration not on the device");
+   if ((n > g_capacity) || (g_batch_nsrc == 0)) return fail ("batch not prepared");
+   bool timed = (slot >= 0) && (slot < kSlots) && !g_capturing;
+   if (timed)
+     {
+        HIPCHECK (hipEventRecord (g_tev[slot][0], g_stream));
+        g_slot_used[slot] = 1;
+     }
+   uint32_t grid = stage_grid (n, MXGD_K_SOURCE);
+   HIPCHECK (launch_batch_source (d_cfg, d_da, g_soa, n, d_batch_off, g_batch_nsrc, d_batch_dir,
+                                  d_batch_tag, exposure, start_time, seed, d_idx[0], d_counts, grid,
+                                  g_stream));
+   if (timed)
+     HIPCHECK (hipEventRecord (g_tev[slot][1], g_stream));
+   g_cur = 0;
+   g_n = n;
+   g_upper = n;
+   return 0;
+}
+
+int mxgd_set_schedule (const mxgd_schedule *s)
+{
+   for (int k = 0; k < MXGD_NUM_K; k++)
+     if ((s->blocks_per_cu[k] < 1) || (s->blocks_per_cu[k] > 32))
+       return fail ("blocks per CU must be 1..32");
+   if ((s->reflect_waves < 2) || (s->reflect_waves > 4))
+     return fail ("reflection kernel waves per SIMD must be 2, 3 or 4");
+   g_sched = *s;
+   return 0;
+}
+
+void mxgd_get_schedule (mxgd_schedule *s) { *s = g_sched; }
+
+struct mxgd_graph
+{
+   hipGraph_t graph;
+   hipGraphExec_t exec;
+   int cur;                 // list state at the end of the captured sequence
+   unsigned int n, upper;
+};
+
+int mxgd_graph_begin (void)
+{
+   if (mxgd_init ()) return -1;
+   if (g_capturing) return fail ("graph capture already running");
+   HIPCHECK (hipStreamSynchronize (g_stream));
+   HIPCHECK (hipStreamBeginCapture (g_stream, hipStreamCaptureModeRelaxed));
+   g_capturing = 1;
+   return 0;
+}
+
+int mxgd_graph_end (mxgd_graph **out)
+{
+   *out = nullptr;
+   if (!g_capturing) return fail ("no graph capture running");
+   g_capturing = 0;
+   hipGraph_t graph = nullptr;
+   HIPCHECK (hipStreamEndCapture (g_stream, &graph));
+   mxgd_graph *g = (mxgd_graph *) calloc (1, sizeof (mxgd_graph));
+   if (g == nullptr) { (void) hipGraphDestroy (graph); return fail ("out of memory"); }
+   g->graph = graph;
+   hipError_t e = hipGraphInstantiate (&g->exec, graph, nullptr, nullptr, 0);
+   if (e != hipSuccess)
+     {
+        (void) hipGraphDestroy (graph);
+        free (g);
+        return fail ("hipGraphInstantiate failed: %s", hipGetErrorString (e));
+     }
+   g->cur = g_cur;
+   g->n = g_n;
+   g->upper = g_upper;
+   *out = g;
+   return 0;
+}
+
+int mxgd_graph_launch (mxgd_graph *g, float *ms)
+{
+   if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
+   HIPCHECK (hipGraphLaunch (g->exec, g_stream));
+   if (ms)
+     {
+        HIPCHECK (hipEventRecord (g_ev1, g_stream));
+        HIPCHECK (hipEventSynchronize (g_ev1));
+        HIPCHECK (hipEventElapsedTime (ms, g_ev0, g_ev1));
+     }
+   g_cur = g->cur;
+   g_n = g->n;
+   g_upper = g->upper;
+   return 0;
+}
+
+void mxgd_graph_destroy (mxgd_graph *g)
+{
+   if (g == nullptr) return;
+   (void) hipStreamSynchronize (g_stream);
+   (void) hipGraphExecDestroy (g->exec);
+   (void) hipGraphDestroy (g->graph);
+   free (g);
 }
 
 } // extern "C"

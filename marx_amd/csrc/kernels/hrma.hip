@@ -254,14 +254,6 @@ void k_hrma_enter (const mxg_config *__restrict__ cfg, Soa s, ListIO io,
    survivors.finish (io);
 }
 
-// Minimum waves per SIMD for k_hrma_reflect (256 lanes per block = one wave
-// per SIMD per block).  Unbounded, the compiler gives it ~200 VGPRs, i.e.
-// two waves per SIMD, too few to hide its LDS and table-load latency; four
-// waves (128 VGPRs, a few spills) measured fastest.
-#ifndef MXG_REFLECT_WAVES
-#define MXG_REFLECT_WAVES 4
-#endif
-
 // Offsets (in floats) of the tables staged in LDS by k_hrma_reflect.
 struct HrmaLds
 {
@@ -274,7 +266,9 @@ struct HrmaLds
 // The block stages the mirror configuration and its small, hot tables (the
 // binary-search targets of every photon) in LDS once, then loops over its
 // range of the middle list.
-__global__ __launch_bounds__ (kBlock, MXG_REFLECT_WAVES)
+// WAVES: minimum waves per SIMD the register allocation must allow.
+template <int WAVES>
+__global__ __launch_bounds__ (kBlock, WAVES)
 void k_hrma_reflect (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
                      const double *__restrict__ da, Soa s, ListIO io,
                      const uint32_t *__restrict__ rng_pos, unsigned long long seed,
@@ -422,6 +416,11 @@ hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const flo
    a.n_out = hs.n_mid;
    b.idx_in = hs.idx_mid;
    b.n_in = hs.n_mid;
+   b.grid = hs.reflect_grid;
+   void (*reflect) (const mxg_config *, const float *, const double *, Soa, ListIO,
+                    const uint32_t *, unsigned long long, uint32_t, HrmaLds)
+     = (hs.reflect_waves <= 2) ? k_hrma_reflect<2>
+       : (hs.reflect_waves == 3) ? k_hrma_reflect<3> : k_hrma_reflect<4>;
    hipError_t e = hipSuccess;
    if (ev) e = hipEventRecord (ev[0], st);
    if (e != hipSuccess) return e;
@@ -432,7 +431,7 @@ hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const flo
    if (e == hipSuccess) e = launch_compact (a, st);
    if ((e == hipSuccess) && ev) e = hipEventRecord (ev[2], st);
    if (e != hipSuccess) return e;
-   hipLaunchKernelGGL (k_hrma_reflect, dim3 (b.grid), dim3 (kBlock), lds_bytes, st, cfg, fa, da, s,
+   hipLaunchKernelGGL (reflect, dim3 (b.grid), dim3 (kBlock), lds_bytes, st, cfg, fa, da, s,
                        b, hs.rng_pos, seed, source_id, lay);
    e = hipGetLastError ();
    if ((e == hipSuccess) && ev) e = hipEventRecord (ev[3], st);

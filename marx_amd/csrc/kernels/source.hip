@@ -20,6 +20,39 @@ namespace mxg {
 constexpr int kTile = 2048;          // photons per scan tile
 constexpr int kPer = kTile / kBlock; // photons per thread
 
+// energy of the FLAT or FILE spectrum (spectrum.c:138-147, 184-189)
+__device__ __forceinline__ double draw_energy (const mxg_source_cfg &src, const double *__restrict__ da,
+                                               Stream &st)
+{
+   if (src.spectrum == MXG_SPECTRUM_FLAT)
+     {
+        double emin = src.emin, de = src.emax - emin;
+        return emin + de * uniform (st);
+     }
+   double r = uniform (st);
+   return interp_d (r, da + src.file_cum_off, da + src.file_e_off, src.file_n);
+}
+
+// every column of photon i set as a fresh photon of the given energy,
+// direction, time and tag
+__device__ __forceinline__ void init_photon (Soa s, uint32_t i, double energy, vec3 p, double t,
+                                             uint32_t tag)
+{
+   s.energy[i] = energy;
+   s.p0[i] = p.x; s.p1[i] = p.y; s.p2[i] = p.z;
+   s.x0[i] = 0.0; s.x1[i] = 0.0; s.x2[i] = 0.0;
+   s.t[i] = t;
+   s.flags[i] = 0u;
+   s.ypix[i] = 0.0f; s.zpix[i] = 0.0f; s.upix[i] = 0.0f; s.vpix[i] = 0.0f;
+   for (int q = 0; q < 6; q++) s.dith[q][i] = 0.0f;
+   s.pi[i] = 0.0f;
+   s.pha[i] = 0;
+   s.shell[i] = 0;
+   s.ccd[i] = 0; s.region[i] = 0; s.order[i] = 0;
+   for (int q = 0; q < 4; q++) s.sup[q][i] = 0;
+   s.tag[i] = tag;
+}
+
 __device__ __forceinline__ uint64_t gap_ticks (double e)
 {
    return (uint64_t) (e * 4294967296.0 + 0.5);
@@ -46,17 +79,7 @@ void k_source (const mxg_config *__restrict__ cfg, const double *__restrict__ da
              const uint32_t tag = tag_start + i;
              Stream st;
              stream_init (st, seed, tag, MXG_STAGE_SOURCE, source_id);
-             double energy;
-             if (src.spectrum == MXG_SPECTRUM_FLAT)
-               {
-                  double emin = src.emin, de = src.emax - emin;
-                  energy = emin + de * uniform (st);
-               }
-             else
-               {
-                  double r = uniform (st);
-                  energy = interp_d (r, da + src.file_cum_off, da + src.file_e_off, src.file_n);
-               }
+             double energy = draw_energy (src, da, st);
              vec3 p = ld3 (src.p);
              if (src.type == MXG_SOURCE_DISK)
                {
@@ -73,19 +96,7 @@ void k_source (const mxg_config *__restrict__ cfg, const double *__restrict__ da
              else
                t = src.uniform_tmax * uniform (st);
 
-             s.energy[i] = energy;
-             s.p0[i] = p.x; s.p1[i] = p.y; s.p2[i] = p.z;
-             s.x0[i] = 0.0; s.x1[i] = 0.0; s.x2[i] = 0.0;
-             s.t[i] = t;
-             s.flags[i] = 0u;
-             s.ypix[i] = 0.0f; s.zpix[i] = 0.0f; s.upix[i] = 0.0f; s.vpix[i] = 0.0f;
-             for (int q = 0; q < 6; q++) s.dith[q][i] = 0.0f;
-             s.pi[i] = 0.0f;
-             s.pha[i] = 0;
-             s.shell[i] = 0;
-             s.ccd[i] = 0; s.region[i] = 0; s.order[i] = 0;
-             for (int q = 0; q < 4; q++) s.sup[q][i] = 0;
-             s.tag[i] = tag;
+             init_photon (s, i, energy, p, t, tag);
           }
         sk[li] = k;
      }
@@ -152,6 +163,44 @@ void k_scan_tiles (uint64_t *__restrict__ tile_tot, uint32_t ntiles)
      }
 }
 
+// INTERNAL dither of photon i (dither.c:167-182, 551-581, 609-658): the
+// Lissajous pointing offset at its arrival time (stored as float, as the
+// reference stores the dither state), Gaussian aspect blur, and the rotation
+// of its direction.
+__device__ __forceinline__ void apply_internal_dither (const mxg_source_cfg &src, Soa s, uint32_t i,
+                                                       double t, double start_time,
+                                                       unsigned long long seed, uint32_t source_id)
+{
+   Stream st;
+   stream_init (st, seed, s.tag[i], MXG_STAGE_DITHER, source_id);
+   double tt = (2.0 * MXG_PI) * (start_time + t);
+   float fra = (float) (src.ra_amp * sin (tt / src.ra_period + src.ra_phase));
+   float fdec = (float) (src.dec_amp * sin (tt / src.dec_period + src.dec_phase));
+   float froll = (float) (src.nominal_roll + src.roll_amp * sin (tt / src.roll_period + src.roll_phase));
+   s.dith[0][i] = fra; s.dith[1][i] = fdec; s.dith[2][i] = froll;
+   s.dith[3][i] = 0.0f; s.dith[4][i] = 0.0f; s.dith[5][i] = 0.0f;
+   double delta_ra = src.aspect_blur * gaussian (st);
+   double delta_dec = src.aspect_blur * gaussian (st);
+   double ra = (double) fra + delta_ra, dec = (double) fdec + delta_dec, roll = froll;
+
+   vec3 p = v3 (s.p0[i], s.p1[i], s.p2[i]);
+   p = rotate_unit_vector (p, v3 (1, 0, 0), -roll);
+   double sin_ra, cos_ra, sin_dec, cos_dec;
+   sincos (ra, &sin_ra, &cos_ra);
+   sincos (dec, &sin_dec, &cos_dec);
+   double cos_theta = cos_dec * cos_ra;
+   vec3 nv = v3 (0, sin_dec, -cos_dec * sin_ra);
+   double sin_theta = length (nv);
+   if (sin_theta > 1e-20)
+     {
+        nv.x /= sin_theta;
+        nv.y /= sin_theta;
+        nv.z /= sin_theta;
+        p = rotate_unit_vector1 (p, nv, cos_theta, sin_theta);
+     }
+   s.p0[i] = p.x; s.p1[i] = p.y; s.p2[i] = p.z;
+}
+
 // arrival times, first photon at or past the exposure, dither
 __global__ __launch_bounds__ (kBlock)
 void k_source_finish (const mxg_config *__restrict__ cfg, Soa s, uint32_t n,
@@ -186,37 +235,7 @@ void k_source_finish (const mxg_config *__restrict__ cfg, Soa s, uint32_t n,
      t = s.t[i];
 
    if (src.dither_mode == MXG_DITHER_INTERNAL)
-     {
-        Stream st;
-        stream_init (st, seed, s.tag[i], MXG_STAGE_DITHER, source_id);
-        double tt = (2.0 * MXG_PI) * (start_time + t);
-        // the dither state is float and the rotation uses the stored value
-        float fra = (float) (src.ra_amp * sin (tt / src.ra_period + src.ra_phase));
-        float fdec = (float) (src.dec_amp * sin (tt / src.dec_period + src.dec_phase));
-        float froll = (float) (src.nominal_roll + src.roll_amp * sin (tt / src.roll_period + src.roll_phase));
-        s.dith[0][i] = fra; s.dith[1][i] = fdec; s.dith[2][i] = froll;
-        s.dith[3][i] = 0.0f; s.dith[4][i] = 0.0f; s.dith[5][i] = 0.0f;
-        double delta_ra = src.aspect_blur * gaussian (st);
-        double delta_dec = src.aspect_blur * gaussian (st);
-        double ra = (double) fra + delta_ra, dec = (double) fdec + delta_dec, roll = froll;
-
-        vec3 p = v3 (s.p0[i], s.p1[i], s.p2[i]);
-        p = rotate_unit_vector (p, v3 (1, 0, 0), -roll);
-        double sin_ra, cos_ra, sin_dec, cos_dec;
-        sincos (ra, &sin_ra, &cos_ra);
-        sincos (dec, &sin_dec, &cos_dec);
-        double cos_theta = cos_dec * cos_ra;
-        vec3 nv = v3 (0, sin_dec, -cos_dec * sin_ra);
-        double sin_theta = length (nv);
-        if (sin_theta > 1e-20)
-          {
-             nv.x /= sin_theta;
-             nv.y /= sin_theta;
-             nv.z /= sin_theta;
-             p = rotate_unit_vector1 (p, nv, cos_theta, sin_theta);
-          }
-        s.p0[i] = p.x; s.p1[i] = p.y; s.p2[i] = p.z;
-     }
+     apply_internal_dither (src, s, i, t, start_time, seed, source_id);
 }
 
 __global__ void k_source_count (uint32_t n, const uint32_t *__restrict__ first_over,
@@ -253,6 +272,68 @@ hipError_t launch_source (const mxg_config *cfg, const double *da, Soa s, uint32
                        first_over);
    hipLaunchKernelGGL (k_source_count, dim3 ((n + kBlock - 1) / kBlock), dim3 (kBlock), 0, st, n,
                        first_over, count, idx);
+   return hipGetLastError ();
+}
+
+// ---- batched point sources ------------------------------------------------------
+//
+// Photon i (0 <= i < n) of a batch belongs to the source k with
+// off[k] <= i < off[k + 1]: direction dir[3k..3k+2] (unit vector of travel),
+// tag *tag_start + i, arrival time uniform over [0, exposure), energy from
+// the configured spectrum, INTERNAL dither when configured.  The tag alone
+// keys the photon's random streams, so tracing a source alone with
+// tag_start = off[k] gives the same photons.  tag_start is read from device
+// memory so that a captured launch sequence can be replayed for new tags.
+// The kernel also writes the identity active list and its length.
+
+__device__ __forceinline__ uint32_t source_of (const uint32_t *__restrict__ off, uint32_t nsrc, uint32_t i)
+{
+   uint32_t lo = 0, hi = nsrc;         // off[lo] <= i < off[hi]
+   while (hi > lo + 1)
+     {
+        uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid;
+     }
+   return lo;
+}
+
+__global__ __launch_bounds__ (kBlock)
+void k_batch_source (const mxg_config *__restrict__ cfg, const double *__restrict__ da, Soa s,
+                     uint32_t n, const uint32_t *__restrict__ off, uint32_t nsrc,
+                     const double *__restrict__ dir, const uint32_t *__restrict__ tag_start,
+                     double exposure, double start_time, unsigned long long seed,
+                     uint32_t *__restrict__ idx, uint32_t *__restrict__ count)
+{
+   const mxg_source_cfg &src = cfg->src;
+   const uint32_t t0 = *tag_start;
+   const uint32_t stride = gridDim.x * blockDim.x;
+   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+     {
+        const uint32_t k = source_of (off, nsrc, i);
+        const uint32_t tag = t0 + i;
+        Stream st;
+        stream_init (st, seed, tag, MXG_STAGE_SOURCE, 0u);
+        double energy = draw_energy (src, da, st);
+        vec3 p = v3 (dir[3 * k], dir[3 * k + 1], dir[3 * k + 2]);
+        double t = exposure * uniform (st);
+        init_photon (s, i, energy, p, t, tag);
+        if (src.dither_mode == MXG_DITHER_INTERNAL)
+          apply_internal_dither (src, s, i, t, start_time, seed, 0u);
+        idx[i] = i;
+     }
+   if ((blockIdx.x == 0) && (threadIdx.x == 0)) *count = n;
+}
+
+hipError_t launch_batch_source (const mxg_config *cfg, const double *da, Soa s, uint32_t n,
+                                const uint32_t *off, uint32_t nsrc, const double *dir,
+                                const uint32_t *tag_start, double exposure, double start_time,
+                                unsigned long long seed, uint32_t *idx, uint32_t *count,
+                                uint32_t grid, hipStream_t st)
+{
+   if (grid == 0) grid = 1;
+   hipLaunchKernelGGL (k_batch_source, dim3 (grid), dim3 (kBlock), 0, st, cfg, da, s, n, off, nsrc,
+                       dir, tag_start, exposure, start_time, seed, idx, count);
    return hipGetLastError ();
 }
 

@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import ATTR_DTYPE, PAR_FILE, PhotonList, check
+from ._lib import ATTR_DTYPE, PAR_FILE, PhotonList, Schedule, check
 
 
 class ParamFile:
@@ -147,6 +147,13 @@ class Tracer:
         self.lib.mxg_last_stage_counts(c)
         return list(c)
 
+    def stage_ms(self):
+        """Device ms of [source, mirror, grating, detector] in the last timed,
+        uncaptured pass."""
+        m = (ctypes.c_float * 4)()
+        self.lib.mxg_last_stage_ms(m)
+        return list(m)
+
     def kernel_stats(self):
         """(ms, photons_in) of the mirror's two kernels [aperture + paraboloid,
         reflections] in the last timed trace_device call."""
@@ -172,6 +179,56 @@ class Tracer:
                                         ctypes.byref(ncol), ctypes.byref(ndet), ms), "mxg_trace_device")
         return ncol.value, ndet.value, (list(ms) if timing else None)
 
+    @staticmethod
+    def _batch_args(dirs, counts):
+        d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1)
+        c = np.ascontiguousarray(counts, dtype=np.uint32)
+        if len(d) != 3 * len(c):
+            raise ValueError("dirs must hold 3 components per source")
+        return d, c
+
+    def trace_batch(self, dirs, counts, tag_start=0, start_time=0.0, exposure=1.0e4, use_graph=True):
+        """mxg_trace_batch: point sources with directions dirs (k, 3) and
+        photon counts (k,), traced in one device pass.  Returns (detected,
+        device ms)."""
+        d, c = self._batch_args(dirs, counts)
+        ndet, ms = ctypes.c_uint(), ctypes.c_float()
+        check(self.lib.mxg_trace_batch(len(c), d.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                       c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), int(tag_start),
+                                       float(start_time), float(exposure), int(bool(use_graph)),
+                                       ctypes.byref(ndet), ctypes.byref(ms)), "mxg_trace_batch")
+        return ndet.value, ms.value
+
+    def autotune_batch(self, dirs, counts, tag_start=0, start_time=0.0, exposure=1.0e4, repeats=2):
+        """mxg_autotune_batch: installs and returns (schedule dict, graph pass ms)."""
+        d, c = self._batch_args(dirs, counts)
+        best, ms = Schedule(), ctypes.c_float()
+        check(self.lib.mxg_autotune_batch(len(c), d.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                          c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), int(tag_start),
+                                          float(start_time), float(exposure), int(repeats),
+                                          ctypes.byref(best), ctypes.byref(ms)), "mxg_autotune_batch")
+        return best.as_dict(), ms.value
+
+    def get_schedule(self):
+        s = Schedule()
+        check(self.lib.mxg_get_schedule(ctypes.byref(s)), "mxg_get_schedule")
+        return s.as_dict()
+
+    def set_schedule(self, blocks_per_cu=None, reflect_waves=None, **per_kernel):
+        """Sets the launch schedule: blocks_per_cu for every kernel, or per
+        kernel (source=, enter=, reflect=, grating=, detect=), and the
+        reflection kernel's waves per SIMD."""
+        s = Schedule()
+        check(self.lib.mxg_get_schedule(ctypes.byref(s)), "mxg_get_schedule")
+        for i, k in enumerate(Schedule.KERNELS):
+            if blocks_per_cu is not None:
+                s.blocks_per_cu[i] = int(blocks_per_cu)
+            if k in per_kernel:
+                s.blocks_per_cu[i] = int(per_kernel[k])
+        if reflect_waves is not None:
+            s.reflect_waves = int(reflect_waves)
+        check(self.lib.mxg_set_schedule(ctypes.byref(s)), "mxg_set_schedule")
+
     def download(self, photons):
         check(self.lib.mxg_download_photons(photons.ptr), "mxg_download_photons")
 
@@ -194,6 +251,15 @@ class Tracer:
         self.lib.mxg_get_config(ctypes.byref(cfg), ctypes.byref(fa), ctypes.byref(nf),
                                 ctypes.byref(da), ctypes.byref(nd))
         return cfg.value, fa.value, nf.value, da.value, nd.value
+
+
+def offaxis_directions(theta, phi):
+    """Unit directions of travel (k, 3) of photons from point sources at
+    off-axis angles theta (radians) and azimuths phi: an on-axis source
+    sends its photons along (-1, 0, 0)."""
+    theta = np.asarray(theta, dtype=np.float64)
+    phi = np.asarray(phi, dtype=np.float64)
+    return -np.stack([np.cos(theta), np.sin(theta) * np.cos(phi), np.sin(theta) * np.sin(phi)], axis=-1)
 
 
 def on_axis_photons(n, energy, tag_start=0, rng=None, time_span=1.0e4):

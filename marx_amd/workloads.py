@@ -1,7 +1,9 @@
 """Named workloads (parameter sets) of the benchmark and the tests."""
 import os
 
-from .sim import Tracer
+import numpy as np
+
+from .sim import Tracer, offaxis_directions
 from .spectra import write_power_law
 
 # on-axis: the source at the nominal pointing of the standard parameter file
@@ -35,6 +37,23 @@ def letg_hrc_s(tmpdir=None, photons=50_000_000, exposure=50_000.0, seed=1):
     flux = photons / (exposure * t.geometric_area)
     t.init_source(SourceFlux=flux)
     return t
+
+
+def batched_acis_i(tmpdir, num_sources=4096, seed=1, layout_seed=0, min_count=100, max_count=100_000,
+                   max_offaxis_arcmin=15.0):
+    """Batched: num_sources point sources at off-axis angles uniform over
+    0-max_offaxis_arcmin with random azimuths, photon counts log-uniform
+    over [min_count, max_count], power law of photon index 2 over 0.3-10 keV,
+    HRMA + ACIS-I, no grating, no dither.  Returns (tracer, dirs, counts)."""
+    spec = write_power_law(os.path.join(tmpdir, "powerlaw_g2_0.3_10.dat"), gamma=2.0, emin=0.3, emax=10.0)
+    t = Tracer(seed=seed, GratingType="NONE", DetectorType="ACIS-I", SourceType="POINT",
+               SpectrumType="FILE", SpectrumFile=spec, DitherModel="NONE", **ON_AXIS)
+    t.init_source()
+    rng = np.random.default_rng(layout_seed)
+    theta = np.radians(rng.uniform(0.0, max_offaxis_arcmin, num_sources) / 60.0)
+    phi = rng.uniform(0.0, 2.0 * np.pi, num_sources)
+    counts = np.round(10.0 ** rng.uniform(np.log10(min_count), np.log10(max_count), num_sources))
+    return t, offaxis_directions(theta, phi), counts.astype(np.uint32)
 
 
 WORKLOADS = {"hetg-acis-s": hetg_acis_s, "letg-hrc-s": letg_hrc_s}

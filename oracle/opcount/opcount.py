@@ -7,7 +7,7 @@ transcendental calls per photon entering the stage.  These are the
 algorithmic work figures behind the benchmark's roofline; they count the
 restated reference algorithm and nothing of the GPU code.
 
-    python -m oracle.opcount.opcount [--photons 100000] [--workload hetg-acis-s|letg-hrc-s]
+    python -m oracle.opcount.opcount [--photons 100000] [--workload hetg-acis-s|letg-hrc-s|batched-acis-i]
 """
 import argparse
 import ctypes
@@ -36,6 +36,10 @@ def load_opcount():
     lib.orc_create_photons.argtypes = [ctypes.POINTER(O.OrcCtx), ctypes.c_void_p, ctypes.c_uint,
                                        ctypes.POINTER(ctypes.c_uint), ctypes.c_void_p]
     lib.orc_create_photons.restype = ctypes.c_int
+    lib.orc_create_batch.argtypes = [ctypes.POINTER(O.OrcCtx), ctypes.c_void_p, ctypes.c_uint,
+                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint),
+                                     ctypes.c_uint, ctypes.c_double, ctypes.c_double]
+    lib.orc_create_batch.restype = ctypes.c_int
     lib.orc_opcount_reset.restype = None
     lib.orc_opcount_get.argtypes = [ctypes.POINTER(ctypes.c_ulonglong)]
     lib.orc_opcount_get_phase.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_ulonglong)]
@@ -52,8 +56,9 @@ def _per_photon(c, entering):
     return per
 
 
-def count_stages(tracer, num, seed=1):
-    """Per-stage counts per entering photon for one chunk of num photons."""
+def count_stages(tracer, num, seed=1, batch=None):
+    """Per-stage counts per entering photon for one chunk of num photons, or
+    for a batch of point sources given as batch = (dirs, counts)."""
     from oracle import oracle as O
     from marx_amd import Photons
     lib = load_opcount()
@@ -85,8 +90,16 @@ def count_stages(tracer, num, seed=1):
             out["mirror_reflect"]["photons_in"] = hits
 
     n = ctypes.c_uint()
-    run("source", lambda: lib.orc_create_photons(ctypes.byref(ctx), ctypes.cast(ph.ptr, ctypes.c_void_p),
-                                                 num, ctypes.byref(n), None), num)
+    if batch is None:
+        run("source", lambda: lib.orc_create_photons(ctypes.byref(ctx), ctypes.cast(ph.ptr, ctypes.c_void_p),
+                                                     num, ctypes.byref(n), None), num)
+    else:
+        d = np.ascontiguousarray(batch[0], dtype=np.float64).reshape(-1)
+        c = np.ascontiguousarray(batch[1], dtype=np.uint32)
+        run("source", lambda: lib.orc_create_batch(
+            ctypes.byref(ctx), ctypes.cast(ph.ptr, ctypes.c_void_p), len(c),
+            d.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)),
+            0, 0.0, 1.0e4), int(c.sum()))
     alive = lambda: int(((ph.active["flags"] & 0xFF) == 0).sum())
     e = alive()
     run("mirror", lambda: lib.orc_mirror_reflect(ctypes.byref(ctx), ctypes.cast(ph.ptr, ctypes.c_void_p)), e)
@@ -104,10 +117,16 @@ def main():
     ap.add_argument("--json", default=None)
     ap.add_argument("--workload", default="hetg-acis-s")
     a = ap.parse_args()
-    from marx_amd.workloads import WORKLOADS
+    from marx_amd.workloads import WORKLOADS, batched_acis_i
     with tempfile.TemporaryDirectory() as tmp:
-        tracer = WORKLOADS[a.workload](tmp, photons=a.photons)
-        res = count_stages(tracer, a.photons)
+        if a.workload == "batched-acis-i":
+            # the batch's sources with their counts scaled to about a.photons
+            tracer, dirs, counts = batched_acis_i(tmp)
+            c = np.maximum(1, np.round(counts * (a.photons / counts.sum()))).astype(np.uint32)
+            res = count_stages(tracer, int(c.sum()), batch=(dirs, c))
+        else:
+            tracer = WORKLOADS[a.workload](tmp, photons=a.photons)
+            res = count_stages(tracer, a.photons)
     for st in ("source", "mirror", "mirror_enter", "mirror_reflect", "grating", "detector"):
         r = res[st]
         print(f"{st:14s} photons_in={r['photons_in']:8d} flops/photon={r['flops']:8.1f} "

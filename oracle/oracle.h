@@ -84,6 +84,10 @@ int orc_interpolate_n_fvector (const float *xp, float **yp, unsigned int n,
 /* Stage-major passes, in the order marx/src/marx.c:240-273 calls them. */
 int orc_create_photons (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int num,
                         unsigned int *num_collected, double *exposure_time);
+/* The photons of a batch of point sources (mxg_trace_batch). */
+int orc_create_batch (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int nsrc,
+                      const double *dirs, const unsigned int *counts, unsigned int tag_start,
+                      double start_time, double exposure);
 int orc_mirror_reflect (const orc_ctx *c, Marx_Photon_Type *pt);
 int orc_grating_diffract (const orc_ctx *c, Marx_Photon_Type *pt);
 int orc_detect (const orc_ctx *c, Marx_Photon_Type *pt);
@@ -98,6 +102,10 @@ void orc_prune_photons (Marx_Photon_Type *pt);
  * Returns the number of photons detected, -1 on error. */
 long orc_trace_parallel (const orc_ctx *c, unsigned int num, unsigned int chunk,
                          int nthreads);
+/* A batch of point sources (orc_create_batch), one OpenMP task per source. */
+long orc_trace_batch_parallel (const orc_ctx *c, unsigned int nsrc, const double *dirs,
+                               const unsigned int *counts, unsigned int tag_start,
+                               double exposure, int nthreads);
 
 #ifdef __cplusplus
 }

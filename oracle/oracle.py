@@ -63,6 +63,12 @@ def load():
     lib.orc_create_photons.argtypes = [ctypes.POINTER(OrcCtx), ctypes.c_void_p, ctypes.c_uint,
                                        u32p, f64p]
     lib.orc_create_photons.restype = ctypes.c_int
+    lib.orc_create_batch.argtypes = [ctypes.POINTER(OrcCtx), ctypes.c_void_p, ctypes.c_uint, f64p, u32p,
+                                     ctypes.c_uint, ctypes.c_double, ctypes.c_double]
+    lib.orc_create_batch.restype = ctypes.c_int
+    lib.orc_trace_batch_parallel.argtypes = [ctypes.POINTER(OrcCtx), ctypes.c_uint, f64p, u32p, ctypes.c_uint,
+                                             ctypes.c_double, ctypes.c_int]
+    lib.orc_trace_batch_parallel.restype = ctypes.c_long
     lib.orc_sort_photons.argtypes = [ctypes.c_void_p]
     lib.orc_sort_photons.restype = ctypes.c_int
     lib.orc_trace_parallel.argtypes = [ctypes.POINTER(OrcCtx), ctypes.c_uint, ctypes.c_uint, ctypes.c_int]
@@ -170,6 +176,22 @@ class Oracle:
         if st != 0:
             raise RuntimeError("oracle create_photons failed")
         return n.value
+
+    def create_batch(self, photons, dirs, counts, tag_start=0, start_time=0.0, exposure=1.0e4):
+        """The photons of a batch of point sources (mxg_trace_batch)."""
+        d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1)
+        c = np.ascontiguousarray(counts, dtype=np.uint32)
+        st = self.lib.orc_create_batch(ctypes.byref(self.ctx), ctypes.cast(photons.ptr, ctypes.c_void_p),
+                                       len(c), _arr(d, ctypes.c_double), _arr(c, ctypes.c_uint),
+                                       tag_start, start_time, exposure)
+        if st != 0:
+            raise RuntimeError("oracle create_batch failed")
+
+    def trace_batch_parallel(self, dirs, counts, tag_start=0, exposure=1.0e4, nthreads=0):
+        d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1)
+        c = np.ascontiguousarray(counts, dtype=np.uint32)
+        return self.lib.orc_trace_batch_parallel(ctypes.byref(self.ctx), len(c), _arr(d, ctypes.c_double),
+                                                 _arr(c, ctypes.c_uint), tag_start, exposure, nthreads)
 
     def trace_parallel(self, num, chunk=100000, nthreads=0):
         return self.lib.orc_trace_parallel(ctypes.byref(self.ctx), num, chunk, nthreads)

@@ -6,17 +6,21 @@
  *          (qsort of indices by energy, index tie-break)
  *   prune: marx/libsrc/photon.c:40-63
  */
+#ifndef _GNU_SOURCE
+# define _GNU_SOURCE   /* qsort_r */
+#endif
 #include <stdlib.h>
 #include <string.h>
 
 #include "oracle_int.h"
 
-static const double *Sort_Energies;
-
-static int energy_cmp (const void *a, const void *b)
+/* the energies travel with the sort (qsort_r), so that concurrent sorts on
+ * OpenMP threads do not share them */
+static int energy_cmp (const void *a, const void *b, void *energies)
 {
+   const double *en = (const double *) energies;
    unsigned int ia = *(const unsigned int *) a, ib = *(const unsigned int *) b;
-   double x = Sort_Energies[ia], y = Sort_Energies[ib];
+   double x = en[ia], y = en[ib];
    if (x > y) return 1;
    if (x < y) return -1;
    if (ia > ib) return 1;
@@ -43,8 +47,7 @@ int orc_sort_photons (Marx_Photon_Type *pt)
         e[i] = pt->attributes[i].energy;
         idx[i] = i;
      }
-   Sort_Energies = e;
-   qsort (idx, n, sizeof (unsigned int), energy_cmp);
+   qsort_r (idx, n, sizeof (unsigned int), energy_cmp, e);
    free (pt->sorted_index);
    pt->sorted_index = idx;
    if (pt->sorted_energies == NULL)

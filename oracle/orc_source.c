@@ -40,6 +40,36 @@ static orc_vec apply_dither (double ra, double dec, double roll, orc_vec p)
    return orc_v_rotate_unit_vector1 (p, n, cos_theta, sin_theta);
 }
 
+static double draw_energy (const orc_ctx *c, const mxg_source_cfg *s, orc_stream *st)
+{
+   if (s->spectrum == MXG_SPECTRUM_FLAT)
+     {
+        double emin = s->emin, de = s->emax - emin;
+        return emin + de * orc_uniform (st);
+     }
+   return orc_interpolate_d (orc_uniform (st), c->da + s->file_cum_off, c->da + s->file_e_off,
+                             s->file_n);
+}
+
+/* INTERNAL dither of one photon (dither.c:167-182, 551-581, 609-658) */
+static void dither_photon (const orc_ctx *c, const mxg_source_cfg *s, Marx_Photon_Attr_Type *at,
+                           double start_time)
+{
+   orc_stream st;
+   double t, ra, dec, roll, delta_ra, delta_dec;
+   Marx_Dither_Type *d = &at->dither_state;
+   orc_stream_init (&st, c->seed, at->tag, MXG_STAGE_DITHER, c->source_id);
+   t = start_time + at->arrival_time;
+   t = (2.0 * PI) * t;
+   ra = d->ra = (float) (s->ra_amp * sin (t / s->ra_period + s->ra_phase));
+   dec = d->dec = (float) (s->dec_amp * sin (t / s->dec_period + s->dec_phase));
+   roll = d->roll = (float) (s->nominal_roll + s->roll_amp * sin (t / s->roll_period + s->roll_phase));
+   d->dy = d->dz = d->dtheta = 0;
+   delta_ra = s->aspect_blur * orc_gaussian (&st);
+   delta_dec = s->aspect_blur * orc_gaussian (&st);
+   at->p = apply_dither (ra + delta_ra, dec + delta_dec, roll, at->p);
+}
+
 int orc_create_photons (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int num,
                         unsigned int *num_collected, double *exposure_time)
 {
@@ -63,16 +93,7 @@ int orc_create_photons (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int num
         orc_stream *st = streams + i;
         at[i].tag = pt->tag_start + i;
         orc_stream_init (st, c->seed, at[i].tag, MXG_STAGE_SOURCE, c->source_id);
-        if (s->spectrum == MXG_SPECTRUM_FLAT)
-          {
-             double emin = s->emin, de = s->emax - emin;
-             at[i].energy = emin + de * orc_uniform (st);
-          }
-        else
-          {
-             double r = orc_uniform (st);
-             at[i].energy = orc_interpolate_d (r, c->da + s->file_cum_off, c->da + s->file_e_off, s->file_n);
-          }
+        at[i].energy = draw_energy (c, s, st);
         at[i].p = orc_v (s->p[0], s->p[1], s->p[2]);
         if (s->type == MXG_SOURCE_DISK)
           {
@@ -117,25 +138,50 @@ int orc_create_photons (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int num
    /* dither (dither.c:609-658) */
    if ((n > 0) && (s->dither_mode == MXG_DITHER_INTERNAL))
      for (i = 0; i < n; i++)
-       {
-          orc_stream st;
-          double t, ra, dec, roll, delta_ra, delta_dec;
-          Marx_Dither_Type *d = &at[i].dither_state;
-          orc_stream_init (&st, c->seed, at[i].tag, MXG_STAGE_DITHER, c->source_id);
-          t = pt->start_time + at[i].arrival_time;
-          t = (2.0 * PI) * t;
-          ra = d->ra = (float) (s->ra_amp * sin (t / s->ra_period + s->ra_phase));
-          dec = d->dec = (float) (s->dec_amp * sin (t / s->dec_period + s->dec_phase));
-          roll = d->roll = (float) (s->nominal_roll + s->roll_amp * sin (t / s->roll_period + s->roll_phase));
-          d->dy = d->dz = d->dtheta = 0;
-          delta_ra = s->aspect_blur * orc_gaussian (&st);
-          delta_dec = s->aspect_blur * orc_gaussian (&st);
-          at[i].p = apply_dither (ra + delta_ra, dec + delta_dec, roll, at[i].p);
-       }
+       dither_photon (c, s, at + i, pt->start_time);
 
    if (-1 == orc_sort_photons (pt)) return -1;
    if (n == 0) pt->total_time = 0.0;
    else pt->total_time = (s->time_mode == MXG_TIME_UNIFORM) ? s->uniform_tmax : at[n - 1].arrival_time;
    *num_collected = n;
    return 0;
+}
+
+/* Batched point sources: the photons mxg_trace_batch creates.  Photon i of
+ * the batch (sources in order) has tag tag_start + i, the direction of its
+ * source, an energy from the configured spectrum, an arrival time uniform
+ * over [0, exposure) and the configured dither.  The list is energy-sorted
+ * for the stage passes like a created chunk. */
+int orc_create_batch (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int nsrc,
+                      const double *dirs, const unsigned int *counts, unsigned int tag_start,
+                      double start_time, double exposure)
+{
+   const mxg_source_cfg *s = &c->cfg->src;
+   Marx_Photon_Attr_Type *at = pt->attributes;
+   unsigned long long total = 0;
+   unsigned int i = 0, k, j;
+
+   for (k = 0; k < nsrc; k++) total += counts[k];
+   if (total > pt->max_n_photons) return -1;
+   memset (at, 0, total * sizeof (Marx_Photon_Attr_Type));
+   pt->source_distance = s->distance;
+   pt->history = 0;
+   pt->start_time = start_time;
+   pt->tag_start = tag_start;
+   for (k = 0; k < nsrc; k++)
+     for (j = 0; j < counts[k]; j++, i++)
+       {
+          orc_stream st;
+          at[i].tag = tag_start + i;
+          orc_stream_init (&st, c->seed, at[i].tag, MXG_STAGE_SOURCE, c->source_id);
+          at[i].energy = draw_energy (c, s, &st);
+          at[i].p = orc_v (dirs[3 * k], dirs[3 * k + 1], dirs[3 * k + 2]);
+          at[i].arrival_time = exposure * orc_uniform (&st);
+          if (s->dither_mode == MXG_DITHER_INTERNAL)
+            dither_photon (c, s, at + i, start_time);
+       }
+   pt->n_photons = (unsigned int) total;
+   pt->history |= (MARX_ENERGY_OK | MARX_TIME_OK | MARX_X_VECTOR_OK | MARX_P_VECTOR_OK | MARX_TAG_OK);
+   pt->total_time = exposure;
+   return orc_sort_photons (pt);
 }

@@ -69,3 +69,43 @@ long orc_trace_parallel (const orc_ctx *c, unsigned int num, unsigned int chunk,
      }
    return failed ? -1 : detected;
 }
+
+/* Batched point sources traced source by source, one OpenMP task per
+ * source (the reference runs one source per process).  Source k's photons
+ * get tags tag_start + off[k] ..., as in one batch pass.  Returns the
+ * number of photons detected, -1 on error. */
+long orc_trace_batch_parallel (const orc_ctx *c, unsigned int nsrc, const double *dirs,
+                               const unsigned int *counts, unsigned int tag_start,
+                               double exposure, int nthreads)
+{
+   unsigned int *off = (unsigned int *) malloc ((nsrc + 1) * sizeof (unsigned int));
+   long detected = 0;
+   int failed = 0;
+   unsigned int k;
+
+   if (off == NULL) return -1;
+   off[0] = 0;
+   for (k = 0; k < nsrc; k++) off[k + 1] = off[k] + counts[k];
+#ifdef _OPENMP
+   if (nthreads > 0) omp_set_num_threads (nthreads);
+#else
+   (void) nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic, 1) reduction(+:detected)
+   for (k = 0; k < nsrc; k++)
+     {
+        Marx_Photon_Type *pt = alloc_list (counts[k] ? counts[k] : 1);
+        if (pt == NULL) { failed = 1; continue; }
+        if ((0 != orc_create_batch (c, pt, 1, dirs + 3 * k, counts + k, tag_start + off[k], 0.0,
+                                    exposure))
+            || (0 != orc_mirror_reflect (c, pt))
+            || (0 != orc_grating_diffract (c, pt))
+            || (0 != orc_detect (c, pt)))
+          failed = 1;
+        orc_prune_photons (pt);
+        detected += pt->num_sorted;
+        free_list (pt);
+     }
+   free (off);
+   return failed ? -1 : detected;
+}

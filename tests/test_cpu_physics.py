@@ -159,3 +159,19 @@ def test_opcount_build_computes_what_the_oracle_computes():
     for fn in (opc.orc_mirror_reflect, opc.orc_grating_diffract, opc.orc_detect):
         assert fn(ctypes.byref(ctx), ctypes.cast(b.ptr, ctypes.c_void_p)) == 0
     assert a.active.tobytes() == b.active.tobytes()
+
+
+def test_parallel_oracle_is_independent_of_thread_count():
+    """The CPU baseline's OpenMP traces (chunks, and one task per source of a
+    batch) give the same detections on 1 and 8 threads: nothing is shared
+    between concurrently traced chunks."""
+    import tempfile
+    from marx_amd.workloads import batched_acis_i, hetg_acis_s
+    with tempfile.TemporaryDirectory() as tmp:
+        t = hetg_acis_s(tmp, photons=1e6)
+        o = Oracle(t)
+        assert o.trace_parallel(200000, 20000, 1) == o.trace_parallel(200000, 20000, 8)
+        t, dirs, counts = batched_acis_i(tmp, num_sources=64, max_count=3000)
+        o = Oracle(t)
+        one = o.trace_batch_parallel(dirs, counts, 0, 1.0e4, 1)
+        assert one > 0 and one == o.trace_batch_parallel(dirs, counts, 0, 1.0e4, 8)

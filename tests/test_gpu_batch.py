@@ -1,0 +1,128 @@
+"""Batched point sources (mxg_trace_batch): one device pass over many sources
+with uneven photon counts, against the CPU oracle; the same photons as a
+per-source loop; independent of the launch schedule; HIP-graph replays equal
+to eager passes; the autotuner installs a valid schedule."""
+import numpy as np
+import pytest
+
+from marx_amd import Photons, Tracer, offaxis_directions
+from marx_amd.spectra import write_power_law
+from oracle.oracle import Oracle
+from tests.parity import compare
+
+ON_AXIS = dict(SourceRA=250.2134679741175, SourceDEC=-53.75743813458669)
+
+
+def _batch(tmp_path, nsrc, lo=100, hi=5000, seed=41, dither="NONE", grating="NONE", det="ACIS-I"):
+    spec = write_power_law(str(tmp_path / "pl.dat"), gamma=2.0, emin=0.3, emax=10.0)
+    t = Tracer(seed=seed, GratingType=grating, DetectorType=det, SourceType="POINT", SpectrumType="FILE",
+               SpectrumFile=spec, DitherModel=dither, **ON_AXIS)
+    t.init_source()
+    rng = np.random.default_rng(nsrc)
+    dirs = offaxis_directions(np.radians(rng.uniform(0, 15, nsrc) / 60), rng.uniform(0, 2 * np.pi, nsrc))
+    counts = np.round(10 ** rng.uniform(np.log10(lo), np.log10(hi), nsrc)).astype(np.uint32)
+    return t, dirs, counts
+
+
+def _download(t, n):
+    p = Photons(n)
+    t.download(p)
+    return p.active.copy()
+
+
+def assert_same_records(a, b, what):
+    """Bit-identical photon records, field by field (numpy copies of this
+    dtype do not carry its padding bytes); on failure names the fields and
+    the first differing photon."""
+    assert len(a) == len(b), what
+    bad = {}
+    for f in a.dtype.names:
+        x = np.ascontiguousarray(a[f]).view(np.uint8).reshape(len(a), -1)
+        y = np.ascontiguousarray(b[f]).view(np.uint8).reshape(len(b), -1)
+        i = np.nonzero(np.any(x != y, axis=1))[0]
+        if len(i):
+            bad[f] = (len(i), int(i[0]))
+    assert not bad, f"{what}: records differ (field: (photons, first index)) {bad}"
+
+
+def _oracle(t, dirs, counts, tag_start, start_time, exposure, grating):
+    po = Photons(int(counts.sum()))
+    o = Oracle(t)
+    o.create_batch(po, dirs, counts, tag_start, start_time, exposure)
+    o.reflect(po)
+    if grating != "NONE":
+        o.diffract(po)
+    o.detect(po)
+    return po.active.copy()         # the view would dangle once po is freed
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("grating,det,dither", [("NONE", "ACIS-I", "NONE"), ("HETG", "ACIS-S", "INTERNAL")])
+def test_batch_parity(gpu, tmp_path, grating, det, dither):
+    t, dirs, counts = _batch(tmp_path, 48, grating=grating, det=det, dither=dither)
+    n = int(counts.sum())
+    ndet, ms = t.trace_batch(dirs, counts, tag_start=1000, start_time=500.0, exposure=3.0e4,
+                             use_graph=False)
+    g = _download(t, n)
+    o = _oracle(t, dirs, counts, 1000, 500.0, 3.0e4, grating)
+    compare(g, o, max_flips=2, label=f"batch {grating}/{det}")
+    assert ndet == int(((g["flags"] & 0xFF) == 0).sum())
+    assert t.stage_counts()[0] == n
+
+
+@pytest.mark.gpu
+def test_per_source_loop_gives_the_same_photons(gpu, tmp_path):
+    t, dirs, counts = _batch(tmp_path, 12)
+    n = int(counts.sum())
+    t.trace_batch(dirs, counts, tag_start=77, exposure=1.0e4, use_graph=False)
+    whole = _download(t, n)
+    parts, off = [], 77
+    for k in range(len(counts)):
+        t.trace_batch(dirs[k:k + 1], counts[k:k + 1], tag_start=off, exposure=1.0e4, use_graph=False)
+        parts.append(_download(t, int(counts[k])))
+        off += int(counts[k])
+    assert_same_records(np.concatenate(parts), whole, "per-source loop vs batch")
+
+
+@pytest.mark.gpu
+def test_batch_is_independent_of_the_schedule(gpu, tmp_path):
+    t, dirs, counts = _batch(tmp_path, 64)
+    n = int(counts.sum())
+    outs = []
+    for bpc, waves in ((2, 2), (16, 4), (5, 3)):
+        t.set_schedule(blocks_per_cu=bpc, reflect_waves=waves)
+        t.trace_batch(dirs, counts, tag_start=5, exposure=1.0e4, use_graph=False)
+        outs.append(_download(t, n))
+    t.set_schedule(blocks_per_cu=8, reflect_waves=4)
+    assert_same_records(outs[0], outs[1], "schedule (2, 2) vs (16, 4)")
+    assert_same_records(outs[0], outs[2], "schedule (2, 2) vs (5, 3)")
+
+
+@pytest.mark.gpu
+def test_graph_replays_equal_eager_passes(gpu, tmp_path):
+    t, dirs, counts = _batch(tmp_path, 40)
+    n = int(counts.sum())
+    for rep, tag in enumerate((0, 123456, 0, 999)):
+        ndet_g, _ = t.trace_batch(dirs, counts, tag_start=tag, exposure=2.0e4, use_graph=True)
+        g = _download(t, n)
+        ndet_e, _ = t.trace_batch(dirs, counts, tag_start=tag, exposure=2.0e4, use_graph=False)
+        e = _download(t, n)
+        assert ndet_g == ndet_e, f"replay {rep}"
+        assert_same_records(g, e, f"replay {rep} (tag_start {tag})")
+        assert t.stage_counts()[3] == ndet_e
+
+
+@pytest.mark.gpu
+def test_autotuner_installs_a_valid_schedule(gpu, tmp_path):
+    t, dirs, counts = _batch(tmp_path, 32)
+    n = int(counts.sum())
+    t.trace_batch(dirs, counts, use_graph=False)
+    ref = _download(t, n)
+    sched, ms = t.autotune_batch(dirs, counts, repeats=1)
+    assert sched == t.get_schedule()
+    assert all(1 <= sched[k] <= 32 for k in ("source", "enter", "reflect", "grating", "detect"))
+    assert sched["reflect_waves"] in (2, 3, 4)
+    assert ms > 0
+    t.trace_batch(dirs, counts, use_graph=True)
+    assert_same_records(_download(t, n), ref, "autotuned graph pass")
+    t.set_schedule(blocks_per_cu=8, reflect_waves=4)
