@@ -8,7 +8,9 @@ source with a photon-index-2 power law over 0.4-8 keV, INTERNAL dither
 (16 arcsec, 1000 / 707 s), HETG and ACIS-S, 50 ks of arrivals;
 --config letg-hrc-s is 5e7 photons of a 2 arcmin disk, flat 0.07-10 keV,
 LETG and HRC-S.  `value` is photons created and traced per second of wall
-time over the timed steps.
+time over the timed steps.  --config batched-acis-i traces 4096 point
+sources in one HIP-graph-launched pass; --config diff times gradient
+evaluations of the differentiable mirror (gradient evals/s).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME] [--photons N]
 """
@@ -38,6 +40,12 @@ FLOPS_PER_PHOTON = {
                    "grating": 1422.2, "detector": 278.0},
     "batched-acis-i": {"source": 11.0, "k_hrma_enter": 130.2, "k_hrma_reflect": 576.9,
                        "grating": 0.0, "detector": 172.4},
+    # per photon of the pass (`--workload diff`): the weighted forward pass
+    # with the tilt tangents (a source at infinity), split into its part up
+    # to the paraboloid intersection (k_diff_enter, values only, every
+    # photon) and the rest (k_diff_forward, the photons that get there), the
+    # oracle's finite-difference reflectivity slope left out; and the adjoint
+    "diff": {"k_diff_enter": 122.7, "k_diff_forward": 1804.2 - 122.7, "k_diff_adjoint": 42.7},
 }
 TRANSCENDENTALS_PER_PHOTON = {
     "hetg-acis-s": {"source": 11.0, "k_hrma_enter": 4.00, "k_hrma_reflect": 13.62,
@@ -46,6 +54,7 @@ TRANSCENDENTALS_PER_PHOTON = {
                    "grating": 13.69, "detector": 0.64},
     "batched-acis-i": {"source": 0.0, "k_hrma_enter": 4.00, "k_hrma_reflect": 13.11,
                        "grating": 0.0, "detector": 0.17},
+    "diff": {"k_diff_enter": 4.00, "k_diff_forward": 11.91 - 4.00, "k_diff_adjoint": 0.0},
 }
 # Algorithmic HBM bytes per photon entering each kernel: the structure-of-
 # arrays columns it must read and write, plus the active-list index in and
@@ -58,6 +67,12 @@ BYTES_PER_PHOTON = {
     "k_hrma_reflect": (4 + 4 + 1 + 4 + 4 + 24 + 24 + 8) + (24 + 24 + 4 + 4),
     "grating": (4 + 4 + 4 + 1 + 8 + 24 + 24 + 1) + (24 + 24 + 4 + 1 + 4),
     "detector": (4 + 4 + 4 + 8 + 8 + 24 + 24) + (24 + 24 + 4 + 1 + 4 + 4 + 4),
+    # per photon entering: in: list index; out: w, y, z, their 3 x 5
+    # derivatives, shell (the image stays in L2 / LDS)
+    "k_diff_forward": 4 + 8 * (3 + 15) + 1,
+    # per photon of weight != 0 (the others: their weight, 8 B): in: y, z,
+    # the 15 derivatives, shell
+    "k_diff_adjoint": 8 * (2 + 15) + 1,
 }
 # HBM bytes per photon entering a kernel, measured with rocprofv3 counters
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per dispatch, the factors that make a
@@ -78,7 +93,14 @@ DESCRIPTION["batched-acis-i"] = (
     "photon counts log-uniform over 1e2-1e5 per source, power law index 2 over 0.3-10 keV, HRMA + "
     "ACIS-I, arrival times uniform over the exposure, all sources in one HIP-graph-launched pass "
     "with the autotuned schedule")
-DEFAULT_PHOTONS = {"hetg-acis-s": 10_000_000, "letg-hrc-s": 50_000_000, "batched-acis-i": 0}
+DESCRIPTION["diff"] = (
+    "Differentiable: 20 misalignment parameters (tilt az/el, decenter y/z, despace x of the 4 shell "
+    "pairs) drawn from N(0, 1 arcsec / 10 um), 1e7 survival-weighted photons of an on-axis point "
+    "source at 1.49 keV through the mirror, loss = squared difference of FP32-accumulated 256x256 "
+    "focal-plane images (0.004 mm pixels) against the image at perturbed parameters")
+DEFAULT_PHOTONS = {"hetg-acis-s": 10_000_000, "letg-hrc-s": 50_000_000, "batched-acis-i": 0,
+                   "diff": 10_000_000}
+DIFF_NPIX, DIFF_PIXEL_MM = 256, 0.004
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
 STAGES = ["source", "mirror", "grating", "detector"]
@@ -205,6 +227,166 @@ def measure_batched(args, tmp, torch):
     return out
 
 
+def measure_diff(args, torch):
+    """Differentiable mirror: one step is one gradient evaluation through the
+    torch.autograd.Function (weighted forward pass with tangents, FP32 loss,
+    loss.backward() running the adjoint kernel).  Beside it: the plain
+    alternative on the same GPU, a central-difference gradient from 40
+    value-only forward passes (mxg_diff_image), with its error against the
+    adjoint gradient; and the FP64 CPU oracle (forward and adjoint, OpenMP)
+    on a sample of the photons."""
+    from marx_amd.diff import NUM_PARAMS, DiffMirror
+    from marx_amd.workloads import diff_mirror
+    cfg, K = "diff", args.steps
+    N = args.photons or DEFAULT_PHOTONS[cfg]
+    progress = lambda msg: print(f"bench diff: {msg}", file=sys.stderr, flush=True)
+    tracer, p0, pt = diff_mirror()
+    dm = DiffMirror(tracer, photons=N, tag_start=0, npix=DIFF_NPIX, pixel_size=DIFF_PIXEL_MM)
+    with torch.no_grad():
+        target = dm.image(torch.as_tensor(pt, dtype=torch.float64)).clone()
+
+    def step():
+        params = torch.tensor(p0, dtype=torch.float64, requires_grad=True)
+        loss = ((dm.image(params) - target) ** 2).sum()
+        loss.backward()
+        return params.grad, loss.detach()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    fwd_ms = adj_ms = ent_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(K):
+        grad, loss = step()
+        f, a, e, listed = dm.kernel_ms()
+        fwd_ms += f
+        adj_ms += a
+        ent_ms += e
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = K / elapsed
+    grad = grad.detach().numpy().copy()
+    loss = float(loss)
+    weighted = int((dm.state(tangents=False)["w"] != 0.0).sum())
+    progress(f"{K} gradient evaluations: {elapsed / K * 1e3:.2f} ms each")
+
+    # the plain loop: central differences of value-only passes
+    img = torch.empty((DIFF_NPIX, DIFF_NPIX), dtype=torch.float32, device="cuda")
+    t64 = target.double()
+
+    def loss_at(p):
+        dm.forward_into(p, img, tangents=False)
+        return float(((img.double() - t64) ** 2).sum())
+
+    steps_fd = np.where(np.arange(NUM_PARAMS) % 5 < 2, 1e-2, 1e-4)    # arcsec, mm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fd = np.zeros(NUM_PARAMS)
+    for j in range(NUM_PARAMS):
+        dp = np.zeros(NUM_PARAMS)
+        dp[j] = steps_fd[j]
+        fd[j] = (loss_at(p0 + dp) - loss_at(p0 - dp)) / (2 * steps_fd[j])
+    torch.cuda.synchronize()
+    fd_s = time.perf_counter() - t0
+    value_ms, _, value_enter_ms, _ = dm.kernel_ms()
+    progress(f"finite-difference gradient (40 value-only passes): {fd_s * 1e3:.1f} ms")
+
+    # per-kernel figures: algorithmic FLOPs per photon of the pass x photons
+    # of the pass; bytes per photon entering x photons entering
+    kernels = {}
+    byt = {"k_diff_enter": None,
+           "k_diff_forward": BYTES_PER_PHOTON["k_diff_forward"] * listed,
+           "k_diff_adjoint": 8 * N + BYTES_PER_PHOTON["k_diff_adjoint"] * weighted}
+    for name, ms_k, n_in in (("k_diff_enter", ent_ms / K, N), ("k_diff_forward", fwd_ms / K, listed),
+                             ("k_diff_adjoint", adj_ms / K, N)):
+        t = ms_k * 1e-3
+        kernels[name] = {"ms": ms_k, "photons_in": n_in,
+                         "flops_per_pass_photon": FLOPS_PER_PHOTON[cfg][name],
+                         "tflops": FLOPS_PER_PHOTON[cfg][name] * N / t / 1e12 if t > 0 else None,
+                         "gbs": byt[name] / t / 1e9 if (t > 0 and byt[name]) else None}
+    kernels["k_diff_enter"]["note"] = "with the list compaction after it"
+    kernels["value_only_pass"] = {"ms": value_enter_ms + value_ms, "k_diff_forward<0>_ms": value_ms,
+                                  "k_diff_enter_ms": value_enter_ms}
+    fk = kernels["k_diff_forward"]
+    rf = {"bound": "mfma", "pipe": "fp64 vector ALU (no matrix work on this path; 78.6 TFLOP/s is MI355X's "
+                                   "FP64 peak)",
+          "kernel": "k_diff_forward<2>", "achieved": fk["tflops"], "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+          "frac": fk["tflops"] / PEAK_FP64_TFLOPS, "traffic": None,
+          "flops_per_pass_photon": FLOPS_PER_PHOTON[cfg]["k_diff_forward"],
+          "transcendentals_per_pass_photon": TRANSCENDENTALS_PER_PHOTON[cfg]["k_diff_forward"],
+          "pass_photons": N, "photons_per_launch": listed, "launch_ms": fk["ms"],
+          "hbm": {"achieved": fk["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": fk["gbs"] / PEAK_HBM_GBS,
+                  "bytes_per_photon": BYTES_PER_PHOTON["k_diff_forward"]}}
+    adj = kernels["k_diff_adjoint"]
+    out = {
+        "metric": "photons/s traced (HRMA+grating+detector) 1 GPU vs host CPU; gradient evals/s",
+        "value": value, "unit": "gradient evals/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": elapsed / K * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": DESCRIPTION[cfg], "name": cfg, "photons_per_eval": N, "params": NUM_PARAMS,
+                   "image": f"{DIFF_NPIX}x{DIFF_NPIX} FP32, {DIFF_PIXEL_MM} mm pixels",
+                   "total_weight": dm.total_weight, "loss": loss},
+        "photons_per_s": value * N,
+        "roofline": rf,
+        "adjoint_hbm": {"achieved": adj["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": adj["gbs"] / PEAK_HBM_GBS if adj["gbs"] else None,
+                        "bytes": byt["k_diff_adjoint"], "weighted_photons": weighted},
+        "kernels": kernels,
+        "fd_loop": {"value": 1.0 / fd_s, "unit": "gradient evals/s", "passes": 2 * NUM_PARAMS,
+                    "steps": "1e-2 arcsec, 1e-4 mm", "seconds": fd_s,
+                    "max_abs_err_over_max_grad": float(np.abs(fd - grad).max() / np.abs(grad).max())},
+        "gradient_over_fd_loop": value * fd_s,
+    }
+    if not args.no_cpu:
+        progress("CPU baseline")
+        out["cpu_baseline"] = cpu_baseline_diff(tracer, p0, target.cpu().numpy(), N)
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    return out
+
+
+def cpu_baseline_diff(tracer, params, target, photons, target_seconds=20.0):
+    """The FP64 oracle's forward and adjoint passes (OpenMP over the host
+    cores) over a sample of the photons, scaled to gradient evaluations of
+    `photons` photons per second.  Two probes size the sample to about
+    target_seconds: a part of the photons, or whole evaluations repeated."""
+    from marx_amd._lib import ImageSpec
+    from marx_amd.diff import setup
+    from oracle.oracle import Oracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    o = Oracle(tracer)
+    spec = ImageSpec(DIFF_NPIX, DIFF_PIXEL_MM, 0.0, 0.0)
+    d = setup(params)
+    t64 = np.asarray(target, dtype=np.float64)
+
+    def timed(n):
+        t0 = time.perf_counter()
+        img, st = o.diff_forward(d, n, 0, spec, nthreads=threads)
+        o.diff_adjoint(st, spec, 2.0 * (img - t64), nthreads=threads)
+        return time.perf_counter() - t0
+
+    n = 20_000 * threads
+    for target_s in (0.5, 2.0):
+        dt = timed(n)
+        rate = n / max(dt, 1e-3)
+        n = int(min(max(rate * target_s, 1000 * threads), photons))
+    if rate * target_seconds < photons:
+        sample = int(max(rate * target_seconds, 1000 * threads))
+        dt = timed(sample)
+        what = f"the first {sample} of the {photons} photons"
+        evals = sample / photons
+    else:
+        reps = int(max(1, min(5, rate * target_seconds // photons)))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            timed(photons)
+        dt = time.perf_counter() - t0
+        what = f"{reps} whole evaluation(s) of the {photons} photons"
+        evals = reps
+    return {"value": evals / dt, "unit": "gradient evals/s", "cores": threads, "kind": "port",
+            "sample": f"{what} (FP64 CPU oracle forward with tangents and adjoint, {threads} OpenMP "
+                      f"threads), {dt:.1f} s"}
+
+
 def cpu_baseline_batch(tracer, dirs, counts, exposure, target_seconds=20.0):
     """The FP64 oracle source by source (one OpenMP task per source) over the
     first sources of the batch, as many as take about target_seconds."""
@@ -286,6 +468,8 @@ def measure():
     tmp = tempfile.mkdtemp(prefix="marx_bench_")
     if args.config == "batched-acis-i":
         return measure_batched(args, tmp, torch)
+    if args.config == "diff":
+        return measure_diff(args, torch)
     cfg = args.config
     N, T = (args.photons or DEFAULT_PHOTONS[cfg]), args.exposure
     tracer = WORKLOADS[cfg](tmp, photons=N, exposure=T)

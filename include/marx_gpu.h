@@ -250,6 +250,68 @@ int mxg_last_stage_counts (unsigned int counts[4]);
  * detector (0 for a stage not run or a graph-launched pass). */
 int mxg_last_stage_ms (float ms[4]);
 
+/* ---- differentiable mirror (no counterpart in the reference) ---- */
+
+/* A focal-plane image: npix x npix pixels of pixel_size mm centred on
+ * (center_y, center_z) mm at x = 0 (the HRMA focus). */
+typedef struct
+{
+   unsigned int npix;
+   double pixel_size;
+   double center_y, center_z;
+}
+mxg_image_spec;
+
+/* Fills *d with the initialised mirror's shell geometry at misalignment
+ * params[20] (MXG_DIFF_NPAR per shell pair, see mxg_diff_cfg) and its
+ * derivatives.  Host only. */
+int mxg_diff_setup (const double *params, void *d /* mxg_diff_cfg */);
+
+/* Weighted-photon forward pass: num photons (tags tag_start...) of the
+ * initialised source (its energies and direction) go through the mirror at
+ * misalignment params[20], with survival weights in place of the vignetting
+ * and reflectivity draws (vignetting x reflectivity x scale factor per
+ * surface; a photon that misses a surface, hits a strut or scatters past
+ * pi/4 gets weight 0), and are projected to the focal plane.  Each photon's
+ * weight is spread bilinearly over the four pixels around it and summed in
+ * FP32 into image (a device pointer of spec->npix^2 floats, overwritten).
+ * The photons' weights, positions and their derivatives with respect to
+ * their shell pair's parameters stay in HBM for mxg_diff_adjoint.
+ * *total_weight, when non-NULL, receives the summed weight. */
+int mxg_diff_forward (const double *params, unsigned int num, unsigned int tag_start,
+                      const mxg_image_spec *spec, float *image, double *total_weight);
+
+/* The same pass without the derivatives: the same image, weights and
+ * positions (mxg_diff_state's w, y, z and shell) at a fraction of the cost,
+ * for evaluations that need no gradient (targets, line searches, finite
+ * differences).  mxg_diff_adjoint refuses to follow it. */
+int mxg_diff_image (const double *params, unsigned int num, unsigned int tag_start,
+                    const mxg_image_spec *spec, float *image, double *total_weight);
+
+/* Adjoint of the last mxg_diff_forward: grad[20] = d(sum_pix grad_image *
+ * image)/d(params), i.e. the gradient of a loss whose derivative with
+ * respect to the image is grad_image (device pointer, npix^2 floats).
+ * Sums per-block partials in a fixed order (deterministic). */
+int mxg_diff_adjoint (const float *grad_image, double *grad);
+
+/* ms[0..2]: device time in ms of the last mxg_diff_forward's or
+ * mxg_diff_image's main kernel (both reflections, focal plane, image), of
+ * the last mxg_diff_adjoint's kernel, and of the last forward pass's first
+ * kernel (every photon to its paraboloid intersection) with the list
+ * compaction after it; HIP events on the library's stream around each
+ * launch, 0 before the first.  *listed (may be NULL): the photons of the
+ * last forward pass that reached the paraboloid, which the main kernel ran
+ * on. */
+int mxg_diff_kernel_ms (float *ms, unsigned int *listed);
+
+/* Host copies of the per-photon state of the last mxg_diff_forward (n =
+ * its photon count): weight w, focal-plane position y, z, and their
+ * derivatives dw, dy, dz (MXG_DIFF_NPAR x n, parameter-major) by the
+ * photon's shell-pair parameters; shell (0..3) of each photon.  Any
+ * pointer may be NULL. */
+int mxg_diff_state (unsigned int n, double *w, double *y, double *z, double *dw, double *dy,
+                    double *dz, unsigned char *shell);
+
 /* ---- batched point sources (no counterpart in the reference, which runs
  * one source per process) ---- */
 

@@ -48,6 +48,7 @@ typedef struct
    double fwd[2][9], bwd[2][9]; /* tilt matrices, row major */
    double blur[2];              /* arcsec */
    double scat_factor[2];
+   double tilt_deg[2][2];       /* [surface][az, el] misalignment (geometry file), degrees */
    double front_position;
    double area_fraction;        /* cumulative, normalised */
    double min_radius, max_radius;
@@ -217,6 +218,28 @@ mxg_config;
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+/* Differentiable mirror: the shell geometry at given misalignment
+ * parameters, with the derivatives of each piece with respect to them.
+ * Parameters per shell pair, in this order: tilt az, tilt el (arcsec, added
+ * to both surfaces' yaw / pitch misalignment), decenter y, decenter z,
+ * despace x (mm: the pair moves along MARX +y, +z, +x). */
+#define MXG_DIFF_NPAR 5
+typedef struct
+{
+   double fwd[2][9], bwd[2][9];              /* tilt matrices at the parameters */
+   double dfwd[2][2][9], dbwd[2][2][9];      /* [surface][d/d az, d/d el per arcsec] */
+   double to_osac[2][3];                     /* d/d(decenter y, z, despace x) = -1 */
+   double front_position;                    /* d/d despace x = +1 */
+}
+mxg_diff_shell;
+
+typedef struct
+{
+   double params[MXG_NUM_SHELLS * MXG_DIFF_NPAR];
+   mxg_diff_shell shell[MXG_NUM_SHELLS];
+}
+mxg_diff_cfg;
 
 /* The library's current configuration and arenas (host copies). */
 int mxg_get_config (const mxg_config **cfg,

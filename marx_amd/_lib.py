@@ -88,6 +88,28 @@ class Schedule(ctypes.Structure):
         return d
 
 
+class ImageSpec(ctypes.Structure):
+    """mxg_image_spec: npix x npix pixels of pixel_size mm centred on
+    (center_y, center_z) mm in the focal plane."""
+    _fields_ = [("npix", ctypes.c_uint), ("pixel_size", ctypes.c_double),
+                ("center_y", ctypes.c_double), ("center_z", ctypes.c_double)]
+
+
+class DiffShell(ctypes.Structure):
+    """mxg_diff_shell"""
+    _fields_ = [("fwd", (ctypes.c_double * 9) * 2), ("bwd", (ctypes.c_double * 9) * 2),
+                ("dfwd", ((ctypes.c_double * 9) * 2) * 2), ("dbwd", ((ctypes.c_double * 9) * 2) * 2),
+                ("to_osac", (ctypes.c_double * 3) * 2), ("front_position", ctypes.c_double)]
+
+
+DIFF_NPAR = 5
+
+
+class DiffCfg(ctypes.Structure):
+    """mxg_diff_cfg: the shell geometry at given misalignment parameters."""
+    _fields_ = [("params", ctypes.c_double * (4 * DIFF_NPAR)), ("shell", DiffShell * 4)]
+
+
 PPhotonList = ctypes.POINTER(PhotonList)
 
 # (name, restype, argtypes) of every exported C-ABI symbol of include/marx_gpu.h
@@ -130,6 +152,17 @@ _SIGNATURES = [
                                        ctypes.c_uint, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_float)]),
     ("mxg_set_schedule", ctypes.c_int, [ctypes.POINTER(Schedule)]),
+    ("mxg_diff_setup", ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]),
+    ("mxg_diff_forward", ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.c_uint, ctypes.c_uint,
+                                        ctypes.POINTER(ImageSpec), ctypes.c_void_p,
+                                        ctypes.POINTER(ctypes.c_double)]),
+    ("mxg_diff_image", ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.c_uint, ctypes.c_uint,
+                                      ctypes.POINTER(ImageSpec), ctypes.c_void_p,
+                                      ctypes.POINTER(ctypes.c_double)]),
+    ("mxg_diff_kernel_ms", ctypes.c_int, [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint)]),
+    ("mxg_diff_adjoint", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    ("mxg_diff_state", ctypes.c_int, [ctypes.c_uint] + [ctypes.POINTER(ctypes.c_double)] * 6
+                                      + [ctypes.POINTER(ctypes.c_ubyte)]),
     ("mxg_get_schedule", ctypes.c_int, [ctypes.POINTER(Schedule)]),
     ("mxg_autotune_batch", ctypes.c_int, [ctypes.c_uint, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_uint), ctypes.c_uint, ctypes.c_double,

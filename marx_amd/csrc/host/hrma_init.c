@@ -198,6 +198,8 @@ int mxg_hrma_init (Param_File_Type *pf)
              s->to_osac[k][2] = -(-g[1]);
              s->to_osac[k][0] -= cap_pos;
              yaw_pitch_matrices (g[6], g[7], s->fwd[k], s->bwd[k]);
+             s->tilt_deg[k][0] = g[6];
+             s->tilt_deg[k][1] = g[7];
           }
         s->front_position = cap_pos - geom[i][0][2] + s->conic_xmax[0];
      }

@@ -98,6 +98,25 @@ int mxgd_graph_end (mxgd_graph **g);
 int mxgd_graph_launch (mxgd_graph *g, float *ms);
 void mxgd_graph_destroy (mxgd_graph *g);
 
+/* ---- differentiable mirror ---- */
+
+/* Weighted forward pass over n photons with the shell geometry d; image is
+ * a device pointer of spec->npix^2 floats; keeps the per-photon weights and
+ * positions, and with tangents != 0 their derivatives.  Synchronises before
+ * returning. */
+int mxgd_diff_forward (const mxg_diff_cfg *d, unsigned int n, unsigned int tag_start,
+                       unsigned long long seed, const mxg_image_spec *spec, float *image,
+                       double *total_weight, int tangents);
+/* grad[MXG_NUM_SHELLS * MXG_DIFF_NPAR] of the last forward pass contracted
+ * with g (device, npix^2 floats).  Synchronises before returning. */
+int mxgd_diff_adjoint (const float *g, double *grad);
+/* Device times (ms) of the last k_diff_forward, k_diff_adjoint, and
+ * k_diff_enter + its list compaction; photons k_diff_forward ran on. */
+void mxgd_diff_kernel_ms (float ms[3], unsigned int *listed);
+/* Host copies of the per-photon state of the last forward pass. */
+int mxgd_diff_state (unsigned int n, double *w, double *y, double *z, double *dw, double *dy,
+                     double *dz, unsigned char *shell);
+
 /* Number of photons in the active list (device -> host copy). */
 int mxgd_active_count (unsigned int *n);
 /* Active list indices (host copy, n = mxgd_active_count). */

@@ -436,4 +436,45 @@ __device__ __forceinline__ double wfold_interp (const mxg_wfold &w, const float 
    return theta_0 + (theta_1 - theta_0) * (e_alpha - e0) / (e1 - e0);
 }
 
+// ---- shared by the mirror and source kernels --------------------------------------
+
+// precollimator / cap / post-collimator strut shadows (hrma.c:900-981)
+__device__ __forceinline__ bool in_strut_shadow (const mxg_hrma_cfg &h, vec3 x, vec3 p,
+                                                 const mxg_strut *s)
+{
+   const double cos_theta = h.strut_cos, sin_theta = h.strut_sin;
+   for (int k = 0; k < 2; k++)
+     {
+        double half_width = s[k].half_width;
+        double t = (s[k].xpos + h.cap_position - x.x) / p.x;
+        double y = x.y + p.y * t;
+        double z = x.z + p.z * t;
+        for (int i = 0; i < 3; i++)
+          {
+             if (i != 0)
+               {
+                  double tmp = cos_theta * y - sin_theta * z;
+                  z = sin_theta * y + cos_theta * z;
+                  y = tmp;
+               }
+             if (((-half_width < y) && (y < half_width)) || ((-half_width < z) && (z < half_width)))
+               return true;
+          }
+     }
+   return false;
+}
+
+// energy of the FLAT or FILE spectrum (spectrum.c:138-147, 184-189)
+__device__ __forceinline__ double draw_energy (const mxg_source_cfg &src, const double *__restrict__ da,
+                                               Stream &st)
+{
+   if (src.spectrum == MXG_SPECTRUM_FLAT)
+     {
+        double emin = src.emin, de = src.emax - emin;
+        return emin + de * uniform (st);
+     }
+   double r = uniform (st);
+   return interp_d (r, da + src.file_cum_off, da + src.file_e_off, src.file_n);
+}
+
 } // namespace mxg

@@ -156,4 +156,22 @@ hipError_t launch_batch_source (const mxg_config *cfg, const double *da, Soa s, 
                                 unsigned long long seed, uint32_t *idx, uint32_t *count,
                                 uint32_t grid, hipStream_t st);
 
+// The differentiable mirror (diff.hip).
+struct DiffOut
+{
+   double *w, *y, *z;           // n values each
+   double *dw, *dy, *dz;        // MXG_DIFF_NPAR x n derivatives each
+   uint8_t *shell;
+   double *block_w;             // per-block summed weight
+};
+hipError_t launch_diff_forward (const mxg_config *cfg, const float *fa, const double *da,
+                                const mxg_diff_cfg *dcfg, uint32_t n, uint32_t tag_start,
+                                unsigned long long seed, const mxg_image_spec *spec, float *image,
+                                float *win_partial, float *slice_sum, DiffOut out, const ListIO &enter,
+                                uint32_t grid, double *wsum_dev, int tangents, hipStream_t st,
+                                hipEvent_t *ev);
+hipError_t launch_diff_adjoint (uint32_t n, DiffOut in, const mxg_image_spec *spec, const float *g,
+                                double *partial, uint32_t grid, double *grad_dev, hipStream_t st,
+                                hipEvent_t *ev);
+
 } // namespace mxg

@@ -56,6 +56,25 @@ uint32_t *d_batch_off = nullptr;   // nsrc + 1 prefix offsets
 double *d_batch_dir = nullptr;     // 3 nsrc directions
 uint32_t *d_batch_tag = nullptr;   // tag of the first photon of the launch
 unsigned int g_batch_cap = 0, g_batch_nsrc = 0;
+// differentiable mirror: per-photon weights, positions and derivatives
+void *d_diff = nullptr;
+size_t g_diff_cap = 0;
+DiffOut g_diff_out;
+unsigned int g_diff_n = 0;
+mxg_image_spec g_diff_spec;
+uint32_t g_diff_grid = 0;
+bool g_diff_tangents = false;      // the last forward pass kept derivatives
+hipEvent_t g_diff_ev[6];           // around k_diff_enter + k_compact, k_diff_forward, k_diff_adjoint
+float g_diff_ms[3] = {0.0f, 0.0f, 0.0f};
+uint32_t *d_diff_list = nullptr;   // photons past the paraboloid: segments, list
+uint32_t *d_diff_seg_count = nullptr;
+uint32_t *d_diff_u32 = nullptr;    // [0] photons of the pass, [1] listed
+uint32_t g_diff_listed = 0;
+float *d_diff_win = nullptr;       // per-block image windows
+float *d_diff_slices = nullptr;    // their partial sums (64 slices)
+double *d_diff_partial = nullptr;  // per-block gradient partials
+double *d_diff_small = nullptr;    // summed weight, gradient
+mxg_diff_cfg *d_diff_cfg = nullptr;
 // launch schedule and graph capture
 mxgd_schedule g_sched = {{8, 8, 8, 8, 8}, MXG_REFLECT_WAVES_DEFAULT};
 int g_capturing = 0;
@@ -563,6 +582,161 @@ int mxgd_run_source (unsigned int n, unsigned int tag_start, double start_time,
    g_upper = m;
    if (n_out) *n_out = m;
    if (last_time) *last_time = t;
+   return 0;
+}
+
+/* ---- differentiable mirror ------------------------------------------------------ */
+
+namespace {
+
+constexpr int kDiffWin = 64 * 64;       // floats of one block's image window
+constexpr int kDiffGrad = MXG_NUM_SHELLS * MXG_DIFF_NPAR;
+
+int diff_buffers (size_t n, uint32_t grid)
+{
+   if ((n > g_diff_cap) || (d_diff_win == nullptr))
+     {
+        size_t cap = g_diff_cap ? g_diff_cap : 1024;
+        while (cap < n) cap *= 2;
+        if (d_diff) { HIPCHECK (hipStreamSynchronize (g_stream)); (void) hipFree (d_diff); d_diff = nullptr; }
+        g_diff_cap = 0;
+        // 3 + 3 * MXG_DIFF_NPAR doubles, one byte and two list words per photon
+        size_t dbl = align256 (cap * sizeof (double));
+        size_t u32 = align256 (cap * sizeof (uint32_t));
+        size_t total = (3 + 3 * MXG_DIFF_NPAR) * dbl + align256 (cap) + 2 * u32;
+        HIPCHECK (hipMalloc (&d_diff, total));
+        char *q = (char *) d_diff;
+        g_diff_out.w = (double *) q; q += dbl;
+        g_diff_out.y = (double *) q; q += dbl;
+        g_diff_out.z = (double *) q; q += dbl;
+        g_diff_out.dw = (double *) q; q += MXG_DIFF_NPAR * dbl;
+        g_diff_out.dy = (double *) q; q += MXG_DIFF_NPAR * dbl;
+        g_diff_out.dz = (double *) q; q += MXG_DIFF_NPAR * dbl;
+        g_diff_out.shell = (uint8_t *) q; q += align256 (cap);
+        d_diff_list = (uint32_t *) q;      // segments, then the list
+        g_diff_cap = cap;
+     }
+   if (d_diff_win == nullptr)
+     {
+        HIPCHECK (hipMalloc (&d_diff_win, (size_t) kMaxSegments * kDiffWin * sizeof (float)));
+        HIPCHECK (hipMalloc (&d_diff_slices, (size_t) 64 * kDiffWin * sizeof (float)));
+        HIPCHECK (hipMalloc (&d_diff_partial, (size_t) kMaxSegments * kDiffGrad * sizeof (double)));
+        HIPCHECK (hipMalloc (&g_diff_out.block_w, (size_t) kMaxSegments * sizeof (double)));
+        HIPCHECK (hipMalloc (&d_diff_small, 64 * sizeof (double)));
+        HIPCHECK (hipMalloc (&d_diff_cfg, sizeof (mxg_diff_cfg)));
+        HIPCHECK (hipMalloc (&d_diff_seg_count, kMaxSegments * sizeof (uint32_t)));
+        HIPCHECK (hipMalloc (&d_diff_u32, 4 * sizeof (uint32_t)));
+        for (int k = 0; k < 6; k++) HIPCHECK (hipEventCreate (&g_diff_ev[k]));
+     }
+   (void) grid;
+   return 0;
+}
+
+} // namespace
+
+int mxgd_diff_forward (const mxg_diff_cfg *d, unsigned int n, unsigned int tag_start,
+                       unsigned long long seed, const mxg_image_spec *spec, float *image,
+                       double *total_weight, int tangents)
+{
+   if (!d_cfg) return fail ("configuration not on the device");
+   if (n > (1u << 28)) return fail ("%u photons exceed 2^28", n);
+   if (spec->npix > 16384) return fail ("image of %u pixels a side exceeds 16384", spec->npix);
+   uint32_t grid = stage_grid (n, MXGD_K_REFLECT);
+   if (diff_buffers (n ? n : 1, grid)) return -1;
+   g_diff_grid = 0;
+   HIPCHECK (hipMemcpyAsync (d_diff_cfg, d, sizeof (mxg_diff_cfg), hipMemcpyHostToDevice, g_stream));
+   // tangents by the tilts only for a source at infinity (the translations
+   // follow at the focal plane), by all five parameters otherwise
+   const int nt = (tangents == 0) ? 0 : (g_hcfg.src.distance > 0.0) ? MXG_DIFF_NPAR : 2;
+   // k_diff_enter lists the photons that reach the paraboloid
+   ListIO enter;
+   enter.idx_in = nullptr;
+   enter.n_in = d_diff_u32;
+   enter.seg = d_diff_list;
+   enter.idx_out = d_diff_list + align256 (g_diff_cap * sizeof (uint32_t)) / sizeof (uint32_t);
+   enter.n_out = d_diff_u32 + 1;
+   enter.seg_count = d_diff_seg_count;
+   enter.grid = stage_grid (n, MXGD_K_ENTER);
+   hipLaunchKernelGGL (k_set_u32, dim3 (1), dim3 (1), 0, g_stream, d_diff_u32, (uint32_t) n);
+   HIPCHECK (hipGetLastError ());
+   HIPCHECK (launch_diff_forward (d_cfg, d_fa, d_da, d_diff_cfg, n, tag_start, seed, spec, image,
+                                  d_diff_win, d_diff_slices, g_diff_out, enter, grid, d_diff_small, nt,
+                                  g_stream, g_diff_ev));
+   double w = 0.0;
+   HIPCHECK (hipMemcpyAsync (&w, d_diff_small, sizeof (double), hipMemcpyDeviceToHost, g_stream));
+   HIPCHECK (hipMemcpyAsync (&g_diff_listed, d_diff_u32 + 1, sizeof (uint32_t), hipMemcpyDeviceToHost,
+                             g_stream));
+   HIPCHECK (hipStreamSynchronize (g_stream));
+   HIPCHECK (hipEventElapsedTime (&g_diff_ms[2], g_diff_ev[0], g_diff_ev[1]));
+   HIPCHECK (hipEventElapsedTime (&g_diff_ms[0], g_diff_ev[2], g_diff_ev[3]));
+   if (total_weight) *total_weight = w;
+   g_diff_n = n;
+   g_diff_spec = *spec;
+   g_diff_grid = grid;
+   g_diff_tangents = (tangents != 0);
+   return 0;
+}
+
+int mxgd_diff_adjoint (const float *g, double *grad)
+{
+   if (g_diff_grid == 0) return fail ("no forward pass to take the adjoint of");
+   if (!g_diff_tangents)
+     return fail ("the last forward pass (mxg_diff_image) kept no derivatives; call mxg_diff_forward");
+   HIPCHECK (launch_diff_adjoint (g_diff_n, g_diff_out, &g_diff_spec, g, d_diff_partial, g_diff_grid,
+                                  d_diff_small + 8, g_stream, g_diff_ev + 4));
+   HIPCHECK (hipMemcpyAsync (grad, d_diff_small + 8, kDiffGrad * sizeof (double),
+                             hipMemcpyDeviceToHost, g_stream));
+   HIPCHECK (hipStreamSynchronize (g_stream));
+   HIPCHECK (hipEventElapsedTime (&g_diff_ms[1], g_diff_ev[4], g_diff_ev[5]));
+   return 0;
+}
+
+void mxgd_diff_kernel_ms (float ms[3], unsigned int *listed)
+{
+   ms[0] = g_diff_ms[0];
+   ms[1] = g_diff_ms[1];
+   ms[2] = g_diff_ms[2];
+   if (listed) *listed = g_diff_listed;
+}
+
+int mxgd_diff_state (unsigned int n, double *w, double *y, double *z, double *dw, double *dy,
+                     double *dz, unsigned char *shell)
+{
+   if ((g_diff_grid == 0) || (n != g_diff_n))
+     return fail ("no forward pass of %u photons", n);
+   if (!g_diff_tangents && (dw || dy || dz))
+     return fail ("the last forward pass (mxg_diff_image) kept no derivatives");
+   const size_t b = (size_t) n * sizeof (double);
+   HIPCHECK (hipStreamSynchronize (g_stream));
+   if (w) HIPCHECK (hipMemcpy (w, g_diff_out.w, b, hipMemcpyDeviceToHost));
+   if (y) HIPCHECK (hipMemcpy (y, g_diff_out.y, b, hipMemcpyDeviceToHost));
+   if (z) HIPCHECK (hipMemcpy (z, g_diff_out.z, b, hipMemcpyDeviceToHost));
+   if (dw) HIPCHECK (hipMemcpy (dw, g_diff_out.dw, MXG_DIFF_NPAR * b, hipMemcpyDeviceToHost));
+   if (dy) HIPCHECK (hipMemcpy (dy, g_diff_out.dy, MXG_DIFF_NPAR * b, hipMemcpyDeviceToHost));
+   if (dz) HIPCHECK (hipMemcpy (dz, g_diff_out.dz, MXG_DIFF_NPAR * b, hipMemcpyDeviceToHost));
+   if (dw || dy || dz)
+     {
+        // photons lost before the paraboloid have no derivative columns on
+        // the device (see k_diff_enter): their derivatives are 0
+        double *wv = w;
+        if (wv == nullptr)
+          {
+             wv = (double *) malloc (b);
+             if (wv == nullptr) return fail ("out of memory");
+             hipError_t e = hipMemcpy (wv, g_diff_out.w, b, hipMemcpyDeviceToHost);
+             if (e != hipSuccess) { free (wv); return fail ("hipMemcpy: %s", hipGetErrorString (e)); }
+          }
+        for (unsigned int i = 0; i < n; i++)
+          if (wv[i] == 0.0)
+            for (int j = 0; j < MXG_DIFF_NPAR; j++)
+              {
+                 if (dw) dw[(size_t) j * n + i] = 0.0;
+                 if (dy) dy[(size_t) j * n + i] = 0.0;
+                 if (dz) dz[(size_t) j * n + i] = 0.0;
+              }
+        if (wv != w) free (wv);
+     }
+   if (shell) HIPCHECK (hipMemcpy (shell, g_diff_out.shell, n, hipMemcpyDeviceToHost));
    return 0;
 }
 

@@ -13,31 +13,6 @@
 
 namespace mxg {
 
-__device__ __forceinline__ bool in_strut_shadow (const mxg_hrma_cfg &h, vec3 x, vec3 p,
-                                                 const mxg_strut *s)
-{
-   const double cos_theta = h.strut_cos, sin_theta = h.strut_sin;
-   for (int k = 0; k < 2; k++)
-     {
-        double half_width = s[k].half_width;
-        double t = (s[k].xpos + h.cap_position - x.x) / p.x;
-        double y = x.y + p.y * t;
-        double z = x.z + p.z * t;
-        for (int i = 0; i < 3; i++)
-          {
-             if (i != 0)
-               {
-                  double tmp = cos_theta * y - sin_theta * z;
-                  z = sin_theta * y + cos_theta * z;
-                  y = tmp;
-               }
-             if (((-half_width < y) && (y < half_width)) || ((-half_width < z) && (z < half_width)))
-               return true;
-          }
-     }
-   return false;
-}
-
 // The hit of a ray on the quadric x^2+y^2+z^2 = a x^2 + b x + c between
 // xmin <= x < xmax (the larger valid root in x).  On success x0 becomes the
 // hit; on a miss x0 is unchanged and -1 is returned.

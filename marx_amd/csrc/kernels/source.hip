@@ -20,19 +20,6 @@ namespace mxg {
 constexpr int kTile = 2048;          // photons per scan tile
 constexpr int kPer = kTile / kBlock; // photons per thread
 
-// energy of the FLAT or FILE spectrum (spectrum.c:138-147, 184-189)
-__device__ __forceinline__ double draw_energy (const mxg_source_cfg &src, const double *__restrict__ da,
-                                               Stream &st)
-{
-   if (src.spectrum == MXG_SPECTRUM_FLAT)
-     {
-        double emin = src.emin, de = src.emax - emin;
-        return emin + de * uniform (st);
-     }
-   double r = uniform (st);
-   return interp_d (r, da + src.file_cum_off, da + src.file_e_off, src.file_n);
-}
-
 // every column of photon i set as a fresh photon of the given energy,
 // direction, time and tag
 __device__ __forceinline__ void init_photon (Soa s, uint32_t i, double energy, vec3 p, double t,

@@ -56,6 +56,29 @@ def batched_acis_i(tmpdir, num_sources=4096, seed=1, layout_seed=0, min_count=10
     return t, offaxis_directions(theta, phi), counts.astype(np.uint32)
 
 
+def diff_mirror(tmpdir=None, seed=1, param_seed=0, tilt_sigma=1.0, shift_sigma=0.01):
+    """Differentiable: on-axis point source at 1.49 keV, mirror only (no
+    grating or detector, no dither), for the weighted-photon images of
+    marx_amd.diff.  The 20 misalignment parameters (tilt az, tilt el in
+    arcsec; decenter y, decenter z, despace x in mm; per shell pair) are drawn
+    from N(0, tilt_sigma arcsec / shift_sigma mm): `params` where the
+    gradient is taken, and `target_params` = params plus a second draw, whose
+    image is the target.  Returns (tracer, params, target_params)."""
+    t = Tracer(seed=seed, source=True, GratingType="NONE", DetectorType="NONE", SourceType="POINT",
+               SpectrumType="FLAT", MinEnergy=1.49, MaxEnergy=1.49, DitherModel="NONE", **ON_AXIS)
+    rng = np.random.default_rng(param_seed)
+
+    def draw():
+        p = np.zeros(20)
+        for s in range(4):
+            p[5 * s:5 * s + 2] = rng.normal(0.0, tilt_sigma, 2)
+            p[5 * s + 2:5 * s + 5] = rng.normal(0.0, shift_sigma, 3)
+        return p
+
+    params = draw()
+    return t, params, params + draw()
+
+
 WORKLOADS = {"hetg-acis-s": hetg_acis_s, "letg-hrc-s": letg_hrc_s}
 
 

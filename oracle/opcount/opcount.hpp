@@ -19,7 +19,7 @@
 
 enum { OC_ADD, OC_MUL, OC_DIV, OC_SQRT, OC_SIN, OC_COS, OC_TAN, OC_ATAN, OC_ATAN2, OC_ASIN,
        OC_ACOS, OC_EXP, OC_EXPM1, OC_LOG, OC_LOG1P, OC_POW, OC_FMOD, OC_N };
-enum { OC_PHASES = 3 };
+enum { OC_PHASES = 8 };
 extern "C" unsigned long long Orc_Opcount[OC_PHASES][OC_N];
 extern "C" unsigned long long Orc_Opcount_Marks[OC_PHASES];
 extern "C" int Orc_Opcount_Phase;
@@ -83,6 +83,7 @@ OC_F1 (log, OC_LOG)
 OC_F1 (log1p, OC_LOG1P)
 #undef OC_F1
 inline orc_fp fabs (orc_fp a) { return orc_fp (::fabs (a.v)); }
+inline orc_fp floor (orc_fp a) { return orc_fp (::floor (a.v)); }
 inline orc_fp atan2 (orc_fp a, orc_fp b) { OC_INC (OC_ATAN2); return orc_fp (::atan2 (a.v, b.v)); }
 inline orc_fp pow (orc_fp a, orc_fp b) { OC_INC (OC_POW); return orc_fp (::pow (a.v, b.v)); }
 inline orc_fp fmod (orc_fp a, orc_fp b) { OC_INC (OC_FMOD); return orc_fp (::fmod (a.v, b.v)); }

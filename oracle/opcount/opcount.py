@@ -7,7 +7,13 @@ transcendental calls per photon entering the stage.  These are the
 algorithmic work figures behind the benchmark's roofline; they count the
 restated reference algorithm and nothing of the GPU code.
 
-    python -m oracle.opcount.opcount [--photons 100000] [--workload hetg-acis-s|letg-hrc-s|batched-acis-i]
+    python -m oracle.opcount.opcount [--photons 100000] [--workload hetg-acis-s|letg-hrc-s|batched-acis-i|diff]
+
+The diff workload counts the weighted forward pass (orc_diff_forward, per
+photon of the launch) in parts: the values, the tilt tangents, the
+translation tangents, and the oracle's finite-difference reflectivity slope
+(not the path's: the GPU differentiates the Fresnel equations); and the
+adjoint (orc_diff_adjoint) per photon.
 """
 import argparse
 import ctypes
@@ -111,6 +117,57 @@ def count_stages(tracer, num, seed=1, batch=None):
     return out
 
 
+def count_diff(tracer, num, params, npix=256, pixel_size=0.004, seed=1):
+    """Per-photon counts of the weighted forward pass and of the adjoint."""
+    from oracle import oracle as O
+    from marx_amd._lib import ImageSpec
+    from marx_amd.diff import setup
+    lib = load_opcount()
+    f64p = ctypes.POINTER(ctypes.c_double)
+    lib.orc_diff_forward.argtypes = ([ctypes.POINTER(O.OrcCtx), ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint,
+                                      ctypes.c_void_p] + [f64p] * 7 + [ctypes.POINTER(ctypes.c_ubyte)])
+    lib.orc_diff_adjoint.argtypes = ([ctypes.c_uint] + [f64p] * 6
+                                     + [ctypes.POINTER(ctypes.c_ubyte), ctypes.c_void_p, f64p, f64p])
+    cfg, fa, _, da, _ = tracer.config_pointers()
+    ctx = O.OrcCtx(cfg, fa, da, seed, 0)
+    spec = ImageSpec(npix, pixel_size, 0.0, 0.0)
+    d = setup(params)
+    img = np.zeros(npix * npix)
+    st = {k: np.zeros(n) for k, n in (("w", num), ("y", num), ("z", num), ("dw", 5 * num), ("dy", 5 * num),
+                                       ("dz", 5 * num))}
+    shell = np.zeros(num, dtype=np.uint8)
+    p = lambda a: a.ctypes.data_as(f64p)
+    cols = [p(st[k]) for k in ("w", "y", "z", "dw", "dy", "dz")]
+    sh = shell.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))
+    lib.orc_opcount_reset()
+    if lib.orc_diff_forward(ctypes.byref(ctx), ctypes.byref(d), num, 0, ctypes.byref(spec), p(img), *cols, sh):
+        raise RuntimeError("op-count oracle diff forward failed")
+    ph = [(ctypes.c_ulonglong * 17)() for _ in range(8)]
+    for k in range(8):
+        lib.orc_opcount_get_phase(k, ph[k])
+    add = lambda *xs: {k: sum(x[k] for x in xs) for k in xs[0]}
+    # bucket part + kind: part 0 up to the paraboloid hit, 4 after it; kind 0
+    # values, 1 tilt tangents, 2 translation tangents, 3 the oracle's slope
+    c = [_per_photon(ph[k], num) for k in range(8)]
+    out = {"values": add(c[0], c[4]), "tilt_tangents": add(c[1], c[5]),
+           "translation_tangents": add(c[2], c[6]), "reflectivity_slope_fd": add(c[3], c[7]),
+           # the whole forward pass of each tangent count (value only, tilts,
+           # all five), per photon of the pass
+           "forward<0>": add(c[0], c[4]), "forward<2>": add(c[0], c[1], c[4], c[5]),
+           "forward<5>": add(c[0], c[1], c[2], c[4], c[5], c[6]),
+           # its part up to the paraboloid hit (k_diff_enter, values only)
+           "k_diff_enter": c[0]}
+    g = np.random.default_rng(0).normal(size=npix * npix)
+    grad = np.zeros(20)
+    lib.orc_opcount_reset()
+    lib.orc_diff_adjoint(num, *cols, sh, ctypes.byref(spec), p(g), p(grad))
+    c = (ctypes.c_ulonglong * 17)()
+    lib.orc_opcount_get(c)
+    out["k_diff_adjoint"] = _per_photon(c, num)
+    out["alive"] = int((st["w"] > 0).sum())
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--photons", type=int, default=100000)
@@ -118,6 +175,20 @@ def main():
     ap.add_argument("--workload", default="hetg-acis-s")
     a = ap.parse_args()
     from marx_amd.workloads import WORKLOADS, batched_acis_i
+    if a.workload == "diff":
+        from marx_amd.workloads import diff_mirror
+        tracer, params, _ = diff_mirror()
+        res = count_diff(tracer, a.photons, params)
+        for st in ("values", "tilt_tangents", "translation_tangents", "reflectivity_slope_fd",
+                   "forward<0>", "forward<2>", "forward<5>", "k_diff_enter", "k_diff_adjoint"):
+            r = res[st]
+            print(f"{st:22s} flops/photon={r['flops']:8.1f} (add {r['add']:.1f} mul {r['mul']:.1f} "
+                  f"div {r['div']:.1f} sqrt {r['sqrt']:.1f}) transcendental/photon={r['transcendental']:.2f}")
+        print(f"alive {res['alive']} of {a.photons}")
+        if a.json:
+            with open(a.json, "w") as fp:
+                json.dump(res, fp, indent=1)
+        return
     with tempfile.TemporaryDirectory() as tmp:
         if a.workload == "batched-acis-i":
             # the batch's sources with their counts scaled to about a.photons

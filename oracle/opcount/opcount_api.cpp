@@ -1,7 +1,7 @@
 // opcount_api.cpp -- counters of the op-counting oracle build (test
 // infrastructure).
 #include <string.h>
-enum { OC_N_ = 17, OC_PHASES_ = 3 };
+enum { OC_N_ = 17, OC_PHASES_ = 8 };
 extern "C" {
 unsigned long long Orc_Opcount[OC_PHASES_][OC_N_];
 unsigned long long Orc_Opcount_Marks[OC_PHASES_];

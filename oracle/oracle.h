@@ -84,6 +84,28 @@ int orc_interpolate_n_fvector (const float *xp, float **yp, unsigned int n,
 /* Stage-major passes, in the order marx/src/marx.c:240-273 calls them. */
 int orc_create_photons (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int num,
                         unsigned int *num_collected, double *exposure_time);
+/* The differentiable mirror (mxg_diff_forward / mxg_diff_adjoint): weights,
+ * focal-plane positions and their derivatives per photon, an FP64 image,
+ * and the gradient for d(loss)/d(image) = g. */
+int orc_diff_forward (const orc_ctx *c, const mxg_diff_cfg *d, unsigned int n,
+                      unsigned int tag_start, const mxg_image_spec *sp, double *image,
+                      double *w, double *y, double *z, double *dw, double *dy, double *dz,
+                      unsigned char *shell);
+int orc_diff_adjoint (unsigned int n, const double *w, const double *y, const double *z,
+                      const double *dw, const double *dy, const double *dz,
+                      const unsigned char *shell, const mxg_image_spec *sp, const double *g,
+                      double *grad);
+/* The same on nthreads OpenMP threads (0: all); per-thread images and
+ * partial gradients are summed in thread order. */
+int orc_diff_forward_parallel (const orc_ctx *c, const mxg_diff_cfg *d, unsigned int n,
+                               unsigned int tag_start, const mxg_image_spec *sp, double *image,
+                               double *w, double *y, double *z, double *dw, double *dy,
+                               double *dz, unsigned char *shell, int nthreads);
+int orc_diff_adjoint_parallel (unsigned int n, const double *w, const double *y, const double *z,
+                               const double *dw, const double *dy, const double *dz,
+                               const unsigned char *shell, const mxg_image_spec *sp,
+                               const double *g, double *grad, int nthreads);
+
 /* The photons of a batch of point sources (mxg_trace_batch). */
 int orc_create_batch (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int nsrc,
                       const double *dirs, const unsigned int *counts, unsigned int tag_start,

@@ -69,6 +69,17 @@ def load():
     lib.orc_trace_batch_parallel.argtypes = [ctypes.POINTER(OrcCtx), ctypes.c_uint, f64p, u32p, ctypes.c_uint,
                                              ctypes.c_double, ctypes.c_int]
     lib.orc_trace_batch_parallel.restype = ctypes.c_long
+    lib.orc_diff_forward.argtypes = [ctypes.POINTER(OrcCtx), ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint,
+                                     ctypes.c_void_p, f64p, f64p, f64p, f64p, f64p, f64p, f64p,
+                                     ctypes.POINTER(ctypes.c_ubyte)]
+    lib.orc_diff_forward.restype = ctypes.c_int
+    lib.orc_diff_adjoint.argtypes = [ctypes.c_uint, f64p, f64p, f64p, f64p, f64p, f64p,
+                                     ctypes.POINTER(ctypes.c_ubyte), ctypes.c_void_p, f64p, f64p]
+    lib.orc_diff_adjoint.restype = ctypes.c_int
+    lib.orc_diff_forward_parallel.argtypes = lib.orc_diff_forward.argtypes + [ctypes.c_int]
+    lib.orc_diff_forward_parallel.restype = ctypes.c_int
+    lib.orc_diff_adjoint_parallel.argtypes = lib.orc_diff_adjoint.argtypes + [ctypes.c_int]
+    lib.orc_diff_adjoint_parallel.restype = ctypes.c_int
     lib.orc_sort_photons.argtypes = [ctypes.c_void_p]
     lib.orc_sort_photons.restype = ctypes.c_int
     lib.orc_trace_parallel.argtypes = [ctypes.POINTER(OrcCtx), ctypes.c_uint, ctypes.c_uint, ctypes.c_int]
@@ -186,6 +197,36 @@ class Oracle:
                                        tag_start, start_time, exposure)
         if st != 0:
             raise RuntimeError("oracle create_batch failed")
+
+    def diff_forward(self, diff_cfg, n, tag_start, spec, nthreads=1):
+        """The weighted trace of n photons: (FP64 image, per-photon state);
+        nthreads != 1 runs it on that many OpenMP threads (0: all)."""
+        img = np.zeros((spec.npix, spec.npix), dtype=np.float64)
+        st = {k: np.zeros(n) for k in ("w", "y", "z")}
+        st.update({k: np.zeros(5 * n) for k in ("dw", "dy", "dz")})
+        st["shell"] = np.zeros(n, dtype=np.uint8)
+        args = [ctypes.byref(self.ctx), ctypes.byref(diff_cfg), n, tag_start, ctypes.byref(spec),
+                _arr(img, ctypes.c_double), *[_arr(st[k], ctypes.c_double) for k in ("w", "y", "z", "dw", "dy", "dz")],
+                _arr(st["shell"], ctypes.c_ubyte)]
+        r = (self.lib.orc_diff_forward(*args) if nthreads == 1
+             else self.lib.orc_diff_forward_parallel(*args, nthreads))
+        if r != 0:
+            raise RuntimeError("oracle diff_forward failed")
+        st["n"] = n
+        return img, st
+
+    def diff_adjoint(self, st, spec, g, nthreads=1):
+        """The FP64 gradient (20,) for d(loss)/d(image) = g."""
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        grad = np.zeros(20)
+        args = [st["n"], *[_arr(st[k], ctypes.c_double) for k in ("w", "y", "z", "dw", "dy", "dz")],
+                _arr(st["shell"], ctypes.c_ubyte), ctypes.byref(spec), _arr(g, ctypes.c_double),
+                _arr(grad, ctypes.c_double)]
+        r = (self.lib.orc_diff_adjoint(*args) if nthreads == 1
+             else self.lib.orc_diff_adjoint_parallel(*args, nthreads))
+        if r != 0:
+            raise RuntimeError("oracle diff_adjoint failed")
+        return grad
 
     def trace_batch_parallel(self, dirs, counts, tag_start=0, exposure=1.0e4, nthreads=0):
         d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1)

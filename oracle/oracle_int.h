@@ -51,5 +51,9 @@ orc_vec orc_v_rotate_vector1 (orc_vec p, orc_vec n, double c, double s);
 orc_vec orc_v_rotate_unit_vector1 (orc_vec p, orc_vec n, double c, double s);
 orc_vec orc_v_rotate_unit_vector (orc_vec p, orc_vec n, double theta);
 orc_vec orc_m3_vmul (const double a[9], orc_vec v);
+/* 1 when the ray x + t p is in the shadow of one of the num struts s */
+int orc_strut_shadow (const mxg_hrma_cfg *h, orc_vec x, orc_vec p, const mxg_strut *s,
+                      unsigned int num);
+double orc_fold_theta (const orc_ctx *c, const double *fa, double p);
 
 #endif

@@ -14,8 +14,8 @@
 
 #include "oracle_int.h"
 
-static int intersects_struts (const mxg_hrma_cfg *h, Marx_Photon_Attr_Type *at,
-                              const mxg_strut *s, unsigned int num)
+int orc_strut_shadow (const mxg_hrma_cfg *h, orc_vec xv, orc_vec pv, const mxg_strut *s,
+                      unsigned int num)
 {
    double theta = 30.0 * (PI / 180.0);
    double cos_theta = cos (theta);
@@ -26,8 +26,8 @@ static int intersects_struts (const mxg_hrma_cfg *h, Marx_Photon_Attr_Type *at,
    while (s < smax)
      {
         double half_width = s->half_width;
-        double x = at->x.x, y = at->x.y, z = at->x.z;
-        double px = at->p.x, py = at->p.y, pz = at->p.z;
+        double x = xv.x, y = xv.y, z = xv.z;
+        double px = pv.x, py = pv.y, pz = pv.z;
         double t = (s->xpos + h->cap_position - x) / px;
         y += py * t;
         z += pz * t;
@@ -41,14 +41,19 @@ static int intersects_struts (const mxg_hrma_cfg *h, Marx_Photon_Attr_Type *at,
                }
              if (((-half_width < y) && (y < half_width))
                  || ((-half_width < z) && (z < half_width)))
-               {
-                  at->flags |= PHOTON_MIRROR_VBLOCKED;
-                  return 1;
-               }
+               return 1;
           }
         s++;
      }
    return 0;
+}
+
+static int intersects_struts (const mxg_hrma_cfg *h, Marx_Photon_Attr_Type *at,
+                              const mxg_strut *s, unsigned int num)
+{
+   if (0 == orc_strut_shadow (h, at->x, at->p, s, num)) return 0;
+   at->flags |= PHOTON_MIRROR_VBLOCKED;
+   return 1;
 }
 
 static int project_photon_to_hrma (const mxg_hrma_cfg *h, Marx_Photon_Attr_Type *at,

@@ -491,7 +491,7 @@ double orc_reflectivity (double cos_theta, double beta, double delta)
 
 /* ---- WFold ----------------------------------------------------------------------- */
 
-static double fold_interpolate_theta (const orc_ctx *c, const double *fa, double p)
+double orc_fold_theta (const orc_ctx *c, const double *fa, double p)
 {
    double p_min = fa[1], delta_p = fa[2], p_max = fa[3];
    unsigned int num = (unsigned int) fa[6];
@@ -517,13 +517,13 @@ double orc_wfold_interp (const orc_ctx *c, const mxg_wfold *w, double energy,
    unsigned int i;
 
    if (w->n_arrays == 0) return 0.0;
-   if (w->n_arrays == 1) return fold_interpolate_theta (c, arrays, r);
+   if (w->n_arrays == 1) return orc_fold_theta (c, arrays, r);
    e_alpha = energy * sin_alpha;
    i = orc_binary_search_d (e_alpha, e_alphas, w->n_arrays);
    if (i == w->n_arrays) i--;
    if (i == 0) i++;
-   theta_0 = fold_interpolate_theta (c, arrays + (i - 1) * MXG_FOLD_STRIDE, r);
-   theta_1 = fold_interpolate_theta (c, arrays + i * MXG_FOLD_STRIDE, r);
+   theta_0 = orc_fold_theta (c, arrays + (i - 1) * MXG_FOLD_STRIDE, r);
+   theta_1 = orc_fold_theta (c, arrays + i * MXG_FOLD_STRIDE, r);
    return theta_0 + (theta_1 - theta_0) * (e_alpha - e_alphas[i - 1])
      / (e_alphas[i] - e_alphas[i - 1]);
 }

@@ -16,7 +16,7 @@ class Wfold(ctypes.Structure):
 class ShellCfg(ctypes.Structure):
     _fields_ = [("conic_a", D * 2), ("conic_b", D * 2), ("conic_c", D * 2),
                 ("conic_xmin", D * 2), ("conic_xmax", D * 2), ("to_osac", (D * 3) * 2),
-                ("fwd", (D * 9) * 2), ("bwd", (D * 9) * 2), ("blur", D * 2), ("scat_factor", D * 2),
+                ("fwd", (D * 9) * 2), ("bwd", (D * 9) * 2), ("blur", D * 2), ("scat_factor", D * 2), ("tilt_deg", (D * 2) * 2),
                 ("front_position", D), ("area_fraction", D), ("min_radius", D), ("max_radius", D),
                 ("shutter_bitmap", U), ("num_open_shutters", U), ("corr_n", U), ("corr_e_off", U),
                 ("corr_f_off", U), ("mirror_number", U), ("wfold", Wfold * 2)]
@@ -99,7 +99,9 @@ class Config(ctypes.Structure):
     _fields_ = [("hrma", HrmaCfg), ("grating", GratingCfg), ("det", DetectorCfg), ("src", SourceCfg)]
 
 
-STRUCTS = {"mxg_wfold": Wfold, "mxg_shell_cfg": ShellCfg, "mxg_strut": Strut, "mxg_hrma_cfg": HrmaCfg,
+from marx_amd._lib import DiffCfg, DiffShell, ImageSpec  # noqa: E402
+
+STRUCTS = {"mxg_diff_shell": DiffShell, "mxg_diff_cfg": DiffCfg, "mxg_image_spec": ImageSpec,"mxg_wfold": Wfold, "mxg_shell_cfg": ShellCfg, "mxg_strut": Strut, "mxg_hrma_cfg": HrmaCfg,
            "mxg_grating": Grating, "mxg_sectors": Sectors, "mxg_grating_cfg": GratingCfg,
            "mxg_chip": Chip, "mxg_qe_table": QeTable, "mxg_rect": Rect,
            "mxg_detector_cfg": DetectorCfg, "mxg_source_cfg": SourceCfg, "mxg_config": Config}
