@@ -98,8 +98,12 @@ DESCRIPTION["diff"] = (
     "pairs) drawn from N(0, 1 arcsec / 10 um), 1e7 survival-weighted photons of an on-axis point "
     "source at 1.49 keV through the mirror, loss = squared difference of FP32-accumulated 256x256 "
     "focal-plane images (0.004 mm pixels) against the image at perturbed parameters")
+DESCRIPTION["output"] = (
+    "Output files: the column files (marx_write_photons format, every column) of one traced HETG + "
+    "ACIS-S chunk of 1e7 photons, written from HBM by mxg_write_device, against downloading the chunk "
+    "and writing it with marx_write_photons")
 DEFAULT_PHOTONS = {"hetg-acis-s": 10_000_000, "letg-hrc-s": 50_000_000, "batched-acis-i": 0,
-                   "diff": 10_000_000}
+                   "diff": 10_000_000, "output": 10_000_000}
 DIFF_NPIX, DIFF_PIXEL_MM = 256, 0.004
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
@@ -387,6 +391,56 @@ def cpu_baseline_diff(tracer, params, target, photons, target_seconds=20.0):
                       f"threads), {dt:.1f} s"}
 
 
+def measure_output(args, tmp, torch):
+    """Output files of one traced chunk: one step writes every column file
+    of the chunk's detected photons from HBM (mxg_write_device: rows
+    selected and bytes packed on the GPU, one copy to the host, one write
+    per file).  Beside it the AoS route on the same chunk: download the list
+    and write it with marx_write_photons.  Files go to a temporary directory
+    (page cache; the figures are not disk speeds)."""
+    from marx_amd import Photons
+    from marx_amd._lib import WRITE_ALL
+    from marx_amd.workloads import hetg_acis_s
+    cfg, K = "output", args.steps
+    N = args.photons or DEFAULT_PHOTONS[cfg]
+    progress = lambda msg: print(f"bench output: {msg}", file=sys.stderr, flush=True)
+    t = hetg_acis_s(tmp, photons=N, exposure=args.exposure)
+    ncol, ndet, _ = t.trace_device(N)
+    a, b = os.path.join(tmp, "device"), os.path.join(tmp, "aos")
+    os.makedirs(a)
+    os.makedirs(b)
+    for _ in range(args.warmup):
+        t.write_device(a)
+    t0 = time.perf_counter()
+    for _ in range(K):
+        rows = t.write_device(a)
+    dev_s = (time.perf_counter() - t0) / K
+    nbytes = sum(os.path.getsize(os.path.join(a, f)) for f in os.listdir(a))
+    progress(f"{rows} rows, {nbytes / 1e6:.1f} MB per write; device path {dev_s * 1e3:.1f} ms")
+    p = Photons(N)
+    reps = max(1, min(K, 3))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        t.download(p)
+        p.write(b)
+    aos_s = (time.perf_counter() - t0) / reps
+    progress(f"download + marx_write_photons {aos_s * 1e3:.1f} ms")
+    same = all(open(os.path.join(a, f), "rb").read() == open(os.path.join(b, f), "rb").read()
+               for f in os.listdir(a))
+    return {
+        "metric": "photons/s traced (HRMA+grating+detector) 1 GPU vs host CPU; gradient evals/s",
+        "value": rows / dev_s, "unit": "rows/s written (all columns)", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": dev_s * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": DESCRIPTION[cfg], "name": cfg, "photons_traced": ncol, "rows": rows,
+                   "files": len(os.listdir(a)), "bytes_per_write": nbytes},
+        "file_gb_per_s": nbytes / dev_s / 1e9,
+        "aos_route": {"ms": aos_s * 1e3, "value": rows / aos_s, "unit": "rows/s written"},
+        "device_over_aos_route": aos_s / dev_s,
+        "files_identical": same,
+    }
+
+
 def cpu_baseline_batch(tracer, dirs, counts, exposure, target_seconds=20.0):
     """The FP64 oracle source by source (one OpenMP task per source) over the
     first sources of the batch, as many as take about target_seconds."""
@@ -470,6 +524,8 @@ def measure():
         return measure_batched(args, tmp, torch)
     if args.config == "diff":
         return measure_diff(args, torch)
+    if args.config == "output":
+        return measure_output(args, tmp, torch)
     cfg = args.config
     N, T = (args.photons or DEFAULT_PHOTONS[cfg]), args.exposure
     tracer = WORKLOADS[cfg](tmp, photons=N, exposure=T)

@@ -18,6 +18,8 @@
 #ifndef MARX_GPU_H
 #define MARX_GPU_H
 
+#include <stdio.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -195,6 +197,64 @@ int marx_create_photons (Marx_Source_Type *st, Marx_Photon_Type *pt,
                          double *exposure_time);
 
 /* ------------------------------------------------------------------------ */
+/* Output files (marx/libsrc/marxio.c, declared at marx.h:161-186, 451-479;
+ * format: marx/doc/dump.txt).  One file per column: a 32-byte header (magic
+ * 0x83 0x13 0x89 0x8D, type 'A' int8 / 'I' int16 / 'J' int32 / 'E' float32
+ * / 'D' float64, the column name in 15 bytes padded with 0, the number of
+ * rows and of columns (0: a vector) as big-endian int32, 4 reserved bytes)
+ * followed by the values, big-endian.                                      */
+
+#define MARX_MAX_COLUMN_NAME_LEN 16
+typedef struct Marx_Dump_File_Type
+{
+   FILE *fp;
+   char type;                          /* A, I, J, E, D */
+   char colname[MARX_MAX_COLUMN_NAME_LEN];
+   int num_rows;
+   int num_cols;                       /* 0 ==> vector */
+   struct Marx_Dump_File_Type *next;
+}
+Marx_Dump_File_Type;
+
+/* Replace marxio.c:466-520 and :523-533: open a column file and read its
+ * header (NULL on error), close it. */
+Marx_Dump_File_Type *marx_open_read_dump_file (char *file);
+void marx_close_read_dump_file (Marx_Dump_File_Type *dft);
+/* Replace marxio.c:143-200 and :79-115: create a column file and write the
+ * header of dft (type, colname, num_rows, num_cols; dft->fp is set), and
+ * write the row count n into the header of such a file and close it. */
+FILE *marx_create_write_dump_file (char *filename, Marx_Dump_File_Type *dft);
+int marx_close_write_dump_file (FILE *fp, unsigned long n);
+
+/* Replaces marxio.c:402-453 (marx.c:157-170 calls it once per chunk).
+ * Writes the photons of p whose flags & BAD_PHOTON_MASK is zero, in list
+ * order, into the column files of dir whose history bits are in
+ * write_mask & p->history (plus the sky and detector dither columns when
+ * dither is on), as in marxio.c:259-289: b_energy energy time tag
+ * xpos ypos zpos xcos ycos zcos pha detector xpixel ypixel hrc_u hrc_v
+ * mirror hrcregion order ofine ocoarse1-3 sky_ra sky_dec sky_roll det_dy
+ * det_dz det_theta (.dat).  time = arrival_time + total_time.  open_mode
+ * != 0 creates the files; 0 appends to them and updates their row counts.
+ * The directory must exist.  Returns 0 or -1. */
+int marx_write_photons (Marx_Photon_Type *p, unsigned long write_mask, char *dir,
+                        int open_mode, double total_time);
+
+/* Replaces marxio.c:697-756: prints the column files named in argv[2..]
+ * side by side as text on stdout ("marx --dump file..."), flushed before
+ * returning. */
+int marx_dump (int argc, char **argv);
+
+/* Ray files (marx/libsrc/s-rayfile.c:43-145): the magic 0xF8161961 and the
+ * list's history as native unsigned longs, then the photons whose flags &
+ * BAD_PHOTON_MASK is zero as native Marx_Photon_Attr_Type records.  As in
+ * the reference, writing adds total_time to those photons' arrival_time in
+ * p.  new_file != 0 creates the file, 0 appends.  marx_dump_rayfile prints
+ * a ray file as text on stdout. */
+#define MARX_RAYFILE_MAGIC 0xF8161961UL
+int marx_dump_to_rayfile (char *file, int new_file, Marx_Photon_Type *p, double total_time);
+int marx_dump_rayfile (char *file);
+
+/* ------------------------------------------------------------------------ */
 /* mxg_ extensions (no counterpart in the reference).                       */
 
 /* Last error message of this library (also printed on stderr). */
@@ -369,6 +429,16 @@ int mxg_last_kernel_stats (float ms[2], unsigned int photons_in[2]);
 /* Copies the device-resident photons of the last mxg_trace_device call into
  * pt (AoS), setting n_photons, history and the pruned sorted_index. */
 int mxg_download_photons (Marx_Photon_Type *pt);
+
+/* marx_write_photons for the device-resident photons of the last
+ * mxg_trace_device or mxg_trace_batch call, without the AoS copy: the GPU
+ * selects the photons whose flags & BAD_PHOTON_MASK is zero (in tag order),
+ * packs each requested column big-endian in HBM, and only the file bytes
+ * cross to the host.  The files are byte-identical to marx_write_photons on
+ * the downloaded list with the same arguments.  *num_written (may be NULL)
+ * receives the rows written.  Returns 0 or -1. */
+int mxg_write_device (unsigned long write_mask, char *dir, int open_mode, double total_time,
+                      unsigned int *num_written);
 
 /* Device pointers of the resident structure-of-arrays columns.  Returned
  * pointers stay valid until the next call that grows the store. */

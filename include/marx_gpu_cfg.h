@@ -178,6 +178,7 @@ mxg_detector_cfg;
 
 #define MXG_SOURCE_POINT   0
 #define MXG_SOURCE_DISK    1
+#define MXG_SOURCE_RAYFILE 2       /* replayed on the host by marx_create_photons */
 #define MXG_SPECTRUM_FLAT  1
 #define MXG_SPECTRUM_FILE  2
 #define MXG_TIME_POISSON   0

@@ -112,6 +112,18 @@ class DiffCfg(ctypes.Structure):
 
 PPhotonList = ctypes.POINTER(PhotonList)
 
+
+class DumpFile(ctypes.Structure):
+    """Marx_Dump_File_Type (marx/libsrc/marx.h:161-173)."""
+
+
+DumpFile._fields_ = [("fp", ctypes.c_void_p), ("type", ctypes.c_char), ("colname", ctypes.c_char * 16),
+                     ("num_rows", ctypes.c_int), ("num_cols", ctypes.c_int),
+                     ("next", ctypes.POINTER(DumpFile))]
+
+# history bits of Marx_Photon_Type (marx/libsrc/marx.h:128-147): every column
+WRITE_ALL = 0x01FFFFFF
+
 # (name, restype, argtypes) of every exported C-ABI symbol of include/marx_gpu.h
 _SIGNATURES = [
     ("marx_alloc_photon_type", PPhotonList, [ctypes.c_uint]),
@@ -152,6 +164,17 @@ _SIGNATURES = [
                                        ctypes.c_uint, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_float)]),
     ("mxg_set_schedule", ctypes.c_int, [ctypes.POINTER(Schedule)]),
+    ("marx_write_photons", ctypes.c_int, [PPhotonList, ctypes.c_ulong, ctypes.c_char_p, ctypes.c_int,
+                                          ctypes.c_double]),
+    ("mxg_write_device", ctypes.c_int, [ctypes.c_ulong, ctypes.c_char_p, ctypes.c_int, ctypes.c_double,
+                                        ctypes.POINTER(ctypes.c_uint)]),
+    ("marx_open_read_dump_file", ctypes.POINTER(DumpFile), [ctypes.c_char_p]),
+    ("marx_close_read_dump_file", None, [ctypes.POINTER(DumpFile)]),
+    ("marx_create_write_dump_file", ctypes.c_void_p, [ctypes.c_char_p, ctypes.POINTER(DumpFile)]),
+    ("marx_close_write_dump_file", ctypes.c_int, [ctypes.c_void_p, ctypes.c_ulong]),
+    ("marx_dump", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]),
+    ("marx_dump_to_rayfile", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, PPhotonList, ctypes.c_double]),
+    ("marx_dump_rayfile", ctypes.c_int, [ctypes.c_char_p]),
     ("mxg_diff_setup", ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]),
     ("mxg_diff_forward", ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.c_uint, ctypes.c_uint,
                                         ctypes.POINTER(ImageSpec), ctypes.c_void_p,

@@ -29,6 +29,9 @@ static int Lazy_Sync = 0;
 
 /* the list whose photons are currently resident in HBM */
 static const Marx_Photon_Type *Resident = NULL;
+/* the device store holds the photons of the last mxg_trace_device /
+ * mxg_trace_batch call (cleared when an AoS list is uploaded over them) */
+static int Device_Trace = 0;
 static unsigned int Resident_Tag_Start = 0;
 
 void mxg_set_seed (unsigned long long seed) { Seed = seed; Seed_Set = 1; }
@@ -119,6 +122,7 @@ static int make_resident (Marx_Photon_Type *pt)
         mxg_error ("photon list has num_sorted > n_photons");
         return -1;
      }
+   Device_Trace = 0;
    if (-1 == mxgd_upload_aos (pt->attributes, pt->n_photons, pt->sorted_index, pt->num_sorted))
      {
         mxg_error ("%s", mxgd_last_error ());
@@ -329,6 +333,66 @@ static int sort_by_energy (Marx_Photon_Type *pt)
    return 0;
 }
 
+/* A RAYFILE source (source.c:268-382 over s-rayfile.c:176-207): the records
+ * are read on the host, cut at the exposure, given new tags, dithered again
+ * on the device (the reference applies the dither to replayed photons as to
+ * new ones) and sorted by energy. */
+static int replay_rayfile (Marx_Source_Type *st, Marx_Photon_Type *pt, unsigned int num,
+                           unsigned int *num_collected, double *exposure_time)
+{
+   unsigned int n = 0, i;
+   unsigned long hist = 0;
+
+   memset (pt->attributes, 0, (size_t) num * sizeof (Marx_Photon_Attr_Type));
+   if (-1 == mxg_rayfile_read (st, pt->attributes, num, &n, &hist))
+     {
+        mxg_error ("marx_create_photons: the source is not an open ray file");
+        return -1;
+     }
+   pt->history = hist;
+   if (0 == (hist & MARX_TIME_OK))
+     {
+        mxg_error ("ray file without arrival times (history lacks MARX_TIME_OK)");
+        return -1;
+     }
+   if (exposure_time != NULL)
+     {
+        double tmax = _Marx_TStart_MJDsecs + *exposure_time;
+        for (i = 0; i < n; i++)
+          if (pt->attributes[i].arrival_time >= tmax)
+            {
+               n = i + 1;
+               break;
+            }
+     }
+   if (n)
+     {
+        for (i = 0; i < n; i++) pt->attributes[i].tag = pt->tag_start + i;
+        pt->history |= MARX_TAG_OK;
+        pt->tag_start += n;
+        if (Mxg_Cfg.src.dither_mode != MXG_DITHER_NONE)
+          {
+             if (-1 == sync_config ())
+               return -1;
+             Device_Trace = 0;
+             if ((-1 == mxgd_upload_aos (pt->attributes, n, NULL, 0))
+                 || (-1 == mxgd_dither_store (pt->start_time, Seed))
+                 || (-1 == mxgd_download_aos (pt->attributes, n)))
+               {
+                  mxg_error ("%s", mxgd_last_error ());
+                  return -1;
+               }
+          }
+     }
+   pt->n_photons = n;
+   Resident = NULL;
+   if (-1 == sort_by_energy (pt))
+     return -1;
+   pt->total_time = n ? pt->attributes[n - 1].arrival_time : 0.0;
+   *num_collected = n;
+   return 0;
+}
+
 int marx_create_photons (Marx_Source_Type *st, Marx_Photon_Type *pt, unsigned int num,
                          unsigned int *num_collected, double *exposure_time)
 {
@@ -347,9 +411,12 @@ int marx_create_photons (Marx_Source_Type *st, Marx_Photon_Type *pt, unsigned in
    pt->source_distance = s->distance;
    pt->history = 0;
    pt->start_time += pt->total_time;
+   if (s->type == MXG_SOURCE_RAYFILE)
+     return replay_rayfile (st, pt, num, num_collected, exposure_time);
 
    if (-1 == sync_config ())
      return -1;
+   Device_Trace = 0;
    if (-1 == mxgd_run_source (num, pt->tag_start, pt->start_time,
                               (exposure_time != NULL) ? *exposure_time : -1.0,
                               Seed, 0, -1, &n, &last_t, NULL))
@@ -408,6 +475,12 @@ int mxg_trace_device (unsigned int num, unsigned int tag_start, double start_tim
         mxg_error ("Mirror not initialized.");
         return -1;
      }
+   if (Mxg_Cfg.src.type == MXG_SOURCE_RAYFILE)
+     {
+        mxg_error ("mxg_trace_device creates its photons on the device; a RAYFILE source is "
+                   "replayed through marx_create_photons");
+        return -1;
+     }
    /* timing slots: 0 source+dither, 1 mirror, 2 grating, 3 detector,
     * 4 and 5 the two mirror kernels alone */
    if ((-1 == mxgd_run_source (num, tag_start, start_time, exposure_time, Seed, 0, 0, &n, &last_t, NULL))
@@ -437,6 +510,7 @@ int mxg_trace_device (unsigned int num, unsigned int tag_start, double start_tim
    Last_Collected = n;
    Last_Tag_Start = tag_start;
    Last_History = hist;
+   Device_Trace = 1;
    if (num_collected) *num_collected = n;
    if (num_detected) *num_detected = ndet;
    if (stage_ms)
@@ -502,6 +576,7 @@ static int batch_finish (unsigned int n, unsigned int tag_start, int with_gratin
    Last_Collected = n;
    Last_Tag_Start = tag_start;
    Last_History = hist;
+   Device_Trace = 1;
    memcpy (Last_Slot_Ms, ms, sizeof (ms));
    if (num_detected) *num_detected = Stage_Counts[3];
    return 0;
@@ -519,7 +594,7 @@ static int batch_setup (unsigned int num_sources, const double *dirs, const unsi
      }
    if ((Mxg_Cfg.src.spectrum != MXG_SPECTRUM_FLAT) && (Mxg_Cfg.src.spectrum != MXG_SPECTRUM_FILE))
      {
-        mxg_error ("Source not initialized (marx_create_source).");
+        mxg_error ("Source not initialized (marx_create_source of a POINT or DISK source with a spectrum).");
         return -1;
      }
    if ((num_sources == 0) || (dirs == NULL) || (counts == NULL))
@@ -754,6 +829,17 @@ int mxg_download_photons (Marx_Photon_Type *pt)
    pt->history = Last_History;
    if (-1 == sort_by_energy (pt)) return -1;
    marx_prune_photons (pt);
+   return 0;
+}
+
+int mxg_resident_history (unsigned long *history)
+{
+   if (!Device_Trace)
+     {
+        mxg_error ("no device-resident trace (mxg_trace_device or mxg_trace_batch) to write");
+        return -1;
+     }
+   *history = Last_History;
    return 0;
 }
 

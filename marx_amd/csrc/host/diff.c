@@ -136,7 +136,7 @@ static int diff_pass (const char *name, const double *params, unsigned int num,
      }
    if ((Mxg_Cfg.src.spectrum != MXG_SPECTRUM_FLAT) && (Mxg_Cfg.src.spectrum != MXG_SPECTRUM_FILE))
      {
-        mxg_error ("Source not initialized (marx_create_source).");
+        mxg_error ("Source not initialized (marx_create_source of a POINT or DISK source with a spectrum).");
         return -1;
      }
    if ((-1 == mxg_diff_setup (params, &d))

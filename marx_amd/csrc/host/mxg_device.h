@@ -117,6 +117,19 @@ void mxgd_diff_kernel_ms (float ms[3], unsigned int *listed);
 int mxgd_diff_state (unsigned int n, double *w, double *y, double *z, double *dw, double *dy,
                      double *dz, unsigned char *shell);
 
+/* The configured dither applied to every photon of the store at its own
+ * arrival time (photons replayed from a ray file).  Synchronises. */
+int mxgd_dither_store (double start_time, unsigned long long seed);
+
+/* ---- output files ---- */
+
+/* Packs the columns cols[0..ncols) (MXG_COL_*) of the store's photons whose
+ * flags & BAD_PHOTON_MASK is zero, in order, big-endian, into pinned host
+ * memory: *bytes + col_off[k] holds column k's *n_rows values.  The memory
+ * stays valid until the next call.  Synchronises before returning. */
+int mxgd_pack_output (const int *cols, int ncols, double total_time, unsigned int *n_rows,
+                      const unsigned char **bytes, size_t *col_off);
+
 /* Number of photons in the active list (device -> host copy). */
 int mxgd_active_count (unsigned int *n);
 /* Active list indices (host copy, n = mxgd_active_count). */

@@ -125,6 +125,19 @@ int mxg_dither_init (Param_File_Type *pf, double *yoff, double *zoff);
 void mxg_get_nominal_pointing (double *ra, double *dec, double *roll);
 void mxg_get_pointing (double *ra, double *dec, double *roll);
 
+/* History bits of the device-resident photons of the last mxg_trace_device
+ * or mxg_trace_batch call; -1 (with the error set) when the store no longer
+ * holds them. */
+int mxg_resident_history (unsigned long *history);
+/* Opens a ray file and reads its header (s-rayfile.c:43-65); NULL with
+ * the error set when it is not one. */
+FILE *mxg_open_rayfile (const char *file, unsigned long *history);
+/* Up to num records of a RAYFILE source, times relative to the previous
+ * call's last photon (s-rayfile.c:176-207). */
+int mxg_rayfile_read (Marx_Source_Type *st, Marx_Photon_Attr_Type *at, unsigned int num,
+                      unsigned int *n, unsigned long *history);
+extern double _Marx_TStart_MJDsecs;
+
 /* ---- photon store on the device (implemented in the HIP layer) ---------- */
 #include "mxg_device.h"
 

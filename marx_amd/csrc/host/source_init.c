@@ -20,7 +20,19 @@
 struct _Marx_Source_Type
 {
    int open;
+   /* RAYFILE sources (s-rayfile.c:148-254) */
+   FILE *ray_fp;
+   unsigned long ray_history;
+   double ray_last_time;
 };
+
+/* The observation start in MJD seconds, as the reference's driver sets it
+ * (marx.c:285-330 -> util.c:55).  It enters the exposure cut of sources
+ * whose photons carry their own times (source.c:295-309).  A driver that
+ * defines it supersedes this definition at link time. */
+double _Marx_TStart_MJDsecs = 0.0;
+
+static char Ray_File[1024];
 
 static double Nominal_Ra, Nominal_Dec, Nominal_Roll;
 static double Pointing_Ra, Pointing_Dec, Pointing_Roll;
@@ -281,10 +293,28 @@ int mxg_source_init (Param_File_Type *pf)
      return -1;
    if (!strcmp (name, "POINT")) s->type = MXG_SOURCE_POINT;
    else if (!strcmp (name, "DISK")) s->type = MXG_SOURCE_DISK;
+   else if (!strcmp (name, "RAYFILE")) s->type = MXG_SOURCE_RAYFILE;
    else
      {
-        mxg_error ("SourceType %s not supported (POINT and DISK are)", name);
+        mxg_error ("SourceType %s not supported (POINT, DISK and RAYFILE are)", name);
         return -1;
+     }
+   if (s->type == MXG_SOURCE_RAYFILE)
+     {
+        /* s-rayfile.c:226-254: the photons come from the file */
+        int dump;
+        if ((-1 != pf_get_boolean (pf, "DumpToRayFile", &dump)) && dump)
+          {
+             mxg_error ("You cannot specify a RAYFILE source AND dump to a ray file.");
+             return -1;
+          }
+        if (-1 == pf_get_string (pf, "RayFile", Ray_File, sizeof (Ray_File)))
+          return -1;
+        if (-1 == mxg_dither_init (pf, &yoff, &zoff))
+          return -1;
+        s->distance = dist * 1000.0;
+        Mxg_Cfg_Generation++;
+        return 0;
      }
    dist *= 1000.0;
    ra *= (PI / 180.0);
@@ -340,7 +370,35 @@ Marx_Source_Type *marx_create_source (Param_File_Type *pf)
    if (-1 == mxg_source_init (pf))
      return NULL;
    st = (Marx_Source_Type *) calloc (1, sizeof (Marx_Source_Type));
+   if ((st != NULL) && (Mxg_Cfg.src.type == MXG_SOURCE_RAYFILE))
+     {
+        st->ray_fp = mxg_open_rayfile (Ray_File, &st->ray_history);
+        if (st->ray_fp == NULL)
+          {
+             free (st);
+             return NULL;
+          }
+     }
    return st;
+}
+
+/* s-rayfile.c:176-207: up to num records of the ray file, their arrival
+ * times made relative to the last photon of the previous call. */
+int mxg_rayfile_read (Marx_Source_Type *st, Marx_Photon_Attr_Type *at, unsigned int num,
+                      unsigned int *n, unsigned long *history)
+{
+   unsigned int i;
+   double t = 0.0;
+   if ((st == NULL) || (st->ray_fp == NULL)) return -1;
+   *n = (unsigned int) fread (at, sizeof (Marx_Photon_Attr_Type), num, st->ray_fp);
+   for (i = 0; i < *n; i++)
+     {
+        t = at[i].arrival_time;
+        at[i].arrival_time -= st->ray_last_time;
+     }
+   st->ray_last_time = t;
+   *history = st->ray_history;
+   return 0;
 }
 
 int marx_open_source (Marx_Source_Type *st)
@@ -353,6 +411,7 @@ int marx_open_source (Marx_Source_Type *st)
 int marx_close_source (Marx_Source_Type *st)
 {
    if (st == NULL) return -1;
+   if (st->ray_fp != NULL) fclose (st->ray_fp);
    free (st);
    return 0;
 }

@@ -110,6 +110,20 @@ struct SegWriter
 // Launches k_compact over the segments a stage kernel of io.grid blocks wrote.
 hipError_t launch_compact (const ListIO &io, hipStream_t st);
 
+// Output files (output.hip): the photons not flagged bad, in order, then the
+// requested columns of them packed big-endian, column k at out + offset[k].
+constexpr int kMaxPackCols = 32;
+struct PackArgs
+{
+   int32_t ncols;
+   int32_t col[kMaxPackCols];
+   uint64_t offset[kMaxPackCols];
+   double total_time;
+};
+hipError_t launch_keep_list (const uint32_t *flags, uint32_t n, const ListIO &io, hipStream_t st);
+hipError_t launch_pack_columns (const Soa &s, const uint32_t *idx, const uint32_t *n_kept, uint32_t m,
+                                const PackArgs &a, uint8_t *out, hipStream_t st);
+
 // Minimum waves per SIMD of the mirror's reflection kernel unless the
 // launch schedule says otherwise (256 lanes per block = one wave per SIMD
 // per block).  Unbounded, the compiler gives it ~200 VGPRs, i.e. two waves
@@ -142,6 +156,8 @@ hipError_t launch_detect (const mxg_config *cfg, const float *fa, const double *
                           ListIO io, uint32_t n_upper, double tstart,
                           unsigned long long seed, uint32_t source_id, hipStream_t st);
 
+hipError_t launch_dither_store (const mxg_config *cfg, Soa s, uint32_t n, double start_time,
+                                unsigned long long seed, hipStream_t st);
 hipError_t launch_source (const mxg_config *cfg, const double *da, Soa s, uint32_t n,
                           uint32_t tag_start, double exposure_time, double start_time,
                           unsigned long long seed, uint32_t source_id, uint64_t *ticks,

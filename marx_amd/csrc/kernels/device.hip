@@ -9,6 +9,7 @@
 #include <stdint.h>
 
 #include "dev_store.h"
+#include "mxg_columns.h"
 #include "mxg_device.h"
 
 using namespace mxg;
@@ -427,6 +428,15 @@ int mxgd_download_aos (Marx_Photon_Attr_Type *at, unsigned int n)
    HIPCHECK (hipGetLastError ());
    HIPCHECK (hipMemcpyAsync (at, d_aos, (size_t) n * sizeof (Marx_Photon_Attr_Type),
                              hipMemcpyDeviceToHost, g_stream));
+   HIPCHECK (hipStreamSynchronize (g_stream));
+   return 0;
+}
+
+int mxgd_dither_store (double start_time, unsigned long long seed)
+{
+   if (mxgd_init ()) return -1;
+   if (!d_cfg) return fail ("configuration not on the device");
+   HIPCHECK (launch_dither_store (d_cfg, g_soa, g_n, start_time, seed, g_stream));
    HIPCHECK (hipStreamSynchronize (g_stream));
    return 0;
 }
@@ -900,3 +910,91 @@ void mxgd_graph_destroy (mxgd_graph *g)
 
 } // extern "C"
 
+/* ---- output files ------------------------------------------------------------------ */
+
+namespace {
+uint32_t *d_out_list = nullptr;      // segments, then the list of photons written
+size_t g_out_list_cap = 0;
+uint32_t *d_out_u32 = nullptr;       // [0] photons in the store, [1] photons written
+uint8_t *d_out_bytes = nullptr;      // packed columns
+size_t g_out_bytes_cap = 0;
+unsigned char *h_out_bytes = nullptr;    // their pinned host copy
+size_t g_out_host_cap = 0;
+} // namespace
+
+extern "C" {
+
+int mxgd_pack_output (const int *cols, int ncols, double total_time, unsigned int *n_rows,
+                      const unsigned char **bytes, size_t *col_off)
+{
+   if (mxgd_init ()) return -1;
+   if ((ncols < 0) || (ncols > kMaxPackCols)) return fail ("%d output columns (at most %d)", ncols, kMaxPackCols);
+   if (g_capturing) return fail ("output cannot be written during graph capture");
+   *n_rows = 0;
+   *bytes = nullptr;
+   const uint32_t n = g_n;
+   if ((n == 0) || (ncols == 0)) return 0;
+   if (g_out_list_cap < n)
+     {
+        HIPCHECK (hipStreamSynchronize (g_stream));
+        if (d_out_list) (void) hipFree (d_out_list);
+        d_out_list = nullptr;
+        g_out_list_cap = 0;
+        HIPCHECK (hipMalloc (&d_out_list, 2 * align256 ((size_t) n * sizeof (uint32_t))));
+        g_out_list_cap = n;
+     }
+   if (!d_out_u32) HIPCHECK (hipMalloc (&d_out_u32, 4 * sizeof (uint32_t)));
+   ListIO io;
+   io.idx_in = nullptr;
+   io.n_in = d_out_u32;
+   io.seg = d_out_list;
+   io.idx_out = d_out_list + align256 (g_out_list_cap * sizeof (uint32_t)) / sizeof (uint32_t);
+   io.n_out = d_out_u32 + 1;
+   io.seg_count = d_seg_count;
+   io.grid = stage_grid (n, MXGD_K_DETECT);
+   hipLaunchKernelGGL (k_set_u32, dim3 (1), dim3 (1), 0, g_stream, d_out_u32, n);
+   HIPCHECK (hipGetLastError ());
+   HIPCHECK (launch_keep_list (g_soa.flags, n, io, g_stream));
+   uint32_t m = 0;
+   HIPCHECK (hipMemcpyAsync (&m, d_out_u32 + 1, sizeof (uint32_t), hipMemcpyDeviceToHost, g_stream));
+   HIPCHECK (hipStreamSynchronize (g_stream));
+   if (m == 0) return 0;
+
+   PackArgs a;
+   memset (&a, 0, sizeof (a));
+   a.ncols = ncols;
+   a.total_time = total_time;
+   size_t off = 0;
+   for (int k = 0; k < ncols; k++)
+     {
+        if ((cols[k] < 0) || (cols[k] >= MXG_NUM_COLS)) return fail ("unknown output column %d", cols[k]);
+        a.col[k] = cols[k];
+        a.offset[k] = off;
+        col_off[k] = off;
+        off += align256 ((size_t) m * MXG_COL_WIDTH (cols[k]));
+     }
+   if (g_out_bytes_cap < off)
+     {
+        if (d_out_bytes) (void) hipFree (d_out_bytes);
+        d_out_bytes = nullptr;
+        g_out_bytes_cap = 0;
+        HIPCHECK (hipMalloc (&d_out_bytes, off));
+        g_out_bytes_cap = off;
+     }
+   if (g_out_host_cap < off)
+     {
+        if (h_out_bytes) (void) hipHostFree (h_out_bytes);
+        h_out_bytes = nullptr;
+        g_out_host_cap = 0;
+        HIPCHECK (hipHostMalloc ((void **) &h_out_bytes, off, hipHostMallocDefault));
+        g_out_host_cap = off;
+     }
+   HIPCHECK (launch_pack_columns (g_soa, io.idx_out, io.n_out, m, a, d_out_bytes, g_stream));
+   HIPCHECK (hipMemcpyAsync (h_out_bytes, d_out_bytes, off, hipMemcpyDeviceToHost, g_stream));
+   HIPCHECK (hipStreamSynchronize (g_stream));
+   *n_rows = m;
+   *bytes = h_out_bytes;
+   return 0;
+}
+
+} // extern "C"

@@ -225,6 +225,27 @@ void k_source_finish (const mxg_config *__restrict__ cfg, Soa s, uint32_t n,
      apply_internal_dither (src, s, i, t, start_time, seed, source_id);
 }
 
+// the configured dither applied to photons 0..n-1 of the store, each at its
+// own arrival time (dither.c:630-658 on photons read from a ray file)
+__global__ __launch_bounds__ (kBlock)
+void k_dither_store (const mxg_config *__restrict__ cfg, Soa s, uint32_t n, double start_time,
+                     unsigned long long seed)
+{
+   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= n) return;
+   if (cfg->src.dither_mode == MXG_DITHER_INTERNAL)
+     apply_internal_dither (cfg->src, s, i, s.t[i], start_time, seed, 0u);
+}
+
+hipError_t launch_dither_store (const mxg_config *cfg, Soa s, uint32_t n, double start_time,
+                                unsigned long long seed, hipStream_t st)
+{
+   if (n == 0) return hipSuccess;
+   hipLaunchKernelGGL (k_dither_store, dim3 ((n + kBlock - 1) / kBlock), dim3 (kBlock), 0, st, cfg, s, n,
+                       start_time, seed);
+   return hipGetLastError ();
+}
+
 __global__ void k_source_count (uint32_t n, const uint32_t *__restrict__ first_over,
                                 uint32_t *__restrict__ count, uint32_t *__restrict__ idx)
 {

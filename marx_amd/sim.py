@@ -8,6 +8,7 @@ Mirrors the reference's operator interface: a parameter file drives
 -1 raises ``RuntimeError`` carrying the library's error message.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -102,6 +103,17 @@ class Photons:
         idx[:n] = order.astype(np.uint32)
         en = np.ctypeslib.as_array(p.sorted_energies, shape=(self.capacity,))
         en[:n] = records["energy"][order]
+
+    def write(self, directory, write_mask=_lib.WRITE_ALL, open_mode=1, total_time=0.0):
+        """marx_write_photons: this list's column files into directory."""
+        check(self.lib.marx_write_photons(self.ptr, int(write_mask), os.fsencode(directory), int(open_mode),
+                                          float(total_time)), "marx_write_photons")
+
+    def dump_to_rayfile(self, path, new_file=1, total_time=0.0):
+        """marx_dump_to_rayfile (adds total_time to the written photons'
+        arrival times in this list, as the reference does)."""
+        check(self.lib.marx_dump_to_rayfile(os.fsencode(path), int(new_file), self.ptr, float(total_time)),
+              "marx_dump_to_rayfile")
 
     def __del__(self):
         try:
@@ -231,6 +243,14 @@ class Tracer:
 
     def download(self, photons):
         check(self.lib.mxg_download_photons(photons.ptr), "mxg_download_photons")
+
+    def write_device(self, directory, write_mask=_lib.WRITE_ALL, open_mode=1, total_time=0.0):
+        """mxg_write_device: the column files of the last device-resident
+        trace into directory (which must exist).  Returns the rows written."""
+        n = ctypes.c_uint()
+        check(self.lib.mxg_write_device(int(write_mask), os.fsencode(directory), int(open_mode),
+                                        float(total_time), ctypes.byref(n)), "mxg_write_device")
+        return n.value
 
     def set_seed(self, seed):
         self.lib.mxg_set_seed(int(seed))

@@ -84,6 +84,8 @@ int orc_interpolate_n_fvector (const float *xp, float **yp, unsigned int n,
 /* Stage-major passes, in the order marx/src/marx.c:240-273 calls them. */
 int orc_create_photons (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int num,
                         unsigned int *num_collected, double *exposure_time);
+/* The dither applied to the photons of pt as replayed from a ray file. */
+int orc_dither_photons (const orc_ctx *c, Marx_Photon_Type *pt);
 /* The differentiable mirror (mxg_diff_forward / mxg_diff_adjoint): weights,
  * focal-plane positions and their derivatives per photon, an FP64 image,
  * and the gradient for d(loss)/d(image) = g. */

@@ -80,6 +80,8 @@ def load():
     lib.orc_diff_forward_parallel.restype = ctypes.c_int
     lib.orc_diff_adjoint_parallel.argtypes = lib.orc_diff_adjoint.argtypes + [ctypes.c_int]
     lib.orc_diff_adjoint_parallel.restype = ctypes.c_int
+    lib.orc_dither_photons.argtypes = [ctypes.POINTER(OrcCtx), ctypes.c_void_p]
+    lib.orc_dither_photons.restype = ctypes.c_int
     lib.orc_sort_photons.argtypes = [ctypes.c_void_p]
     lib.orc_sort_photons.restype = ctypes.c_int
     lib.orc_trace_parallel.argtypes = [ctypes.POINTER(OrcCtx), ctypes.c_uint, ctypes.c_uint, ctypes.c_int]
@@ -172,6 +174,10 @@ class Oracle:
 
     def reflect(self, photons):
         self._run(self.lib.orc_mirror_reflect, photons)
+
+    def dither(self, photons):
+        """The dither of a replayed ray file applied to photons in place."""
+        self._run(self.lib.orc_dither_photons, photons)
 
     def diffract(self, photons):
         self._run(self.lib.orc_grating_diffract, photons)

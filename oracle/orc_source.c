@@ -70,6 +70,19 @@ static void dither_photon (const orc_ctx *c, const mxg_source_cfg *s, Marx_Photo
    at->p = apply_dither (ra + delta_ra, dec + delta_dec, roll, at->p);
 }
 
+/* The configured dither applied to every photon of pt at its own arrival
+ * time and pt->start_time (dither.c:630-658), as marx_create_photons applies
+ * it to photons replayed from a ray file (source.c:344-359). */
+int orc_dither_photons (const orc_ctx *c, Marx_Photon_Type *pt)
+{
+   const mxg_source_cfg *s = &c->cfg->src;
+   unsigned int i;
+   if (s->dither_mode == MXG_DITHER_INTERNAL)
+     for (i = 0; i < pt->n_photons; i++)
+       dither_photon (c, s, pt->attributes + i, pt->start_time);
+   return 0;
+}
+
 int orc_create_photons (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int num,
                         unsigned int *num_collected, double *exposure_time)
 {

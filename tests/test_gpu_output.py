@@ -1,0 +1,138 @@
+"""Output files and ray-file replay on the GPU.
+
+- mxg_write_device (rows selected and columns packed big-endian in HBM)
+  against marx_write_photons on the downloaded list and against the numpy
+  restatement of the format (oracle/dumpfile.py): the same bytes, over two
+  chunks written and appended, with the dither columns;
+- ray files: photons dumped by marx_dump_to_rayfile and replayed by a
+  RAYFILE source (marx_create_photons) are the dumped records with new tags
+  and relative times; without dither they trace through mirror, grating and
+  detector exactly as the originals; with dither the replay applies it once
+  more, as the reference does (source.c:344-359), equal to the oracle's."""
+import os
+
+import numpy as np
+import pytest
+
+from marx_amd import Photons, Tracer
+from marx_amd._lib import WRITE_ALL
+from marx_amd.workloads import WORKLOADS, batched_acis_i
+from oracle import dumpfile as D
+from oracle.oracle import Oracle
+from tests.test_gpu_batch import assert_same_records
+
+ON_AXIS = dict(SourceRA=250.2134679741175, SourceDEC=-53.75743813458669)
+
+
+def _read_dir(d):
+    return {f: open(os.path.join(d, f), "rb").read() for f in sorted(os.listdir(d))}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("workload", ["hetg-acis-s", "letg-hrc-s"])
+def test_device_writer_equals_the_aos_writer(gpu, tmp_path, workload):
+    n = 200_000
+    t = WORKLOADS[workload](str(tmp_path), photons=n, exposure=1.0e4)
+    a, b = tmp_path / "device", tmp_path / "aos"
+    a.mkdir()
+    b.mkdir()
+    p = Photons(n)
+    chunks = []
+    for k, (mode, total_time) in enumerate(((1, 0.0), (0, 1.0e4))):
+        ncol, ndet, _ = t.trace_device(n, tag_start=k * n, start_time=k * 1.0e4)
+        written = t.write_device(str(a), open_mode=mode, total_time=total_time)
+        assert written == ndet
+        t.download(p)
+        p.write(str(b), open_mode=mode, total_time=total_time)
+        chunks.append((p.active.copy(), p.pt.history, total_time))
+    got, aos = _read_dir(a), _read_dir(b)
+    assert sorted(got) == sorted(aos)
+    for f in aos:
+        assert got[f] == aos[f], f
+    want = D.column_files(chunks, WRITE_ALL, dither_on=True)
+    assert sorted(want) == sorted(got)
+    for f in want:
+        assert got[f] == want[f], f
+    assert D.read_column(got["tag.dat"])[2] == sum(int(((c[0]["flags"] & 0xFF) == 0).sum()) for c in chunks)
+
+
+@pytest.mark.gpu
+def test_device_writer_after_a_batch(gpu, tmp_path):
+    t, dirs, counts = batched_acis_i(str(tmp_path), num_sources=64, max_count=3000)
+    n = int(counts.sum())
+    t.trace_batch(dirs, counts, use_graph=False)
+    a, b = tmp_path / "device", tmp_path / "aos"
+    a.mkdir()
+    b.mkdir()
+    t.write_device(str(a), total_time=5.0)
+    p = Photons(n)
+    t.download(p)
+    p.write(str(b), total_time=5.0)
+    assert _read_dir(a) == _read_dir(b)
+    assert _read_dir(a) == D.column_files([(p.active.copy(), p.pt.history, 5.0)], WRITE_ALL, dither_on=False)
+
+
+@pytest.mark.gpu
+def test_device_writer_needs_a_device_trace(gpu, tmp_path):
+    t = Tracer(seed=2, source=True, GratingType="NONE", DetectorType="ACIS-S", SourceType="POINT",
+               SpectrumType="FLAT", MinEnergy=1.0, MaxEnergy=2.0, DitherModel="NONE", **ON_AXIS)
+    p = Photons(1000)
+    t.create_photons(p, 1000)
+    t.reflect(p)            # an AoS list now occupies the device store
+    with pytest.raises(RuntimeError, match="no device-resident trace"):
+        t.write_device(str(tmp_path))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dither", ["NONE", "INTERNAL"])
+def test_rayfile_replay(gpu, tmp_path, dither):
+    n = 20_000
+    ray = str(tmp_path / "rays.dat")
+    t = Tracer(seed=5, GratingType="HETG", DetectorType="ACIS-S", SourceType="POINT", SpectrumType="FLAT",
+               MinEnergy=0.5, MaxEnergy=7.0, DitherModel=dither, **ON_AXIS)
+    t.init_source()
+    src = Photons(n)
+    m = t.create_photons(src, n)
+    src.dump_to_rayfile(ray, new_file=1, total_time=0.0)
+    dumped = src.active.copy()
+
+    t.init_source(SourceType="RAYFILE", RayFile=ray)
+    rep = Photons(n)
+    assert t.create_photons(rep, n) == m
+    assert rep.pt.history & 0x1F == 0x1F
+    r = rep.active.copy()
+    if dither == "NONE":
+        assert_same_records(r, dumped, "replayed records")
+        assert np.array_equal(rep.sorted_index(), src.sorted_index())
+        for q in (src, rep):
+            t.reflect(q)
+            t.diffract(q)
+            t.detect(q)
+        assert_same_records(rep.active.copy(), src.active.copy(), "replayed photons traced")
+    else:
+        po = Photons(n)
+        po.load(dumped, sort=False)
+        po.pt.start_time = 0.0
+        Oracle(t).dither(po)
+        o = po.active.copy()
+        for f in ("energy", "x", "arrival_time", "tag", "flags", "dither"):
+            assert np.array_equal(r[f], o[f]), f
+        np.testing.assert_allclose(r["p"], o["p"], rtol=0, atol=1e-12)
+        # dithered twice: the directions moved again
+        assert np.abs(r["p"] - dumped["p"]).max() > 1e-9
+    # a second read continues after the records already replayed
+    assert t.create_photons(rep, n) == 0
+
+
+@pytest.mark.gpu
+def test_rayfile_source_is_not_traced_on_the_device(gpu, tmp_path):
+    ray = str(tmp_path / "rays.dat")
+    t = Tracer(seed=5, GratingType="NONE", DetectorType="ACIS-S", SourceType="POINT", SpectrumType="FLAT",
+               MinEnergy=1.0, MaxEnergy=2.0, DitherModel="NONE", **ON_AXIS)
+    t.init_source()
+    p = Photons(100)
+    t.create_photons(p, 100)
+    p.dump_to_rayfile(ray)
+    t.init_source(SourceType="RAYFILE", RayFile=ray)
+    with pytest.raises(RuntimeError, match="RAYFILE"):
+        t.trace_device(100)
