@@ -102,8 +102,15 @@ DESCRIPTION["output"] = (
     "Output files: the column files (marx_write_photons format, every column) of one traced HETG + "
     "ACIS-S chunk of 1e7 photons, written from HBM by mxg_write_device, against downloading the chunk "
     "and writing it with marx_write_photons")
+DESCRIPTION["sources"] = (
+    "Source models: 1e7 photons per step of an on-axis POINT, DISK (2 arcmin), GAUSS (60 arcsec), BETA "
+    "(10 arcsec core, beta 0.7) and LINE (30 arcmin) source, flat 0.3-10 keV, INTERNAL dither, created on "
+    "the device and traced through HRMA + ACIS-S")
 DEFAULT_PHOTONS = {"hetg-acis-s": 10_000_000, "letg-hrc-s": 50_000_000, "batched-acis-i": 0,
-                   "diff": 10_000_000, "output": 10_000_000}
+                   "diff": 10_000_000, "output": 10_000_000, "sources": 10_000_000}
+SOURCE_SHAPES = {"POINT": {}, "DISK": {"S-DiskTheta0": 0.0, "S-DiskTheta1": 120.0},
+                 "GAUSS": {"S-GaussSigma": 60.0}, "BETA": {"S-BetaCoreRadius": 10.0, "S-BetaBeta": 0.7},
+                 "LINE": {"S-LineTheta": 1800.0, "S-LinePhi": 30.0}}
 DIFF_NPIX, DIFF_PIXEL_MM = 256, 0.004
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
@@ -441,6 +448,43 @@ def measure_output(args, tmp, torch):
     }
 
 
+def measure_sources(args, torch):
+    """Source models: per shape, steps of 1e7 photons created on the device
+    (k_source, k_scan_tiles, k_source_finish) and traced through HRMA and
+    ACIS-S; the source stage's device time and the whole step's."""
+    from marx_amd import Tracer
+    from marx_amd.workloads import ON_AXIS
+    cfg, K = "sources", args.steps
+    N = args.photons or DEFAULT_PHOTONS[cfg]
+    res = {}
+    for shape, par in SOURCE_SHAPES.items():
+        t = Tracer(seed=1, GratingType="NONE", DetectorType="ACIS-S", SourceType=shape, SpectrumType="FLAT",
+                   MinEnergy=0.3, MaxEnergy=10.0, DitherModel="INTERNAL", **par, **ON_AXIS)
+        t.init_source(SourceFlux=N / (args.exposure * t.geometric_area))
+        for k in range(args.warmup):
+            t.trace_device(N, tag_start=k * N, start_time=k * args.exposure)
+        torch.cuda.synchronize()
+        src_ms = 0.0
+        t0 = time.perf_counter()
+        for k in range(K):
+            _, _, ms = t.trace_device(N, tag_start=k * N, start_time=k * args.exposure, timing=True)
+            src_ms += ms[0]
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / K
+        res[shape] = {"step_ms": el * 1e3, "photons_per_s": N / el, "source_stage_ms": src_ms / K,
+                      "source_photons_per_s": N / (src_ms / K * 1e-3)}
+        print(f"bench sources: {shape} {res[shape]}", file=sys.stderr, flush=True)
+    v = min(r["photons_per_s"] for r in res.values())
+    return {
+        "metric": "photons/s traced (HRMA+grating+detector) 1 GPU vs host CPU; gradient evals/s",
+        "value": v, "unit": "photons/s (slowest shape)", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": max(r["step_ms"] for r in res.values()), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": DESCRIPTION[cfg], "name": cfg, "photons_per_step": N},
+        "shapes": res,
+    }
+
+
 def cpu_baseline_batch(tracer, dirs, counts, exposure, target_seconds=20.0):
     """The FP64 oracle source by source (one OpenMP task per source) over the
     first sources of the batch, as many as take about target_seconds."""
@@ -526,6 +570,8 @@ def measure():
         return measure_diff(args, torch)
     if args.config == "output":
         return measure_output(args, tmp, torch)
+    if args.config == "sources":
+        return measure_sources(args, torch)
     cfg = args.config
     N, T = (args.photons or DEFAULT_PHOTONS[cfg]), args.exposure
     tracer = WORKLOADS[cfg](tmp, photons=N, exposure=T)

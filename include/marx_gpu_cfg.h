@@ -179,6 +179,9 @@ mxg_detector_cfg;
 #define MXG_SOURCE_POINT   0
 #define MXG_SOURCE_DISK    1
 #define MXG_SOURCE_RAYFILE 2       /* replayed on the host by marx_create_photons */
+#define MXG_SOURCE_GAUSS   3
+#define MXG_SOURCE_BETA    4
+#define MXG_SOURCE_LINE    5
 #define MXG_SPECTRUM_FLAT  1
 #define MXG_SPECTRUM_FILE  2
 #define MXG_TIME_POISSON   0
@@ -204,6 +207,11 @@ typedef struct
    double ra_phase, dec_phase, roll_phase;
    double nominal_roll;
    double aspect_blur;              /* radians, sigma */
+   double gauss_sigma;              /* GAUSS: radians (s-gauss.c) */
+   double beta_core, beta_xpon;     /* BETA: core radius (radians), 1 / (1 - alpha) (s-beta.c) */
+   double line_theta;               /* LINE: half length (radians) (s-line.c) */
+   double line_cos_phi, line_sin_phi;    /* of its orientation angle */
+   double line_alpha, line_normal[3];    /* the rotation of -x onto the source direction */
 }
 mxg_source_cfg;
 

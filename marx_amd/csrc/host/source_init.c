@@ -278,6 +278,33 @@ static int spectrum_init (Param_File_Type *pf, mxg_source_cfg *s, const char *na
    return -1;
 }
 
+/* The angle and unit axis that rotate a onto b (JDMv_find_rotation_axis,
+ * jdmath/src/vector.c:289-331, with its choice of axis when a and b are
+ * parallel). */
+static double rotation_axis (JDMVector_Type a, JDMVector_Type b, JDMVector_Type *c)
+{
+   double theta = mxg_v_dot (a, b), len;
+   JDMVector_Type cc;
+   if (theta > 1.0) theta = 1.0;
+   else if (theta < -1.0) theta = -1.0;
+   theta = acos (theta);
+   cc = mxg_v_cross (a, b);
+   len = mxg_v_length (cc);
+   if (len > 0.0)
+     {
+        c->x = cc.x / len;
+        c->y = cc.y / len;
+        c->z = cc.z / len;
+        return theta;
+     }
+   if ((a.x != 0.0) || (a.y != 0.0))
+     *c = mxg_v (-a.y, -a.x, 0.0);
+   else
+     *c = mxg_v (-a.z, 0.0, 0.0);
+   mxg_v_normalize (c);
+   return theta;
+}
+
 int mxg_source_init (Param_File_Type *pf)
 {
    mxg_source_cfg *s = &Mxg_Cfg.src;
@@ -294,9 +321,12 @@ int mxg_source_init (Param_File_Type *pf)
    if (!strcmp (name, "POINT")) s->type = MXG_SOURCE_POINT;
    else if (!strcmp (name, "DISK")) s->type = MXG_SOURCE_DISK;
    else if (!strcmp (name, "RAYFILE")) s->type = MXG_SOURCE_RAYFILE;
+   else if (!strcmp (name, "GAUSS")) s->type = MXG_SOURCE_GAUSS;
+   else if (!strcmp (name, "BETA")) s->type = MXG_SOURCE_BETA;
+   else if (!strcmp (name, "LINE")) s->type = MXG_SOURCE_LINE;
    else
      {
-        mxg_error ("SourceType %s not supported (POINT, DISK and RAYFILE are)", name);
+        mxg_error ("SourceType %s not supported (POINT, DISK, GAUSS, BETA, LINE and RAYFILE are)", name);
         return -1;
      }
    if (s->type == MXG_SOURCE_RAYFILE)
@@ -351,6 +381,45 @@ int mxg_source_init (Param_File_Type *pf)
           return -1;
         s->disk_theta = th1 * (1.0 / 3600.0 * PI / 180.0);
         s->disk_theta_min = th0 * (1.0 / 3600.0 * PI / 180.0);
+     }
+   else if (s->type == MXG_SOURCE_GAUSS)
+     {
+        /* s-gauss.c:146-160, 87-88 */
+        if (-1 == pf_get_double (pf, "S-GaussSigma", &s->gauss_sigma))
+          return -1;
+        s->gauss_sigma *= (1.0 / 3600.0 * PI / 180.0);
+     }
+   else if (s->type == MXG_SOURCE_BETA)
+     {
+        /* s-beta.c:132-162, 99 */
+        double core, beta, alpha;
+        if ((-1 == pf_get_double (pf, "S-BetaCoreRadius", &core))
+            || (-1 == pf_get_double (pf, "S-BetaBeta", &beta)))
+          return -1;
+        alpha = 3.0 * beta - 0.5;
+        if ((alpha <= 1.0) || (core <= 0.0))
+          {
+             mxg_error ("Beta parameters out of range.  Must be greater than 1/2.");
+             return -1;
+          }
+        s->beta_core = core * (1.0 / 3600.0) * (PI / 180.0);
+        s->beta_xpon = 1.0 / (1.0 - alpha);
+     }
+   else if (s->type == MXG_SOURCE_LINE)
+     {
+        /* s-line.c:104-122, 75-80: the line along +-theta in the plane of
+         * angle phi, rotated from -x onto the source direction */
+        double theta, phi;
+        JDMVector_Type a = mxg_v (-1.0, 0.0, 0.0), b = mxg_v (s->p[0], s->p[1], s->p[2]), c;
+        if ((-1 == pf_get_double (pf, "S-LinePhi", &phi))
+            || (-1 == pf_get_double (pf, "S-LineTheta", &theta)))
+          return -1;
+        s->line_theta = theta * (1.0 / 3600.0 * PI / 180.0);
+        phi = phi * (PI / 180.0);
+        s->line_cos_phi = cos (phi);
+        s->line_sin_phi = sin (phi);
+        s->line_alpha = rotation_axis (a, b, &c);
+        s->line_normal[0] = c.x; s->line_normal[1] = c.y; s->line_normal[2] = c.z;
      }
    if (-1 == spectrum_init (pf, s, name))
      return -1;

@@ -244,6 +244,19 @@ __device__ __forceinline__ double length (vec3 a)
    return z * sqrt (1.0 + x * x + y * y);
 }
 
+#ifdef MXG_FAST_NORMALIZE
+__device__ __forceinline__ void normalize (vec3 &a)
+{
+   double l2 = a.x * a.x + a.y * a.y + a.z * a.z;
+   if (l2 != 0.0)
+     {
+        double inv = 1.0 / sqrt (l2);
+        a.x = a.x * inv;
+        a.y = a.y * inv;
+        a.z = a.z * inv;
+     }
+}
+#else
 __device__ __forceinline__ void normalize (vec3 &a)
 {
    double len = length (a);
@@ -254,6 +267,7 @@ __device__ __forceinline__ void normalize (vec3 &a)
         a.z = a.z / len;
      }
 }
+#endif
 
 __device__ __forceinline__ vec3 unit (vec3 a) { normalize (a); return a; }
 

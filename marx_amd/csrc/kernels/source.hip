@@ -2,9 +2,11 @@
 // times, exposure cut and the INTERNAL dither model.
 //
 // Per photon (marx/libsrc/source.c:268-384, s-point.c:59-83, s-disk.c:63-110,
+// s-gauss.c:78-143, s-beta.c:81-130, s-line.c:60-101,
 // spectrum.c:138-189, prob.c:46-58, dither.c:167-182, 551-581, 609-658):
 //   energy from a FLAT or tabulated (inverse-CDF) spectrum; direction of a
-//   point source, or uniform over a disk; exponential gap to the previous
+//   point source, uniform over a disk, Gaussian or beta-model profile about
+//   the source, or uniform along a line; exponential gap to the previous
 //   photon (Poisson arrivals) or a uniform time; then Lissajous dither with
 //   Gaussian aspect blur applied to the direction.
 //
@@ -40,6 +42,67 @@ __device__ __forceinline__ void init_photon (Soa s, uint32_t i, double energy, v
    s.tag[i] = tag;
 }
 
+// The direction of a photon of the configured source shape about the source
+// direction p (p_normal: a unit vector normal to p; line_normal and
+// line_alpha: the rotation of -x onto p): a point source's p; uniform over
+// a disk (s-disk.c:95-106); a Gaussian or beta-model profile
+// (s-gauss.c:121-137, s-beta.c:108-124); uniform along a line
+// (s-line.c:89-98).  The draws follow the energy in the source stream.
+__device__ __forceinline__ vec3 extended_direction (const mxg_source_cfg &src, vec3 p, vec3 p_normal,
+                                                    vec3 line_normal, double line_alpha, Stream &st)
+{
+   if (src.type == MXG_SOURCE_DISK)
+     {
+        vec3 normal = rotate_unit_vector (p_normal, p, 2.0 * MXG_PI * uniform (st));
+        double x0 = src.disk_theta_min / src.disk_theta;
+        x0 = x0 * x0;
+        double x1 = 1.0 - x0;
+        double rnd = src.disk_theta * sqrt (x0 + x1 * uniform (st));
+        return rotate_unit_vector (p, normal, rnd);
+     }
+   if ((src.type == MXG_SOURCE_GAUSS) || (src.type == MXG_SOURCE_BETA))
+     {
+        vec3 normal = rotate_unit_vector (p_normal, p, 2.0 * MXG_PI * uniform (st));
+        double rnd;
+        do rnd = uniform (st);
+        while (rnd == 0.0);
+        if (src.type == MXG_SOURCE_GAUSS)
+          rnd = src.gauss_sigma * sqrt (-log (rnd));
+        else
+          rnd = src.beta_core * sqrt (pow (rnd, src.beta_xpon) - 1.0);
+        return rotate_unit_vector (p, normal, rnd);
+     }
+   if (src.type == MXG_SOURCE_LINE)
+     {
+        double theta = src.line_theta * (-1.0 + 2.0 * uniform (st));
+        double sn, cs;
+        sincos (theta, &sn, &cs);
+        const double sin_theta = -sn;
+        vec3 q = v3 (-cs, sin_theta * src.line_cos_phi, sin_theta * src.line_sin_phi);
+        return rotate_unit_vector (q, line_normal, line_alpha);
+     }
+   return p;
+}
+
+// the angle and unit axis that rotate a onto b (jdmath/src/vector.c:289-331)
+__device__ __forceinline__ double rotation_axis (vec3 a, vec3 b, vec3 &c)
+{
+   double theta = dot (a, b);
+   if (theta > 1.0) theta = 1.0;
+   else if (theta < -1.0) theta = -1.0;
+   theta = acos (theta);
+   vec3 cc = cross (a, b);
+   double len = length (cc);
+   if (len > 0.0)
+     {
+        c = v3 (cc.x / len, cc.y / len, cc.z / len);
+        return theta;
+     }
+   c = ((a.x != 0.0) || (a.y != 0.0)) ? v3 (-a.y, -a.x, 0.0) : v3 (-a.z, 0.0, 0.0);
+   normalize (c);
+   return theta;
+}
+
 __device__ __forceinline__ uint64_t gap_ticks (double e)
 {
    return (uint64_t) (e * 4294967296.0 + 0.5);
@@ -67,16 +130,8 @@ void k_source (const mxg_config *__restrict__ cfg, const double *__restrict__ da
              Stream st;
              stream_init (st, seed, tag, MXG_STAGE_SOURCE, source_id);
              double energy = draw_energy (src, da, st);
-             vec3 p = ld3 (src.p);
-             if (src.type == MXG_SOURCE_DISK)
-               {
-                  vec3 normal = rotate_unit_vector (ld3 (src.p_normal), p, 2.0 * MXG_PI * uniform (st));
-                  double x0 = src.disk_theta_min / src.disk_theta;
-                  x0 = x0 * x0;
-                  double x1 = 1.0 - x0;
-                  double rnd = src.disk_theta * sqrt (x0 + x1 * uniform (st));
-                  p = rotate_unit_vector (p, normal, rnd);
-               }
+             vec3 p = extended_direction (src, ld3 (src.p), ld3 (src.p_normal), ld3 (src.line_normal),
+                                          src.line_alpha, st);
              double t = 0.0;
              if (poisson)
                k = gap_ticks (expn (st));
@@ -324,6 +379,17 @@ void k_batch_source (const mxg_config *__restrict__ cfg, const double *__restric
         stream_init (st, seed, tag, MXG_STAGE_SOURCE, 0u);
         double energy = draw_energy (src, da, st);
         vec3 p = v3 (dir[3 * k], dir[3 * k + 1], dir[3 * k + 2]);
+        if (src.type != MXG_SOURCE_POINT)
+          {
+             // the shape about this source's direction: its normal as the
+             // source set-up makes it, and the rotation of -x onto it
+             vec3 nrm = v3 (-p.y / p.x, 1.0, 0.0);
+             normalize (nrm);
+             vec3 ln = v3 (0.0, 0.0, 0.0);
+             double la = 0.0;
+             if (src.type == MXG_SOURCE_LINE) la = rotation_axis (v3 (-1.0, 0.0, 0.0), p, ln);
+             p = extended_direction (src, p, nrm, ln, la, st);
+          }
         double t = exposure * uniform (st);
         init_photon (s, i, energy, p, t, tag);
         if (src.dither_mode == MXG_DITHER_INTERNAL)

@@ -4,12 +4,14 @@
  *   marx/libsrc/source.c:268-384  marx_create_photons (source call, arrival
  *                                 times with the exposure cut, tags, dither,
  *                                 energy sort)
- *   marx/libsrc/s-point.c:59-83, s-disk.c:63-110   directions
+ *   marx/libsrc/s-point.c:59-83, s-disk.c:63-110, s-gauss.c:78-143,
+ *   s-beta.c:81-130, s-line.c:60-101             directions
  *   marx/libsrc/spectrum.c:138-189, prob.c:46-58  FLAT / FILE energies
  *   marx/libsrc/dither.c:167-182, 551-581, 609-658  INTERNAL dither
  * Deviations fixed by the per-photon stream design, shared with the GPU:
- *  - the disk azimuth of each photon is drawn independently (the reference
- *    rotates one normal cumulatively from photon to photon, s-disk.c:99);
+ *  - the disk, Gaussian and beta-model azimuth of each photon is drawn
+ *    independently (the reference rotates one normal cumulatively from
+ *    photon to photon, s-disk.c:99, s-gauss.c:124, s-beta.c:111);
  *  - the exponential gaps are summed as 64-bit integers in units of 2^-32
  *    of the mean gap, so that a parallel scan gives the same times;
  *  - MXG_TIME_UNIFORM draws times uniformly over [0, tmax).
@@ -49,6 +51,65 @@ static double draw_energy (const orc_ctx *c, const mxg_source_cfg *s, orc_stream
      }
    return orc_interpolate_d (orc_uniform (st), c->da + s->file_cum_off, c->da + s->file_e_off,
                              s->file_n);
+}
+
+/* The direction of a photon of the configured shape about the source
+ * direction p (s-disk.c:95-106, s-gauss.c:121-137, s-beta.c:108-124,
+ * s-line.c:89-98); p_normal is normal to p, line_normal and line_alpha
+ * rotate -x onto p. */
+static orc_vec extended_direction (const mxg_source_cfg *s, orc_vec p, orc_vec p_normal,
+                                   orc_vec line_normal, double line_alpha, orc_stream *st)
+{
+   if (s->type == MXG_SOURCE_DISK)
+     {
+        double x0 = s->disk_theta_min / s->disk_theta, x1, rnd;
+        orc_vec normal;
+        x0 = x0 * x0;
+        x1 = 1.0 - x0;
+        normal = orc_v_rotate_unit_vector (p_normal, p, 2.0 * PI * orc_uniform (st));
+        rnd = s->disk_theta * sqrt (x0 + x1 * orc_uniform (st));
+        return orc_v_rotate_unit_vector (p, normal, rnd);
+     }
+   if ((s->type == MXG_SOURCE_GAUSS) || (s->type == MXG_SOURCE_BETA))
+     {
+        orc_vec normal = orc_v_rotate_unit_vector (p_normal, p, 2.0 * PI * orc_uniform (st));
+        double rnd;
+        do rnd = orc_uniform (st);
+        while (rnd == 0.0);
+        if (s->type == MXG_SOURCE_GAUSS)
+          rnd = s->gauss_sigma * sqrt (-log (rnd));
+        else
+          rnd = s->beta_core * sqrt (pow (rnd, s->beta_xpon) - 1.0);
+        return orc_v_rotate_unit_vector (p, normal, rnd);
+     }
+   if (s->type == MXG_SOURCE_LINE)
+     {
+        double theta = s->line_theta * (-1.0 + 2.0 * orc_uniform (st));
+        double sin_theta = -sin (theta);
+        orc_vec q = orc_v (-cos (theta), sin_theta * s->line_cos_phi, sin_theta * s->line_sin_phi);
+        return orc_v_rotate_unit_vector (q, line_normal, line_alpha);
+     }
+   return p;
+}
+
+/* the angle and unit axis rotating a onto b (jdmath/src/vector.c:289-331) */
+static double rotation_axis (orc_vec a, orc_vec b, orc_vec *c)
+{
+   double theta = orc_v_dot (a, b), len;
+   orc_vec cc;
+   if (theta > 1.0) theta = 1.0;
+   else if (theta < -1.0) theta = -1.0;
+   theta = acos (theta);
+   cc = orc_v_cross (a, b);
+   len = orc_v_length (cc);
+   if (len > 0.0)
+     {
+        *c = orc_v (cc.x / len, cc.y / len, cc.z / len);
+        return theta;
+     }
+   *c = ((a.x != 0.0) || (a.y != 0.0)) ? orc_v (-a.y, -a.x, 0.0) : orc_v (-a.z, 0.0, 0.0);
+   orc_v_normalize (c);
+   return theta;
 }
 
 /* INTERNAL dither of one photon (dither.c:167-182, 551-581, 609-658) */
@@ -107,17 +168,10 @@ int orc_create_photons (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int num
         at[i].tag = pt->tag_start + i;
         orc_stream_init (st, c->seed, at[i].tag, MXG_STAGE_SOURCE, c->source_id);
         at[i].energy = draw_energy (c, s, st);
-        at[i].p = orc_v (s->p[0], s->p[1], s->p[2]);
-        if (s->type == MXG_SOURCE_DISK)
-          {
-             orc_vec normal = orc_v (s->p_normal[0], s->p_normal[1], s->p_normal[2]);
-             double x0 = s->disk_theta_min / s->disk_theta, x1, rnd;
-             x0 = x0 * x0;
-             x1 = 1.0 - x0;
-             normal = orc_v_rotate_unit_vector (normal, at[i].p, 2.0 * PI * orc_uniform (st));
-             rnd = s->disk_theta * sqrt (x0 + x1 * orc_uniform (st));
-             at[i].p = orc_v_rotate_unit_vector (at[i].p, normal, rnd);
-          }
+        at[i].p = extended_direction (s, orc_v (s->p[0], s->p[1], s->p[2]),
+                                      orc_v (s->p_normal[0], s->p_normal[1], s->p_normal[2]),
+                                      orc_v (s->line_normal[0], s->line_normal[1], s->line_normal[2]),
+                                      s->line_alpha, st);
      }
 
    /* arrival times */
@@ -189,6 +243,14 @@ int orc_create_batch (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int nsrc,
           orc_stream_init (&st, c->seed, at[i].tag, MXG_STAGE_SOURCE, c->source_id);
           at[i].energy = draw_energy (c, s, &st);
           at[i].p = orc_v (dirs[3 * k], dirs[3 * k + 1], dirs[3 * k + 2]);
+          if (s->type != MXG_SOURCE_POINT)
+            {
+               orc_vec p = at[i].p, nrm = orc_v (-p.y / p.x, 1.0, 0.0), ln = orc_v (0, 0, 0);
+               double la = 0.0;
+               orc_v_normalize (&nrm);
+               if (s->type == MXG_SOURCE_LINE) la = rotation_axis (orc_v (-1.0, 0.0, 0.0), p, &ln);
+               at[i].p = extended_direction (s, p, nrm, ln, la, &st);
+            }
           at[i].arrival_time = exposure * orc_uniform (&st);
           if (s->dither_mode == MXG_DITHER_INTERNAL)
             dither_photon (c, s, at + i, start_time);

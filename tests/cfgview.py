@@ -92,7 +92,9 @@ class SourceCfg(ctypes.Structure):
                 ("ra_amp", D), ("dec_amp", D), ("roll_amp", D),
                 ("ra_period", D), ("dec_period", D), ("roll_period", D),
                 ("ra_phase", D), ("dec_phase", D), ("roll_phase", D),
-                ("nominal_roll", D), ("aspect_blur", D)]
+                ("nominal_roll", D), ("aspect_blur", D), ("gauss_sigma", D), ("beta_core", D), ("beta_xpon", D),
+                ("line_theta", D), ("line_cos_phi", D), ("line_sin_phi", D), ("line_alpha", D),
+                ("line_normal", D * 3)]
 
 
 class Config(ctypes.Structure):

@@ -13,10 +13,11 @@ from tests.parity import compare
 ON_AXIS = dict(SourceRA=250.2134679741175, SourceDEC=-53.75743813458669)
 
 
-def _batch(tmp_path, nsrc, lo=100, hi=5000, seed=41, dither="NONE", grating="NONE", det="ACIS-I"):
+def _batch(tmp_path, nsrc, lo=100, hi=5000, seed=41, dither="NONE", grating="NONE", det="ACIS-I",
+           shape="POINT", **shape_par):
     spec = write_power_law(str(tmp_path / "pl.dat"), gamma=2.0, emin=0.3, emax=10.0)
-    t = Tracer(seed=seed, GratingType=grating, DetectorType=det, SourceType="POINT", SpectrumType="FILE",
-               SpectrumFile=spec, DitherModel=dither, **ON_AXIS)
+    t = Tracer(seed=seed, GratingType=grating, DetectorType=det, SourceType=shape, SpectrumType="FILE",
+               SpectrumFile=spec, DitherModel=dither, **shape_par, **ON_AXIS)
     t.init_source()
     rng = np.random.default_rng(nsrc)
     dirs = offaxis_directions(np.radians(rng.uniform(0, 15, nsrc) / 60), rng.uniform(0, 2 * np.pi, nsrc))
@@ -68,6 +69,22 @@ def test_batch_parity(gpu, tmp_path, grating, det, dither):
     compare(g, o, max_flips=2, label=f"batch {grating}/{det}")
     assert ndet == int(((g["flags"] & 0xFF) == 0).sum())
     assert t.stage_counts()[0] == n
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape,par", [("DISK", {"S-DiskTheta0": 5.0, "S-DiskTheta1": 40.0}),
+                                       ("GAUSS", {"S-GaussSigma": 20.0}),
+                                       ("BETA", {"S-BetaCoreRadius": 6.0, "S-BetaBeta": 0.7}),
+                                       ("LINE", {"S-LineTheta": 300.0, "S-LinePhi": 60.0})])
+def test_batch_of_extended_sources(gpu, tmp_path, shape, par):
+    """Every source of the batch with the configured shape about its own
+    direction (s-disk.c, s-gauss.c, s-beta.c, s-line.c)."""
+    t, dirs, counts = _batch(tmp_path, 40, shape=shape, **par)
+    n = int(counts.sum())
+    t.trace_batch(dirs, counts, tag_start=9, start_time=0.0, exposure=2.0e4, use_graph=False)
+    g = _download(t, n)
+    o = _oracle(t, dirs, counts, 9, 0.0, 2.0e4, "NONE")
+    compare(g, o, max_flips=2, label=f"batch {shape}")
 
 
 @pytest.mark.gpu

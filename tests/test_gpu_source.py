@@ -1,6 +1,7 @@
 """Photon creation: GPU source kernels vs the CPU oracle through the C-ABI
 (marx_create_photons, replacing marx/libsrc/source.c:268-384 with
-s-point.c, s-disk.c, spectrum.c and the INTERNAL dither of dither.c), and
+s-point.c, s-disk.c, s-gauss.c, s-beta.c, s-line.c, spectrum.c and the
+INTERNAL dither of dither.c), and
 the whole chain source -> HRMA -> HETG -> ACIS-S, through the AoS entry
 points and through the device-resident mxg_trace_device."""
 import numpy as np
@@ -26,7 +27,8 @@ def _both(t, num, exposure=None):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind", ["flat-point-dither", "file-point", "flat-disk-dither", "uniform-times"])
+@pytest.mark.parametrize("kind", ["flat-point-dither", "file-point", "flat-disk-dither", "uniform-times",
+                                  "gauss-dither", "beta-offaxis", "line-file"])
 def test_source_parity(gpu, tmp_path, kind):
     extra = dict(ON_AXIS)
     utimes = 0.0
@@ -38,6 +40,18 @@ def test_source_parity(gpu, tmp_path, kind):
     elif kind == "flat-disk-dither":
         extra.update(SourceType="DISK", **{"S-DiskTheta0": 10.0, "S-DiskTheta1": 120.0},
                      SpectrumType="FLAT", MinEnergy=0.07, MaxEnergy=10.0, DitherModel="INTERNAL")
+    elif kind == "gauss-dither":
+        extra.update(SourceType="GAUSS", **{"S-GaussSigma": 45.0},
+                     SpectrumType="FLAT", MinEnergy=0.3, MaxEnergy=9.0, DitherModel="INTERNAL")
+    elif kind == "beta-offaxis":
+        extra = dict(SourceRA=ON_AXIS["SourceRA"] + 0.04, SourceDEC=ON_AXIS["SourceDEC"] - 0.02,
+                     SourceType="BETA", **{"S-BetaCoreRadius": 8.0, "S-BetaBeta": 0.67},
+                     SpectrumType="FLAT", MinEnergy=0.5, MaxEnergy=7.0, DitherModel="NONE")
+    elif kind == "line-file":
+        f = write_power_law(str(tmp_path / "pl.dat"))
+        extra = dict(SourceRA=ON_AXIS["SourceRA"] - 0.03, SourceDEC=ON_AXIS["SourceDEC"] + 0.01,
+                     SourceType="LINE", **{"S-LineTheta": 900.0, "S-LinePhi": 35.0},
+                     SpectrumType="FILE", SpectrumFile=f, DitherModel="NONE")
     else:
         extra.update(SpectrumType="FLAT", MinEnergy=1.49, MaxEnergy=1.49, DitherModel="NONE")
         utimes = 1.0e4
