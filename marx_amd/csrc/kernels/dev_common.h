@@ -154,6 +154,26 @@ __device__ __forceinline__ float interp_f (float x, const float *__restrict__ xp
    return (float) ((double) y0 + (double) dy / (x1 - x0) * ((double) x - x0));
 }
 
+// interp_f of two tables on one abscissa with one search (the same values)
+__device__ __forceinline__ void interp_f2 (float x, const float *__restrict__ xp, const float *__restrict__ ya,
+                                           const float *__restrict__ yb, unsigned int n, float &a, float &b)
+{
+   if (n == 1) { a = ya[0]; b = yb[0]; return; }
+   unsigned int n1 = search_f (x, xp, n);
+   unsigned int n0 = n1 - 1;
+   if ((n1 < n) && (x == xp[n1])) { a = ya[n1]; b = yb[n1]; return; }
+   if (n1 == n) { n1--; n0--; }
+   if (n1 == 0) n0 = 1;
+   double x0 = xp[n0], x1 = xp[n1];
+   if (x1 == x0) { a = ya[n1]; b = yb[n1]; return; }
+   float y0 = ya[n0];
+   float dy = ya[n1] - y0;
+   a = (float) ((double) y0 + (double) dy / (x1 - x0) * ((double) x - x0));
+   y0 = yb[n0];
+   dy = yb[n1] - y0;
+   b = (float) ((double) y0 + (double) dy / (x1 - x0) * ((double) x - x0));
+}
+
 __device__ __forceinline__ unsigned int search_d (double x, const double *__restrict__ xp, unsigned int n)
 {
    unsigned int n0 = 0, n1 = n;
@@ -244,7 +264,9 @@ __device__ __forceinline__ double length (vec3 a)
    return z * sqrt (1.0 + x * x + y * y);
 }
 
-#ifdef MXG_FAST_NORMALIZE
+// a / |a|: one square root and one division (the reference divides each
+// component by its overflow-safe length, vector.c; for the unit-scale
+// directions and normals normalised here the results agree to an ulp)
 __device__ __forceinline__ void normalize (vec3 &a)
 {
    double l2 = a.x * a.x + a.y * a.y + a.z * a.z;
@@ -256,18 +278,6 @@ __device__ __forceinline__ void normalize (vec3 &a)
         a.z = a.z * inv;
      }
 }
-#else
-__device__ __forceinline__ void normalize (vec3 &a)
-{
-   double len = length (a);
-   if (len != 0.0)
-     {
-        a.x = a.x / len;
-        a.y = a.y / len;
-        a.z = a.z / len;
-     }
-}
-#endif
 
 __device__ __forceinline__ vec3 unit (vec3 a) { normalize (a); return a; }
 

@@ -310,9 +310,10 @@ void k_hrma_reflect (const mxg_config *__restrict__ cfg, const float *__restrict
              double beta = 0.0, delta = 1.0, corr = 1.0;
              if (h.opt_n != 0)
                {
-                  float ef = (float) energy;
-                  beta = interp_f (ef, opt_e, opt_b, h.opt_n);
-                  delta = interp_f (ef, opt_e, opt_d, h.opt_n);
+                  float ef = (float) energy, fb, fd;
+                  interp_f2 (ef, opt_e, opt_b, opt_d, h.opt_n, fb, fd);
+                  beta = fb;
+                  delta = fd;
                   if (h.use_scale)
                     {
                        corr = interp_f (ef, ltab + lay.corr_e[si], ltab + lay.corr_f[si], S.corr_n);
