@@ -23,6 +23,7 @@
 #define MXG_STAGE_MIRROR   3u
 #define MXG_STAGE_GRATING  4u
 #define MXG_STAGE_DETECT   5u
+#define MXG_STAGE_EVENT    6u       /* event coordinates: grade choice, pixel randomisation */
 
 /* One scattering array of a WFold table (wfold.c:42-57), stored in the
  * double arena as 8 consecutive values:
@@ -223,6 +224,25 @@ typedef struct
    mxg_source_cfg src;
 }
 mxg_config;
+
+/* Event coordinates (mxg_event_coords): the settings with the ACIS tiling
+ * and focal-plane system, and the CALDB subpixel (EDSER) tables: for table
+ * t (0 front-, 1 back-illuminated chips) and flight grade g, sub_n[t][g]
+ * points (0: no offset) at sub_off[t][g] in a float array holding the
+ * energies, then the chip x offsets, then the chip y offsets. */
+typedef struct
+{
+   double focal_length, det_offset[3];
+   double cos_roll, sin_roll;
+   double fp_delta_s0, fp_x0, fp_y0;    /* focal-plane system AXAF-FP-1.1 (detpix.c:43-49) */
+   double tdet_xoff[MXG_MAX_CHIPS], tdet_yoff[MXG_MAX_CHIPS];   /* by chip id */
+   int chip_index[MXG_MAX_CHIPS];       /* chip id -> mxg_detector_cfg.chip[], -1 */
+   int is_acis_i;
+   int pix_adj, dither_used;
+   int pad;
+   unsigned int sub_off[2][256], sub_n[2][256];
+}
+mxg_event_params;
 
 #ifdef __cplusplus
 extern "C" {

@@ -112,6 +112,24 @@ class DiffCfg(ctypes.Structure):
 
 PPhotonList = ctypes.POINTER(PhotonList)
 
+# pixel adjustments of mxg_event_cfg.pix_adj (marx2fits --pixadj)
+PIX_ADJ = {"NONE": 0, "RANDOMIZE": 1, "EDSER": 2, "EXACT": 3}
+
+
+class EventCfg(ctypes.Structure):
+    """mxg_event_cfg: the marx2fits settings of mxg_event_coords."""
+    _fields_ = [("focal_length", ctypes.c_double), ("det_offset", ctypes.c_double * 3),
+                ("roll_nom", ctypes.c_double), ("exposure_time", ctypes.c_double),
+                ("frame_transfer_time", ctypes.c_double), ("total_time", ctypes.c_double),
+                ("seed", ctypes.c_ulonglong), ("pix_adj", ctypes.c_int), ("dither_used", ctypes.c_int)]
+
+
+class EventColumns(ctypes.Structure):
+    """mxg_event_columns: host arrays (NULL ones skipped)."""
+    _fields_ = [("tag", ctypes.c_void_p), ("ccd_id", ctypes.c_void_p), ("tdetx", ctypes.c_void_p),
+                ("tdety", ctypes.c_void_p), ("detx", ctypes.c_void_p), ("dety", ctypes.c_void_p),
+                ("x", ctypes.c_void_p), ("y", ctypes.c_void_p), ("fltgrade", ctypes.c_void_p)]
+
 
 class DumpFile(ctypes.Structure):
     """Marx_Dump_File_Type (marx/libsrc/marx.h:161-173)."""
@@ -168,6 +186,9 @@ _SIGNATURES = [
                                           ctypes.c_double]),
     ("mxg_write_device", ctypes.c_int, [ctypes.c_ulong, ctypes.c_char_p, ctypes.c_int, ctypes.c_double,
                                         ctypes.POINTER(ctypes.c_uint)]),
+    ("mxg_event_coords", ctypes.c_int, [ctypes.POINTER(EventCfg), ctypes.POINTER(EventColumns), ctypes.c_uint,
+                                        ctypes.POINTER(ctypes.c_uint)]),
+    ("mxg_event_kernel_ms", ctypes.c_int, [ctypes.POINTER(ctypes.c_float)]),
     ("marx_open_read_dump_file", ctypes.POINTER(DumpFile), [ctypes.c_char_p]),
     ("marx_close_read_dump_file", None, [ctypes.POINTER(DumpFile)]),
     ("marx_create_write_dump_file", ctypes.c_void_p, [ctypes.c_char_p, ctypes.POINTER(DumpFile)]),

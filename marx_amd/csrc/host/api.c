@@ -266,6 +266,18 @@ int marx_detector_init (Param_File_Type *pf)
    return type;
 }
 
+/* the columns a detector fills (acis-s.c:189-190, acis-i.c:108-109,
+ * hrc-s.c:245-248, hrc-i.c:128-129) */
+static unsigned long detector_history (void)
+{
+   if ((Detector_Type == MARX_DETECTOR_ACIS_S) || (Detector_Type == MARX_DETECTOR_ACIS_I))
+     return MARX_DET_PIXEL_OK | MARX_DET_NUM_OK | MARX_PULSEHEIGHT_OK | MARX_PI_OK;
+   if (Detector_Type == MARX_DETECTOR_HRC_S)
+     return MARX_DET_REGION_OK | MARX_PULSEHEIGHT_OK | MARX_DET_PIXEL_OK | MARX_DET_NUM_OK
+       | MARX_DET_UV_PIXEL_OK;
+   return MARX_DET_REGION_OK | MARX_PULSEHEIGHT_OK | MARX_DET_PIXEL_OK | MARX_DET_NUM_OK;
+}
+
 int marx_detect (Marx_Photon_Type *pt, int verbose)
 {
    if (Detector_Type == -1)
@@ -283,12 +295,7 @@ int marx_detect (Marx_Photon_Type *pt, int verbose)
    if (verbose > 0) mxg_message (0, "Detecting\n");
    if (pt->history & MARX_DET_NUM_OK)
      return 0;
-   if ((Detector_Type == MARX_DETECTOR_ACIS_S) || (Detector_Type == MARX_DETECTOR_ACIS_I))
-     pt->history |= (MARX_DET_PIXEL_OK | MARX_DET_NUM_OK | MARX_PULSEHEIGHT_OK | MARX_PI_OK);
-   else
-     pt->history |= (MARX_DET_REGION_OK | MARX_PULSEHEIGHT_OK | MARX_DET_PIXEL_OK | MARX_DET_NUM_OK);
-   if (Detector_Type == MARX_DETECTOR_HRC_S)
-     pt->history |= MARX_DET_UV_PIXEL_OK;
+   pt->history |= detector_history ();
    if ((-1 == sync_config ()) || (-1 == make_resident (pt)))
      return -1;
    if (-1 == mxgd_run_detect (pt->start_time, Seed, -1, NULL))
@@ -494,7 +501,7 @@ int mxg_trace_device (unsigned int num, unsigned int tag_start, double start_tim
      }
    hist |= MARX_MIRROR_SHELL_OK;
    if (with_grating) hist |= MARX_ORDER_OK;
-   if (Detector_Type > 0) hist |= MARX_DET_NUM_OK | MARX_DET_PIXEL_OK;
+   if (Detector_Type > 0) hist |= detector_history ();
    /* photons alive after each stage (a skipped stage passes them through) */
    Stage_Counts[0] = n;
    Stage_Counts[1] = counts[1];
@@ -567,7 +574,7 @@ static int batch_finish (unsigned int n, unsigned int tag_start, int with_gratin
    Stage_Counts[2] = with_grating ? counts[2] : Stage_Counts[1];
    Stage_Counts[3] = (Detector_Type > 0) ? counts[3] : Stage_Counts[2];
    if (with_grating) hist |= MARX_ORDER_OK;
-   if (Detector_Type > 0) hist |= MARX_DET_NUM_OK | MARX_DET_PIXEL_OK;
+   if (Detector_Type > 0) hist |= detector_history ();
    Kernel_Ms[0] = ms[MXGD_SLOT_HRMA_ENTER];
    Kernel_Ms[1] = ms[MXGD_SLOT_HRMA_REFLECT];
    Kernel_In[0] = counts[MXGD_SLOT_HRMA_ENTER];

@@ -130,6 +130,18 @@ int mxgd_dither_store (double start_time, unsigned long long seed);
 int mxgd_pack_output (const int *cols, int ncols, double total_time, unsigned int *n_rows,
                       const unsigned char **bytes, size_t *col_off);
 
+/* Event coordinates of the rows of mxgd_pack_output (same selection and
+ * order) into the host arrays of *out (NULL ones skipped): the frame heads
+ * of the held dither state, then k_event_coords.  subtab holds n_subtab
+ * floats of subpixel tables indexed by par->sub_off / sub_n.  Fails when the
+ * rows exceed capacity.  Synchronises. */
+int mxgd_event_coords (const mxg_event_params *par, const float *subtab, unsigned int n_subtab,
+                       double total_time, double time_del, unsigned long long seed,
+                       const mxg_event_columns *out, unsigned int capacity, unsigned int *num_rows);
+
+/* GPU time (ms) of the last mxgd_event_coords kernels (heads and coordinates). */
+float mxgd_event_ms (void);
+
 /* Number of photons in the active list (device -> host copy). */
 int mxgd_active_count (unsigned int *n);
 /* Active list indices (host copy, n = mxgd_active_count). */

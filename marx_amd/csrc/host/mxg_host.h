@@ -37,7 +37,7 @@ const double *mxg_darena_base (void);
 /* ---- data files -------------------------------------------------------- */
 /* Malloced absolute path of a calibration file (data.c:57-81). */
 char *mxg_data_file_name (const char *file);
-/* CALDB object (GEOM, AIMPTS, ACISQE, HETGEFF, LETGEFF) -> malloced path
+/* CALDB object (GEOM, AIMPTS, ACISQE, HETGEFF, LETGEFF, ACISSUBPIX) -> malloced path
  * (caldb.c:37-133). */
 char *mxg_caldb_file (const char *object);
 

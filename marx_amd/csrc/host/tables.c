@@ -104,6 +104,7 @@ char *mxg_caldb_file (const char *object)
           {"ACISQE", "caldb/acisD1997-04-17qeN0006.fits"},
           {"HETGEFF", "caldb/hetgD1996-11-01greffpr001N0007.fits"},
           {"LETGEFF", "caldb/letgD1996-11-01greffpr001N0007.fits"},
+          {"ACISSUBPIX", "caldb/acisD1999-07-22subpixN0001_marx.fits"},
           {NULL, NULL}
      };
    unsigned int i;

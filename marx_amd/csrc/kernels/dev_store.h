@@ -121,6 +121,21 @@ struct PackArgs
    double total_time;
 };
 hipError_t launch_keep_list (const uint32_t *flags, uint32_t n, const ListIO &io, hipStream_t st);
+
+// Event coordinates (events.hip): row q of each column for photon idx[q].
+struct EventOut
+{
+   uint32_t *tag;
+   int32_t *ccd, *tdetx, *tdety;
+   double *detx, *dety, *x, *y;
+   int16_t *fltgrade;
+};
+// The frame heads (first row of each row's frame, for the held dither
+// state) take head[m], chunk_max[ceil(m / kBlock)] and chunk_pre[same].
+hipError_t launch_event_coords (const mxg_config *cfg, const mxg_event_params *params, const float *subtab,
+                                const Soa &s, const uint32_t *idx, uint32_t m, double total_time, double time_del,
+                                unsigned long long seed, uint32_t *head, uint32_t *chunk_max, uint32_t *chunk_pre,
+                                const EventOut &out, hipStream_t st);
 hipError_t launch_pack_columns (const Soa &s, const uint32_t *idx, const uint32_t *n_kept, uint32_t m,
                                 const PackArgs &a, uint8_t *out, hipStream_t st);
 

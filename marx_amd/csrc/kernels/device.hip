@@ -920,6 +920,44 @@ uint8_t *d_out_bytes = nullptr;      // packed columns
 size_t g_out_bytes_cap = 0;
 unsigned char *h_out_bytes = nullptr;    // their pinned host copy
 size_t g_out_host_cap = 0;
+// event coordinates
+void *d_ev = nullptr;                // frame heads, chunk maxima and prefixes, columns
+size_t g_ev_cap = 0;
+void *d_ev_par = nullptr;            // mxg_event_params, then the subpixel tables
+size_t g_ev_par_cap = 0;
+hipEvent_t g_ev_ev[2] = {nullptr, nullptr};
+float g_ev_ms = 0.0f;
+
+// The list of the store's photons whose flags & BAD_PHOTON_MASK is zero, in
+// store order: io.idx_out, *m rows.  Synchronises.
+int keep_rows (ListIO &io, uint32_t *m)
+{
+   const uint32_t n = g_n;
+   if (g_out_list_cap < n)
+     {
+        HIPCHECK (hipStreamSynchronize (g_stream));
+        if (d_out_list) (void) hipFree (d_out_list);
+        d_out_list = nullptr;
+        g_out_list_cap = 0;
+        HIPCHECK (hipMalloc (&d_out_list, 2 * align256 ((size_t) n * sizeof (uint32_t))));
+        g_out_list_cap = n;
+     }
+   if (!d_out_u32) HIPCHECK (hipMalloc (&d_out_u32, 4 * sizeof (uint32_t)));
+   io.idx_in = nullptr;
+   io.n_in = d_out_u32;
+   io.seg = d_out_list;
+   io.idx_out = d_out_list + align256 (g_out_list_cap * sizeof (uint32_t)) / sizeof (uint32_t);
+   io.n_out = d_out_u32 + 1;
+   io.seg_count = d_seg_count;
+   io.grid = stage_grid (n, MXGD_K_DETECT);
+   hipLaunchKernelGGL (k_set_u32, dim3 (1), dim3 (1), 0, g_stream, d_out_u32, n);
+   HIPCHECK (hipGetLastError ());
+   HIPCHECK (launch_keep_list (g_soa.flags, n, io, g_stream));
+   *m = 0;
+   HIPCHECK (hipMemcpyAsync (m, d_out_u32 + 1, sizeof (uint32_t), hipMemcpyDeviceToHost, g_stream));
+   HIPCHECK (hipStreamSynchronize (g_stream));
+   return 0;
+}
 } // namespace
 
 extern "C" {
@@ -932,32 +970,10 @@ int mxgd_pack_output (const int *cols, int ncols, double total_time, unsigned in
    if (g_capturing) return fail ("output cannot be written during graph capture");
    *n_rows = 0;
    *bytes = nullptr;
-   const uint32_t n = g_n;
-   if ((n == 0) || (ncols == 0)) return 0;
-   if (g_out_list_cap < n)
-     {
-        HIPCHECK (hipStreamSynchronize (g_stream));
-        if (d_out_list) (void) hipFree (d_out_list);
-        d_out_list = nullptr;
-        g_out_list_cap = 0;
-        HIPCHECK (hipMalloc (&d_out_list, 2 * align256 ((size_t) n * sizeof (uint32_t))));
-        g_out_list_cap = n;
-     }
-   if (!d_out_u32) HIPCHECK (hipMalloc (&d_out_u32, 4 * sizeof (uint32_t)));
+   if ((g_n == 0) || (ncols == 0)) return 0;
    ListIO io;
-   io.idx_in = nullptr;
-   io.n_in = d_out_u32;
-   io.seg = d_out_list;
-   io.idx_out = d_out_list + align256 (g_out_list_cap * sizeof (uint32_t)) / sizeof (uint32_t);
-   io.n_out = d_out_u32 + 1;
-   io.seg_count = d_seg_count;
-   io.grid = stage_grid (n, MXGD_K_DETECT);
-   hipLaunchKernelGGL (k_set_u32, dim3 (1), dim3 (1), 0, g_stream, d_out_u32, n);
-   HIPCHECK (hipGetLastError ());
-   HIPCHECK (launch_keep_list (g_soa.flags, n, io, g_stream));
    uint32_t m = 0;
-   HIPCHECK (hipMemcpyAsync (&m, d_out_u32 + 1, sizeof (uint32_t), hipMemcpyDeviceToHost, g_stream));
-   HIPCHECK (hipStreamSynchronize (g_stream));
+   if (keep_rows (io, &m)) return -1;
    if (m == 0) return 0;
 
    PackArgs a;
@@ -995,6 +1011,89 @@ int mxgd_pack_output (const int *cols, int ncols, double total_time, unsigned in
    *n_rows = m;
    *bytes = h_out_bytes;
    return 0;
+}
+
+int mxgd_event_coords (const mxg_event_params *par, const float *subtab, unsigned int n_subtab,
+                       double total_time, double time_del, unsigned long long seed,
+                       const mxg_event_columns *out, unsigned int capacity, unsigned int *num_rows)
+{
+   if (mxgd_init ()) return -1;
+   if (g_capturing) return fail ("event coordinates cannot be computed during graph capture");
+   *num_rows = 0;
+   g_ev_ms = 0.0f;
+   if (g_n == 0) return 0;
+   ListIO io;
+   uint32_t m = 0;
+   if (keep_rows (io, &m)) return -1;
+   if (m > capacity) return fail ("%u event rows exceed the capacity of %u", m, capacity);
+   if (m == 0) return 0;
+
+   const size_t par_bytes = align256 (sizeof (mxg_event_params)) + (size_t) n_subtab * sizeof (float);
+   if (g_ev_par_cap < par_bytes)
+     {
+        HIPCHECK (hipStreamSynchronize (g_stream));
+        if (d_ev_par) (void) hipFree (d_ev_par);
+        d_ev_par = nullptr;
+        g_ev_par_cap = 0;
+        HIPCHECK (hipMalloc (&d_ev_par, par_bytes));
+        g_ev_par_cap = par_bytes;
+     }
+   mxg_event_params *d_par = (mxg_event_params *) d_ev_par;
+   float *d_sub = (float *) ((char *) d_ev_par + align256 (sizeof (mxg_event_params)));
+   HIPCHECK (hipMemcpyAsync (d_par, par, sizeof (mxg_event_params), hipMemcpyHostToDevice, g_stream));
+   if (n_subtab)
+     HIPCHECK (hipMemcpyAsync (d_sub, subtab, n_subtab * sizeof (float), hipMemcpyHostToDevice, g_stream));
+
+   // frame heads, chunk maxima, chunk prefixes, then the columns
+   const size_t nc = (m + kBlock - 1) / kBlock;
+   const size_t w4 = align256 ((size_t) m * 4), w8 = align256 ((size_t) m * 8), w2 = align256 ((size_t) m * 2);
+   const size_t wc = align256 (nc * 4);
+   const size_t need = w4 + 2 * wc + 4 * w4 + 4 * w8 + w2;
+   if (g_ev_cap < need)
+     {
+        HIPCHECK (hipStreamSynchronize (g_stream));
+        if (d_ev) (void) hipFree (d_ev);
+        d_ev = nullptr;
+        g_ev_cap = 0;
+        HIPCHECK (hipMalloc (&d_ev, need));
+        g_ev_cap = need;
+     }
+   char *b = (char *) d_ev;
+   uint32_t *head = (uint32_t *) b; b += w4;
+   uint32_t *cmax = (uint32_t *) b; b += wc;
+   uint32_t *cpre = (uint32_t *) b; b += wc;
+   EventOut eo;
+   eo.tag = (uint32_t *) b; b += w4;
+   eo.ccd = (int32_t *) b; b += w4;
+   eo.tdetx = (int32_t *) b; b += w4;
+   eo.tdety = (int32_t *) b; b += w4;
+   eo.detx = (double *) b; b += w8;
+   eo.dety = (double *) b; b += w8;
+   eo.x = (double *) b; b += w8;
+   eo.y = (double *) b; b += w8;
+   eo.fltgrade = (int16_t *) b;
+   if (!g_ev_ev[0])
+     for (int k = 0; k < 2; k++) HIPCHECK (hipEventCreate (&g_ev_ev[k]));
+   HIPCHECK (hipEventRecord (g_ev_ev[0], g_stream));
+   HIPCHECK (launch_event_coords (d_cfg, d_par, d_sub, g_soa, io.idx_out, m, total_time, time_del, seed, head,
+                                  cmax, cpre, eo, g_stream));
+   HIPCHECK (hipEventRecord (g_ev_ev[1], g_stream));
+   struct { void *h; const void *d; size_t w; } cp[9] = {
+      {out->tag, eo.tag, 4}, {out->ccd_id, eo.ccd, 4}, {out->tdetx, eo.tdetx, 4}, {out->tdety, eo.tdety, 4},
+      {out->detx, eo.detx, 8}, {out->dety, eo.dety, 8}, {out->x, eo.x, 8}, {out->y, eo.y, 8},
+      {out->fltgrade, eo.fltgrade, 2}};
+   for (int k = 0; k < 9; k++)
+     if (cp[k].h)
+       HIPCHECK (hipMemcpyAsync (cp[k].h, cp[k].d, (size_t) m * cp[k].w, hipMemcpyDeviceToHost, g_stream));
+   HIPCHECK (hipStreamSynchronize (g_stream));
+   HIPCHECK (hipEventElapsedTime (&g_ev_ms, g_ev_ev[0], g_ev_ev[1]));
+   *num_rows = m;
+   return 0;
+}
+
+float mxgd_event_ms (void)
+{
+   return g_ev_ms;
 }
 
 } // extern "C"

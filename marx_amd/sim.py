@@ -252,6 +252,45 @@ class Tracer:
                                         float(total_time), ctypes.byref(n)), "mxg_write_device")
         return n.value
 
+    # (name, numpy dtype) of the columns of event_coords, in mxg_event_columns order
+    EVENT_COLUMNS = (("tag", np.uint32), ("ccd_id", np.int32), ("tdetx", np.int32), ("tdety", np.int32),
+                     ("detx", np.float64), ("dety", np.float64), ("x", np.float64), ("y", np.float64),
+                     ("fltgrade", np.int16))
+
+    def event_cfg(self, pix_adj="EDSER", seed=1, total_time=0.0):
+        """The mxg_event_cfg that marx2fits would use for this simulation:
+        FocalLength, DetOffsetX/Y/Z, Roll_Nom, the ACIS frame times and
+        whether DitherModel is other than NONE, from the parameter file."""
+        c = _lib.EventCfg()
+        c.focal_length = self.par.get_double("FocalLength")
+        for k, a in enumerate("XYZ"):
+            c.det_offset[k] = self.par.get_double("DetOffset" + a)
+        c.roll_nom = self.par.get_double("Roll_Nom")
+        c.exposure_time = self.par.get_double("ACIS_Exposure_Time")
+        c.frame_transfer_time = self.par.get_double("ACIS_Frame_Transfer_Time")
+        c.total_time = float(total_time)
+        c.seed = int(seed)
+        c.pix_adj = _lib.PIX_ADJ[pix_adj.upper()]
+        c.dither_used = int(self.par.get_string("DitherModel") != "NONE")
+        return c
+
+    def event_coords(self, capacity, pix_adj="EDSER", seed=1, total_time=0.0, cfg=None):
+        """mxg_event_coords: the marx2fits event columns (tag, ccd_id,
+        tdetx/y, detx/y, sky x/y, fltgrade) of the last device-resident ACIS
+        trace, as a dict of numpy arrays, in the rows of write_device."""
+        cfg = cfg if cfg is not None else self.event_cfg(pix_adj, seed, total_time)
+        arrays = {k: np.empty(int(capacity), dt) for k, dt in self.EVENT_COLUMNS}
+        cols = _lib.EventColumns(*[a.ctypes.data for a in arrays.values()])
+        n = ctypes.c_uint()
+        check(self.lib.mxg_event_coords(ctypes.byref(cfg), ctypes.byref(cols), int(capacity), ctypes.byref(n)),
+              "mxg_event_coords")
+        return {k: a[:n.value] for k, a in arrays.items()}
+
+    def event_kernel_ms(self):
+        ms = ctypes.c_float()
+        self.lib.mxg_event_kernel_ms(ctypes.byref(ms))
+        return ms.value
+
     def set_seed(self, seed):
         self.lib.mxg_set_seed(int(seed))
 

@@ -81,6 +81,23 @@ int orc_interpolate_n_fvector (const float *xp, float **yp, unsigned int n,
                                const float *oldxp, float **oldyp,
                                unsigned int oldn, unsigned int n_yp);
 
+/* marx2fits over the n rows of an ACIS simulation's dump columns, in row
+ * order (time as read from time.dat, chip pixels from xpixel/ypixel.dat,
+ * b_energy.dat, the six dither columns or NULL): tiled, focal-plane and sky
+ * pixels and flight grades.  subtab / sub_off / sub_n: subpixel tables
+ * indexed [256 * t + fltgrade] (t 0 front-, 1 back-illuminated), each
+ * table its energies, then its chip x, then its chip y offsets. */
+int orc_event_coords (const mxg_config *cfg, const mxg_event_cfg *ec, unsigned int n,
+                      const float *time, const unsigned int *tag, const signed char *ccd,
+                      const float *xpixel, const float *ypixel, const float *b_energy,
+                      const float *const *dither, const float *subtab,
+                      const unsigned int *sub_off, const unsigned int *sub_n,
+                      int *tdetx, int *tdety, double *detx, double *dety,
+                      double *sky_x, double *sky_y, short *fltgrade);
+/* acis_subpix.c:283-322 over one table of n energies (then n chip x and n
+ * chip y offsets). */
+int orc_subpix (const float *e, unsigned int n, float energy, float *dx, float *dy);
+
 /* Stage-major passes, in the order marx/src/marx.c:240-273 calls them. */
 int orc_create_photons (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int num,
                         unsigned int *num_collected, double *exposure_time);

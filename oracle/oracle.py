@@ -86,6 +86,14 @@ def load():
     lib.orc_sort_photons.restype = ctypes.c_int
     lib.orc_trace_parallel.argtypes = [ctypes.POINTER(OrcCtx), ctypes.c_uint, ctypes.c_uint, ctypes.c_int]
     lib.orc_trace_parallel.restype = ctypes.c_long
+    i32p = ctypes.POINTER(ctypes.c_int)
+    lib.orc_event_coords.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint, f32p, u32p,
+                                     ctypes.POINTER(ctypes.c_byte), f32p, f32p, f32p, ctypes.POINTER(f32p),
+                                     f32p, u32p, u32p, i32p, i32p, f64p, f64p, f64p, f64p,
+                                     ctypes.POINTER(ctypes.c_short)]
+    lib.orc_event_coords.restype = ctypes.c_int
+    lib.orc_subpix.argtypes = [f32p, ctypes.c_uint, ctypes.c_float, f32p, f32p]
+    lib.orc_subpix.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -156,6 +164,15 @@ def interpolate_n_fvector(x, oldx, oldy):
     if st != 0:
         raise ValueError("interpolate_n_fvector needs at least 2 table rows")
     return np.array(out)
+
+
+def subpix(table, n, energy):
+    """acis_subpix.c:283-322 over one table (n energies, then n chip x and n
+    chip y offsets): (dx, dy)."""
+    t = np.ascontiguousarray(table, dtype=np.float32)
+    dx, dy = ctypes.c_float(), ctypes.c_float()
+    load().orc_subpix(_arr(t, ctypes.c_float), int(n), float(energy), ctypes.byref(dx), ctypes.byref(dy))
+    return dx.value, dy.value
 
 
 class Oracle:
@@ -233,6 +250,42 @@ class Oracle:
         if r != 0:
             raise RuntimeError("oracle diff_adjoint failed")
         return grad
+
+    def event_coords(self, ecfg, cols, subtab=None, sub_off=None, sub_n=None):
+        """marx2fits over the rows of a dump: ecfg an mxg_event_cfg (its
+        total_time is unused: the time column holds it), cols the columns as
+        read from the dump files (TIME, TAG, CCDID, CHIPX, CHIPY, B_ENERGY
+        and, when the simulation dithered, RA, DEC, ROLL, DET_DY, DET_DZ,
+        DET_THETA).  Returns the dict of event columns."""
+        n = len(cols["TAG"])
+        c = {k: np.ascontiguousarray(cols[k], dtype=dt) for k, dt in
+             (("TIME", np.float32), ("TAG", np.uint32), ("CCDID", np.int8), ("CHIPX", np.float32),
+              ("CHIPY", np.float32), ("B_ENERGY", np.float32))}
+        names = ("RA", "DEC", "ROLL", "DET_DY", "DET_DZ", "DET_THETA")
+        dcols = [np.ascontiguousarray(cols[k], dtype=np.float32) if k in cols else np.zeros(n, np.float32)
+                 for k in names]
+        f32p = ctypes.POINTER(ctypes.c_float)
+        dptr = (f32p * 6)(*[_arr(a, ctypes.c_float) for a in dcols])
+        if subtab is None:
+            subtab, sub_off, sub_n = np.zeros(1, np.float32), np.zeros(512, np.uint32), np.zeros(512, np.uint32)
+        subtab = np.ascontiguousarray(subtab, dtype=np.float32)
+        sub_off = np.ascontiguousarray(sub_off, dtype=np.uint32)
+        sub_n = np.ascontiguousarray(sub_n, dtype=np.uint32)
+        out = {"tag": c["TAG"].copy(), "ccd_id": c["CCDID"].astype(np.int32),
+               "tdetx": np.zeros(n, np.int32), "tdety": np.zeros(n, np.int32),
+               "detx": np.zeros(n), "dety": np.zeros(n), "x": np.zeros(n), "y": np.zeros(n),
+               "fltgrade": np.zeros(n, np.int16)}
+        st = self.lib.orc_event_coords(
+            ctypes.c_void_p(self.ctx.cfg), ctypes.byref(ecfg), n, _arr(c["TIME"], ctypes.c_float),
+            _arr(c["TAG"], ctypes.c_uint), _arr(c["CCDID"], ctypes.c_byte), _arr(c["CHIPX"], ctypes.c_float),
+            _arr(c["CHIPY"], ctypes.c_float), _arr(c["B_ENERGY"], ctypes.c_float), dptr,
+            _arr(subtab, ctypes.c_float), _arr(sub_off, ctypes.c_uint), _arr(sub_n, ctypes.c_uint),
+            _arr(out["tdetx"], ctypes.c_int), _arr(out["tdety"], ctypes.c_int), _arr(out["detx"], ctypes.c_double),
+            _arr(out["dety"], ctypes.c_double), _arr(out["x"], ctypes.c_double), _arr(out["y"], ctypes.c_double),
+            _arr(out["fltgrade"], ctypes.c_short))
+        if st != 0:
+            raise RuntimeError("oracle event_coords failed")
+        return out
 
     def trace_batch_parallel(self, dirs, counts, tag_start=0, exposure=1.0e4, nthreads=0):
         d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1)

@@ -101,9 +101,9 @@ class Config(ctypes.Structure):
     _fields_ = [("hrma", HrmaCfg), ("grating", GratingCfg), ("det", DetectorCfg), ("src", SourceCfg)]
 
 
-from marx_amd._lib import DiffCfg, DiffShell, ImageSpec  # noqa: E402
+from marx_amd._lib import DiffCfg, DiffShell, EventCfg, EventColumns, ImageSpec  # noqa: E402
 
-STRUCTS = {"mxg_diff_shell": DiffShell, "mxg_diff_cfg": DiffCfg, "mxg_image_spec": ImageSpec,"mxg_wfold": Wfold, "mxg_shell_cfg": ShellCfg, "mxg_strut": Strut, "mxg_hrma_cfg": HrmaCfg,
+STRUCTS = {"mxg_event_cfg": EventCfg, "mxg_event_columns": EventColumns, "mxg_diff_shell": DiffShell, "mxg_diff_cfg": DiffCfg, "mxg_image_spec": ImageSpec,"mxg_wfold": Wfold, "mxg_shell_cfg": ShellCfg, "mxg_strut": Strut, "mxg_hrma_cfg": HrmaCfg,
            "mxg_grating": Grating, "mxg_sectors": Sectors, "mxg_grating_cfg": GratingCfg,
            "mxg_chip": Chip, "mxg_qe_table": QeTable, "mxg_rect": Rect,
            "mxg_detector_cfg": DetectorCfg, "mxg_source_cfg": SourceCfg, "mxg_config": Config}

@@ -125,13 +125,20 @@ def test_created_header_matches_the_format(tmp_path):
     assert open(path, "rb").read() == D.header("D", "SOME_LONG_NAME_", 41, 3)
 
 
+def _drain_stdio(capfd):
+    """Empties the C library's stdout buffer (messages of earlier calls)
+    and the captured output, so that what follows is the call's own."""
+    ctypes.CDLL(None).fflush(None)
+    capfd.readouterr()
+
+
 def test_dump_text(tmp_path, capfd):
     lib = load()
     r = _records(40, 6)
     _photons(r, ALL_HISTORY).write(str(tmp_path))
     names = ["energy.dat", "tag.dat", "order.dat", "pha.dat", "mirror.dat"]
     argv = (ctypes.c_char_p * (len(names) + 2))(b"marx", b"--dump", *[os.fsencode(tmp_path / f) for f in names])
-    capfd.readouterr()
+    _drain_stdio(capfd)
     assert lib.marx_dump(len(names) + 2, argv) == 0
     out = capfd.readouterr().out
     files = [(f, open(tmp_path / f, "rb").read()) for f in names]
@@ -153,7 +160,7 @@ def test_rayfile(tmp_path, capfd):
     # the reference adds total_time to the written photons in the caller's list
     assert np.array_equal(p2.active["arrival_time"], mutated[1]["arrival_time"])
     lib = load()
-    capfd.readouterr()
+    _drain_stdio(capfd)
     assert lib.marx_dump_rayfile(os.fsencode(path)) == 0
     lines = capfd.readouterr().out.splitlines()
     kept = ((r1["flags"] & 0xFF) == 0).sum() + ((r2["flags"] & 0xFF) == 0).sum()
