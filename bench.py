@@ -106,8 +106,17 @@ DESCRIPTION["sources"] = (
     "Source models: 1e7 photons per step of an on-axis POINT, DISK (2 arcmin), GAUSS (60 arcsec), BETA "
     "(10 arcsec core, beta 0.7) and LINE (30 arcmin) source, flat 0.3-10 keV, INTERNAL dither, created on "
     "the device and traced through HRMA + ACIS-S")
+DESCRIPTION["events"] = (
+    "Event coordinates: marx2fits's tiled, focal-plane and sky pixels, flight grades and EDSER "
+    "subpixel offsets of the detected photons of one traced HETG + ACIS-S chunk of 1e7 photons "
+    "(dither held per 3.241 s frame), computed in HBM by mxg_event_coords")
 DEFAULT_PHOTONS = {"hetg-acis-s": 10_000_000, "letg-hrc-s": 50_000_000, "batched-acis-i": 0,
-                   "diff": 10_000_000, "output": 10_000_000, "sources": 10_000_000}
+                   "diff": 10_000_000, "output": 10_000_000, "sources": 10_000_000, "events": 10_000_000}
+# algorithmic HBM bytes per event row: k_event_heads reads the row's list
+# entry (4) and time (8) and writes its frame head (4); k_event_coords reads
+# the list entry and head (8), tag, chip id, chip x/y and PI (4+1+4+4+4; the
+# held dither state is one read per frame) and writes nine columns (50)
+EVENT_BYTES = {"k_event_heads": 16, "k_event_coords": 75}
 SOURCE_SHAPES = {"POINT": {}, "DISK": {"S-DiskTheta0": 0.0, "S-DiskTheta1": 120.0},
                  "GAUSS": {"S-GaussSigma": 60.0}, "BETA": {"S-BetaCoreRadius": 10.0, "S-BetaBeta": 0.7},
                  "LINE": {"S-LineTheta": 1800.0, "S-LinePhi": 30.0}}
@@ -448,6 +457,73 @@ def measure_output(args, tmp, torch):
     }
 
 
+def measure_events(args, tmp, torch):
+    """Event coordinates of one traced chunk: each step computes every event
+    column of the chunk's detected photons (mxg_event_coords, EDSER) in HBM;
+    the device time of its kernels (frame heads and coordinates) is the
+    figure, the call's wall time with the copies of the columns to host
+    memory is reported beside it.  CPU baseline: the oracle's row loop (one
+    thread) over the first rows of the same columns, as written to the
+    dump files."""
+    from marx_amd.workloads import hetg_acis_s
+    cfg, K = "events", args.steps
+    N = args.photons or DEFAULT_PHOTONS[cfg]
+    progress = lambda msg: print(f"bench events: {msg}", file=sys.stderr, flush=True)
+    t = hetg_acis_s(tmp, photons=N, exposure=args.exposure)
+    ncol, ndet, _ = t.trace_device(N)
+    for _ in range(args.warmup):
+        t.event_coords(ndet)
+    kern = 0.0
+    t0 = time.perf_counter()
+    for _ in range(K):
+        out = t.event_coords(ndet)
+        kern += t.event_kernel_ms()
+    wall = (time.perf_counter() - t0) / K
+    kern /= K
+    rows = len(out["tag"])
+    progress(f"{rows} rows; kernels {kern:.3f} ms, call with host copies {wall * 1e3:.1f} ms")
+    res = {
+        "metric": "photons/s traced (HRMA+grating+detector) 1 GPU vs host CPU; gradient evals/s",
+        "value": rows / (kern * 1e-3), "unit": "event rows/s (device)", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": kern, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": DESCRIPTION[cfg], "name": cfg, "photons_traced": ncol, "rows": rows,
+                   "pix_adj": "EDSER"},
+        "call_ms_with_host_copies": wall * 1e3,
+        "roofline": {"bound": "hbm", "achieved": rows * sum(EVENT_BYTES.values()) / (kern * 1e-3) / 1e9,
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "traffic": None,
+                     "kernels": "k_event_heads + k_event_chunk_prefix + k_event_coords"},
+    }
+    res["roofline"]["frac"] = res["roofline"]["achieved"] / PEAK_HBM_GBS
+    if not args.no_cpu:
+        from oracle import dumpfile as D
+        from oracle.oracle import Oracle
+        from oracle.subpix import subpix_tables
+        from marx_amd._lib import DATA_DIR
+        d = os.path.join(tmp, "cols")
+        os.makedirs(d)
+        t.write_device(d)
+        m = min(rows, 2_000_000)
+        col = lambda f: D.read_column(open(os.path.join(d, f), "rb").read())[4][:m]
+        cols = {"TIME": col("time.dat"), "TAG": col("tag.dat"), "CCDID": col("detector.dat"),
+                "CHIPX": col("xpixel.dat"), "CHIPY": col("ypixel.dat"), "B_ENERGY": col("b_energy.dat")}
+        for k, f in (("RA", "sky_ra.dat"), ("DEC", "sky_dec.dat"), ("ROLL", "sky_roll.dat"),
+                     ("DET_DY", "det_dy.dat"), ("DET_DZ", "det_dz.dat"), ("DET_THETA", "det_theta.dat")):
+            cols[k] = col(f)
+        tables = subpix_tables(os.path.join(DATA_DIR, "caldb", "acisD1999-07-22subpixN0001_marx.fits"))
+        o = Oracle(t)
+        ec = t.event_cfg("EDSER")
+        t0 = time.perf_counter()
+        e = o.event_coords(ec, cols, *tables)
+        cpu_s = time.perf_counter() - t0
+        same = all(np.array_equal(e[k], out[k][:m]) for k in ("tag", "tdetx", "tdety", "fltgrade"))
+        res["cpu_baseline"] = {"value": m / cpu_s, "unit": "event rows/s", "cores": 1, "kind": "port",
+                               "sample": f"first {m} of the {rows} rows, oracle row loop (marx2fits order)",
+                               "integer_columns_identical": bool(same)}
+        progress(f"cpu baseline {m / cpu_s:.3g} rows/s")
+    return res
+
+
 def measure_sources(args, torch):
     """Source models: per shape, steps of 1e7 photons created on the device
     (k_source, k_scan_tiles, k_source_finish) and traced through HRMA and
@@ -572,6 +648,8 @@ def measure():
         return measure_output(args, tmp, torch)
     if args.config == "sources":
         return measure_sources(args, torch)
+    if args.config == "events":
+        return measure_events(args, tmp, torch)
     cfg = args.config
     N, T = (args.photons or DEFAULT_PHOTONS[cfg]), args.exposure
     tracer = WORKLOADS[cfg](tmp, photons=N, exposure=T)
