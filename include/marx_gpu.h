@@ -431,21 +431,24 @@ int mxg_last_kernel_stats (float ms[2], unsigned int photons_in[2]);
 int mxg_download_photons (Marx_Photon_Type *pt);
 
 /* Event coordinates (marx/src/marx2fits.c:3584-3735, 3813-3910 over
- * marx/libsrc/pixlib.c, detpix.c:43-209, acis_geom.c:37-230,
- * acis_subpix.c:270-322, dither.c:583-607) of the detected photons of an
- * ACIS simulation: tiled (TDETX/Y), focal-plane (DETX/Y) and sky (X/Y)
- * pixels and the flight grade, from the values marx2fits reads (the float
- * chip pixels plus one, the float dither state, the PI energy).  A pixel is
- * placed within its chip pixel as chosen by pix_adj: EXACT (the simulated
- * position), NONE (the centre), RANDOMIZE (uniform), EDSER (the centre plus
- * the CALDB subpixel offset of the event's energy and flight grade).  The
- * flight grade is one of the four of the event's 3x3 subpixel
- * (marx2fits.c:3822-3866).  Except in EXACT mode the dither state that DET
- * and sky pixels use is held over a frame, as marx2fits re-reads it only when
- * the exposure number (long)(time / TimeDel) changes from one row to the next
+ * marx/libsrc/pixlib.c, detpix.c:41-209, acis_geom.c:37-230,
+ * hrc_s_geom.c:321-510, hrc_i_geom.c:130-236, acis_subpix.c:270-322,
+ * dither.c:583-607) of the detected photons of an ACIS or HRC simulation:
+ * tiled (TDETX/Y), focal-plane (DETX/Y) and sky (X/Y) pixels and, for ACIS,
+ * the flight grade, from the values marx2fits reads (the float chip pixels
+ * plus one, the float dither state, the PI energy).  A pixel is placed
+ * within its chip pixel as chosen by pix_adj: EXACT (the simulated
+ * position), NONE (the centre), RANDOMIZE (uniform), EDSER (ACIS: the
+ * centre plus the CALDB subpixel offset of the event's energy and flight
+ * grade; HRC: RANDOMIZE, as marx2fits.c:3307-3308).  The flight grade is
+ * one of the four of the event's 3x3 subpixel (marx2fits.c:3822-3866).
+ * For ACIS, except in EXACT mode, the dither state that DET and sky pixels
+ * use is held over a frame, as marx2fits re-reads it only when the
+ * exposure number (long)(time / TimeDel) changes from one row to the next
  * (marx2fits.c:3571-3578, 3741-3760), TimeDel being the ACIS exposure time
- * plus the frame transfer time.  The random choices come from per-event
- * streams (seed, tag), not from a serial generator. */
+ * plus the frame transfer time; HRC rows use their own (:2850-2851).  The
+ * random choices come from per-event streams (seed, tag), not from a
+ * serial generator. */
 #define MXG_PIX_ADJ_NONE       0
 #define MXG_PIX_ADJ_RANDOMIZE  1
 #define MXG_PIX_ADJ_EDSER      2
@@ -479,7 +482,7 @@ mxg_event_columns;
 /* The event coordinates of the device-resident photons of the last
  * mxg_trace_device / mxg_trace_batch call whose flags & BAD_PHOTON_MASK is
  * zero (the rows of mxg_write_device, in the same order), computed on the
- * GPU.  The detector must be ACIS-S or ACIS-I.  *num_rows receives the rows;
+ * GPU.  The detector must be ACIS or HRC.  *num_rows receives the rows;
  * -1 with the error set when they exceed capacity. */
 int mxg_event_coords (const mxg_event_cfg *cfg, mxg_event_columns *out, unsigned int capacity,
                       unsigned int *num_rows);

@@ -225,21 +225,25 @@ typedef struct
 }
 mxg_config;
 
-/* Event coordinates (mxg_event_coords): the settings with the ACIS tiling
- * and focal-plane system, and the CALDB subpixel (EDSER) tables: for table
- * t (0 front-, 1 back-illuminated chips) and flight grade g, sub_n[t][g]
- * points (0: no offset) at sub_off[t][g] in a float array holding the
- * energies, then the chip x offsets, then the chip y offsets. */
+/* Event coordinates (mxg_event_coords): the settings with the detector's
+ * tiling and focal-plane system, and the CALDB subpixel (EDSER) tables of
+ * ACIS: for table t (0 front-, 1 back-illuminated chips) and flight grade
+ * g, sub_n[t][g] points (0: no offset) at sub_off[t][g] in a float array
+ * holding the energies, then the chip x offsets, then the chip y offsets. */
+#define MXG_TILE_SHIFT 0                 /* tiled = chip + offset (ACIS-S, HRC) */
+#define MXG_TILE_ACIS_I 1                /* chips 0, 2 and 1, 3 turned (acis_geom.c:112-141) */
 typedef struct
 {
    double focal_length, det_offset[3];
    double cos_roll, sin_roll;
-   double fp_delta_s0, fp_x0, fp_y0;    /* focal-plane system AXAF-FP-1.1 (detpix.c:43-49) */
+   double fp_delta_s0, fp_x0, fp_y0;    /* the detector's focal-plane system (detpix.c:41-61) */
    double tdet_xoff[MXG_MAX_CHIPS], tdet_yoff[MXG_MAX_CHIPS];   /* by chip id */
+   double pixel_xoff[MXG_MAX_CHIPS], pixel_yoff[MXG_MAX_CHIPS]; /* chip pixel of the LL corner, by index */
    int chip_index[MXG_MAX_CHIPS];       /* chip id -> mxg_detector_cfg.chip[], -1 */
-   int is_acis_i;
+   int tiling;                          /* MXG_TILE_* */
    int pix_adj, dither_used;
-   int pad;
+   int has_grade;                       /* ACIS: flight grades (and EDSER) */
+   int held;                            /* dither state held over a frame (ACIS) */
    unsigned int sub_off[2][256], sub_n[2][256];
 }
 mxg_event_params;

@@ -1,11 +1,14 @@
 /* events.c -- event coordinates (marx2fits) of the detected photons of an
- * ACIS simulation, computed on the GPU from the device-resident trace.
+ * ACIS or HRC simulation, computed on the GPU from the device-resident trace.
  *
  * The host side gathers what marx2fits reads besides the photon columns:
  * the tiling offsets of the tiled detector system (acis_geom.c:37-49,
- * 104-141, 175-196), the focal-plane system AXAF-FP-1.1 (detpix.c:43-49),
- * the frame length TimeDel (marx2fits.c:2536-2545) and the CALDB subpixel
- * tables (acis_subpix.c:121-268), and hands them to the device layer. */
+ * 104-141, 175-196; hrc_s_geom.c:321-327, 477-510 for AXAF-HRC-2.7S;
+ * hrc_i_geom.c:130-140, 216-236), the chip pixel of each facet's lower-left
+ * corner (hrc_s_geom.c:329-336, 344-394; hrc_i_geom.c:139-140), the
+ * detector's focal-plane system (detpix.c:41-61), the frame length TimeDel
+ * (marx2fits.c:2536-2545) and the CALDB subpixel tables
+ * (acis_subpix.c:121-268), and hands them to the device layer. */
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -14,9 +17,11 @@
 #include "mxg_host.h"
 #include "mxg_device.h"
 
-/* tiled-system offsets by chip id (acis_geom.c:38-48) */
+/* ACIS tiled-system offsets by chip id (acis_geom.c:38-48) */
 static const double Tdet_Xoff[MXG_MAX_CHIPS] = {3061, 5131, 3061, 5131, 791, 1833, 2875, 3917, 4959, 6001};
 static const double Tdet_Yoff[MXG_MAX_CHIPS] = {5131, 4107, 4085, 3061, 1702, 1702, 1702, 1702, 1702, 1702};
+/* HRC-S (AXAF-HRC-2.7S) by chip id 1-3, HRC-I (AXAF-HRC-2.4I) chip 0 */
+static const double Hrc_S_Tdet_Yoff[3] = {-12.0, 16444.0, 32900.0};
 
 /* subpixel tables: for table t (0 FI, 1 BI) and flight grade g, Sub_N[t][g]
  * energies at Sub_Off[t][g] in Sub_Tab, then as many chip x and chip y
@@ -169,7 +174,7 @@ int mxg_event_coords (const mxg_event_cfg *cfg, mxg_event_columns *out, unsigned
    mxg_event_params par;
    unsigned long history;
    double time_del, roll;
-   int k;
+   int k, is_acis;
 
    if ((cfg == NULL) || (out == NULL) || (num_rows == NULL))
      {
@@ -179,9 +184,10 @@ int mxg_event_coords (const mxg_event_cfg *cfg, mxg_event_columns *out, unsigned
    *num_rows = 0;
    if (-1 == mxg_resident_history (&history))
      return -1;
-   if ((d->type != MARX_DETECTOR_ACIS_S) && (d->type != MARX_DETECTOR_ACIS_I))
+   is_acis = (d->type == MARX_DETECTOR_ACIS_S) || (d->type == MARX_DETECTOR_ACIS_I);
+   if (!is_acis && (d->type != MARX_DETECTOR_HRC_S) && (d->type != MARX_DETECTOR_HRC_I))
      {
-        mxg_error ("event coordinates need an ACIS-S or ACIS-I simulation");
+        mxg_error ("event coordinates need an ACIS or HRC simulation");
         return -1;
      }
    if ((cfg->pix_adj < MXG_PIX_ADJ_NONE) || (cfg->pix_adj > MXG_PIX_ADJ_EXACT))
@@ -189,36 +195,71 @@ int mxg_event_coords (const mxg_event_cfg *cfg, mxg_event_columns *out, unsigned
         mxg_error ("Unsupported pixel adjustment %d", cfg->pix_adj);
         return -1;
      }
-   if ((cfg->pix_adj == MXG_PIX_ADJ_EDSER) && (-1 == load_subpix ()))
-     return -1;
 
    memset (&par, 0, sizeof (par));
+   /* EDSER is for ACIS; marx2fits randomizes HRC pixels instead
+    * (marx2fits.c:3296-3308) */
+   par.pix_adj = cfg->pix_adj;
+   if (!is_acis && (par.pix_adj == MXG_PIX_ADJ_EDSER))
+     par.pix_adj = MXG_PIX_ADJ_RANDOMIZE;
+   if ((par.pix_adj == MXG_PIX_ADJ_EDSER) && (-1 == load_subpix ()))
+     return -1;
    par.focal_length = cfg->focal_length;
    for (k = 0; k < 3; k++) par.det_offset[k] = cfg->det_offset[k];
    roll = cfg->roll_nom * PI / 180.0;
    par.cos_roll = cos (roll);
    par.sin_roll = sin (roll);
-   par.fp_delta_s0 = 0.492 * (PI / (3600.0 * 180.0));
-   par.fp_x0 = 4096.5;
-   par.fp_y0 = 4096.5;
-   par.is_acis_i = (d->type == MARX_DETECTOR_ACIS_I);
-   par.pix_adj = cfg->pix_adj;
+   par.tiling = (d->type == MARX_DETECTOR_ACIS_I) ? MXG_TILE_ACIS_I : MXG_TILE_SHIFT;
    par.dither_used = (cfg->dither_used != 0);
+   par.has_grade = is_acis;
+   par.held = is_acis;          /* marx2fits.c:2850-2851 */
    for (k = 0; k < MXG_MAX_CHIPS; k++)
+     par.chip_index[k] = -1;
+   if (is_acis)
      {
-        par.tdet_xoff[k] = Tdet_Xoff[k];
-        par.tdet_yoff[k] = Tdet_Yoff[k];
-        par.chip_index[k] = -1;
+        par.fp_delta_s0 = 0.492 * (PI / (3600.0 * 180.0));
+        par.fp_x0 = par.fp_y0 = 4096.5;
+        for (k = 0; k < MXG_MAX_CHIPS; k++)
+          {
+             par.tdet_xoff[k] = Tdet_Xoff[k];
+             par.tdet_yoff[k] = Tdet_Yoff[k];
+          }
+     }
+   else if (d->type == MARX_DETECTOR_HRC_S)
+     {
+        par.fp_delta_s0 = 0.1318 * (PI / (3600.0 * 180.0));     /* AXAF-FP-2.3 */
+        par.fp_x0 = par.fp_y0 = 32768.5;
+        for (k = 1; k <= 3; k++)
+          par.tdet_yoff[k] = Hrc_S_Tdet_Yoff[k - 1];
+     }
+   else
+     {
+        par.fp_delta_s0 = 0.13175 * (PI / (3600.0 * 180.0));    /* AXAF-FP-2.1 */
+        par.fp_x0 = par.fp_y0 = 16384.5;
+        par.tdet_xoff[0] = -1.0;
      }
    for (k = 0; k < d->num_chips; k++)
      {
-        int id = d->chip[k].id;
-        /* the detector's own chips: 0-3 for ACIS-I, 4-9 for ACIS-S */
+        const mxg_chip *g = d->chip + k;
+        int id = g->id;
         if ((id < 0) || (id >= MXG_MAX_CHIPS)) continue;
-        if (par.is_acis_i ? (id > 3) : (id < 4)) continue;
+        /* the detector's own chips: 0-3 ACIS-I, 4-9 ACIS-S, 1-3 HRC-S, 0 HRC-I */
+        if ((d->type == MARX_DETECTOR_ACIS_I) && (id > 3)) continue;
+        if ((d->type == MARX_DETECTOR_ACIS_S) && (id < 4)) continue;
         par.chip_index[id] = k;
+        if (d->type == MARX_DETECTOR_HRC_S)
+          {
+             /* the pixel of the corner: _marx_hrc_s_compute_pixel (id, 0, 0) */
+             par.pixel_xoff[k] = d->pix_x0[id] + (d->pix_u[id] - d->pix_u0[id]);
+             par.pixel_yoff[k] = d->pix_y0[id] + (d->pix_v[id] - d->pix_v0[id]);
+          }
+        else
+          {
+             par.pixel_xoff[k] = g->xpixel_offset;
+             par.pixel_yoff[k] = g->ypixel_offset;
+          }
      }
-   if (cfg->pix_adj == MXG_PIX_ADJ_EDSER)
+   if (par.pix_adj == MXG_PIX_ADJ_EDSER)
      {
         memcpy (par.sub_off, Sub_Off, sizeof (Sub_Off));
         memcpy (par.sub_n, Sub_N, sizeof (Sub_N));
@@ -229,8 +270,8 @@ int mxg_event_coords (const mxg_event_cfg *cfg, mxg_event_columns *out, unsigned
    if ((cfg->exposure_time > 0.0) && (cfg->frame_transfer_time > 0.0))
      time_del += cfg->frame_transfer_time;
 
-   if (-1 == mxgd_event_coords (&par, (cfg->pix_adj == MXG_PIX_ADJ_EDSER) ? Sub_Tab : NULL,
-                                (cfg->pix_adj == MXG_PIX_ADJ_EDSER) ? Sub_Len : 0,
+   if (-1 == mxgd_event_coords (&par, (par.pix_adj == MXG_PIX_ADJ_EDSER) ? Sub_Tab : NULL,
+                                (par.pix_adj == MXG_PIX_ADJ_EDSER) ? Sub_Len : 0,
                                 cfg->total_time, time_del, cfg->seed, out, capacity, num_rows))
      {
         mxg_error ("%s", mxgd_last_error ());

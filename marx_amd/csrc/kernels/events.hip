@@ -1,8 +1,9 @@
-// events.hip -- event coordinates of detected ACIS photons: tiled, focal-plane
-// and sky pixels and the flight grade, as marx2fits computes them from the
+// events.hip -- event coordinates of detected photons: tiled, focal-plane and
+// sky pixels and (ACIS) the flight grade, as marx2fits computes them from the
 // simulation's output columns (marx/src/marx2fits.c:3584-3735, 3813-3910,
 // marx/libsrc/pixlib.c:134-238, 486-573, detpix.c:182-209,
-// acis_geom.c:112-200, acis_subpix.c:283-322, dither.c:583-607).
+// acis_geom.c:112-200, hrc_s_geom.c:477-510, hrc_i_geom.c:216-236,
+// acis_subpix.c:283-322, dither.c:583-607).
 #include "dev_common.h"
 #include "dev_store.h"
 
@@ -191,7 +192,8 @@ void k_event_coords (const mxg_config *__restrict__ cfg, const mxg_event_params 
         const uint32_t i = idx[q];
         // the row whose dither state this row uses: its own in EXACT mode,
         // else the first of its frame
-        const uint32_t h = (P.pix_adj == MXG_PIX_ADJ_EXACT) ? i : idx[max (head[q], chunk_pre[q / kBlock])];
+        const uint32_t h = ((P.pix_adj == MXG_PIX_ADJ_EXACT) || !P.held) ? i
+          : idx[max (head[q], chunk_pre[q / kBlock])];
         const int ccd = s.ccd[i];
         out.tag[q] = s.tag[i];
         out.ccd[q] = ccd;
@@ -211,7 +213,7 @@ void k_event_coords (const mxg_config *__restrict__ cfg, const mxg_event_params 
         {
            const unsigned int ux = (unsigned int) chipx, uy = (unsigned int) chipy;
            float xf, yf;
-           if (!P.is_acis_i)
+           if (P.tiling == MXG_TILE_SHIFT)
              {
                 xf = (float) (ux + P.tdet_xoff[ccd]);
                 yf = (float) (uy + P.tdet_yoff[ccd]);
@@ -232,15 +234,19 @@ void k_event_coords (const mxg_config *__restrict__ cfg, const mxg_event_params 
            out.tdety[q] = (int) ((unsigned int) yf) + 1;
         }
 
-        // flight grade: one of the four of its 3x3 subpixel
+        // flight grade (ACIS): one of the four of its 3x3 subpixel
         Stream st;
         stream_init (st, seed, s.tag[i], MXG_STAGE_EVENT, 0u);
         const double cx = (double) chipx, cy = (double) chipy;
-        const int sx = (int) (3.0 * (cx - floor (cx)));
-        const int sy = (int) (3.0 * (cy - floor (cy)));
-        // the uniform is in [0, 1]: a draw of exactly 1 takes the fourth grade
-        // (marx2fits.c:3864 would index one past the row)
-        const int fltgrade = kFltGrades[3 * sy + sx][min ((int) (4 * uniform (st)), 3)];
+        int fltgrade = -1;
+        if (P.has_grade)
+          {
+             const int sx = (int) (3.0 * (cx - floor (cx)));
+             const int sy = (int) (3.0 * (cy - floor (cy)));
+             // the uniform is in [0, 1]: a draw of exactly 1 takes the fourth
+             // grade (marx2fits.c:3864 would index one past the row)
+             fltgrade = kFltGrades[3 * sy + sx][min ((int) (4 * uniform (st)), 3)];
+          }
         out.fltgrade[q] = (short) fltgrade;
 
         // the position within the chip pixel (marx2fits.c:3693-3717)
@@ -279,8 +285,8 @@ void k_event_coords (const mxg_config *__restrict__ cfg, const mxg_event_params 
         vec3 e2 = v3 (g.yhat[0], ct * g.yhat[1] - stt * g.yhat[2], stt * g.yhat[1] + ct * g.yhat[2]);
         e1 = v3 (g.x_pixel_size * e1.x, g.x_pixel_size * e1.y, g.x_pixel_size * e1.z);
         e2 = v3 (g.y_pixel_size * e2.x, g.y_pixel_size * e2.y, g.y_pixel_size * e2.z);
-        x -= g.xpixel_offset;
-        y -= g.ypixel_offset;
+        x -= P.pixel_xoff[k];
+        y -= P.pixel_yoff[k];
         vec3 mnc = v3 (ofs.x + e1.x * x + e2.x * y, ofs.y + e1.y * x + e2.y * y, ofs.z + e1.z * x + e2.z * y);
         normalize_div (mnc);
         const double factor = 1.0 / (P.fp_delta_s0 * mnc.x);

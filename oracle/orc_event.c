@@ -1,15 +1,19 @@
 /* orc_event.c -- oracle of the marx2fits event coordinates (test
  * infrastructure only, see oracle.h).
  *
- * A row-by-row restatement of what marx2fits does for each row of an ACIS
- * simulation's dump files, in its order and with its serial state
+ * A row-by-row restatement of what marx2fits does for each row of an ACIS or
+ * HRC simulation's dump files, in its order and with its serial state
  * (marx/src/marx2fits.c): the exposure number and the held dither state
- * (compute_expno :3741-3760, read_dither_value :3567-3581), the tiled pixel
- * (compute_tdetxy :3584-3597 over acis_geom.c:112-200), the flight grade
+ * (compute_expno :3741-3760, read_dither_value :3567-3581; HRC re-reads
+ * the dither on every row, :2850-2851), the tiled pixel (compute_tdetxy
+ * :3584-3597 over acis_geom.c:112-200, hrc_s_geom.c:477-510,
+ * hrc_i_geom.c:216-236), the flight grade of ACIS
  * (compute_fltgrade :3855-3866), the focal-plane pixel (compute_detxy
  * :3681-3735 over pixlib.c:133-238 and detpix.c:182-209, with the EDSER
- * offsets of acis_subpix.c:283-322) and the sky pixel (compute_xy_sky
- * :3869-3910 over dither.c:583-607, 703-707 and pixlib.c:498-573).
+ * offsets of acis_subpix.c:283-322; HRC randomizes instead, :3307-3308)
+ * and the sky pixel (compute_xy_sky :3869-3910 over dither.c:583-607,
+ * 703-707 and pixlib.c:498-573), in the detector's focal-plane system
+ * (detpix.c:41-61).
  * Random choices come from the per-event streams (seed, tag, stage 6). */
 #include <math.h>
 #include <string.h>
@@ -21,6 +25,9 @@
 /* acis_geom.c:38-48, by chip id */
 static const float Tdet_Xoff[10] = {3061, 5131, 3061, 5131, 791, 1833, 2875, 3917, 4959, 6001};
 static const float Tdet_Yoff[10] = {5131, 4107, 4085, 3061, 1702, 1702, 1702, 1702, 1702, 1702};
+
+/* HRC-S AXAF-HRC-2.7S y offsets of chips 1-3 (hrc_s_geom.c:321-327) */
+static const float Hrc_S_Tdet_Yoff[3] = {-12.0f, 16444.0f, 32900.0f};
 
 /* marx2fits.c:3822-3833 */
 static const int Flight_Grades[9][4] =
@@ -110,14 +117,36 @@ int orc_event_coords (const mxg_config *cfg, const mxg_event_cfg *ec, unsigned i
                       double *sky_x, double *sky_y, short *fltgrade)
 {
    const mxg_detector_cfg *d = &cfg->det;
-   const int is_acis_i = (d->type == MARX_DETECTOR_ACIS_I);
-   const double fp_delta_s0 = 0.492 * (PI / (3600.0 * 180.0)), fp_x0 = 4096.5, fp_y0 = 4096.5;
+   const int type = d->type;
+   const int is_acis_i = (type == MARX_DETECTOR_ACIS_I);
+   const int is_acis = is_acis_i || (type == MARX_DETECTOR_ACIS_S);
+   double fp_delta_s0, fp_x0, fp_y0;
    double time_del = ec->exposure_time;
    long last_expno = -1;
+   int pix_adj = ec->pix_adj;
    dither_state ds;
    unsigned int i;
 
-   if ((d->type != MARX_DETECTOR_ACIS_S) && !is_acis_i) return -1;
+   /* detpix.c:43-61: AXAF-FP-1.1 (ACIS), 2.1 (HRC-I), 2.3 (HRC-S) */
+   if (is_acis)
+     {
+        fp_delta_s0 = 0.492 * (PI / (3600.0 * 180.0));
+        fp_x0 = fp_y0 = 4096.5;
+     }
+   else if (type == MARX_DETECTOR_HRC_I)
+     {
+        fp_delta_s0 = 0.13175 * (PI / (3600.0 * 180.0));
+        fp_x0 = fp_y0 = 16384.5;
+     }
+   else if (type == MARX_DETECTOR_HRC_S)
+     {
+        fp_delta_s0 = 0.1318 * (PI / (3600.0 * 180.0));
+        fp_x0 = fp_y0 = 32768.5;
+     }
+   else
+     return -1;
+   if (!is_acis && (pix_adj == MXG_PIX_ADJ_EDSER))
+     pix_adj = MXG_PIX_ADJ_RANDOMIZE;
    if ((ec->exposure_time > 0.0) && (ec->frame_transfer_time > 0.0))
      time_del += ec->frame_transfer_time;
    memset (&ds, 0, sizeof (ds));
@@ -130,8 +159,10 @@ int orc_event_coords (const mxg_config *cfg, const mxg_event_cfg *ec, unsigned i
         orc_stream st;
         const mxg_chip *g;
 
-        /* compute_expno */
-        if (time_del <= 0.0)
+        /* compute_expno (ACIS; HRC updates the dither on every row) */
+        if (!is_acis)
+          update = 1;
+        else if (time_del <= 0.0)
           {
              last_expno++;
              update = 1;
@@ -146,7 +177,7 @@ int orc_event_coords (const mxg_config *cfg, const mxg_event_cfg *ec, unsigned i
          * or in EXACT mode */
         if (!ec->dither_used)
           memset (&ds, 0, sizeof (ds));
-        else if (update || (ec->pix_adj == MXG_PIX_ADJ_EXACT))
+        else if (update || (pix_adj == MXG_PIX_ADJ_EXACT))
           {
              ds.ra = dither[0][i];
              ds.dec = dither[1][i];
@@ -157,7 +188,9 @@ int orc_event_coords (const mxg_config *cfg, const mxg_event_cfg *ec, unsigned i
           }
 
         for (k = 0; k < d->num_chips; k++)
-          if ((d->chip[k].id == id) && (is_acis_i ? (id <= 3) : (id >= 4))) kc = k;
+          if ((d->chip[k].id == id) && ((type != MARX_DETECTOR_ACIS_I) || (id <= 3))
+              && ((type != MARX_DETECTOR_ACIS_S) || (id >= 4)))
+            kc = k;
         if (kc < 0)
           {
              tdetx[i] = tdety[i] = -1;
@@ -171,7 +204,18 @@ int orc_event_coords (const mxg_config *cfg, const mxg_event_cfg *ec, unsigned i
         {
            unsigned int ux = (unsigned int) chipx, uy = (unsigned int) chipy;
            float xf, yf;
-           if (!is_acis_i)
+           if (type == MARX_DETECTOR_HRC_S)
+             {
+                yf = uy;
+                yf = yf + Hrc_S_Tdet_Yoff[id - 1];
+                xf = ux + 0.0f;
+             }
+           else if (type == MARX_DETECTOR_HRC_I)
+             {
+                xf = ux + (-1.0f);
+                yf = uy + 0.0f;
+             }
+           else if (!is_acis_i)
              {
                 xf = ux + Tdet_Xoff[id];
                 yf = uy + Tdet_Yoff[id];
@@ -192,8 +236,10 @@ int orc_event_coords (const mxg_config *cfg, const mxg_event_cfg *ec, unsigned i
            tdety[i] = (int) (unsigned int) yf + 1;
         }
 
-        /* compute_fltgrade */
+        /* compute_fltgrade (ACIS) */
         orc_stream_init (&st, ec->seed, tag[i], EVENT_STAGE, 0);
+        fltgrade[i] = -1;
+        if (is_acis)
         {
            int dx = (int) (3.0 * (chipx - floor (chipx)));
            int dy = (int) (3.0 * (chipy - floor (chipy)));
@@ -220,7 +266,7 @@ int orc_event_coords (const mxg_config *cfg, const mxg_event_cfg *ec, unsigned i
 
            x = chipx - 1.0;
            y = chipy - 1.0;
-           switch (ec->pix_adj)
+           switch (pix_adj)
              {
               case MXG_PIX_ADJ_EXACT:
                 break;
@@ -243,8 +289,19 @@ int orc_event_coords (const mxg_config *cfg, const mxg_event_cfg *ec, unsigned i
                 }
                 break;
              }
-           x -= g->xpixel_offset;
-           y -= g->ypixel_offset;
+           if (type == MARX_DETECTOR_HRC_S)
+             {
+                /* the chip pixel of the facet's corner, as
+                 * _marx_hrc_s_compute_pixel (id, 0, 0) gives it
+                 * (hrc_s_geom.c:329-336, 344-394) */
+                x -= d->pix_x0[id] + (d->pix_u[id] - d->pix_u0[id]);
+                y -= d->pix_y0[id] + (d->pix_v[id] - d->pix_v0[id]);
+             }
+           else
+             {
+                x -= g->xpixel_offset;
+                y -= g->ypixel_offset;
+             }
            mnc.x = ofs.x + e1.x * x + e2.x * y;
            mnc.y = ofs.y + e1.y * x + e2.y * y;
            mnc.z = ofs.z + e1.z * x + e2.z * y;

@@ -5,12 +5,14 @@ oracle/subpix.py), pinned by what the geometry and the tables fix:
   between them and extends the end segments (acis_subpix.c:283-322);
 - the tiled system lays the chips out as acis_geom.c:104-109 describes
   (ACIS-I 01/23 with chips 0, 2 turned one way and 1, 3 the other; ACIS-S
-  side by side), each chip onto a disjoint 1024 x 1024 tile;
+  side by side), each chip onto a disjoint 1024 x 1024 tile, and the three
+  HRC-S segments one after the other along the tiled y axis (AXAF-HRC-2.7S);
 - a focal-plane pixel of the chip pixel of a simulated photon is the
   projection of that photon's own position through the mirror node;
 - undithering puts the events of a dithered on-axis source back on the sky
-  position of the source while their detector pixels spread over the
-  dither pattern, and the undithered sky pixels of a zero-amplitude dither
+  position of the source (the centre of each detector's focal-plane
+  system) while their detector pixels spread over the dither pattern, for
+  ACIS, HRC-S and HRC-I, and the undithered sky pixels of a zero-amplitude dither
   equal the roll rotation of the detector pixels (both sky formulas of
   marx2fits.c:3869-3910 agree);
 - the dither state is held over a frame: the result equals per-row dither
@@ -36,9 +38,9 @@ GRADES = [[10, 11, 138, 139], [2, 34, 130, 162], [18, 22, 50, 54], [8, 12, 136, 
           [16, 17, 48, 49], [72, 76, 104, 108], [64, 65, 68, 69], [80, 81, 208, 209]]
 
 
-def _trace(det="ACIS-S", dither="INTERNAL", n=40000, seed=5, **kw):
+def _trace(det="ACIS-S", dither="INTERNAL", n=40000, seed=5, emin=0.5, emax=7.0, **kw):
     t = Tracer(seed=seed, source=True, GratingType="NONE", DetectorType=det, SourceType="POINT",
-               SpectrumType="FLAT", MinEnergy=0.5, MaxEnergy=7.0, DitherModel=dither, **ON_AXIS, **kw)
+               SpectrumType="FLAT", MinEnergy=emin, MaxEnergy=emax, DitherModel=dither, **ON_AXIS, **kw)
     t.init_source(SourceFlux=n / (2e4 * t.geometric_area))
     o = Oracle(t, seed=seed)
     p = Photons(n)
@@ -58,8 +60,10 @@ def _cols(r):
     return c
 
 
-@pytest.fixture(scope="module")
+@pytest.fixture
 def dithered():
+    # per test: the library holds one configuration, which other tests'
+    # tracers replace
     return _trace()
 
 
@@ -118,12 +122,13 @@ def _grid_rows(ids, pix):
             "CHIPX": xy[:, 0], "CHIPY": xy[:, 1], "B_ENERGY": np.ones(n, np.float32)}
 
 
-@pytest.mark.parametrize("det,ids", [("ACIS-S", range(4, 10)), ("ACIS-I", range(4))])
+@pytest.mark.parametrize("det,ids", [("ACIS-S", range(4, 10)), ("ACIS-I", range(4)), ("HRC-S", range(1, 4))])
 def test_tiled_pixels(det, ids):
     t = Tracer(GratingType="NONE", DetectorType=det, DitherModel="NONE")
     o = Oracle(t, seed=1)
-    corners = [(0.2, 0.2), (1023.7, 0.2), (0.2, 1023.7), (1023.7, 1023.7), (100.5, 300.5), (101.5, 300.5),
-               (100.5, 301.5)]
+    nx, ny = (4096, 16384) if det == "HRC-S" else (1024, 1024)
+    corners = [(12.2, 12.2), (nx - 0.3, 12.2), (12.2, ny - 0.3), (nx - 0.3, ny - 0.3), (100.5, 300.5),
+               (101.5, 300.5), (100.5, 301.5)]
     out = o.event_coords(t.event_cfg("EXACT"), _grid_rows(list(ids), corners))
     tx = out["tdetx"].reshape(len(ids), -1)
     ty = out["tdety"].reshape(len(ids), -1)
@@ -132,7 +137,7 @@ def test_tiled_pixels(det, ids):
         # one chip pixel step moves one tiled pixel along one axis
         d1 = (tx[k, 5] - tx[k, 4], ty[k, 5] - ty[k, 4])
         d2 = (tx[k, 6] - tx[k, 4], ty[k, 6] - ty[k, 4])
-        if det == "ACIS-S":
+        if det in ("ACIS-S", "HRC-S"):
             assert d1 == (1, 0) and d2 == (0, 1)
         elif cid in (0, 2):
             # (0,0) at the upper left, chip x running down, chip y right
@@ -140,7 +145,8 @@ def test_tiled_pixels(det, ids):
         else:
             assert d1 == (0, 1) and d2 == (-1, 0)
         boxes[cid] = (tx[k, :4].min(), tx[k, :4].max(), ty[k, :4].min(), ty[k, :4].max())
-        assert boxes[cid][1] - boxes[cid][0] == 1023 and boxes[cid][3] - boxes[cid][2] == 1023
+        span = sorted((boxes[cid][1] - boxes[cid][0], boxes[cid][3] - boxes[cid][2]))
+        assert span == sorted((nx - 13, ny - 13))
     b = list(boxes.values())
     for i in range(len(b)):
         for j in range(i + 1, len(b)):
@@ -149,6 +155,10 @@ def test_tiled_pixels(det, ids):
         # 01 over 23
         assert boxes[0][1] < boxes[1][0] and boxes[2][1] < boxes[3][0]
         assert boxes[0][2] > boxes[2][3] and boxes[1][2] > boxes[3][3]
+    elif det == "HRC-S":
+        # the segments follow one another along y, the same x range
+        assert boxes[1][3] < boxes[2][2] and boxes[2][3] < boxes[3][2]
+        assert len({boxes[c][0] for c in ids}) == 1
     else:
         xs = [boxes[c][0] for c in ids]
         assert xs == sorted(xs) and len({boxes[c][2] for c in ids}) == 1
@@ -195,6 +205,24 @@ def test_sky_pixels_undo_the_dither(dithered):
         assert abs(np.median(sx) - 4096.5) < 0.5 and abs(np.median(sy) - 4096.5) < 0.5
         core = np.hypot(sx - np.median(sx), sy - np.median(sy))
         assert np.percentile(core, 50) < 1.5
+
+
+@pytest.mark.parametrize("det,centre,pix", [("HRC-S", 32768.5, 0.1318), ("HRC-I", 16384.5, 0.13175)])
+def test_hrc_sky_pixels_undo_the_dither(det, centre, pix):
+    """The same for HRC, in its own focal-plane system (detpix.c:50-61):
+    0.13 arcsec pixels about the centre pixel, the 16 arcsec dither being
+    about 120 of them."""
+    t, o, r = _trace(det=det, n=60000, emin=0.5, emax=2.0)
+    assert len(r) > 2000
+    for mode in ("EXACT", "EDSER"):
+        out = o.event_coords(t.event_cfg(mode), _cols(r))
+        assert (out["fltgrade"] == -1).all()
+        assert np.std(out["detx"]) > 8 / pix * 0.5 and np.std(out["dety"]) > 8 / pix * 0.5
+        sx, sy = out["x"], out["y"]
+        assert abs(np.median(sx) - centre) < 2 and abs(np.median(sy) - centre) < 2
+        core = np.hypot(sx - np.median(sx), sy - np.median(sy))
+        # the HRMA core and the HRC blur: about half an arcsec
+        assert np.percentile(core, 50) < 0.6 / pix
 
 
 def test_both_sky_formulas_agree(dithered):
