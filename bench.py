@@ -104,8 +104,9 @@ DESCRIPTION["output"] = (
     "and writing it with marx_write_photons")
 DESCRIPTION["sources"] = (
     "Source models: 1e7 photons per step of an on-axis POINT, DISK (2 arcmin), GAUSS (60 arcsec), BETA "
-    "(10 arcsec core, beta 0.7) and LINE (30 arcmin) source, flat 0.3-10 keV, INTERNAL dither, created on "
-    "the device and traced through HRMA + ACIS-S")
+    "(10 arcsec core, beta 0.7), LINE (30 arcmin) and IMAGE (512 x 512 pixels of 0.5 arcsec: a 20 arcsec "
+    "Gaussian blob on a 1 percent floor) source, flat 0.3-10 keV, INTERNAL dither, created on the device "
+    "and traced through HRMA + ACIS-S")
 DESCRIPTION["events"] = (
     "Event coordinates: marx2fits's tiled, focal-plane and sky pixels, flight grades and EDSER "
     "subpixel offsets of the detected photons of one traced HETG + ACIS-S chunk of 1e7 photons "
@@ -119,7 +120,10 @@ DEFAULT_PHOTONS = {"hetg-acis-s": 10_000_000, "letg-hrc-s": 50_000_000, "batched
 EVENT_BYTES = {"k_event_heads": 16, "k_event_coords": 75}
 SOURCE_SHAPES = {"POINT": {}, "DISK": {"S-DiskTheta0": 0.0, "S-DiskTheta1": 120.0},
                  "GAUSS": {"S-GaussSigma": 60.0}, "BETA": {"S-BetaCoreRadius": 10.0, "S-BetaBeta": 0.7},
-                 "LINE": {"S-LineTheta": 1800.0, "S-LinePhi": 30.0}}
+                 "LINE": {"S-LineTheta": 1800.0, "S-LinePhi": 30.0}, "IMAGE": {}}
+# the IMAGE shape's synthetic image: 512 x 512 pixels of 0.5 arcsec, a
+# Gaussian blob of 20 arcsec sigma over a flat floor of 1 percent of its peak
+IMAGE_SHAPE = (512, 512, 0.5)
 DIFF_NPIX, DIFF_PIXEL_MM = 256, 0.004
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
@@ -524,6 +528,21 @@ def measure_events(args, tmp, torch):
     return res
 
 
+def _write_source_image(path):
+    ny, nx, pix = IMAGE_SHAPE
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    r2 = ((xx - nx / 2) ** 2 + (yy - ny / 2) ** 2) * (pix / 20.0) ** 2
+    img = (np.exp(-0.5 * r2) + 0.01).astype(">f4")
+    cards = [("SIMPLE", "T"), ("BITPIX", "-32"), ("NAXIS", "2"), ("NAXIS1", str(nx)), ("NAXIS2", str(ny)),
+             ("CDELT1", repr(-pix / 3600)), ("CDELT2", repr(pix / 3600))]
+    hdr = "".join(f"{k:<8}= {v:>20}".ljust(80) for k, v in cards) + "END".ljust(80)
+    hdr = hdr.ljust((len(hdr) + 2879) // 2880 * 2880)
+    body = img.tobytes()
+    with open(path, "wb") as f:
+        f.write(hdr.encode("ascii") + body + b"\0" * ((-len(body)) % 2880))
+    return path
+
+
 def measure_sources(args, torch):
     """Source models: per shape, steps of 1e7 photons created on the device
     (k_source, k_scan_tiles, k_source_finish) and traced through HRMA and
@@ -533,7 +552,10 @@ def measure_sources(args, torch):
     cfg, K = "sources", args.steps
     N = args.photons or DEFAULT_PHOTONS[cfg]
     res = {}
+    image = _write_source_image(os.path.join(tempfile.mkdtemp(prefix="marx_bench_"), "image.fits"))
     for shape, par in SOURCE_SHAPES.items():
+        if shape == "IMAGE":
+            par = {"S-ImageFile": image}
         t = Tracer(seed=1, GratingType="NONE", DetectorType="ACIS-S", SourceType=shape, SpectrumType="FLAT",
                    MinEnergy=0.3, MaxEnergy=10.0, DitherModel="INTERNAL", **par, **ON_AXIS)
         t.init_source(SourceFlux=N / (args.exposure * t.geometric_area))

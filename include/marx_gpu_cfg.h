@@ -183,6 +183,7 @@ mxg_detector_cfg;
 #define MXG_SOURCE_GAUSS   3
 #define MXG_SOURCE_BETA    4
 #define MXG_SOURCE_LINE    5
+#define MXG_SOURCE_IMAGE   6
 #define MXG_SPECTRUM_FLAT  1
 #define MXG_SPECTRUM_FILE  2
 #define MXG_TIME_POISSON   0
@@ -212,7 +213,9 @@ typedef struct
    double beta_core, beta_xpon;     /* BETA: core radius (radians), 1 / (1 - alpha) (s-beta.c) */
    double line_theta;               /* LINE: half length (radians) (s-line.c) */
    double line_cos_phi, line_sin_phi;    /* of its orientation angle */
-   double line_alpha, line_normal[3];    /* the rotation of -x onto the source direction */
+   double line_alpha, line_normal[3];    /* LINE, IMAGE: the rotation of -x onto the source direction */
+   unsigned int image_n, image_nx, image_ny, image_off;   /* IMAGE: cumulative pixel distribution */
+   double image_xscale, image_yscale;    /* radians per pixel (float arena at image_off) */
 }
 mxg_source_cfg;
 

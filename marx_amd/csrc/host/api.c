@@ -609,6 +609,12 @@ static int batch_setup (unsigned int num_sources, const double *dirs, const unsi
         mxg_error ("mxg_trace_batch: no sources");
         return -1;
      }
+   if ((Mxg_Cfg.src.type == MXG_SOURCE_IMAGE) || (Mxg_Cfg.src.type == MXG_SOURCE_RAYFILE))
+     {
+        mxg_error ("mxg_trace_batch: the sources of a batch are points, disks, Gaussians, beta models "
+                   "or lines about their directions, not images or ray files");
+        return -1;
+     }
    if (-1 == mxgd_batch_set_sources (num_sources, dirs, counts, n))
      {
         mxg_error ("%s", mxgd_last_error ());

@@ -305,6 +305,181 @@ static double rotation_axis (JDMVector_Type a, JDMVector_Type b, JDMVector_Type 
    return theta;
 }
 
+/* ---- IMAGE (s-image.c) ------------------------------------------------------------- */
+
+/* the value of keyword key in a FITS header of ncards cards, or -1 */
+static int fits_card (const char *hdr, unsigned int ncards, const char *key, char *val, unsigned int vlen)
+{
+   unsigned int i, n = (unsigned int) strlen (key);
+   for (i = 0; i < ncards; i++)
+     {
+        const char *c = hdr + 80 * i;
+        unsigned int j = 0, k;
+        if ((0 != strncmp (c, key, n)) || ((n < 8) && (c[n] != ' ')) || (c[8] != '=')) continue;
+        for (k = 10; (k < 80) && (c[k] == ' '); k++) ;
+        while ((k < 80) && (c[k] != '/') && (j + 1 < vlen)) val[j++] = c[k++];
+        while ((j > 0) && (val[j - 1] == ' ')) j--;
+        val[j] = 0;
+        return 0;
+     }
+   return -1;
+}
+
+static int fits_double (const char *hdr, unsigned int ncards, const char *key, double *v)
+{
+   char buf[80], *end;
+   if (-1 == fits_card (hdr, ncards, key, buf, sizeof (buf))) return -1;
+   *v = strtod (buf, &end);
+   return (end == buf) ? -1 : 0;
+}
+
+/* The primary image of a FITS file as the cumulative distribution of its
+ * pixels in the float arena, and its pixel scale (s-image.c:44-134,
+ * 162-238): BITPIX 8, 16, 32, -32 or -64 (values as stored), CDELT1/2 or
+ * the CD matrix, else 0.5 arcsec; the running sum is kept in double,
+ * stored as float, then divided by the total. */
+static int read_image_source (const char *file, mxg_source_cfg *s)
+{
+   FILE *fp = fopen (file, "rb");
+   char *hdr = NULL, val[80];
+   unsigned int ncards = 0, done = 0, i, n, nx, ny, bytes;
+   double naxis1, naxis2, bitpix, s1, s2, sum;
+   unsigned char *buf = NULL;
+   float *cdf;
+
+   if (fp == NULL)
+     {
+        mxg_error ("Failed to open %s", file);
+        return -1;
+     }
+   while (!done)
+     {
+        char *h = (char *) realloc (hdr, 80 * (ncards + 36));
+        if (h == NULL) goto nomem;
+        hdr = h;
+        if (2880 != fread (hdr + 80 * ncards, 1, 2880, fp))
+          {
+             mxg_error ("%s: truncated FITS header", file);
+             goto fail;
+          }
+        for (i = 0; i < 36; i++)
+          if (0 == strncmp (hdr + 80 * (ncards + i), "END     ", 8)) done = 1;
+        ncards += 36;
+     }
+   if ((-1 == fits_double (hdr, ncards, "NAXIS1", &naxis1))
+       || (-1 == fits_double (hdr, ncards, "NAXIS2", &naxis2))
+       || (-1 == fits_double (hdr, ncards, "BITPIX", &bitpix)))
+     {
+        mxg_error ("%s: NAXIS1, NAXIS2 or BITPIX missing", file);
+        goto fail;
+     }
+   if (0 == fits_double (hdr, ncards, "CDELT1", &s1))
+     {
+        if (-1 == fits_double (hdr, ncards, "CDELT2", &s2))
+          {
+             mxg_error ("Error locating keyword CDELT2");
+             goto fail;
+          }
+        s1 = fabs (s1);
+        s2 = fabs (s2);
+     }
+   else if (0 == fits_double (hdr, ncards, "CD1_1", &s1))
+     {
+        double c12, c21, c22;
+        if ((-1 == fits_double (hdr, ncards, "CD1_2", &c12))
+            || (-1 == fits_double (hdr, ncards, "CD2_1", &c21))
+            || (-1 == fits_double (hdr, ncards, "CD2_2", &c22)))
+          {
+             mxg_error ("Error locating a CD keyword");
+             goto fail;
+          }
+        s2 = sqrt (c21 * c21 + c22 * c22);
+        s1 = sqrt (s1 * s1 + c12 * c12);
+     }
+   else s1 = s2 = 0.0;
+   if ((s1 == 0.0) || (s2 == 0.0))
+     {
+        mxg_message (0, "Cannot determine scale of image.  Assuming 0.5 arc-sec per pixel\n");
+        s1 = s2 = 0.5 / 3600.0;
+     }
+   (void) val;
+   nx = (unsigned int) naxis1;
+   ny = (unsigned int) naxis2;
+   n = nx * ny;
+   switch ((int) bitpix)
+     {
+      case 8: bytes = 1; break;
+      case 16: bytes = 2; break;
+      case 32: case -32: bytes = 4; break;
+      case -64: bytes = 8; break;
+      default:
+        mxg_error ("BITPIX = %d is not supported", (int) bitpix);
+        goto fail;
+     }
+   if ((n == 0) || (n > (1u << 28)))
+     {
+        mxg_error ("%s: image of %u x %u pixels not supported", file, nx, ny);
+        goto fail;
+     }
+   if (NULL == (buf = (unsigned char *) malloc ((size_t) n * bytes))) goto nomem;
+   if (n != fread (buf, bytes, n, fp))
+     {
+        mxg_error ("Error reading image file");
+        goto fail;
+     }
+   s->image_off = mxg_farena_reserve (n);
+   if (s->image_off == (unsigned int) -1) goto nomem;
+   cdf = mxg_farena_ptr (s->image_off);
+   sum = 0.0;
+   for (i = 0; i < n; i++)
+     {
+        const unsigned char *b = buf + (size_t) i * bytes;
+        double v;
+        if (bytes == 1) v = (double) (float) b[0];
+        else if (bytes == 2) v = (double) (float) (short) ((b[0] << 8) | b[1]);
+        else
+          {
+             unsigned long long u = 0;
+             unsigned int k;
+             for (k = 0; k < bytes; k++) u = (u << 8) | b[k];
+             if (bitpix == 32) v = (double) (float) (int) (unsigned int) u;
+             else if (bitpix == -32)
+               {
+                  unsigned int w = (unsigned int) u;
+                  float f;
+                  memcpy (&f, &w, 4);
+                  v = (double) f;
+               }
+             else
+               {
+                  double d;
+                  memcpy (&d, &u, 8);
+                  v = (double) (float) d;
+               }
+          }
+        sum += v;
+        cdf[i] = (float) sum;
+     }
+   for (i = 0; i < n; i++)
+     cdf[i] = (float) (cdf[i] / sum);
+   s->image_n = n;
+   s->image_nx = nx;
+   s->image_ny = ny;
+   s->image_xscale = s1 * PI / 180.0;
+   s->image_yscale = s2 * PI / 180.0;
+   free (buf);
+   free (hdr);
+   fclose (fp);
+   return 0;
+nomem:
+   mxg_error ("Not enough memory for the image source");
+fail:
+   free (buf);
+   free (hdr);
+   fclose (fp);
+   return -1;
+}
+
 int mxg_source_init (Param_File_Type *pf)
 {
    mxg_source_cfg *s = &Mxg_Cfg.src;
@@ -324,9 +499,11 @@ int mxg_source_init (Param_File_Type *pf)
    else if (!strcmp (name, "GAUSS")) s->type = MXG_SOURCE_GAUSS;
    else if (!strcmp (name, "BETA")) s->type = MXG_SOURCE_BETA;
    else if (!strcmp (name, "LINE")) s->type = MXG_SOURCE_LINE;
+   else if (!strcmp (name, "IMAGE")) s->type = MXG_SOURCE_IMAGE;
    else
      {
-        mxg_error ("SourceType %s not supported (POINT, DISK, GAUSS, BETA, LINE and RAYFILE are)", name);
+        mxg_error ("SourceType %s not supported (POINT, DISK, GAUSS, BETA, LINE, IMAGE and RAYFILE are)",
+                   name);
         return -1;
      }
    if (s->type == MXG_SOURCE_RAYFILE)
@@ -420,6 +597,26 @@ int mxg_source_init (Param_File_Type *pf)
         s->line_sin_phi = sin (phi);
         s->line_alpha = rotation_axis (a, b, &c);
         s->line_normal[0] = c.x; s->line_normal[1] = c.y; s->line_normal[2] = c.z;
+     }
+   else if (s->type == MXG_SOURCE_IMAGE)
+     {
+        /* s-image.c:370-385, 262-282: the image centre is -x, rotated onto
+         * the source direction about cross(-x, p) (left unnormalised, and
+         * unused, when the angle is 0) */
+        char file[1024];
+        JDMVector_Type m = mxg_v (-1.0, 0.0, 0.0), p0 = mxg_v (s->p[0], s->p[1], s->p[2]), nrm;
+        double theta;
+        if (-1 == pf_get_file (pf, "S-ImageFile", file, sizeof (file)))
+          return -1;
+        if (-1 == read_image_source (file, s))
+          return -1;
+        theta = mxg_v_dot (p0, m);
+        if (fabs (theta) > 1.0) theta = (theta < 0) ? -1.0 : 1.0;
+        theta = acos (theta);
+        nrm = mxg_v_cross (m, p0);
+        if (theta != 0.0) mxg_v_normalize (&nrm);
+        s->line_alpha = theta;
+        s->line_normal[0] = nrm.x; s->line_normal[1] = nrm.y; s->line_normal[2] = nrm.z;
      }
    if (-1 == spectrum_init (pf, s, name))
      return -1;

@@ -173,7 +173,7 @@ hipError_t launch_detect (const mxg_config *cfg, const float *fa, const double *
 
 hipError_t launch_dither_store (const mxg_config *cfg, Soa s, uint32_t n, double start_time,
                                 unsigned long long seed, hipStream_t st);
-hipError_t launch_source (const mxg_config *cfg, const double *da, Soa s, uint32_t n,
+hipError_t launch_source (const mxg_config *cfg, const float *fa, const double *da, Soa s, uint32_t n,
                           uint32_t tag_start, double exposure_time, double start_time,
                           unsigned long long seed, uint32_t source_id, uint64_t *ticks,
                           uint64_t *tile_tot, uint32_t *first_over, uint32_t *count,

@@ -568,7 +568,7 @@ int mxgd_run_source (unsigned int n, unsigned int tag_start, double start_time,
         HIPCHECK (hipEventRecord (g_tev[slot][0], g_stream));
         g_slot_used[slot] = 1;
      }
-   HIPCHECK (launch_source (d_cfg, d_da, g_soa, n, tag_start, exposure_time, start_time, seed,
+   HIPCHECK (launch_source (d_cfg, d_fa, d_da, g_soa, n, tag_start, exposure_time, start_time, seed,
                             source_id, d_ticks, d_tile_tot, d_counts + 2, d_counts, d_idx[0],
                             g_stream));
    if (slot >= 0 && slot < kSlots)

@@ -47,9 +47,13 @@ __device__ __forceinline__ void init_photon (Soa s, uint32_t i, double energy, v
 // line_alpha: the rotation of -x onto p): a point source's p; uniform over
 // a disk (s-disk.c:95-106); a Gaussian or beta-model profile
 // (s-gauss.c:121-137, s-beta.c:108-124); uniform along a line
-// (s-line.c:89-98).  The draws follow the energy in the source stream.
-__device__ __forceinline__ vec3 extended_direction (const mxg_source_cfg &src, vec3 p, vec3 p_normal,
-                                                    vec3 line_normal, double line_alpha, Stream &st)
+// (s-line.c:89-98); a pixel of an image drawn from its cumulative
+// distribution (fa + image_off), uniform within the pixel, the image centre
+// at -x rotated onto the source direction (s-image.c:284-324).  The draws
+// follow the energy in the source stream.
+__device__ __forceinline__ vec3 extended_direction (const mxg_source_cfg &src, const float *__restrict__ fa,
+                                                    vec3 p, vec3 p_normal, vec3 line_normal, double line_alpha,
+                                                    Stream &st)
 {
    if (src.type == MXG_SOURCE_DISK)
      {
@@ -71,6 +75,20 @@ __device__ __forceinline__ vec3 extended_direction (const mxg_source_cfg &src, v
         else
           rnd = src.beta_core * sqrt (pow (rnd, src.beta_xpon) - 1.0);
         return rotate_unit_vector (p, normal, rnd);
+     }
+   if (src.type == MXG_SOURCE_IMAGE)
+     {
+        const unsigned int ofs = search_f ((float) uniform (st), fa + src.image_off, src.image_n);
+        double y = (double) (ofs / src.image_nx);
+        double x = (double) (ofs % src.image_nx);
+        y += -0.5 * src.image_ny + (uniform (st) - 0.5);
+        x += -0.5 * src.image_nx + (uniform (st) - 0.5);
+        y = y * src.image_yscale;
+        x = x * src.image_xscale;
+        double sx, cx, sy, cy;
+        sincos (x, &sx, &cx);
+        sincos (y, &sy, &cy);
+        return rotate_unit_vector (v3 (-cy * cx, cy * sx, -sy), line_normal, line_alpha);
      }
    if (src.type == MXG_SOURCE_LINE)
      {
@@ -109,7 +127,7 @@ __device__ __forceinline__ uint64_t gap_ticks (double e)
 }
 
 __global__ __launch_bounds__ (kBlock)
-void k_source (const mxg_config *__restrict__ cfg, const double *__restrict__ da, Soa s,
+void k_source (const mxg_config *__restrict__ cfg, const float *__restrict__ fa, const double *__restrict__ da, Soa s,
                uint32_t n, uint32_t tag_start, unsigned long long seed, uint32_t source_id,
                uint64_t *__restrict__ ticks, uint64_t *__restrict__ tile_tot)
 {
@@ -130,7 +148,7 @@ void k_source (const mxg_config *__restrict__ cfg, const double *__restrict__ da
              Stream st;
              stream_init (st, seed, tag, MXG_STAGE_SOURCE, source_id);
              double energy = draw_energy (src, da, st);
-             vec3 p = extended_direction (src, ld3 (src.p), ld3 (src.p_normal), ld3 (src.line_normal),
+             vec3 p = extended_direction (src, fa, ld3 (src.p), ld3 (src.p_normal), ld3 (src.line_normal),
                                           src.line_alpha, st);
              double t = 0.0;
              if (poisson)
@@ -314,7 +332,7 @@ __global__ void k_source_count (uint32_t n, const uint32_t *__restrict__ first_o
 
 __global__ void k_reset (uint32_t *p, uint32_t v) { *p = v; }
 
-hipError_t launch_source (const mxg_config *cfg, const double *da, Soa s, uint32_t n,
+hipError_t launch_source (const mxg_config *cfg, const float *fa, const double *da, Soa s, uint32_t n,
                           uint32_t tag_start, double exposure_time, double start_time,
                           unsigned long long seed, uint32_t source_id, uint64_t *ticks,
                           uint64_t *tile_tot, uint32_t *first_over, uint32_t *count,
@@ -327,7 +345,7 @@ hipError_t launch_source (const mxg_config *cfg, const double *da, Soa s, uint32
      }
    const uint32_t ntiles = (n + kTile - 1) / kTile;
    hipLaunchKernelGGL (k_reset, dim3 (1), dim3 (1), 0, st, first_over, 0xFFFFFFFFu);
-   hipLaunchKernelGGL (k_source, dim3 (ntiles), dim3 (kBlock), 0, st, cfg, da, s, n, tag_start,
+   hipLaunchKernelGGL (k_source, dim3 (ntiles), dim3 (kBlock), 0, st, cfg, fa, da, s, n, tag_start,
                        seed, source_id, ticks, tile_tot);
    hipLaunchKernelGGL (k_scan_tiles, dim3 (1), dim3 (kBlock), 0, st, tile_tot, ntiles);
    hipLaunchKernelGGL (k_source_finish, dim3 ((n + kBlock - 1) / kBlock), dim3 (kBlock), 0, st,
@@ -388,7 +406,7 @@ void k_batch_source (const mxg_config *__restrict__ cfg, const double *__restric
              vec3 ln = v3 (0.0, 0.0, 0.0);
              double la = 0.0;
              if (src.type == MXG_SOURCE_LINE) la = rotation_axis (v3 (-1.0, 0.0, 0.0), p, ln);
-             p = extended_direction (src, p, nrm, ln, la, st);
+             p = extended_direction (src, nullptr, p, nrm, ln, la, st);
           }
         double t = exposure * uniform (st);
         init_photon (s, i, energy, p, t, tag);

@@ -55,9 +55,10 @@ static double draw_energy (const orc_ctx *c, const mxg_source_cfg *s, orc_stream
 
 /* The direction of a photon of the configured shape about the source
  * direction p (s-disk.c:95-106, s-gauss.c:121-137, s-beta.c:108-124,
- * s-line.c:89-98); p_normal is normal to p, line_normal and line_alpha
- * rotate -x onto p. */
-static orc_vec extended_direction (const mxg_source_cfg *s, orc_vec p, orc_vec p_normal,
+ * s-line.c:89-98, s-image.c:284-324); p_normal is normal to p, line_normal
+ * and line_alpha rotate -x onto p; an image's cumulative pixel
+ * distribution is at fa + image_off. */
+static orc_vec extended_direction (const mxg_source_cfg *s, const float *fa, orc_vec p, orc_vec p_normal,
                                    orc_vec line_normal, double line_alpha, orc_stream *st)
 {
    if (s->type == MXG_SOURCE_DISK)
@@ -81,6 +82,24 @@ static orc_vec extended_direction (const mxg_source_cfg *s, orc_vec p, orc_vec p
         else
           rnd = s->beta_core * sqrt (pow (rnd, s->beta_xpon) - 1.0);
         return orc_v_rotate_unit_vector (p, normal, rnd);
+     }
+   if (s->type == MXG_SOURCE_IMAGE)
+     {
+        /* s-image.c:284-324 */
+        unsigned int ofs = orc_binary_search_f ((float) orc_uniform (st), fa + s->image_off, s->image_n);
+        double x, y, cos_y;
+        orc_vec p1;
+        y = (double) (ofs / s->image_nx);
+        x = (double) (ofs % s->image_nx);
+        y += -0.5 * s->image_ny + (orc_uniform (st) - 0.5);
+        x += -0.5 * s->image_nx + (orc_uniform (st) - 0.5);
+        y = y * s->image_yscale;
+        x = x * s->image_xscale;
+        cos_y = cos (y);
+        p1.x = -cos_y * cos (x);
+        p1.y = cos_y * sin (x);
+        p1.z = -sin (y);
+        return orc_v_rotate_unit_vector (p1, line_normal, line_alpha);
      }
    if (s->type == MXG_SOURCE_LINE)
      {
@@ -168,7 +187,7 @@ int orc_create_photons (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int num
         at[i].tag = pt->tag_start + i;
         orc_stream_init (st, c->seed, at[i].tag, MXG_STAGE_SOURCE, c->source_id);
         at[i].energy = draw_energy (c, s, st);
-        at[i].p = extended_direction (s, orc_v (s->p[0], s->p[1], s->p[2]),
+        at[i].p = extended_direction (s, c->fa, orc_v (s->p[0], s->p[1], s->p[2]),
                                       orc_v (s->p_normal[0], s->p_normal[1], s->p_normal[2]),
                                       orc_v (s->line_normal[0], s->line_normal[1], s->line_normal[2]),
                                       s->line_alpha, st);
@@ -249,7 +268,7 @@ int orc_create_batch (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int nsrc,
                double la = 0.0;
                orc_v_normalize (&nrm);
                if (s->type == MXG_SOURCE_LINE) la = rotation_axis (orc_v (-1.0, 0.0, 0.0), p, &ln);
-               at[i].p = extended_direction (s, p, nrm, ln, la, &st);
+               at[i].p = extended_direction (s, c->fa, p, nrm, ln, la, &st);
             }
           at[i].arrival_time = exposure * orc_uniform (&st);
           if (s->dither_mode == MXG_DITHER_INTERNAL)

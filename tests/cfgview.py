@@ -94,7 +94,8 @@ class SourceCfg(ctypes.Structure):
                 ("ra_phase", D), ("dec_phase", D), ("roll_phase", D),
                 ("nominal_roll", D), ("aspect_blur", D), ("gauss_sigma", D), ("beta_core", D), ("beta_xpon", D),
                 ("line_theta", D), ("line_cos_phi", D), ("line_sin_phi", D), ("line_alpha", D),
-                ("line_normal", D * 3)]
+                ("line_normal", D * 3), ("image_n", U), ("image_nx", U), ("image_ny", U),
+                ("image_off", U), ("image_xscale", D), ("image_yscale", D)]
 
 
 class Config(ctypes.Structure):

@@ -77,3 +77,79 @@ def test_line_geometry(phi):
     half = 600.0 * ARCSEC
     assert np.abs(theta).max() <= half * (1 + 1e-9)
     assert stats.kstest(theta, stats.uniform(-half, 2 * half).cdf).pvalue > 1e-3
+
+
+# ---- IMAGE (s-image.c) ------------------------------------------------------------------
+
+from tests.fitsimg import write_image  # noqa: E402
+
+
+def _image_tracer(path, **kw):
+    return Tracer(seed=19, source=True, GratingType="NONE", DetectorType="NONE", SourceType="IMAGE",
+                  SpectrumType="FLAT", MinEnergy=1.0, MaxEnergy=2.0, DitherModel="NONE",
+                  **{"S-ImageFile": path}, **kw)
+
+
+@pytest.mark.parametrize("bitpix", [8, 16, 32, -32, -64])
+def test_image_distribution_from_every_bitpix(tmp_path, bitpix):
+    """The cumulative pixel distribution in the float arena: the running sum
+    in double of the float pixel values, stored as float, then divided by
+    the total (s-image.c:44-134); the scale from CDELT, the CD matrix, or
+    0.5 arcsec."""
+    from cfgview import ConfigView
+    rng = np.random.default_rng(bitpix + 100)
+    data = rng.integers(0, 120, (7, 11)).astype(np.float64)
+    if bitpix < 0:
+        data = data * 0.37 + rng.uniform(0, 1, data.shape)
+    path = write_image(str(tmp_path / "im.fits"), data, bitpix, cdelt=(-2e-4, 3e-4))
+    t = _image_tracer(path, **ON_AXIS)
+    c = ConfigView(t)
+    s = c.cfg.src
+    assert (s.image_nx, s.image_ny, s.image_n) == (11, 7, 77)
+    vals = np.asarray(data, dtype=np.float32 if bitpix != -64 else np.float64).astype(np.float32)
+    run = np.cumsum(vals.astype(np.float64).ravel())
+    cdf = (run.astype(np.float32).astype(np.float64) / run[-1]).astype(np.float32)
+    assert np.array_equal(c.f(s.image_off, 77), cdf)
+    assert s.image_xscale == pytest.approx(2e-4 * np.pi / 180, rel=1e-15)
+    assert s.image_yscale == pytest.approx(3e-4 * np.pi / 180, rel=1e-15)
+    # the CD matrix: the lengths of its rows; nothing: 0.5 arcsec
+    write_image(path, data, bitpix, cd=(3e-4, 4e-4, 0.0, -2e-4))
+    s = ConfigView(_image_tracer(path, **ON_AXIS)).cfg.src
+    assert s.image_xscale == pytest.approx(5e-4 * np.pi / 180, rel=1e-15)
+    assert s.image_yscale == pytest.approx(2e-4 * np.pi / 180, rel=1e-15)
+    write_image(path, data, bitpix)
+    s = ConfigView(_image_tracer(path, **ON_AXIS)).cfg.src
+    assert s.image_xscale == s.image_yscale == pytest.approx(0.5 * ARCSEC, rel=1e-15)
+
+
+def test_image_photons_land_in_their_pixels(tmp_path):
+    """An on-axis image with two bright pixels of weights 3 and 1 in a 13 x 9
+    image of 1.5 x 2.5 arcsec pixels: inverting the direction of each photon
+    (x = atan(p_y / -p_x), y = -asin(p_z), pixel = round(angle / scale +
+    size / 2)) gives one of the two pixels, in the ratio 3:1, spread
+    uniformly within it.  A transposed image, swapped scales or a sign slip
+    move photons to other pixels."""
+    data = np.zeros((9, 13))
+    data[2, 10] = 3.0
+    data[6, 3] = 1.0
+    path = write_image(str(tmp_path / "im.fits"), data, -32, cdelt=(1.5 / 3600, 2.5 / 3600))
+    t = _image_tracer(path, SourceRA=250.2134679741175, SourceDEC=-53.75743813458669)
+    o = Oracle(t)
+    n = 40000
+    p = Photons(n)
+    o.create_photons(p, n)
+    a = p.active
+    from cfgview import ConfigView
+    s = ConfigView(t).cfg.src
+    # the on-axis source: the image centre stays at -x
+    assert s.line_alpha < 1e-9
+    x = np.arctan2(a["p"][:, 1], -a["p"][:, 0]) / (1.5 * ARCSEC) + 0.5 * 13
+    y = -np.arcsin(a["p"][:, 2]) / (2.5 * ARCSEC) + 0.5 * 9
+    col, row = np.floor(x + 0.5).astype(int), np.floor(y + 0.5).astype(int)
+    pix = set(zip(row.tolist(), col.tolist()))
+    assert pix == {(2, 10), (6, 3)}
+    k = np.sum((row == 2) & (col == 10))
+    assert abs(k / n - 0.75) < 4 * np.sqrt(0.75 * 0.25 / n)
+    # uniform within the pixel
+    fx = x - col + 0.5
+    assert stats.kstest(fx, "uniform").pvalue > 1e-3

@@ -1,6 +1,6 @@
 """Photon creation: GPU source kernels vs the CPU oracle through the C-ABI
 (marx_create_photons, replacing marx/libsrc/source.c:268-384 with
-s-point.c, s-disk.c, s-gauss.c, s-beta.c, s-line.c, spectrum.c and the
+s-point.c, s-disk.c, s-gauss.c, s-beta.c, s-line.c, s-image.c, spectrum.c and the
 INTERNAL dither of dither.c), and
 the whole chain source -> HRMA -> HETG -> ACIS-S, through the AoS entry
 points and through the device-resident mxg_trace_device."""
@@ -10,6 +10,7 @@ import pytest
 from marx_amd import Photons, Tracer
 from marx_amd.spectra import write_power_law
 from oracle.oracle import Oracle
+from tests.fitsimg import write_image
 from tests.parity import compare
 
 ON_AXIS = dict(SourceRA=250.2134679741175, SourceDEC=-53.75743813458669)
@@ -28,7 +29,7 @@ def _both(t, num, exposure=None):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("kind", ["flat-point-dither", "file-point", "flat-disk-dither", "uniform-times",
-                                  "gauss-dither", "beta-offaxis", "line-file"])
+                                  "gauss-dither", "beta-offaxis", "line-file", "image-offaxis-dither"])
 def test_source_parity(gpu, tmp_path, kind):
     extra = dict(ON_AXIS)
     utimes = 0.0
@@ -52,6 +53,13 @@ def test_source_parity(gpu, tmp_path, kind):
         extra = dict(SourceRA=ON_AXIS["SourceRA"] - 0.03, SourceDEC=ON_AXIS["SourceDEC"] + 0.01,
                      SourceType="LINE", **{"S-LineTheta": 900.0, "S-LinePhi": 35.0},
                      SpectrumType="FILE", SpectrumFile=f, DitherModel="NONE")
+    elif kind == "image-offaxis-dither":
+        rng = np.random.default_rng(3)
+        img = rng.gamma(0.3, 1.0, (48, 64))
+        f = write_image(str(tmp_path / "im.fits"), img, -32, cdelt=(-0.4 / 3600, 0.6 / 3600))
+        extra = dict(SourceRA=ON_AXIS["SourceRA"] + 0.02, SourceDEC=ON_AXIS["SourceDEC"] - 0.01,
+                     SourceType="IMAGE", **{"S-ImageFile": f}, SpectrumType="FLAT", MinEnergy=0.3,
+                     MaxEnergy=8.0, DitherModel="INTERNAL")
     else:
         extra.update(SpectrumType="FLAT", MinEnergy=1.49, MaxEnergy=1.49, DitherModel="NONE")
         utimes = 1.0e4
