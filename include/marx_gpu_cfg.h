@@ -184,6 +184,7 @@ mxg_detector_cfg;
 #define MXG_SOURCE_BETA    4
 #define MXG_SOURCE_LINE    5
 #define MXG_SOURCE_IMAGE   6
+#define MXG_SOURCE_USER    7       /* rays from a dlopen()ed plug-in, on the host */
 #define MXG_SPECTRUM_FLAT  1
 #define MXG_SPECTRUM_FILE  2
 #define MXG_TIME_POISSON   0

@@ -400,6 +400,114 @@ static int replay_rayfile (Marx_Source_Type *st, Marx_Photon_Type *pt, unsigned 
    return 0;
 }
 
+/* A USER source (source.c:268-384 over s-user.c:140-205): the device draws
+ * each photon's energy from the spectrum (and Poisson arrival times) in its
+ * own stream, with the dither held back; the plug-in's user_create_ray then
+ * gives the direction, and may replace the energy and give time steps,
+ * photon by photon on the host until it returns -1.  The exposure cut,
+ * tags, dither (on the device) and the energy sort follow as for any
+ * source. */
+static int create_user (Marx_Photon_Type *pt, unsigned int num, unsigned int *num_collected,
+                        double *exposure_time)
+{
+   mxg_source_cfg *s = &Mxg_Cfg.src;
+   Marx_Photon_Attr_Type *at = pt->attributes;
+   unsigned int n = 0, i;
+   double last_t = 0.0, t = -1.0, user_time = 0.0;
+   int dither_mode = s->dither_mode, status;
+
+   memset (at, 0, (size_t) num * sizeof (Marx_Photon_Attr_Type));
+   if (-1 == mxg_user_start_iteration ())
+     {
+        pt->n_photons = 0;
+        pt->total_time = 0.0;
+        return sort_by_energy (pt);
+     }
+   if (num > 0)
+     {
+        s->dither_mode = MXG_DITHER_NONE;
+        Mxg_Cfg_Generation++;
+        status = sync_config ();
+        Device_Trace = 0;
+        if (status == 0)
+          status = mxgd_run_source (num, pt->tag_start, pt->start_time, -1.0, Seed, 0, -1, &n, &last_t, NULL);
+        s->dither_mode = dither_mode;
+        Mxg_Cfg_Generation++;
+        if ((status == -1) || ((n > 0) && (-1 == mxgd_download_aos (at, n))))
+          {
+             mxg_error ("%s", mxgd_last_error ());
+             return -1;
+          }
+     }
+   for (i = 0; i < n; i++)
+     {
+        double energy = at[i].energy, drawn = energy, px, py, pz;
+        if (-1 == mxg_user_create_ray (&t, &energy, &px, &py, &pz))
+          break;
+        at[i].flags = 0;
+        at[i].p.x = px;
+        at[i].p.y = py;
+        at[i].p.z = pz;
+        if (t >= 0.0)
+          {
+             user_time += t;
+             at[i].arrival_time = user_time;
+          }
+        if (energy < 0.0) energy = drawn;
+        at[i].energy = energy;
+     }
+   n = i;
+   if (t >= 0.0)
+     {
+        /* the plug-in's times: cut at the exposure from the start time */
+        if (exposure_time != NULL)
+          {
+             double tmax = _Marx_TStart_MJDsecs + *exposure_time;
+             for (i = 0; i < n; i++)
+               if (at[i].arrival_time >= tmax)
+                 {
+                    n = i + 1;
+                    break;
+                 }
+          }
+     }
+   else if (exposure_time != NULL)
+     {
+        /* Poisson times: the first photon at or past the exposure is the last */
+        for (i = 0; i < n; i++)
+          if (*exposure_time <= at[i].arrival_time)
+            {
+               n = i + 1;
+               break;
+            }
+     }
+   pt->history = MARX_ENERGY_OK | MARX_TIME_OK | MARX_X_VECTOR_OK | MARX_P_VECTOR_OK;
+   if (n)
+     {
+        for (i = 0; i < n; i++) at[i].tag = pt->tag_start + i;
+        pt->history |= MARX_TAG_OK;
+        pt->tag_start += n;
+        if (dither_mode != MXG_DITHER_NONE)
+          {
+             if ((-1 == sync_config ())
+                 || (-1 == mxgd_upload_aos (at, n, NULL, 0))
+                 || (-1 == mxgd_dither_store (pt->start_time, Seed))
+                 || (-1 == mxgd_download_aos (at, n)))
+               {
+                  mxg_error ("%s", mxgd_last_error ());
+                  return -1;
+               }
+          }
+     }
+   pt->n_photons = n;
+   Resident = NULL;
+   if (-1 == sort_by_energy (pt))
+     return -1;
+   pt->total_time = n ? at[n - 1].arrival_time : 0.0;
+   *num_collected = n;
+   return 0;
+}
+
 int marx_create_photons (Marx_Source_Type *st, Marx_Photon_Type *pt, unsigned int num,
                          unsigned int *num_collected, double *exposure_time)
 {
@@ -420,6 +528,8 @@ int marx_create_photons (Marx_Source_Type *st, Marx_Photon_Type *pt, unsigned in
    pt->start_time += pt->total_time;
    if (s->type == MXG_SOURCE_RAYFILE)
      return replay_rayfile (st, pt, num, num_collected, exposure_time);
+   if (s->type == MXG_SOURCE_USER)
+     return create_user (pt, num, num_collected, exposure_time);
 
    if (-1 == sync_config ())
      return -1;
@@ -482,10 +592,10 @@ int mxg_trace_device (unsigned int num, unsigned int tag_start, double start_tim
         mxg_error ("Mirror not initialized.");
         return -1;
      }
-   if (Mxg_Cfg.src.type == MXG_SOURCE_RAYFILE)
+   if ((Mxg_Cfg.src.type == MXG_SOURCE_RAYFILE) || (Mxg_Cfg.src.type == MXG_SOURCE_USER))
      {
-        mxg_error ("mxg_trace_device creates its photons on the device; a RAYFILE source is "
-                   "replayed through marx_create_photons");
+        mxg_error ("mxg_trace_device creates its photons on the device; a RAYFILE or USER source "
+                   "makes its rays on the host, through marx_create_photons");
         return -1;
      }
    /* timing slots: 0 source+dither, 1 mirror, 2 grating, 3 detector,
@@ -609,10 +719,11 @@ static int batch_setup (unsigned int num_sources, const double *dirs, const unsi
         mxg_error ("mxg_trace_batch: no sources");
         return -1;
      }
-   if ((Mxg_Cfg.src.type == MXG_SOURCE_IMAGE) || (Mxg_Cfg.src.type == MXG_SOURCE_RAYFILE))
+   if ((Mxg_Cfg.src.type == MXG_SOURCE_IMAGE) || (Mxg_Cfg.src.type == MXG_SOURCE_RAYFILE)
+       || (Mxg_Cfg.src.type == MXG_SOURCE_USER))
      {
         mxg_error ("mxg_trace_batch: the sources of a batch are points, disks, Gaussians, beta models "
-                   "or lines about their directions, not images or ray files");
+                   "or lines about their directions, not images, ray files or plug-ins");
         return -1;
      }
    if (-1 == mxgd_batch_set_sources (num_sources, dirs, counts, n))

@@ -134,6 +134,10 @@ int mxg_resident_history (unsigned long *history);
 FILE *mxg_open_rayfile (const char *file, unsigned long *history);
 /* Up to num records of a RAYFILE source, times relative to the previous
  * call's last photon (s-rayfile.c:176-207). */
+/* USER sources (s-user.c): the plug-in's user_start_iteration (opening the
+ * source first if needed; 0 when it has none) and user_create_ray. */
+int mxg_user_start_iteration (void);
+int mxg_user_create_ray (double *dt, double *energy, double *cx, double *cy, double *cz);
 int mxg_rayfile_read (Marx_Source_Type *st, Marx_Photon_Attr_Type *at, unsigned int num,
                       unsigned int *n, unsigned long *history);
 extern double _Marx_TStart_MJDsecs;

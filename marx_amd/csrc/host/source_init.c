@@ -11,6 +11,7 @@
  * the mxg_source_cfg fields for what the kernels read.
  */
 #include <stdio.h>
+#include <dlfcn.h>
 #include <stdlib.h>
 #include <string.h>
 #include <math.h>
@@ -305,6 +306,105 @@ static double rotation_axis (JDMVector_Type a, JDMVector_Type b, JDMVector_Type 
    return theta;
 }
 
+/* ---- USER (s-user.c) -------------------------------------------------------------- */
+
+static void *User_Handle;
+static int (*User_Open_Source) (char **, int, double, double, double, double);
+static void (*User_Close_Source) (void);
+static int (*User_Create_Ray) (double *, double *, double *, double *, double *);
+static int (*User_Start_Iteration) (void);
+static char User_Args[1024];
+static char *User_Argv[64];
+static int User_Argc;
+static int User_Opened;
+
+static void user_source_close (void)
+{
+   if (User_Opened && (User_Close_Source != NULL))
+     (*User_Close_Source) ();
+   if (User_Handle != NULL)
+     dlclose (User_Handle);
+   User_Handle = NULL;
+   User_Open_Source = NULL;
+   User_Close_Source = NULL;
+   User_Create_Ray = NULL;
+   User_Start_Iteration = NULL;
+   User_Argc = 0;
+   User_Opened = 0;
+}
+
+/* s-user.c:229-300: link the plug-in named by UserSourceFile; its
+ * arguments are UserSourceArgs split at white space. */
+static int user_source_init (Param_File_Type *pf)
+{
+   char file[1024], *p;
+   user_source_close ();
+   if (-1 == pf_get_file (pf, "UserSourceFile", file, sizeof (file)))
+     {
+        mxg_error ("Unable to find parameter 'UserSourceFile'");
+        return -1;
+     }
+   User_Args[0] = 0;
+   if (-1 == pf_get_string (pf, "UserSourceArgs", User_Args, sizeof (User_Args)))
+     return -1;
+   mxg_message (1, "Dynamically linking to file %s\n", file);
+   if (NULL == (User_Handle = dlopen (file, RTLD_LAZY)))
+     {
+        const char *err = dlerror ();
+        mxg_error ("Error linking to %s\nReason: %s", file, err ? err : "UNKNOWN");
+        return -1;
+     }
+   *(void **) (&User_Open_Source) = dlsym (User_Handle, "user_open_source");
+   *(void **) (&User_Close_Source) = dlsym (User_Handle, "user_close_source");
+   *(void **) (&User_Create_Ray) = dlsym (User_Handle, "user_create_ray");
+   *(void **) (&User_Start_Iteration) = dlsym (User_Handle, "user_start_iteration");
+   if ((User_Open_Source == NULL) || (User_Close_Source == NULL) || (User_Create_Ray == NULL))
+     {
+        mxg_error ("Unable to get symbol '%s'", (User_Open_Source == NULL) ? "user_open_source"
+                   : (User_Close_Source == NULL) ? "user_close_source" : "user_create_ray");
+        user_source_close ();
+        return -1;
+     }
+   p = User_Args;
+   while (*p && (User_Argc < 63))
+     {
+        while ((*p == ' ') || (*p == '\t') || (*p == '\n')) p++;
+        if (*p == 0) break;
+        User_Argv[User_Argc++] = p;
+        while (*p && (*p != ' ') && (*p != '\t') && (*p != '\n')) p++;
+        if (*p) *p++ = 0;
+     }
+   User_Argv[User_Argc] = NULL;
+   return 0;
+}
+
+/* s-user.c:122-130: user_open_source (argv, argc, area, source direction) */
+static int user_source_open (void)
+{
+   const mxg_source_cfg *s = &Mxg_Cfg.src;
+   if (User_Open_Source == NULL) return -1;
+   if (User_Opened) return 0;
+   if (0 != (*User_Open_Source) (User_Argv, User_Argc, Marx_Mirror_Geometric_Area, s->p[0], s->p[1], s->p[2]))
+     {
+        mxg_error ("user_open_source failed");
+        return -1;
+     }
+   User_Opened = 1;
+   return 0;
+}
+
+int mxg_user_start_iteration (void)
+{
+   if (-1 == user_source_open ()) return -1;
+   if (User_Start_Iteration == NULL) return 0;
+   return (*User_Start_Iteration) ();
+}
+
+int mxg_user_create_ray (double *dt, double *energy, double *cx, double *cy, double *cz)
+{
+   return (*User_Create_Ray) (dt, energy, cx, cy, cz);
+}
+
 /* ---- IMAGE (s-image.c) ------------------------------------------------------------- */
 
 /* the value of keyword key in a FITS header of ncards cards, or -1 */
@@ -500,10 +600,11 @@ int mxg_source_init (Param_File_Type *pf)
    else if (!strcmp (name, "BETA")) s->type = MXG_SOURCE_BETA;
    else if (!strcmp (name, "LINE")) s->type = MXG_SOURCE_LINE;
    else if (!strcmp (name, "IMAGE")) s->type = MXG_SOURCE_IMAGE;
+   else if (!strcmp (name, "USER")) s->type = MXG_SOURCE_USER;
    else
      {
-        mxg_error ("SourceType %s not supported (POINT, DISK, GAUSS, BETA, LINE, IMAGE and RAYFILE are)",
-                   name);
+        mxg_error ("SourceType %s not supported (POINT, DISK, GAUSS, BETA, LINE, IMAGE, USER and RAYFILE "
+                   "are)", name);
         return -1;
      }
    if (s->type == MXG_SOURCE_RAYFILE)
@@ -598,6 +699,11 @@ int mxg_source_init (Param_File_Type *pf)
         s->line_alpha = rotation_axis (a, b, &c);
         s->line_normal[0] = c.x; s->line_normal[1] = c.y; s->line_normal[2] = c.z;
      }
+   else if (s->type == MXG_SOURCE_USER)
+     {
+        if (-1 == user_source_init (pf))
+          return -1;
+     }
    else if (s->type == MXG_SOURCE_IMAGE)
      {
         /* s-image.c:370-385, 262-282: the image centre is -x, rotated onto
@@ -670,6 +776,8 @@ int mxg_rayfile_read (Marx_Source_Type *st, Marx_Photon_Attr_Type *at, unsigned 
 int marx_open_source (Marx_Source_Type *st)
 {
    if (st == NULL) return -1;
+   if ((Mxg_Cfg.src.type == MXG_SOURCE_USER) && (-1 == user_source_open ()))
+     return -1;
    st->open = 1;
    return 0;
 }
@@ -678,6 +786,8 @@ int marx_close_source (Marx_Source_Type *st)
 {
    if (st == NULL) return -1;
    if (st->ray_fp != NULL) fclose (st->ray_fp);
+   if (Mxg_Cfg.src.type == MXG_SOURCE_USER)
+     user_source_close ();
    free (st);
    return 0;
 }

@@ -152,6 +152,9 @@ class Tracer:
         self.source = self.lib.marx_create_source(self.par.handle)
         if not self.source:
             raise RuntimeError("marx_create_source failed: " + _lib.last_error())
+        # marx.c opens the source after creating it (a USER plug-in's
+        # user_open_source runs here)
+        check(self.lib.marx_open_source(self.source), "marx_open_source")
         self.lib.mxg_set_uniform_times(float(uniform_times))
 
     def stage_counts(self):

@@ -101,6 +101,11 @@ int orc_subpix (const float *e, unsigned int n, float energy, float *dx, float *
 /* Stage-major passes, in the order marx/src/marx.c:240-273 calls them. */
 int orc_create_photons (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int num,
                         unsigned int *num_collected, double *exposure_time);
+/* A USER source: ray is the plug-in's user_create_ray; tstart the time
+ * origin of the exposure cut of plug-in times (_Marx_TStart_MJDsecs). */
+int orc_create_user (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int num,
+                     unsigned int *num_collected, double *exposure_time, double tstart,
+                     int (*ray) (double *, double *, double *, double *, double *));
 /* The dither applied to the photons of pt as replayed from a ray file. */
 int orc_dither_photons (const orc_ctx *c, Marx_Photon_Type *pt);
 /* The differentiable mirror (mxg_diff_forward / mxg_diff_adjoint): weights,

@@ -66,6 +66,9 @@ def load():
     lib.orc_create_batch.argtypes = [ctypes.POINTER(OrcCtx), ctypes.c_void_p, ctypes.c_uint, f64p, u32p,
                                      ctypes.c_uint, ctypes.c_double, ctypes.c_double]
     lib.orc_create_batch.restype = ctypes.c_int
+    lib.orc_create_user.argtypes = [ctypes.POINTER(OrcCtx), ctypes.c_void_p, ctypes.c_uint, u32p, f64p,
+                                    ctypes.c_double, ctypes.c_void_p]
+    lib.orc_create_user.restype = ctypes.c_int
     lib.orc_trace_batch_parallel.argtypes = [ctypes.POINTER(OrcCtx), ctypes.c_uint, f64p, u32p, ctypes.c_uint,
                                              ctypes.c_double, ctypes.c_int]
     lib.orc_trace_batch_parallel.restype = ctypes.c_long
@@ -209,6 +212,18 @@ class Oracle:
                                          num, ctypes.byref(n), ctypes.byref(exp) if exp is not None else None)
         if st != 0:
             raise RuntimeError("oracle create_photons failed")
+        return n.value
+
+    def create_user(self, photons, num, plugin, exposure_time=None, tstart=0.0):
+        """The photons of a USER source whose plug-in is the loaded CDLL
+        plugin (its user_create_ray)."""
+        n = ctypes.c_uint()
+        exp = ctypes.byref(ctypes.c_double(exposure_time)) if exposure_time is not None else None
+        fn = ctypes.cast(plugin.user_create_ray, ctypes.c_void_p)
+        st = self.lib.orc_create_user(ctypes.byref(self.ctx), ctypes.cast(photons.ptr, ctypes.c_void_p),
+                                      num, ctypes.byref(n), exp, tstart, fn)
+        if st != 0:
+            raise RuntimeError("oracle create_user failed")
         return n.value
 
     def create_batch(self, photons, dirs, counts, tag_start=0, start_time=0.0, exposure=1.0e4):

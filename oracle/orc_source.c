@@ -233,6 +233,81 @@ int orc_create_photons (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int num
    return 0;
 }
 
+/* A USER source (source.c:268-384 over s-user.c:140-205): energies and
+ * Poisson times from the photons' streams for all num photons, then the
+ * plug-in's rays in creation order until it returns -1; its time steps
+ * (when it gives them) replace the times, its energy the drawn one unless
+ * negative; then the exposure cut of the time model in use (tstart +
+ * exposure for the plug-in's times, exposure for Poisson ones), tags,
+ * dither and the energy sort. */
+int orc_create_user (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int num,
+                     unsigned int *num_collected, double *exposure_time, double tstart,
+                     int (*ray) (double *, double *, double *, double *, double *))
+{
+   const mxg_source_cfg *s = &c->cfg->src;
+   Marx_Photon_Attr_Type *at = pt->attributes;
+   unsigned int n, i;
+   unsigned long long S = 0;
+   double t = -1.0, user_time = 0.0;
+
+   *num_collected = 0;
+   if (num > pt->max_n_photons) return -1;
+   pt->source_distance = s->distance;
+   pt->history = 0;
+   pt->start_time += pt->total_time;
+   memset (at, 0, num * sizeof (Marx_Photon_Attr_Type));
+   for (i = 0; i < num; i++)
+     {
+        orc_stream st;
+        at[i].tag = pt->tag_start + i;
+        orc_stream_init (&st, c->seed, at[i].tag, MXG_STAGE_SOURCE, c->source_id);
+        at[i].energy = draw_energy (c, s, &st);
+        if (s->time_mode == MXG_TIME_POISSON)
+          {
+             double e = orc_expn (&st);
+             S += (unsigned long long) (e * 4294967296.0 + 0.5);
+             at[i].arrival_time = s->mean_time * ((double) S * (1.0 / 4294967296.0));
+          }
+        else
+          at[i].arrival_time = s->uniform_tmax * orc_uniform (&st);
+     }
+   for (i = 0; i < num; i++)
+     {
+        double energy = at[i].energy, drawn = energy, px, py, pz;
+        if (-1 == (*ray) (&t, &energy, &px, &py, &pz))
+          break;
+        at[i].flags = 0;
+        at[i].p = orc_v (px, py, pz);
+        if (t >= 0.0)
+          {
+             user_time += t;
+             at[i].arrival_time = user_time;
+          }
+        if (energy < 0.0) energy = drawn;
+        at[i].energy = energy;
+     }
+   n = i;
+   if (exposure_time != NULL)
+     for (i = 0; i < n; i++)
+       if ((t >= 0.0) ? (at[i].arrival_time >= tstart + *exposure_time)
+           : (*exposure_time <= at[i].arrival_time))
+         {
+            n = i + 1;
+            break;
+         }
+   pt->history = MARX_ENERGY_OK | MARX_TIME_OK | MARX_X_VECTOR_OK | MARX_P_VECTOR_OK;
+   if (n) pt->history |= MARX_TAG_OK;
+   pt->n_photons = n;
+   pt->tag_start += n;
+   if ((n > 0) && (s->dither_mode == MXG_DITHER_INTERNAL))
+     for (i = 0; i < n; i++)
+       dither_photon (c, s, at + i, pt->start_time);
+   if (-1 == orc_sort_photons (pt)) return -1;
+   pt->total_time = n ? at[n - 1].arrival_time : 0.0;
+   *num_collected = n;
+   return 0;
+}
+
 /* Batched point sources: the photons mxg_trace_batch creates.  Photon i of
  * the batch (sources in order) has tag tag_start + i, the direction of its
  * source, an energy from the configured spectrum, an arrival time uniform

@@ -153,3 +153,82 @@ def test_image_photons_land_in_their_pixels(tmp_path):
     # uniform within the pixel
     fx = x - col + 0.5
     assert stats.kstest(fx, "uniform").pvalue > 1e-3
+
+
+# ---- USER (s-user.c) --------------------------------------------------------------------
+
+def test_user_source_plugin_is_linked_and_opened(tmp_path):
+    """marx_create_source links the UserSourceFile plug-in and marx_open_source
+    calls its user_open_source with UserSourceArgs split at white space, the
+    mirror's geometric area and the source direction (s-user.c:122-130,
+    229-300); marx_close_source calls user_close_source."""
+    import ctypes
+    from tests.user_plugin import build
+    path, plug = build(str(tmp_path))
+    t = Tracer(seed=3, GratingType="NONE", DetectorType="NONE", SourceType="USER", SpectrumType="FLAT",
+               MinEnergy=1.0, MaxEnergy=2.0, DitherModel="NONE", UserSourceFile=path,
+               UserSourceArgs="  times   stop=9.5 ", **OFF_AXIS)
+    t.init_source()
+    from cfgview import ConfigView
+    src = ConfigView(t).cfg.src
+    assert ctypes.c_int.in_dll(plug, "Open_Count").value == 1
+    assert ctypes.c_int.in_dll(plug, "Open_Argc").value == 2
+    assert ctypes.c_char_p(ctypes.addressof(ctypes.c_char.in_dll(plug, "Open_Argv0"))).value == b"times"
+    assert ctypes.c_char_p(ctypes.addressof(ctypes.c_char.in_dll(plug, "Open_Argv1"))).value == b"stop=9.5"
+    assert ctypes.c_double.in_dll(plug, "Open_Area").value == t.geometric_area
+    d = (ctypes.c_double * 3).in_dll(plug, "Open_Dir")
+    assert list(d) == list(src.p)
+    t.lib.marx_close_source(t.source)
+    t.source = None
+    assert ctypes.c_int.in_dll(plug, "Close_Count").value == 1
+
+
+def test_user_source_missing_symbols_fail(tmp_path):
+    p = tmp_path / "empty.c"
+    p.write_text("int user_open_source (void) { return 0; }\n")
+    import subprocess
+    so = str(tmp_path / "empty.so")
+    subprocess.run(["gcc", "-shared", "-fPIC", "-o", so, str(p)], check=True)
+    t = Tracer(seed=3, GratingType="NONE", DetectorType="NONE", SourceType="USER", SpectrumType="FLAT",
+               MinEnergy=1.0, MaxEnergy=2.0, DitherModel="NONE", UserSourceFile=so, **OFF_AXIS)
+    with pytest.raises(RuntimeError, match="marx_create_source failed"):
+        t.init_source()
+
+
+def test_user_source_oracle_follows_the_plugin(tmp_path):
+    """The oracle's USER photons: directions and time steps exactly as the
+    plug-in makes them from each drawn energy, its energies where it gives
+    one, the source ended at its first refusal, and the plug-in's times cut
+    at the exposure."""
+    from tests.user_plugin import build
+    path, plug = build(str(tmp_path))
+    t = Tracer(seed=3, GratingType="NONE", DetectorType="NONE", SourceType="USER", SpectrumType="FLAT",
+               MinEnergy=1.0, MaxEnergy=10.0, DitherModel="NONE", UserSourceFile=path,
+               UserSourceArgs="times stop=9.999", **OFF_AXIS)
+    t.init_source()
+    o = Oracle(t)
+    n = 20000
+    p = Photons(n)
+    m = o.create_user(p, n, plug)
+    a = p.active
+    assert 0 < m < n
+    order = np.argsort(a["tag"])
+    a = a[order]
+    assert np.array_equal(a["tag"], np.arange(m))
+    # energies above 6 were lowered by 1 by the plug-in; none reached the stop
+    assert a["energy"].max() < 9.999 and m > 1000
+    dt = np.diff(np.r_[0.0, a["arrival_time"]])
+    assert np.all(dt > 0.25)
+    # re-derive each ray's drawn energy from its time step, then its direction
+    e = (dt - 0.25) / 0.05
+    assert np.allclose(np.where(e > 6.0, e - 1.0, e), a["energy"], atol=1e-9)
+    from cfgview import ConfigView
+    d0 = np.array(ConfigView(t).cfg.src.p)
+    v = np.stack([np.full(m, d0[0]), d0[1] + 2e-4 * np.sin(7 * e), d0[2] + 2e-4 * np.cos(5 * e)], 1)
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    assert np.allclose(a["p"], v, atol=1e-12)
+    # the exposure cut of plug-in times: the first photon at or past it is the last
+    q = Photons(n)
+    k = o.create_user(q, n, plug, exposure_time=50.0)
+    assert q.active["arrival_time"].max() >= 50.0
+    assert k == np.searchsorted(a["arrival_time"], 50.0) + 1

@@ -127,3 +127,32 @@ def test_full_chain_aos_and_device(gpu, tmp_path, workload):
     compare(pd.active, po.active, max_flips=2, label=f"chain device {workload}")
     assert ndet == int(((po.active["flags"] & 0xFF) == 0).sum())
     assert all(m >= 0 for m in ms)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("args,dither,exposure", [("times stop=9.999", "INTERNAL", 50.0), ("", "NONE", None),
+                                                  ("stop=9.999", "INTERNAL", 300.0)])
+def test_user_source_parity(gpu, tmp_path, args, dither, exposure):
+    """A USER plug-in (marx_create_photons over s-user.c): energies drawn on
+    the device, rays from the plug-in on the host, then cut, tags, dither
+    and sort, against the oracle calling the same plug-in."""
+    from tests.user_plugin import build
+    path, plug = build(str(tmp_path))
+    t = Tracer(seed=5, GratingType="NONE", DetectorType="NONE", SourceType="USER", SpectrumType="FLAT",
+               MinEnergy=1.0, MaxEnergy=10.0, DitherModel=dither, UserSourceFile=path, UserSourceArgs=args,
+               SourceRA=ON_AXIS["SourceRA"] + 0.01, SourceDEC=ON_AXIS["SourceDEC"])
+    t.init_source(SourceFlux=20.0)
+    n = 20000
+    pg, po = Photons(n), Photons(n)
+    ng = t.create_photons(pg, n, exposure)
+    no = Oracle(t).create_user(po, n, plug, exposure_time=exposure)
+    assert ng == no > 50
+    assert np.array_equal(pg.sorted_index(), po.sorted_index())
+    a, b = pg.active, po.active
+    compare(a, b, max_flips=0, check_pixels=False, label=f"user source {args}")
+    assert np.array_equal(a["arrival_time"], b["arrival_time"])
+    assert np.array_equal(a["dither"], b["dither"])
+    assert pg.pt.total_time == po.pt.total_time
+    # a device-created trace refuses a plug-in source
+    with pytest.raises(RuntimeError, match="USER"):
+        t.trace_device(1000)
