@@ -174,6 +174,13 @@ typedef struct
    mxg_rect drake[MXG_MAX_DRAKE];
    unsigned int drake_c_n, drake_c_e_off, drake_c_beta_off, drake_c_delta_off;
    unsigned int drake_cr_n, drake_cr_e_off, drake_cr_beta_off, drake_cr_delta_off;
+   /* ACIS FEF (acis_fef.c): fef_on when a FEF file was read.  Double arena
+    * at fef_map_off: by chip id (10) and 32 x 32 regions of 32 pixels
+    * ([id][x/32][y/32]), the FEF index or -1; at fef_desc_off, 5 per FEF:
+    * energies, Gaussians, and the float-arena offsets of its energies, its
+    * channels and its Gaussians ([energy][gaussian][amp, centre, sigma]). */
+   int fef_on, fef_max_gaussians;
+   unsigned int fef_map_off, fef_desc_off;
 }
 mxg_detector_cfg;
 

@@ -276,6 +276,186 @@ static int read_opt_consts (Param_File_Type *pf, const char *pname, unsigned int
 
 /* ---- ACIS ------------------------------------------------------------------------ */
 
+/* ---- ACIS FEF (acis_fef.c:330-899) -------------------------------------------------- */
+
+#define FEF_REGION 32
+#define FEF_NREG (1024 / FEF_REGION)
+#define FEF_FWHM_TO_SIGMA 0.4246609001440095
+
+static int match_function (mxg_fits_table *t, void *client)
+{
+   (void) client;
+   return mxg_fits_keyword_is (t, "EXTNAME", "FUNCTION");
+}
+
+/* One region's rows [r0, r1) of the FUNCTION table into the arenas; returns
+ * its index in the descriptor list or -1. */
+static int fef_region (mxg_fits_table *t, unsigned int r0, unsigned int r1, unsigned int ng,
+                       double *desc, unsigned int *ndesc)
+{
+   unsigned int ne = r1 - r0, i, k, eo, co, go;
+   float *e, *c, *g;
+   if ((ne < 2) || (*ndesc >= 10 * FEF_NREG * FEF_NREG))
+     {
+        mxg_error ("FEF region of rows %u-%u: %s", r0 + 1, r1, (ne < 2) ? "fewer than 2 energies" : "too many regions");
+        return -1;
+     }
+   eo = mxg_farena_reserve (ne);
+   co = mxg_farena_reserve (ne);
+   go = mxg_farena_reserve (3 * ne * ng);
+   if ((eo == (unsigned int) -1) || (co == (unsigned int) -1) || (go == (unsigned int) -1)) return -1;
+   e = mxg_farena_ptr (eo);
+   c = mxg_farena_ptr (co);
+   g = mxg_farena_ptr (go);
+   for (i = 0; i < ne; i++)
+     {
+        if ((-1 == mxg_fits_get_float (t, "ENERGY", r0 + i, 0, e + i))
+            || (-1 == mxg_fits_get_float (t, "CHANNEL", r0 + i, 0, c + i)))
+          return -1;
+        for (k = 0; k < ng; k++)
+          {
+             char name[32];
+             float fwhm, pos, ampl;
+             snprintf (name, sizeof (name), "G%u_FWHM", k + 1);
+             if (-1 == mxg_fits_get_float (t, name, r0 + i, 0, &fwhm)) return -1;
+             snprintf (name, sizeof (name), "G%u_POS", k + 1);
+             if (-1 == mxg_fits_get_float (t, name, r0 + i, 0, &pos)) return -1;
+             snprintf (name, sizeof (name), "G%u_AMPL", k + 1);
+             if (-1 == mxg_fits_get_float (t, name, r0 + i, 0, &ampl)) return -1;
+             g[3 * (i * ng + k) + 0] = ampl;
+             g[3 * (i * ng + k) + 1] = pos;
+             g[3 * (i * ng + k) + 2] = (float) (FEF_FWHM_TO_SIGMA * fwhm);
+          }
+     }
+   /* energies and channels must increase (acis_fef.c:299-312) */
+   for (i = 1; i < ne; i++)
+     if ((e[i] <= e[i - 1]) || (c[i] <= c[i - 1]))
+       {
+          mxg_error ("Fef Map does not have monotonically increasing %s", (e[i] <= e[i - 1]) ? "energies" : "PHAs");
+          return -1;
+       }
+   desc[5 * *ndesc + 0] = ne;
+   desc[5 * *ndesc + 1] = ng;
+   desc[5 * *ndesc + 2] = eo;
+   desc[5 * *ndesc + 3] = co;
+   desc[5 * *ndesc + 4] = go;
+   return (int) (*ndesc)++;
+}
+
+/* The FEF file of the ACIS chips min_id..max_id: rows grouped into regions by
+ * runs of REGNUM, each region mapped onto its 32-pixel cells
+ * (acis_fef.c:194-298, 706-874).  The file is ACISFEFFile when set, else the
+ * CALDB ACISFEF file; when neither exists the FEF is off and PHA and PI stay 0. */
+static int read_acis_fef (Param_File_Type *pf, int min_id, int max_id)
+{
+   mxg_detector_cfg *d = &Mxg_Cfg.det;
+   char buf[1024], *file = NULL;
+   mxg_fits_table *t;
+   unsigned int nrows, ng, r0, i, ndesc = 0, id;
+   double *map, *desc;
+   FILE *fp;
+
+   d->fef_on = 0;
+   buf[0] = 0;
+   if ((-1 == pf_get_string (pf, "ACISFEFFile", buf, sizeof (buf))) || (buf[0] == 0))
+     file = mxg_caldb_file ("ACISFEF");
+   else
+     file = strdup (buf);
+   if ((file == NULL) || (NULL == (fp = fopen (file, "rb"))))
+     {
+        mxg_message (0, "ACIS FEF file not available: PHA and PI are not computed\n");
+        free (file);
+        return 0;
+     }
+   fclose (fp);
+   if (NULL == (t = mxg_fits_find_table (file, match_function, NULL)))
+     {
+        mxg_error ("Unable to open ACIS FEF file %s", file);
+        free (file);
+        return -1;
+     }
+   mxg_message (2, "\t%s\n", file);
+   free (file);
+   for (ng = 0; ng < 64; ng++)
+     {
+        char name[32];
+        snprintf (name, sizeof (name), "G%u_AMPL", ng + 1);
+        if (-1 == mxg_fits_column (t, name, NULL, NULL)) break;
+     }
+   nrows = mxg_fits_num_rows (t);
+   d->fef_map_off = mxg_darena_reserve (10 * FEF_NREG * FEF_NREG);
+   d->fef_desc_off = mxg_darena_reserve (5 * 10 * FEF_NREG * FEF_NREG);
+   if ((ng == 0) || (ng > 16) || (d->fef_map_off == (unsigned int) -1) || (d->fef_desc_off == (unsigned int) -1))
+     {
+        mxg_error ("ACIS FEF file with %u Gaussians (1 to 16 supported)", ng);
+        mxg_fits_close (t);
+        return -1;
+     }
+   map = mxg_darena_ptr (d->fef_map_off);
+   for (i = 0; i < 10 * FEF_NREG * FEF_NREG; i++) map[i] = -1.0;
+   r0 = 0;
+   while (r0 < nrows)
+     {
+        double v[6];
+        unsigned int r1 = r0 + 1;
+        int k, x, y, xmin, xmax, ymin, ymax, ccd, regnum;
+        if ((-1 == mxg_fits_get_double (t, "CCD_ID", r0, 0, v + 0))
+            || (-1 == mxg_fits_get_double (t, "REGNUM", r0, 0, v + 1)))
+          goto fail;
+        ccd = (int) v[0];
+        regnum = (int) v[1];
+        while (r1 < nrows)
+          {
+             double w[2];
+             if ((-1 == mxg_fits_get_double (t, "CCD_ID", r1, 0, w + 0))
+                 || (-1 == mxg_fits_get_double (t, "REGNUM", r1, 0, w + 1)))
+               goto fail;
+             if (((int) w[1] != regnum) || ((int) w[0] != ccd)) break;
+             r1++;
+          }
+        if ((ccd < min_id) || (ccd > max_id))
+          {
+             r0 = r1;
+             continue;
+          }
+        if ((-1 == mxg_fits_get_double (t, "CHIPX_LO", r0, 0, v + 2))
+            || (-1 == mxg_fits_get_double (t, "CHIPX_HI", r0, 0, v + 3))
+            || (-1 == mxg_fits_get_double (t, "CHIPY_LO", r0, 0, v + 4))
+            || (-1 == mxg_fits_get_double (t, "CHIPY_HI", r0, 0, v + 5)))
+          goto fail;
+        xmin = (int) v[2] - 1; xmax = (int) v[3];
+        ymin = (int) v[4] - 1; ymax = (int) v[5];
+        if ((xmin < 0) || (ymin < 0) || (xmax > 1024) || (ymax > 1024)
+            || (xmin % FEF_REGION) || (xmax % FEF_REGION) || (ymin % FEF_REGION) || (ymax % FEF_REGION))
+          {
+             mxg_error ("FEF region CHIPX/Y_LO/HI values must be multiples of %d within 1-1024", FEF_REGION);
+             goto fail;
+          }
+        desc = mxg_darena_ptr (d->fef_desc_off);
+        if (-1 == (k = fef_region (t, r0, r1, ng, desc, &ndesc)))
+          goto fail;
+        map = mxg_darena_ptr (d->fef_map_off);
+        for (x = xmin / FEF_REGION; x < xmax / FEF_REGION; x++)
+          for (y = ymin / FEF_REGION; y < ymax / FEF_REGION; y++)
+            map[(ccd * FEF_NREG + x) * FEF_NREG + y] = k;
+        r0 = r1;
+     }
+   mxg_fits_close (t);
+   for (id = (unsigned int) min_id; id <= (unsigned int) max_id; id++)
+     for (i = 0; i < FEF_NREG * FEF_NREG; i++)
+       if (map[id * FEF_NREG * FEF_NREG + i] < 0)
+         {
+            mxg_message (0, "***Warning: FEF for CCDID=%u contains holes\n", id);
+            break;
+          }
+   d->fef_on = 1;
+   d->fef_max_gaussians = (int) ng;
+   return 0;
+fail:
+   mxg_fits_close (t);
+   return -1;
+}
+
 static int acis_init (Param_File_Type *pf, int is_s)
 {
    mxg_detector_cfg *d = &Mxg_Cfg.det;
@@ -315,7 +495,7 @@ static int acis_init (Param_File_Type *pf, int is_s)
           if (-1 == read_acis_qe (g))
             return -1;
      }
-   return 0;
+   return read_acis_fef (pf, is_s ? 4 : 0, is_s ? 9 : 3);
 }
 
 /* ---- HRC blur (hrcblur.c:115-236) ------------------------------------------------ */

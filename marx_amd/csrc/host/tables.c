@@ -105,6 +105,7 @@ char *mxg_caldb_file (const char *object)
           {"HETGEFF", "caldb/hetgD1996-11-01greffpr001N0007.fits"},
           {"LETGEFF", "caldb/letgD1996-11-01greffpr001N0007.fits"},
           {"ACISSUBPIX", "caldb/acisD1999-07-22subpixN0001_marx.fits"},
+          {"ACISFEF", "caldb/acisfef.fits"},
           {NULL, NULL}
      };
    unsigned int i;

@@ -143,6 +143,162 @@ __device__ __forceinline__ int drake_hit (const mxg_detector_cfg &d, vec3 &x, ve
    return -1;
 }
 
+// ---- ACIS FEF: PHA and PI (acis_fef.c:380-503, 509-579, 910-1079) ---------------
+
+constexpr int kFefMaxGauss = 16;
+constexpr int kFefRetries = 10000;   // the reference's outer sampling loop has no bound
+
+struct FefGauss
+{
+   float amp[kFefMaxGauss], center[kFefMaxGauss], sigma[kFefMaxGauss], cum[kFefMaxGauss];
+   bool tail[kFefMaxGauss];
+};
+
+// the reference's amplitudes carry the 1 / sqrt(2 pi sigma^2) (acis_fef.c:372-392)
+__device__ __forceinline__ double fef_integral (double xmin, double xmax, float amp, float center, float sigma)
+{
+   const double s = sigma * 1.4142135623730951;
+   if (s == 0.0) return 0.0;
+   const double x0 = center;
+   return 0.5 * amp * (erf ((xmax - x0) / s) - erf ((xmin - x0) / s)) * (2.5066282746310002 * sigma);
+}
+
+// one PHA from the positive-amplitude Gaussians; false when none is found
+__device__ bool fef_pos_pha (const FefGauss &G, int ng, Stream &st, double &pha_out)
+{
+   for (int it = 0; it < kFefRetries; it++)
+     {
+        const double r = uniform (st);
+        for (int k = 0; k < ng; k++)
+          {
+             if (G.cum[k] <= r) continue;
+             double pha;
+             if (!G.tail[k])
+               {
+                  int count = 0;
+                  do
+                    {
+                       pha = G.center[k] + G.sigma[k] * gaussian (st);
+                       count++;
+                    }
+                  while ((pha < 0) && (count < 100));
+               }
+             else
+               {
+                  // Gaussian tail beyond 0 (the GSL algorithm, as acis_fef.c:444-458)
+                  const double s = (0 - G.center[k]) / G.sigma[k];
+                  double u, x;
+                  int guard = 0;
+                  do
+                    {
+                       double v;
+                       u = uniform (st);
+                       do v = uniform (st);
+                       while (v == 0.0);
+                       x = sqrt (s * s - 2 * log (v));
+                    }
+                  while ((x * u > s) && (++guard < kFefRetries));
+                  pha = G.center[k] + x * G.sigma[k];
+               }
+             if (pha < 0) break;
+             pha_out = pha;
+             return true;
+          }
+     }
+   return false;
+}
+
+// PHA and PI of a detected photon at chip pixel (x, y); false marks it undetected
+__device__ __noinline__ bool apply_fef (const mxg_detector_cfg &d, const float *__restrict__ fa,
+                           const double *__restrict__ da, int ccd, float x, float y, double energy,
+                           Stream &st, short &pha_out, float &pi_out)
+{
+   if (x < 0) { if (d.extend == 0) return false; x = 0; }
+   else if (x >= 1024) { if (d.extend == 0) return false; x = 1023; }
+   if (y < 0) { if (d.extend == 0) return false; y = 0; }
+   else if (y >= 1024) { if (d.extend == 0) return false; y = 1023; }
+   const unsigned int i = (unsigned int) (x / 32), j = (unsigned int) (y / 32);
+   if ((i >= 32) || (j >= 32) || (ccd < 0) || (ccd >= 10)) return false;
+   const double kf = da[d.fef_map_off + (ccd * 32 + i) * 32 + j];
+   if (kf < 0) return false;
+   const double *desc = da + d.fef_desc_off + 5 * (unsigned int) kf;
+   const unsigned int ne = (unsigned int) desc[0];
+   const int ng = (int) desc[1];
+   const float *e = fa + (unsigned int) desc[2];
+   const float *ch = fa + (unsigned int) desc[3];
+   unsigned int n = search_f ((float) energy, e, ne);
+   if (n == 0) n++;
+   if (n == ne) n--;
+   const double t = (energy - e[n - 1]) / (e[n] - e[n - 1]);
+   const float *g0 = fa + (unsigned int) desc[4] + 3 * (n - 1) * ng, *g1 = g0 + 3 * ng;
+   FefGauss G;
+   double total_pos = 0.0, total_neg = 0.0;
+   for (int k = 0; k < ng; k++)
+     {
+        const float a0 = g0[3 * k], c0 = g0[3 * k + 1], s0 = g0[3 * k + 2];
+        const float a1 = g1[3 * k], c1 = g1[3 * k + 1], s1 = g1[3 * k + 2];
+        G.center[k] = (float) (c0 + t * (c1 - c0));
+        double v = s0 + t * (s1 - s0);
+        if (v <= 0.0)
+          {
+             G.sigma[k] = 0.0f;
+             G.amp[k] = 0.0f;
+          }
+        else
+          {
+             G.sigma[k] = (float) v;
+             v = a0 + t * (a1 - a0);
+             if ((v < 0.0) && ((a1 > 0.0f) || (a0 > 0.0f))) v = 0.0;
+             G.amp[k] = (float) v;
+          }
+        // normalize_gaussians
+        const double area1 = fef_integral (0, 1e37, G.amp[k], G.center[k], G.sigma[k]);
+        const double area2 = fef_integral (-1e37, 0, G.amp[k], G.center[k], G.sigma[k]);
+        G.tail[k] = (area2 > area1) && (area1 / area2 < 0.1);
+        if (area1 >= 0) total_pos += area1;
+        else total_neg -= area1;
+        G.cum[k] = (float) total_pos;
+     }
+   if (total_pos <= total_neg) return false;
+   if (total_pos > 0)
+     for (int k = 0; k < ng; k++) G.cum[k] = (float) (G.cum[k] / total_pos);
+   double pha;
+   if (total_neg == 0.0)
+     {
+        if (!fef_pos_pha (G, ng, st, pha)) return false;
+     }
+   else
+     {
+        // rejection against the full sum, positive parts as the envelope
+        bool ok = false;
+        for (int count = 0; (count < 100) && !ok; count++)
+          {
+             if (!fef_pos_pha (G, ng, st, pha)) return false;
+             double pos_sum = 0.0, sum = 0.0;
+             for (int k = 0; k < ng; k++)
+               {
+                  double ds = 0.0;
+                  if (G.sigma[k] != 0.0f)
+                    {
+                       const double z = (pha - G.center[k]) / G.sigma[k];
+                       ds = G.amp[k] * exp (-0.5 * z * z);
+                    }
+                  sum += ds;
+                  if (ds > 0) pos_sum += ds;
+               }
+             ok = (uniform (st) * pos_sum < sum);
+          }
+        if (!ok) return false;
+     }
+   const short ipha = (short) pha;
+   const double frac = ipha - uniform (st);
+   const float pi = interp_f ((float) frac, ch, e, ne);
+   if (pi < 0) return false;
+   pha_out = ipha;
+   pi_out = pi;
+   return true;
+}
+
 __global__ __launch_bounds__ (kBlock)
 void k_detect (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
                const double *__restrict__ da, Soa s, ListIO io, double tstart,
@@ -154,8 +310,6 @@ void k_detect (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
    __shared__ uint32_t seg_wc[2 * (kBlock / 64)];
    const BlockRange range = block_range (*io.n_in, blockIdx.x, gridDim.x);
    SegWriter survivors (io, range, seg_wc);
-   (void) da;
-
    for (uint32_t base = range.lo; base < range.hi; base += kBlock)
      {
         uint32_t i = base + threadIdx.x;
@@ -235,6 +389,23 @@ void k_detect (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
                                    qe = interp_f ((float) energy, fa + g.qe_e_off, fa + g.qe_off, g.qe_n);
                                  if (r >= qe * 1.0 * 1.0)
                                    {
+                                      flags |= PHOTON_UNDETECTED;
+                                      detected = false;
+                                   }
+                              }
+                            if (detected && d.fef_on)
+                              {
+                                 short pha = -1;
+                                 float pi = 0.0f;
+                                 if (apply_fef (d, fa, da, g.id, ypix, zpix, energy, st, pha, pi))
+                                   {
+                                      s.pha[id] = pha;
+                                      s.pi[id] = pi;
+                                   }
+                                 else
+                                   {
+                                      s.pha[id] = -1;
+                                      s.pi[id] = 0.0f;
                                       flags |= PHOTON_UNDETECTED;
                                       detected = false;
                                    }

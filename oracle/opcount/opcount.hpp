@@ -18,7 +18,7 @@
 #endif
 
 enum { OC_ADD, OC_MUL, OC_DIV, OC_SQRT, OC_SIN, OC_COS, OC_TAN, OC_ATAN, OC_ATAN2, OC_ASIN,
-       OC_ACOS, OC_EXP, OC_EXPM1, OC_LOG, OC_LOG1P, OC_POW, OC_FMOD, OC_N };
+       OC_ACOS, OC_EXP, OC_EXPM1, OC_LOG, OC_LOG1P, OC_POW, OC_FMOD, OC_ERF, OC_N };
 enum { OC_PHASES = 8 };
 extern "C" unsigned long long Orc_Opcount[OC_PHASES][OC_N];
 extern "C" unsigned long long Orc_Opcount_Marks[OC_PHASES];
@@ -81,6 +81,7 @@ OC_F1 (exp, OC_EXP)
 OC_F1 (expm1, OC_EXPM1)
 OC_F1 (log, OC_LOG)
 OC_F1 (log1p, OC_LOG1P)
+OC_F1 (erf, OC_ERF)
 #undef OC_F1
 inline orc_fp fabs (orc_fp a) { return orc_fp (::fabs (a.v)); }
 inline orc_fp floor (orc_fp a) { return orc_fp (::floor (a.v)); }

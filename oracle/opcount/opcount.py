@@ -30,7 +30,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 NAMES = ["add", "mul", "div", "sqrt", "sin", "cos", "tan", "atan", "atan2", "asin", "acos",
-         "exp", "expm1", "log", "log1p", "pow", "fmod"]
+         "exp", "expm1", "log", "log1p", "pow", "fmod", "erf"]
 
 
 def load_opcount():

@@ -122,6 +122,169 @@ static const mxg_chip *intersect_with_detector (const mxg_detector_cfg *d, orc_v
 
 /* ---- ACIS ------------------------------------------------------------------------ */
 
+/* ---- ACIS FEF (acis_fef.c:380-503, 509-579, 910-1079) ---------------------------- */
+
+typedef struct
+{
+   float amp, center, sigma, cum_area;
+   int use_tail_dist;
+}
+fef_gauss;
+
+static double fef_integral (double xmin, double xmax, const fef_gauss *g)
+{
+   double sigma, x0;
+   if (0.0 == (sigma = g->sigma * 1.4142135623730951)) return 0.0;
+   x0 = g->center;
+   return 0.5 * g->amp * (erf ((xmax - x0) / sigma) - erf ((xmin - x0) / sigma)) * (2.5066282746310002 * g->sigma);
+}
+
+static double fef_evaluate (double x, const fef_gauss *g)
+{
+   double sigma = g->sigma;
+   if (sigma == 0.0) return 0.0;
+   x = (x - g->center) / sigma;
+   return g->amp * exp (-0.5 * x * x);
+}
+
+/* compute_pha_with_pos_amps, its unbounded outer loop bounded */
+static int fef_pos_pha (const fef_gauss *gs, unsigned int ng, orc_stream *st, double *phap)
+{
+   unsigned int it;
+   for (it = 0; it < 10000; it++)
+     {
+        double r = orc_uniform (st);
+        const fef_gauss *g = gs, *gmax = gs + ng;
+        while (g < gmax)
+          {
+             double pha;
+             if (g->cum_area <= r)
+               {
+                  g++;
+                  continue;
+               }
+             if (g->use_tail_dist == 0)
+               {
+                  unsigned int count = 0;
+                  do
+                    {
+                       pha = g->center + g->sigma * orc_gaussian (st);
+                       count++;
+                    }
+                  while ((pha < 0) && (count < 100));
+               }
+             else
+               {
+                  double u, v, x;
+                  double s = (0 - g->center) / g->sigma;
+                  unsigned int guard = 0;
+                  do
+                    {
+                       u = orc_uniform (st);
+                       do v = orc_uniform (st);
+                       while (v == 0.0);
+                       x = sqrt (s * s - 2 * log (v));
+                    }
+                  while ((x * u > s) && (++guard < 10000));
+                  pha = g->center + x * g->sigma;
+               }
+             if (pha < 0) break;
+             *phap = pha;
+             return 0;
+          }
+     }
+   return -1;
+}
+
+/* marx_apply_acis_rmf: 0 with *phap and *pip, or -1 (the photon is undetected) */
+static int apply_fef (const orc_ctx *c, const mxg_detector_cfg *d, int ccd, float x, float y, double energy,
+                      orc_stream *st, short *phap, float *pip)
+{
+   fef_gauss gs[16];
+   const double *desc;
+   const float *e, *ch, *g0, *g1;
+   unsigned int i, j, n, ne, ng, k;
+   double t, kf, total_pos = 0.0, total_neg = 0.0, pha;
+
+   if (x < 0) { if (d->extend == 0) return -1; x = 0; }
+   else if (x >= 1024) { if (d->extend == 0) return -1; x = 1023; }
+   if (y < 0) { if (d->extend == 0) return -1; y = 0; }
+   else if (y >= 1024) { if (d->extend == 0) return -1; y = 1023; }
+   i = (unsigned int) (x / 32);
+   j = (unsigned int) (y / 32);
+   if ((i >= 32) || (j >= 32) || (ccd < 0) || (ccd >= 10)) return -1;
+   kf = c->da[d->fef_map_off + (ccd * 32 + i) * 32 + j];
+   if (kf < 0) return -1;
+   desc = c->da + d->fef_desc_off + 5 * (unsigned int) kf;
+   ne = (unsigned int) desc[0];
+   ng = (unsigned int) desc[1];
+   e = c->fa + (unsigned int) desc[2];
+   ch = c->fa + (unsigned int) desc[3];
+   n = orc_binary_search_f ((float) energy, e, ne);
+   if (n == 0) n++;
+   if (n == ne) n--;
+   t = (energy - e[n - 1]) / (e[n] - e[n - 1]);
+   g0 = c->fa + (unsigned int) desc[4] + 3 * (n - 1) * ng;
+   g1 = g0 + 3 * ng;
+   for (k = 0; k < ng; k++)
+     {
+        double v, area1, area2;
+        fef_gauss *g = gs + k;
+        g->center = (float) (g0[3 * k + 1] + t * (g1[3 * k + 1] - g0[3 * k + 1]));
+        v = g0[3 * k + 2] + t * (g1[3 * k + 2] - g0[3 * k + 2]);
+        if (v <= 0.0)
+          {
+             g->sigma = 0.0f;
+             g->amp = 0.0f;
+          }
+        else
+          {
+             g->sigma = (float) v;
+             v = g0[3 * k] + t * (g1[3 * k] - g0[3 * k]);
+             if ((v < 0.0) && ((g1[3 * k] > 0.0f) || (g0[3 * k] > 0.0f))) v = 0.0;
+             g->amp = (float) v;
+          }
+        /* normalize_gaussians */
+        area1 = fef_integral (0, 1e37, g);
+        area2 = fef_integral (-1e37, 0, g);
+        g->use_tail_dist = (area2 > area1) && (area1 / area2 < 0.1);
+        if (area1 >= 0) total_pos += area1;
+        else total_neg -= area1;
+        g->cum_area = (float) total_pos;
+     }
+   if (total_pos <= total_neg) return -1;
+   if (total_pos > 0)
+     for (k = 0; k < ng; k++) gs[k].cum_area = (float) (gs[k].cum_area / total_pos);
+   if (total_neg == 0.0)
+     {
+        if (-1 == fef_pos_pha (gs, ng, st, &pha)) return -1;
+     }
+   else
+     {
+        /* compute_pha_with_neg_amps */
+        int count = 0, ok = 0;
+        while ((count < 100) && !ok)
+          {
+             double pos_sum = 0.0, sum = 0.0;
+             if (-1 == fef_pos_pha (gs, ng, st, &pha)) return -1;
+             for (k = 0; k < ng; k++)
+               {
+                  double ds = fef_evaluate (pha, gs + k);
+                  sum += ds;
+                  if (ds > 0) pos_sum += ds;
+               }
+             ok = (orc_uniform (st) * pos_sum < sum);
+             count++;
+          }
+        if (!ok) return -1;
+     }
+   *phap = (short) pha;
+   pha = *phap - orc_uniform (st);
+   *pip = orc_interpolate_f ((float) pha, ch, e, ne);
+   if (*pip < 0) return -1;
+   return 0;
+}
+
 static void acis_photon (const orc_ctx *c, Marx_Photon_Attr_Type *at, double tstart, orc_stream *st)
 {
    const mxg_detector_cfg *d = &c->cfg->det;
@@ -149,6 +312,17 @@ static void acis_photon (const orc_ctx *c, Marx_Photon_Attr_Type *at, double tst
                qe = orc_interpolate_f ((float) at->energy, c->fa + g->qe_e_off, c->fa + g->qe_off, g->qe_n);
              if (r >= qe * qe_filter * qe_contam)
                {
+                  at->flags |= PHOTON_UNDETECTED;
+                  detected = 0;
+               }
+          }
+        if (detected && d->fef_on)
+          {
+             if (-1 == apply_fef (c, d, g->id, at->y_pixel, at->z_pixel, at->energy, st,
+                                  &at->pulse_height, &at->pi))
+               {
+                  at->pulse_height = -1;
+                  at->pi = 0;
                   at->flags |= PHOTON_UNDETECTED;
                   detected = 0;
                }

@@ -81,7 +81,8 @@ class DetectorCfg(ctypes.Structure):
                 ("use_drake", I), ("drake_n", I), ("drake_cr_width", D), ("drake", Rect * MAX_DRAKE),
                 ("drake_c_n", U), ("drake_c_e_off", U), ("drake_c_beta_off", U), ("drake_c_delta_off", U),
                 ("drake_cr_n", U), ("drake_cr_e_off", U), ("drake_cr_beta_off", U),
-                ("drake_cr_delta_off", U)]
+                ("drake_cr_delta_off", U), ("fef_on", I), ("fef_max_gaussians", I), ("fef_map_off", U),
+                ("fef_desc_off", U)]
 
 
 class SourceCfg(ctypes.Structure):
