@@ -299,7 +299,14 @@ __device__ __noinline__ bool apply_fef (const mxg_detector_cfg &d, const float *
    return true;
 }
 
+// FEF: compiled with the ACIS FEF step (launched only when a FEF file was
+// read, so that the common case keeps the registers of the kernel without it)
+template <bool FEF>
+#if MXG_DETECT_WAVES > 0
+__global__ __launch_bounds__ (kBlock, MXG_DETECT_WAVES)
+#else
 __global__ __launch_bounds__ (kBlock)
+#endif
 void k_detect (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
                const double *__restrict__ da, Soa s, ListIO io, double tstart,
                unsigned long long seed, uint32_t source_id)
@@ -393,7 +400,7 @@ void k_detect (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
                                       detected = false;
                                    }
                               }
-                            if (detected && d.fef_on)
+                            if (FEF && detected && d.fef_on)
                               {
                                  short pha = -1;
                                  float pi = 0.0f;
@@ -540,11 +547,15 @@ void k_detect (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
 
 hipError_t launch_detect (const mxg_config *cfg, const float *fa, const double *da, Soa s,
                           ListIO io, uint32_t n_upper, double tstart, unsigned long long seed,
-                          uint32_t source_id, hipStream_t st)
+                          uint32_t source_id, bool fef, hipStream_t st)
 {
    if (n_upper == 0) return hipSuccess;
-   hipLaunchKernelGGL (k_detect, dim3 (io.grid), dim3 (kBlock), 0, st, cfg, fa, da, s, io, tstart,
-                       seed, source_id);
+   if (fef)
+     hipLaunchKernelGGL (k_detect<true>, dim3 (io.grid), dim3 (kBlock), 0, st, cfg, fa, da, s, io, tstart,
+                         seed, source_id);
+   else
+     hipLaunchKernelGGL (k_detect<false>, dim3 (io.grid), dim3 (kBlock), 0, st, cfg, fa, da, s, io, tstart,
+                         seed, source_id);
    hipError_t e = hipGetLastError ();
    if (e == hipSuccess) e = launch_compact (io, st);
    return e;

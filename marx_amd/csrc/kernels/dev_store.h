@@ -145,6 +145,21 @@ hipError_t launch_pack_columns (const Soa &s, const uint32_t *idx, const uint32_
 // per SIMD, too few to hide its LDS and table-load latency; four waves
 // (128 VGPRs, a few spills) measured fastest on the HETG workload.
 #define MXG_REFLECT_WAVES_DEFAULT 4
+// waves per SIMD that the compiler must leave room for (0: unbounded).
+// k_source at 4 (128 VGPRs instead of 132) measured 6 percent faster; the
+// grating and detector kernels measured slower at 3 and 4 (spills)
+#ifndef MXG_SOURCE_WAVES
+#define MXG_SOURCE_WAVES 4
+#endif
+#ifndef MXG_BATCH_SOURCE_WAVES
+#define MXG_BATCH_SOURCE_WAVES 0
+#endif
+#ifndef MXG_GRATING_WAVES
+#define MXG_GRATING_WAVES 0
+#endif
+#ifndef MXG_DETECT_WAVES
+#define MXG_DETECT_WAVES 0
+#endif
 
 // Device buffers and launch shape of the mirror stage's second kernel.
 struct HrmaScratch
@@ -167,9 +182,10 @@ hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const flo
 hipError_t launch_grating (const mxg_config *cfg, const float *fa, const double *da, Soa s,
                            ListIO io, uint32_t n_upper, unsigned long long seed,
                            uint32_t source_id, hipStream_t st);
+// fef: run the variant with the ACIS FEF step (the configuration read a FEF file)
 hipError_t launch_detect (const mxg_config *cfg, const float *fa, const double *da, Soa s,
                           ListIO io, uint32_t n_upper, double tstart,
-                          unsigned long long seed, uint32_t source_id, hipStream_t st);
+                          unsigned long long seed, uint32_t source_id, bool fef, hipStream_t st);
 
 hipError_t launch_dither_store (const mxg_config *cfg, Soa s, uint32_t n, double start_time,
                                 unsigned long long seed, hipStream_t st);

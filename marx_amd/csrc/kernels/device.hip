@@ -509,7 +509,7 @@ int mxgd_run_detect (double tstart, unsigned long long seed, int slot, float *ms
    if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
    if (begin_stage (slot)) return -1;
    HIPCHECK (launch_detect (d_cfg, d_fa, d_da, g_soa, list_io (MXGD_K_DETECT), g_upper, tstart,
-                            seed, 0, g_stream));
+                            seed, 0, g_hcfg.det.fef_on != 0, g_stream));
    return end_stage (slot, ms);
 }
 

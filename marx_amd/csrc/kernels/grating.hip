@@ -176,7 +176,11 @@ __device__ __forceinline__ int diffract_from (const mxg_grating &g, const mxg_se
    return diffract_photon (g, gs, da, theta, energy, x, p, order, st);
 }
 
+#if MXG_GRATING_WAVES > 0
+__global__ __launch_bounds__ (kBlock, MXG_GRATING_WAVES)
+#else
 __global__ __launch_bounds__ (kBlock)
+#endif
 void k_grating (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
                 const double *__restrict__ da, Soa s, ListIO io,
                 unsigned long long seed, uint32_t source_id)

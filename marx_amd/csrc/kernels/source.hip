@@ -23,17 +23,19 @@ constexpr int kTile = 2048;          // photons per scan tile
 constexpr int kPer = kTile / kBlock; // photons per thread
 
 // every column of photon i set as a fresh photon of the given energy,
-// direction, time and tag
+// direction, time and tag; the time and the dither state are left to a
+// later pass that writes them (with_t, with_dith false)
 __device__ __forceinline__ void init_photon (Soa s, uint32_t i, double energy, vec3 p, double t,
-                                             uint32_t tag)
+                                             uint32_t tag, bool with_t = true, bool with_dith = true)
 {
    s.energy[i] = energy;
    s.p0[i] = p.x; s.p1[i] = p.y; s.p2[i] = p.z;
    s.x0[i] = 0.0; s.x1[i] = 0.0; s.x2[i] = 0.0;
-   s.t[i] = t;
+   if (with_t) s.t[i] = t;
    s.flags[i] = 0u;
    s.ypix[i] = 0.0f; s.zpix[i] = 0.0f; s.upix[i] = 0.0f; s.vpix[i] = 0.0f;
-   for (int q = 0; q < 6; q++) s.dith[q][i] = 0.0f;
+   if (with_dith)
+     for (int q = 0; q < 6; q++) s.dith[q][i] = 0.0f;
    s.pi[i] = 0.0f;
    s.pha[i] = 0;
    s.shell[i] = 0;
@@ -126,7 +128,11 @@ __device__ __forceinline__ uint64_t gap_ticks (double e)
    return (uint64_t) (e * 4294967296.0 + 0.5);
 }
 
+#if MXG_SOURCE_WAVES > 0
+__global__ __launch_bounds__ (kBlock, MXG_SOURCE_WAVES)
+#else
 __global__ __launch_bounds__ (kBlock)
+#endif
 void k_source (const mxg_config *__restrict__ cfg, const float *__restrict__ fa, const double *__restrict__ da, Soa s,
                uint32_t n, uint32_t tag_start, unsigned long long seed, uint32_t source_id,
                uint64_t *__restrict__ ticks, uint64_t *__restrict__ tile_tot)
@@ -156,7 +162,8 @@ void k_source (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
              else
                t = src.uniform_tmax * uniform (st);
 
-             init_photon (s, i, energy, p, t, tag);
+             // k_source_finish writes Poisson times and the INTERNAL dither
+             init_photon (s, i, energy, p, t, tag, !poisson, src.dither_mode != MXG_DITHER_INTERNAL);
           }
         sk[li] = k;
      }
@@ -379,7 +386,11 @@ __device__ __forceinline__ uint32_t source_of (const uint32_t *__restrict__ off,
    return lo;
 }
 
+#if MXG_BATCH_SOURCE_WAVES > 0
+__global__ __launch_bounds__ (kBlock, MXG_BATCH_SOURCE_WAVES)
+#else
 __global__ __launch_bounds__ (kBlock)
+#endif
 void k_batch_source (const mxg_config *__restrict__ cfg, const double *__restrict__ da, Soa s,
                      uint32_t n, const uint32_t *__restrict__ off, uint32_t nsrc,
                      const double *__restrict__ dir, const uint32_t *__restrict__ tag_start,
