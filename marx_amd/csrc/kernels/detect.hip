@@ -300,13 +300,11 @@ __device__ __noinline__ bool apply_fef (const mxg_detector_cfg &d, const float *
 }
 
 // FEF: compiled with the ACIS FEF step (launched only when a FEF file was
-// read, so that the common case keeps the registers of the kernel without it)
-template <bool FEF>
-#if MXG_DETECT_WAVES > 0
-__global__ __launch_bounds__ (kBlock, MXG_DETECT_WAVES)
-#else
-__global__ __launch_bounds__ (kBlock)
-#endif
+// read, so that the common case keeps the registers of the kernel without
+// it); W: waves per SIMD to leave room for (0 unbounded: ACIS and HRC both
+// measured slower bounded)
+template <bool FEF, int W>
+__global__ __launch_bounds__ (kBlock, W > 0 ? W : 1)
 void k_detect (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
                const double *__restrict__ da, Soa s, ListIO io, double tstart,
                unsigned long long seed, uint32_t source_id)
@@ -551,11 +549,11 @@ hipError_t launch_detect (const mxg_config *cfg, const float *fa, const double *
 {
    if (n_upper == 0) return hipSuccess;
    if (fef)
-     hipLaunchKernelGGL (k_detect<true>, dim3 (io.grid), dim3 (kBlock), 0, st, cfg, fa, da, s, io, tstart,
-                         seed, source_id);
+     hipLaunchKernelGGL ((k_detect<true, MXG_DETECT_WAVES>), dim3 (io.grid), dim3 (kBlock), 0, st, cfg, fa, da,
+                         s, io, tstart, seed, source_id);
    else
-     hipLaunchKernelGGL (k_detect<false>, dim3 (io.grid), dim3 (kBlock), 0, st, cfg, fa, da, s, io, tstart,
-                         seed, source_id);
+     hipLaunchKernelGGL ((k_detect<false, MXG_DETECT_WAVES>), dim3 (io.grid), dim3 (kBlock), 0, st, cfg, fa, da,
+                         s, io, tstart, seed, source_id);
    hipError_t e = hipGetLastError ();
    if (e == hipSuccess) e = launch_compact (io, st);
    return e;

@@ -146,13 +146,18 @@ hipError_t launch_pack_columns (const Soa &s, const uint32_t *idx, const uint32_
 // (128 VGPRs, a few spills) measured fastest on the HETG workload.
 #define MXG_REFLECT_WAVES_DEFAULT 4
 // waves per SIMD that the compiler must leave room for (0: unbounded).
-// k_source at 4 (128 VGPRs instead of 132) measured 6 percent faster; the
-// grating and detector kernels measured slower at 3 and 4 (spills)
+// k_source at 4 (128 VGPRs instead of 132) measured 6 percent faster,
+// k_batch_source at 4 (from 175) 29 percent, the LETG grating pass at 4 (from
+// 218) 25 percent; the HETG grating and the ACIS detector kernels measured
+// slower at 3 and 4 (spills)
 #ifndef MXG_SOURCE_WAVES
 #define MXG_SOURCE_WAVES 4
 #endif
 #ifndef MXG_BATCH_SOURCE_WAVES
-#define MXG_BATCH_SOURCE_WAVES 0
+#define MXG_BATCH_SOURCE_WAVES 4
+#endif
+#ifndef MXG_LETG_GRATING_WAVES
+#define MXG_LETG_GRATING_WAVES 4
 #endif
 #ifndef MXG_GRATING_WAVES
 #define MXG_GRATING_WAVES 0
@@ -179,9 +184,10 @@ hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const flo
                         const double *da, Soa s, ListIO io, HrmaScratch hs, uint32_t n_upper,
                         double source_distance, unsigned long long seed, uint32_t source_id,
                         hipStream_t st, hipEvent_t *ev);
+// letg: the instance compiled for the LETG pass
 hipError_t launch_grating (const mxg_config *cfg, const float *fa, const double *da, Soa s,
                            ListIO io, uint32_t n_upper, unsigned long long seed,
-                           uint32_t source_id, hipStream_t st);
+                           uint32_t source_id, bool letg, hipStream_t st);
 // fef: run the variant with the ACIS FEF step (the configuration read a FEF file)
 hipError_t launch_detect (const mxg_config *cfg, const float *fa, const double *da, Soa s,
                           ListIO io, uint32_t n_upper, double tstart,

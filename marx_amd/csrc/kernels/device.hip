@@ -499,6 +499,7 @@ int mxgd_run_grating (unsigned long long seed, int slot, float *ms)
    if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
    if (begin_stage (slot)) return -1;
    HIPCHECK (launch_grating (d_cfg, d_fa, d_da, g_soa, list_io (MXGD_K_GRATING), g_upper, seed, 0,
+                             g_hcfg.grating.type == MARX_GRATING_LETG,
                              g_stream));
    return end_stage (slot, ms);
 }

@@ -176,11 +176,11 @@ __device__ __forceinline__ int diffract_from (const mxg_grating &g, const mxg_se
    return diffract_photon (g, gs, da, theta, energy, x, p, order, st);
 }
 
-#if MXG_GRATING_WAVES > 0
-__global__ __launch_bounds__ (kBlock, MXG_GRATING_WAVES)
-#else
-__global__ __launch_bounds__ (kBlock)
-#endif
+// The same kernel for both gratings, compiled for the occupancy each
+// measured fastest at (W waves per SIMD, 0 unbounded): the LETG pass, with
+// its support-grating draws, gains from 4 waves; the HETG pass loses
+template <int W>
+__global__ __launch_bounds__ (kBlock, W > 0 ? W : 1)
 void k_grating (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
                 const double *__restrict__ da, Soa s, ListIO io,
                 unsigned long long seed, uint32_t source_id)
@@ -259,11 +259,15 @@ void k_grating (const mxg_config *__restrict__ cfg, const float *__restrict__ fa
 
 hipError_t launch_grating (const mxg_config *cfg, const float *fa, const double *da, Soa s,
                            ListIO io, uint32_t n_upper, unsigned long long seed,
-                           uint32_t source_id, hipStream_t st)
+                           uint32_t source_id, bool letg, hipStream_t st)
 {
    if (n_upper == 0) return hipSuccess;
-   hipLaunchKernelGGL (k_grating, dim3 (io.grid), dim3 (kBlock), 0, st, cfg, fa, da, s, io, seed,
-                       source_id);
+   if (letg)
+     hipLaunchKernelGGL (k_grating<MXG_LETG_GRATING_WAVES>, dim3 (io.grid), dim3 (kBlock), 0, st, cfg, fa, da,
+                         s, io, seed, source_id);
+   else
+     hipLaunchKernelGGL (k_grating<MXG_GRATING_WAVES>, dim3 (io.grid), dim3 (kBlock), 0, st, cfg, fa, da, s,
+                         io, seed, source_id);
    hipError_t e = hipGetLastError ();
    if (e == hipSuccess) e = launch_compact (io, st);
    return e;
