@@ -77,9 +77,9 @@ BYTES_PER_PHOTON = {
 # HBM bytes per photon entering a kernel, measured with rocprofv3 counters
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per dispatch, the factors that make a
 # 1 GiB device copy read back 1 GiB each way); commands and figures in
-# profiles/pmc_hetg_acis_s_1e7.md.  Absent when not measured for the current
-# kernels.
-TRAFFIC_BYTES_PER_PHOTON = {("hetg-acis-s", "k_hrma_reflect"): 317.3}
+# profiles/pmc_hetg_acis_s_1e7.md and profiles/pmc_letg_diff.md.  Absent
+# when not measured for the current kernels.
+TRAFFIC_BYTES_PER_PHOTON = {("hetg-acis-s", "k_hrma_reflect"): 321.9, ("letg-hrc-s", "grating"): 876.9}
 DESCRIPTION = {
     "hetg-acis-s": "HETG + ACIS-S: on-axis point source, power law index 2 over 0.4-8 keV (FILE table), "
                    "HEG/MEG orders from the CALDB efficiency tables, INTERNAL dither 16 arcsec "
