@@ -156,6 +156,15 @@ hipError_t launch_pack_columns (const Soa &s, const uint32_t *idx, const uint32_
 #ifndef MXG_BATCH_SOURCE_WAVES
 #define MXG_BATCH_SOURCE_WAVES 4
 #endif
+// waves of the reflection kernel of the schedule's largest setting (4; 5
+// measured 21 percent slower), and of the aperture kernel (unbounded, 95
+// VGPRs; 6 and 8 measured slower)
+#ifndef MXG_REFLECT_WAVES_TOP
+#define MXG_REFLECT_WAVES_TOP 4
+#endif
+#ifndef MXG_ENTER_WAVES
+#define MXG_ENTER_WAVES 0
+#endif
 #ifndef MXG_LETG_GRATING_WAVES
 #define MXG_LETG_GRATING_WAVES 4
 #endif

@@ -135,7 +135,11 @@ __device__ __forceinline__ void stage_hrma_cfg (const mxg_config *__restrict__ c
 // next unused word of the current block, packed as (c2 << 3) | pos.  No
 // Gaussian variate is pending at the split (none is drawn before it), and
 // the buffered block is recomputed from c2 - 1 on resumption.
+#if MXG_ENTER_WAVES > 0
+__global__ __launch_bounds__ (kBlock, MXG_ENTER_WAVES)
+#else
 __global__ __launch_bounds__ (kBlock)
+#endif
 void k_hrma_enter (const mxg_config *__restrict__ cfg, Soa s, ListIO io,
                    uint32_t *__restrict__ rng_pos, double source_distance,
                    unsigned long long seed, uint32_t source_id)
@@ -396,7 +400,7 @@ hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const flo
    void (*reflect) (const mxg_config *, const float *, const double *, Soa, ListIO,
                     const uint32_t *, unsigned long long, uint32_t, HrmaLds)
      = (hs.reflect_waves <= 2) ? k_hrma_reflect<2>
-       : (hs.reflect_waves == 3) ? k_hrma_reflect<3> : k_hrma_reflect<4>;
+       : (hs.reflect_waves == 3) ? k_hrma_reflect<3> : k_hrma_reflect<MXG_REFLECT_WAVES_TOP>;
    hipError_t e = hipSuccess;
    if (ev) e = hipEventRecord (ev[0], st);
    if (e != hipSuccess) return e;
