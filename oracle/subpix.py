@@ -34,7 +34,8 @@ def _cards(buf, pos):
 
 
 def read_bintables(path):
-    """{EXTNAME: {column name (upper case): array (rows,) or (rows, repeat)}}."""
+    """{HDUNAME (else EXTNAME): {column name (upper case): array (rows,) or
+    (rows, repeat)}}."""
     buf = open(path, "rb").read()
     h, pos = _cards(buf, 0)
     out = {}
@@ -53,7 +54,8 @@ def read_bintables(path):
             assert off == width
             dt = np.dtype([(n, c, s) for n, c, s in fields])
             rec = np.frombuffer(buf[pos:pos + size], dtype=dt)
-            out[h.get("EXTNAME", "")] = {n: rec[n].astype(np.dtype(c).newbyteorder("=")) for n, c, _ in fields}
+            key = h.get("HDUNAME", h.get("EXTNAME", ""))
+            out[key] = {n: rec[n].astype(np.dtype(c).newbyteorder("=")) for n, c, _ in fields}
         pos += (size + BLOCK - 1) // BLOCK * BLOCK
     return out
 
