@@ -3,6 +3,7 @@ computation: numpy's searchsorted / textbook linear interpolation, the
 closed forms of the quadratic, Fresnel's equations evaluated with numpy
 complex arithmetic, and the documented WFold table semantics."""
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -205,3 +206,45 @@ def test_wfold_table_semantics(mirror_tables):
                 t1 = _fold_theta(view, arrays[j], r)
                 want = t0 + (t1 - t0) * (e_alpha - ea[j - 1]) / (ea[j] - ea[j - 1])
                 assert got == pytest.approx(want, rel=1e-6, abs=1e-12)
+
+
+def _read_wfold_file(path):
+    """The WFold binary format (wfold.c:110-170), decoded here: records of a
+    big-endian int32 count n (0 or end of file ends the table), six float32
+    (e_alpha, p_min, delta_p, p_max, norm, expon) and n float32 theta values;
+    delta_p is re-derived as (p_max - p_min) / (n - 1)."""
+    b = open(path, "rb").read()
+    off, out = 0, []
+    while off + 4 <= len(b):
+        n = int(np.frombuffer(b, ">i4", 1, off)[0])
+        if n == 0:
+            break
+        hdr = np.frombuffer(b, ">f4", 6, off + 4).astype(np.float64)
+        theta = np.frombuffer(b, ">f4", n, off + 28).astype(np.float32)
+        off += 28 + 4 * n
+        e_alpha, p_min, _, p_max, norm, expon = hdr
+        if n == 1:
+            p_max, dp = p_min, 0.1
+        else:
+            dp = (p_max - p_min) / (n - 1)
+        out.append(((e_alpha, p_min, dp, p_max, norm, expon, n), theta))
+    assert off == len(b) or n == 0
+    return out
+
+
+def test_wfold_tables_are_the_files(mirror_tables):
+    """What the mirror set-up loaded for each shell and surface is exactly
+    the content of its WFold file (WFold_<P|H><shell>_File), decoded here."""
+    from marx_amd._lib import DATA_DIR
+    t, view = mirror_tables
+    for shell, number in enumerate((1, 3, 4, 6)):
+        for k, surf in enumerate("PH"):
+            path = os.path.join(DATA_DIR, t.par.get_string(f"WFold_{surf}{number}_File"))
+            recs = _read_wfold_file(path)
+            w = view.cfg.hrma.shell[shell].wfold[k]
+            assert w.n_arrays == len(recs) > 10, (number, surf)
+            arrays = view.d(w.arrays_off, w.n_arrays * 8).reshape(w.n_arrays, 8)
+            np.testing.assert_array_equal(view.d(w.e_alphas_off, w.n_arrays), [r[0][0] for r in recs])
+            for j, (hdr, theta) in enumerate(recs):
+                np.testing.assert_array_equal(arrays[j, :7], hdr, err_msg=f"{number}{surf} array {j}")
+                np.testing.assert_array_equal(view.f(int(arrays[j, 7]), hdr[6]), theta)
