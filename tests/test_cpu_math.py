@@ -1,7 +1,9 @@
 """The oracle's numerical building blocks, each checked against an independent
 computation: numpy's searchsorted / textbook linear interpolation, the
 closed forms of the quadratic, Fresnel's equations evaluated with numpy
-complex arithmetic, and the documented WFold table semantics."""
+complex arithmetic, the documented WFold table semantics, the WFold tables
+the mirror loads against the test's own decoding of the files, and the
+WFold draw as a rotation in the plane of incidence distributed as the table."""
 import ctypes
 import os
 
@@ -248,3 +250,103 @@ def test_wfold_tables_are_the_files(mirror_tables):
             for j, (hdr, theta) in enumerate(recs):
                 np.testing.assert_array_equal(arrays[j, :7], hdr, err_msg=f"{number}{surf} array {j}")
                 np.testing.assert_array_equal(view.f(int(arrays[j, 7]), hdr[6]), theta)
+
+
+def _fold_theta_grid(view, arr, r):
+    """_fold_theta over an array of p values."""
+    e_alpha, p_min, delta_p, p_max, norm, expon, num, toff = arr
+    vals = view.f(int(toff), int(num)).astype(np.float64)
+    mid = np.interp(r, p_min + delta_p * np.arange(int(num)), vals)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        tail = (norm * (1.0 - r)) ** expon
+    return np.where(r < p_min, 0.0, np.where(r > p_max, tail, mid))
+
+
+def test_wfold_draw_rotates_in_the_plane_of_incidence():
+    """The H-surface WFold scatter (wfold.c:334-369, applied as in hrma.c):
+    with the P-surface factors 0, an unblurred mirror without struts, and the same
+    photons traced with and without WFold, the only difference is a rotation
+    of the reflected direction at the same H hit point, in the plane of the
+    surface normal (n from the conic y^2 + z^2 = (a-1) x^2 + b x + c in the
+    shell frame), by f_H theta(E sin graze, r) for r uniform.  The
+    probability-integral transform of each photon's angle under its own
+    table (randomized over theta's atom at 0) must be uniform; the same
+    angles scaled by 0.8 or 1.25 (a scatter angle off by that factor) must
+    not be."""
+    from scipy import stats
+    from marx_amd import Photons, Tracer, on_axis_photons
+    from cfgview import ConfigView
+    from oracle.oracle import Oracle
+    zero_p = {f"P{k}ScatFactor": 0.0 for k in (1, 3, 4, 6)}
+    common = dict(GratingType="NONE", DetectorType="NONE", HRMA_Ideal="no", HRMA_Use_Blur="no",
+                  HRMA_Use_Struts="no", seed=21, **zero_p)
+    n, energy = 150000, 1.49
+    rec = on_axis_photons(n, energy)
+    out = {}
+    for wf in ("no", "yes"):
+        t = Tracer(HRMA_Use_WFold=wf, **common)
+        ph = Photons(n)
+        ph.load(rec)
+        Oracle(t).reflect(ph)
+        a = ph.active
+        out[wf] = a[np.argsort(a["tag"])]
+    view = ConfigView(t)
+    A, B = out["no"], out["yes"]
+    ok = ((A["flags"] & 0xFF) == 0) & ((B["flags"] & 0xFF) == 0)
+    A, B = A[ok], B[ok]
+    np.testing.assert_array_equal(A["mirror_shell"], B["mirror_shell"])
+    np.testing.assert_allclose(B["x"], A["x"], rtol=0, atol=1e-9)
+    rng = np.random.default_rng(9)
+    grid = (np.arange(4000) + 0.5) / 4000.0
+    scales = (1.0, 0.8, 1.25)
+    pits = {c: [] for c in scales}
+    n_moved, expect_moved, var_moved = 0, 0.0, 0.0
+    for shell in range(4):
+        s = view.cfg.hrma.shell[shell]
+        sel = A["mirror_shell"] == shell
+        assert sel.sum() > 1000
+        fwd = np.array(s.fwd[1]).reshape(3, 3)
+        x = (np.asarray(A["x"][sel]) + np.array(s.to_osac[1])) @ fwd.T
+        pa, pb = np.asarray(A["p"][sel]) @ fwd.T, np.asarray(B["p"][sel]) @ fwd.T
+        nrm = np.stack([(s.conic_a[1] - 1) * x[:, 0] + 0.5 * s.conic_b[1], -x[:, 1], -x[:, 2]], axis=1)
+        nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+        cr = np.cross(pa, pb)
+        ang = np.arctan2(np.linalg.norm(cr, axis=1), np.sum(pa * pb, axis=1))
+        moved = ang > 1e-12
+        # rotation axis perpendicular to the normal: pb stays in the plane of (pa, n)
+        assert np.abs(np.sum(cr[moved] * nrm[moved], axis=1)).max() < 1e-9 * ang[moved].max()
+        sin_graze = np.abs(np.sum(pa * nrm, axis=1))
+        f_h = s.scat_factor[1]
+        assert f_h > 1.0 and s.scat_factor[0] == 0.0
+        w = view.cfg.hrma.shell[shell].wfold[1]
+        if True:
+            arrays = view.d(w.arrays_off, w.n_arrays * 8).reshape(w.n_arrays, 8)
+            ea = arrays[:, 0]
+            theta = np.array([_fold_theta_grid(view, arr, grid) for arr in arrays])
+            e_alpha = energy * sin_graze
+            inside = (e_alpha > ea[0]) & (e_alpha < ea[-1])
+            assert inside.mean() > 0.99
+            j = np.searchsorted(ea, e_alpha[inside])
+            wgt = (e_alpha[inside] - ea[j - 1]) / (ea[j] - ea[j - 1])
+            obs = ang[inside] / f_h
+            for i in range(len(obs)):
+                th = np.maximum.accumulate((1 - wgt[i]) * theta[j[i] - 1] + wgt[i] * theta[j[i]])
+                # F(0): the chance of no scatter (r below p_min)
+                f0 = np.searchsorted(th, 1e-15, "right") / len(grid)
+                expect_moved += 1.0 - f0
+                var_moved += f0 * (1.0 - f0)
+                if obs[i] <= 1e-12:        # unscattered: rounding of the zero P rotation only
+                    continue
+                n_moved += 1
+                for c in scales:
+                    o = c * obs[i]
+                    lo = np.searchsorted(th, o * (1 - 1e-9), "left")
+                    hi = np.searchsorted(th, o * (1 + 1e-9), "right")
+                    u = (lo + rng.uniform() * max(hi - lo, 1)) / len(grid)
+                    pits[c].append((u - f0) / (1.0 - f0))
+    # the scattered fraction is 1 - F(0), and the angles of the scattered
+    # photons follow theta given theta > 0
+    assert abs(n_moved - expect_moved) < 5 * np.sqrt(var_moved) + 1, (n_moved, expect_moved)
+    pv = {c: stats.kstest(pits[c], "uniform").pvalue for c in scales}
+    assert n_moved > 1500
+    assert pv[1.0] > 1e-3 and pv[0.8] < 1e-8 and pv[1.25] < 1e-8, pv
