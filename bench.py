@@ -112,7 +112,8 @@ DESCRIPTION["events"] = (
     "subpixel offsets of the detected photons of one traced HETG + ACIS-S chunk of 1e7 photons "
     "(dither held per 3.241 s frame), computed in HBM by mxg_event_coords")
 DEFAULT_PHOTONS = {"hetg-acis-s": 10_000_000, "letg-hrc-s": 50_000_000, "batched-acis-i": 0,
-                   "diff": 10_000_000, "output": 10_000_000, "sources": 10_000_000, "events": 10_000_000}
+                   "diff": 10_000_000, "output": 10_000_000, "sources": 10_000_000, "events": 10_000_000,
+                   "host-aos": 2_000_000}
 # algorithmic HBM bytes per event row: k_event_heads reads the row's list
 # entry (4) and time (8) and writes its frame head (4); k_event_coords reads
 # the list entry and head (8), tag, chip id, chip x/y and PI (4+1+4+4+4; the
@@ -461,6 +462,48 @@ def measure_output(args, tmp, torch):
     }
 
 
+def measure_host_aos(args, tmp, torch):
+    """The libmarx drop-in route on one chunk, as marx.c drives it: each step
+    is marx_create_photons into host AoS records, then marx_mirror_reflect
+    (upload), marx_grating_diffract and marx_detect (download into the
+    caller's records).  Wall time per chunk, PCIe copies and the AoS <-> SoA
+    conversions included; the device-resident rate is the default line's."""
+    from marx_amd import Photons
+    from marx_amd.workloads import hetg_acis_s
+    cfg, K = "host-aos", args.steps
+    N = args.photons or DEFAULT_PHOTONS[cfg]
+    progress = lambda msg: print(f"bench host-aos: {msg}", file=sys.stderr, flush=True)
+    t = hetg_acis_s(tmp, photons=N, exposure=args.exposure)
+    ph = Photons(N)
+
+    def step():
+        made = t.create_photons(ph, N)
+        t.reflect(ph)
+        t.diffract(ph)
+        t.detect(ph)
+        return made
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    made = 0
+    for _ in range(K):
+        made += step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / K
+    made /= K
+    progress(f"{made:.0f} photons per chunk, {dt * 1e3:.1f} ms per chunk")
+    return {
+        "metric": "photons/s traced (HRMA+grating+detector) 1 GPU vs host CPU; gradient evals/s",
+        "value": made / dt, "unit": "photons/s (host AoS in and out, PCIe included)", "n_gpus": 1,
+        "steps": K, "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": DESCRIPTION["hetg-acis-s"], "name": cfg, "photons_per_step": made,
+                   "route": "marx_create_photons + marx_mirror_reflect + marx_grating_diffract + marx_detect"},
+    }
+
+
 def measure_events(args, tmp, torch):
     """Event coordinates of one traced chunk: each step computes every event
     column of the chunk's detected photons (mxg_event_coords, EDSER) in HBM;
@@ -672,6 +715,8 @@ def measure():
         return measure_sources(args, torch)
     if args.config == "events":
         return measure_events(args, tmp, torch)
+    if args.config == "host-aos":
+        return measure_host_aos(args, tmp, torch)
     cfg = args.config
     N, T = (args.photons or DEFAULT_PHOTONS[cfg]), args.exposure
     tracer = WORKLOADS[cfg](tmp, photons=N, exposure=T)
