@@ -79,7 +79,21 @@ BYTES_PER_PHOTON = {
 # 1 GiB device copy read back 1 GiB each way); commands and figures in
 # profiles/pmc_hetg_acis_s_1e7.md and profiles/pmc_letg_diff.md.  Absent
 # when not measured for the current kernels.
-TRAFFIC_BYTES_PER_PHOTON = {("hetg-acis-s", "k_hrma_reflect"): 321.9, ("letg-hrc-s", "grating"): 876.9}
+# The figure is not measured inside this run (counters need their own
+# profiler runs): each entry names the profile it comes from, and the bench
+# line carries that name beside `traffic`.
+TRAFFIC_BYTES_PER_PHOTON = {
+    ("hetg-acis-s", "k_hrma_reflect"): (321.9, "profiles/pmc_hetg_acis_s_1e7.md"),
+    ("letg-hrc-s", "grating"): (876.9, "profiles/pmc_letg_diff.md"),
+}
+
+
+def traffic_of(cfg, kernel, units):
+    """(HBM bytes per launch, source profile) of a kernel, or (None, None)."""
+    tb = TRAFFIC_BYTES_PER_PHOTON.get((cfg, kernel))
+    if tb is None:
+        return None, None
+    return tb[0] * units, tb[1]
 DESCRIPTION = {
     "hetg-acis-s": "HETG + ACIS-S: on-axis point source, power law index 2 over 0.4-8 keV (FILE table), "
                    "HEG/MEG orders from the CALDB efficiency tables, INTERNAL dither 16 arcsec "
@@ -127,6 +141,9 @@ SOURCE_SHAPES = {"POINT": {}, "DISK": {"S-DiskTheta0": 0.0, "S-DiskTheta1": 120.
 IMAGE_SHAPE = (512, 512, 0.5)
 DIFF_NPIX, DIFF_PIXEL_MM = 256, 0.004
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix)
+# The path does no matrix work: its compute roof is the FP64 vector ALU.
+VALU_PIPE = ("FP64 vector ALU (VALU; no matrix instructions on this path): 78.6 TFLOP/s is MI355X's "
+             "FP64 vector peak")
 PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
 STAGES = ["source", "mirror", "grating", "detector"]
 
@@ -162,14 +179,13 @@ def kernel_roofline(cfg, kernels):
     """The roofline object for the slowest of the timed kernels."""
     dom = max(kernels, key=lambda k: kernels[k]["ms"])
     units = kernels[dom]["photons_in"]
-    tb = TRAFFIC_BYTES_PER_PHOTON.get((cfg, dom))
-    return {"bound": "mfma", "pipe": "fp64 vector ALU (no matrix work on this path; 78.6 TFLOP/s "
-                                     "is MI355X's FP64 peak)",
+    traffic, traffic_src = traffic_of(cfg, dom, units)
+    return {"bound": "valu", "pipe": VALU_PIPE,
             "kernel": "k_grating (stage)" if dom == "grating" else
                       "k_detect (stage)" if dom == "detector" else dom,
             "achieved": kernels[dom]["tflops"], "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "frac": kernels[dom]["tflops"] / PEAK_FP64_TFLOPS,
-            "traffic": tb * units if tb is not None else None,
+            "traffic": traffic, "traffic_source": traffic_src,
             "flops_per_photon": FLOPS_PER_PHOTON[cfg][dom],
             "transcendentals_per_photon": TRANSCENDENTALS_PER_PHOTON[cfg][dom],
             "photons_per_launch": units, "launch_ms": kernels[dom]["ms"],
@@ -333,8 +349,7 @@ def measure_diff(args, torch):
     kernels["value_only_pass"] = {"ms": value_enter_ms + value_ms, "k_diff_forward<0>_ms": value_ms,
                                   "k_diff_enter_ms": value_enter_ms}
     fk = kernels["k_diff_forward"]
-    rf = {"bound": "mfma", "pipe": "fp64 vector ALU (no matrix work on this path; 78.6 TFLOP/s is MI355X's "
-                                   "FP64 peak)",
+    rf = {"bound": "valu", "pipe": VALU_PIPE,
           "kernel": "k_diff_forward<2>", "achieved": fk["tflops"], "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
           "frac": fk["tflops"] / PEAK_FP64_TFLOPS, "traffic": None,
           "flops_per_pass_photon": FLOPS_PER_PHOTON[cfg]["k_diff_forward"],
@@ -772,8 +787,7 @@ def measure():
     units = kernels[dom]["photons_in"]
     achieved = kernels[dom]["tflops"]
     achieved_bw = kernels[dom]["gbs"]
-    tb = TRAFFIC_BYTES_PER_PHOTON.get((cfg, dom))
-    traffic = tb * units if tb is not None else None
+    traffic, traffic_src = traffic_of(cfg, dom, units)
 
     out = {
         "metric": "photons/s traced (HRMA+grating+detector) 1 GPU vs host CPU; gradient evals/s",
@@ -791,12 +805,12 @@ def measure():
         "config": {"workload": DESCRIPTION[cfg], "name": cfg,
                    "photons_per_step": N, "exposure_s": T,
                    "detected_per_step": detected / K},
-        "roofline": {"bound": "mfma", "pipe": "fp64 vector ALU (no matrix work on this path; 78.6 TFLOP/s "
-                                              "is MI355X's FP64 peak)",
+        "roofline": {"bound": "valu", "pipe": VALU_PIPE,
                      "kernel": "k_grating (stage)" if dom == "grating" else
                                "k_detect (stage)" if dom == "detector" else dom,
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "flops_per_photon": FLOPS_PER_PHOTON[cfg][dom],
                      "transcendentals_per_photon": TRANSCENDENTALS_PER_PHOTON[cfg][dom],
                      "photons_per_launch": units, "launch_ms": t_launch * 1e3,

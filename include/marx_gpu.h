@@ -348,11 +348,20 @@ int mxg_diff_forward (const double *params, unsigned int num, unsigned int tag_s
 int mxg_diff_image (const double *params, unsigned int num, unsigned int tag_start,
                     const mxg_image_spec *spec, float *image, double *total_weight);
 
-/* Adjoint of the last mxg_diff_forward: grad[20] = d(sum_pix grad_image *
- * image)/d(params), i.e. the gradient of a loss whose derivative with
- * respect to the image is grad_image (device pointer, npix^2 floats).
- * Sums per-block partials in a fixed order (deterministic). */
-int mxg_diff_adjoint (const float *grad_image, double *grad);
+/* Id of the last forward pass (mxg_diff_forward or mxg_diff_image) made in
+ * this process by any caller; 0 before the first.  The adjoint needs it. */
+unsigned long long mxg_diff_last_pass (void);
+
+/* Adjoint of forward pass `pass` (the value mxg_diff_last_pass gave right
+ * after it): grad[20] = d(sum_pix grad_image * image)/d(params), i.e. the
+ * gradient of a loss whose derivative with respect to the image is
+ * grad_image (device pointer, npix^2 floats).  The device keeps the state
+ * of one forward pass: -1 when another pass ran since (its state would give
+ * another pass's gradient), when npix is not that pass's image side, or
+ * when that pass kept no derivatives.  Sums per-block partials in a fixed
+ * order (deterministic). */
+int mxg_diff_adjoint (unsigned long long pass, unsigned int npix, const float *grad_image,
+                      double *grad);
 
 /* ms[0..2]: device time in ms of the last mxg_diff_forward's or
  * mxg_diff_image's main kernel (both reflections, focal plane, image), of

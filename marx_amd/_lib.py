@@ -204,7 +204,9 @@ _SIGNATURES = [
                                       ctypes.POINTER(ImageSpec), ctypes.c_void_p,
                                       ctypes.POINTER(ctypes.c_double)]),
     ("mxg_diff_kernel_ms", ctypes.c_int, [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint)]),
-    ("mxg_diff_adjoint", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    ("mxg_diff_last_pass", ctypes.c_ulonglong, []),
+    ("mxg_diff_adjoint", ctypes.c_int, [ctypes.c_ulonglong, ctypes.c_uint, ctypes.c_void_p,
+                                        ctypes.POINTER(ctypes.c_double)]),
     ("mxg_diff_state", ctypes.c_int, [ctypes.c_uint] + [ctypes.POINTER(ctypes.c_double)] * 6
                                       + [ctypes.POINTER(ctypes.c_ubyte)]),
     ("mxg_get_schedule", ctypes.c_int, [ctypes.POINTER(Schedule)]),

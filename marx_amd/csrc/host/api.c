@@ -205,6 +205,21 @@ int marx_mirror_reflect (Marx_Photon_Type *pt, int verbose)
 
 /* ---- grating (grating.c:44-114) ------------------------------------------------ */
 
+/* The history bits a grating pass sets: MARX_ORDER_OK, and for LETG the
+ * support-grating order columns of the gratings that have orders
+ * (diffract.c:1098-1117: the fine grating ORDER1, the coarse one ORDER2-4). */
+static unsigned long grating_history (void)
+{
+   unsigned long hist = MARX_ORDER_OK;
+   if (Grating_Type != MARX_GRATING_LETG)
+     return hist;
+   if (Mxg_Cfg.grating.has_fine)
+     hist |= MARX_ORDER1_OK;
+   if (Mxg_Cfg.grating.has_coarse)
+     hist |= (MARX_ORDER2_OK | MARX_ORDER3_OK | MARX_ORDER4_OK);
+   return hist;
+}
+
 int marx_grating_init (Param_File_Type *pf)
 {
    char buf[256];
@@ -244,9 +259,7 @@ int marx_grating_diffract (Marx_Photon_Type *pt, int verbose)
      mxg_message (0, "Diffracting from %s.\n", (Grating_Type == MARX_GRATING_LETG) ? "LETG" : "HETG");
    if (pt->history & MARX_ORDER_OK)
      return 0;
-   pt->history |= MARX_ORDER_OK;
-   if (Grating_Type == MARX_GRATING_LETG)
-     pt->history |= (MARX_ORDER1_OK | MARX_ORDER2_OK | MARX_ORDER3_OK | MARX_ORDER4_OK);
+   pt->history |= grating_history ();
    if ((-1 == sync_config ()) || (-1 == make_resident (pt)))
      return -1;
    if (-1 == mxgd_run_grating (Seed, -1, NULL))
@@ -610,7 +623,7 @@ int mxg_trace_device (unsigned int num, unsigned int tag_start, double start_tim
         return -1;
      }
    hist |= MARX_MIRROR_SHELL_OK;
-   if (with_grating) hist |= MARX_ORDER_OK;
+   if (with_grating) hist |= grating_history ();
    if (Detector_Type > 0) hist |= detector_history ();
    /* photons alive after each stage (a skipped stage passes them through) */
    Stage_Counts[0] = n;
@@ -646,7 +659,7 @@ int mxg_trace_device (unsigned int num, unsigned int tag_start, double start_tim
 
 typedef struct
 {
-   unsigned int nsrc, n, generation, grating, detector;
+   unsigned int nsrc, n, generation, grating, detector, alloc_epoch;
    unsigned long long seed;
    double exposure, start_time, distance;
    mxgd_schedule sched;
@@ -683,7 +696,7 @@ static int batch_finish (unsigned int n, unsigned int tag_start, int with_gratin
    Stage_Counts[1] = counts[1];
    Stage_Counts[2] = with_grating ? counts[2] : Stage_Counts[1];
    Stage_Counts[3] = (Detector_Type > 0) ? counts[3] : Stage_Counts[2];
-   if (with_grating) hist |= MARX_ORDER_OK;
+   if (with_grating) hist |= grating_history ();
    if (Detector_Type > 0) hist |= detector_history ();
    Kernel_Ms[0] = ms[MXGD_SLOT_HRMA_ENTER];
    Kernel_Ms[1] = ms[MXGD_SLOT_HRMA_REFLECT];
@@ -744,6 +757,9 @@ int mxg_trace_batch (unsigned int num_sources, const double *dirs, const unsigne
 
    if (-1 == batch_setup (num_sources, dirs, counts, &n))
      return -1;
+   /* size the store first: the key then names the buffers a capture holds */
+   if (-1 == mxgd_batch_prepare (n))
+     goto device_error;
    memset (&key, 0, sizeof (key));
    key.nsrc = num_sources;
    key.n = n;
@@ -754,6 +770,7 @@ int mxg_trace_batch (unsigned int num_sources, const double *dirs, const unsigne
    key.exposure = exposure;
    key.start_time = start_time;
    key.distance = Mxg_Cfg.src.distance;
+   key.alloc_epoch = mxgd_alloc_epoch ();
    mxgd_get_schedule (&key.sched);
 
    if (use_graph)
@@ -762,8 +779,7 @@ int mxg_trace_batch (unsigned int num_sources, const double *dirs, const unsigne
           {
              mxgd_graph_destroy (Batch_Graph);
              Batch_Graph = NULL;
-             if ((-1 == mxgd_batch_prepare (n))
-                 || (-1 == mxgd_graph_begin ()))
+             if (-1 == mxgd_graph_begin ())
                goto device_error;
              if (-1 == batch_pass (n, start_time, exposure, with_grating))
                {
@@ -782,8 +798,7 @@ int mxg_trace_batch (unsigned int num_sources, const double *dirs, const unsigne
      }
    else
      {
-        if ((-1 == mxgd_batch_prepare (n))
-            || (-1 == mxgd_batch_set_tag_start (tag_start))
+        if ((-1 == mxgd_batch_set_tag_start (tag_start))
             || (-1 == batch_pass (n, start_time, exposure, with_grating))
             || (-1 == batch_finish (n, tag_start, with_grating, num_detected)))
           goto device_error;

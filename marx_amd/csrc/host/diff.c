@@ -164,14 +164,20 @@ int mxg_diff_image (const double *params, unsigned int num, unsigned int tag_sta
    return diff_pass ("mxg_diff_image", params, num, tag_start, spec, image, total_weight, 0);
 }
 
-int mxg_diff_adjoint (const float *grad_image, double *grad)
+unsigned long long mxg_diff_last_pass (void)
+{
+   return mxgd_diff_pass ();
+}
+
+int mxg_diff_adjoint (unsigned long long pass, unsigned int npix, const float *grad_image,
+                      double *grad)
 {
    if ((grad_image == NULL) || (grad == NULL))
      {
         mxg_error ("mxg_diff_adjoint: bad arguments");
         return -1;
      }
-   if (-1 == mxgd_diff_adjoint (grad_image, grad))
+   if (-1 == mxgd_diff_adjoint (pass, npix, grad_image, grad))
      {
         mxg_error ("%s", mxgd_last_error ());
         return -1;

@@ -93,6 +93,9 @@ void mxgd_get_schedule (mxgd_schedule *s);
  * mxgd_graph_launch replays it (ms: its device time, or NULL) and restores
  * the list state the sequence left at capture. */
 typedef struct mxgd_graph mxgd_graph;
+/* Changes whenever a device buffer that a captured sequence may hold is
+ * reallocated; a graph captured under another value must not be replayed. */
+unsigned int mxgd_alloc_epoch (void);
 int mxgd_graph_begin (void);
 int mxgd_graph_end (mxgd_graph **g);
 int mxgd_graph_launch (mxgd_graph *g, float *ms);
@@ -107,9 +110,12 @@ void mxgd_graph_destroy (mxgd_graph *g);
 int mxgd_diff_forward (const mxg_diff_cfg *d, unsigned int n, unsigned int tag_start,
                        unsigned long long seed, const mxg_image_spec *spec, float *image,
                        double *total_weight, int tangents);
-/* grad[MXG_NUM_SHELLS * MXG_DIFF_NPAR] of the last forward pass contracted
- * with g (device, npix^2 floats).  Synchronises before returning. */
-int mxgd_diff_adjoint (const float *g, double *grad);
+/* Forward passes made so far (the id of the last one). */
+unsigned long long mxgd_diff_pass (void);
+/* grad[MXG_NUM_SHELLS * MXG_DIFF_NPAR] of forward pass `pass`, which must be
+ * the last one, contracted with g (device, npix^2 floats, npix that pass's
+ * image side).  Synchronises before returning. */
+int mxgd_diff_adjoint (unsigned long long pass, unsigned int npix, const float *g, double *grad);
 /* Device times (ms) of the last k_diff_forward, k_diff_adjoint, and
  * k_diff_enter + its list compaction; photons k_diff_forward ran on. */
 void mxgd_diff_kernel_ms (float ms[3], unsigned int *listed);

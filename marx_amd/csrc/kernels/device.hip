@@ -65,6 +65,7 @@ unsigned int g_diff_n = 0;
 mxg_image_spec g_diff_spec;
 uint32_t g_diff_grid = 0;
 bool g_diff_tangents = false;      // the last forward pass kept derivatives
+unsigned long long g_diff_pass = 0; // forward passes made (over every caller)
 hipEvent_t g_diff_ev[6];           // around k_diff_enter + k_compact, k_diff_forward, k_diff_adjoint
 float g_diff_ms[3] = {0.0f, 0.0f, 0.0f};
 uint32_t *d_diff_list = nullptr;   // photons past the paraboloid: segments, list
@@ -80,6 +81,10 @@ mxg_diff_cfg *d_diff_cfg = nullptr;
 mxgd_schedule g_sched = {{8, 8, 8, 8, 8}, MXG_REFLECT_WAVES_DEFAULT};
 int g_capturing = 0;
 uint64_t *d_tile_tot = nullptr;
+// bumped whenever a buffer that a captured launch sequence may hold is
+// reallocated (photon store, lists, batch sources, configuration arenas):
+// a graph captured in an older epoch holds freed addresses
+unsigned int g_alloc_epoch = 0;
 
 int fail (const char *fmt, ...)
 {
@@ -205,6 +210,7 @@ int ensure_capacity (size_t n)
    for (int j = 0; j < 4; j++) g_soa.sup[j] = (int8_t *) take ();
    g_soa.tag = (uint32_t *) take ();
    g_capacity = cap;
+   g_alloc_epoch++;
    return 0;
 }
 
@@ -367,6 +373,8 @@ int mxgd_sync_config (const mxg_config *cfg, const float *fa, unsigned int nf,
    if ((generation == g_cfg_gen) && d_cfg) return 0;
    HIPCHECK (hipStreamSynchronize (g_stream));
    if (!d_cfg) HIPCHECK (hipMalloc (&d_cfg, sizeof (mxg_config)));
+   if ((nf + 1 > d_fa_cap) || (nd + 1 > d_da_cap))
+     g_alloc_epoch++;
    if (nf + 1 > d_fa_cap)
      {
         if (d_fa) HIPCHECK (hipFree (d_fa));
@@ -685,12 +693,20 @@ int mxgd_diff_forward (const mxg_diff_cfg *d, unsigned int n, unsigned int tag_s
    g_diff_spec = *spec;
    g_diff_grid = grid;
    g_diff_tangents = (tangents != 0);
+   g_diff_pass++;
    return 0;
 }
 
-int mxgd_diff_adjoint (const float *g, double *grad)
+unsigned long long mxgd_diff_pass (void) { return g_diff_pass; }
+
+int mxgd_diff_adjoint (unsigned long long pass, unsigned int npix, const float *g, double *grad)
 {
    if (g_diff_grid == 0) return fail ("no forward pass to take the adjoint of");
+   if (pass != g_diff_pass)
+     return fail ("the adjoint of forward pass %llu was asked for, but the device holds pass %llu "
+                  "(another forward pass ran since)", pass, g_diff_pass);
+   if (npix != g_diff_spec.npix)
+     return fail ("gradient image of %u pixels a side for a forward pass of %u", npix, g_diff_spec.npix);
    if (!g_diff_tangents)
      return fail ("the last forward pass (mxg_diff_image) kept no derivatives; call mxg_diff_forward");
    HIPCHECK (launch_diff_adjoint (g_diff_n, g_diff_out, &g_diff_spec, g, d_diff_partial, g_diff_grid,
@@ -772,8 +788,13 @@ int mxgd_batch_set_sources (unsigned int nsrc, const double *dirs, const unsigne
         HIPCHECK (hipMalloc (&d_batch_off, (nsrc + 1) * sizeof (uint32_t)));
         HIPCHECK (hipMalloc (&d_batch_dir, 3 * (size_t) nsrc * sizeof (double)));
         g_batch_cap = nsrc;
+        g_alloc_epoch++;
      }
-   if (!d_batch_tag) HIPCHECK (hipMalloc (&d_batch_tag, sizeof (uint32_t)));
+   if (!d_batch_tag)
+     {
+        HIPCHECK (hipMalloc (&d_batch_tag, sizeof (uint32_t)));
+        g_alloc_epoch++;
+     }
    uint32_t *off = (uint32_t *) malloc ((nsrc + 1) * sizeof (uint32_t));
    if (off == nullptr) return fail ("out of memory");
    off[0] = 0;
@@ -841,6 +862,8 @@ int mxgd_set_schedule (const mxgd_schedule *s)
 }
 
 void mxgd_get_schedule (mxgd_schedule *s) { *s = g_sched; }
+
+unsigned int mxgd_alloc_epoch (void) { return g_alloc_epoch; }
 
 struct mxgd_graph
 {

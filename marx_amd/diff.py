@@ -45,7 +45,6 @@ class DiffMirror:
         self.spec = ImageSpec(int(npix), float(pixel_size), float(center[0]), float(center[1]))
         self.npix = int(npix)
         self.total_weight = 0.0
-        self._pass = 0            # forward passes made; the adjoint needs the latest
 
     def forward_into(self, params, image, tangents=True):
         """mxg_diff_forward (or, with tangents=False, mxg_diff_image: no
@@ -62,19 +61,23 @@ class DiffMirror:
         check(fn(p.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), self.photons, self.tag_start,
                  ctypes.byref(self.spec), ctypes.c_void_p(image.data_ptr()), ctypes.byref(tw)), name)
         self.total_weight = tw.value
-        self._pass += 1
-        return self._pass
+        # the library's id of this pass, over every DiffMirror of the process
+        return int(self.lib.mxg_diff_last_pass())
 
     def adjoint(self, grad_image, token):
-        """mxg_diff_adjoint of forward pass `token` for d(loss)/d(image)."""
+        """mxg_diff_adjoint of forward pass `token` for d(loss)/d(image).
+        The device holds the state of the last forward pass of any
+        DiffMirror; the library refuses a token of an older pass and a
+        gradient image of another size."""
         import torch
-        if token != self._pass:
-            raise RuntimeError("the adjoint needs the latest forward pass of this DiffMirror; "
-                               "another forward pass was made since")
+        if tuple(grad_image.shape) != (self.npix, self.npix):
+            raise ValueError(f"gradient image of shape {tuple(grad_image.shape)} for an image of "
+                             f"{self.npix} x {self.npix} pixels")
         g = grad_image.detach().to(torch.float32).contiguous()
         torch.cuda.current_stream().synchronize()
         out = (ctypes.c_double * NUM_PARAMS)()
-        check(self.lib.mxg_diff_adjoint(ctypes.c_void_p(g.data_ptr()), out), "mxg_diff_adjoint")
+        check(self.lib.mxg_diff_adjoint(ctypes.c_ulonglong(token), self.npix, ctypes.c_void_p(g.data_ptr()),
+                                        out), "mxg_diff_adjoint")
         return np.array(out)
 
     def kernel_ms(self):

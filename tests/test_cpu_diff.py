@@ -129,3 +129,21 @@ def test_threaded_oracle_equals_serial(tracer):
     g = np.random.default_rng(6).normal(size=(64, 64))
     np.testing.assert_allclose(o.diff_adjoint(b, spec, g, nthreads=4), o.diff_adjoint(a, spec, g),
                                rtol=1e-12, atol=1e-12)
+
+
+def test_adjoint_guard_without_a_forward_pass(tracer):
+    """The adjoint's guard on the host: before any forward pass the pass id
+    is 0 and the library refuses an adjoint (no forward state, no GPU
+    call), and DiffMirror refuses a gradient image of the wrong shape
+    before it reaches the library."""
+    import ctypes
+    import torch
+    from marx_amd.diff import DiffMirror
+    lib = tracer.lib
+    assert lib.mxg_diff_last_pass() == 0
+    g = np.zeros(16, dtype=np.float32)
+    out = (ctypes.c_double * NUM_PARAMS)()
+    assert lib.mxg_diff_adjoint(1, 4, g.ctypes.data_as(ctypes.c_void_p), out) == -1
+    dm = DiffMirror(tracer, photons=100, npix=8)
+    with pytest.raises(ValueError, match="shape"):
+        dm.adjoint(torch.zeros((4, 4)), 1)

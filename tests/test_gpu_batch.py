@@ -143,3 +143,29 @@ def test_autotuner_installs_a_valid_schedule(gpu, tmp_path):
     t.trace_batch(dirs, counts, use_graph=True)
     assert_same_records(_download(t, n), ref, "autotuned graph pass")
     t.set_schedule(blocks_per_cu=8, reflect_waves=4)
+
+
+@pytest.mark.gpu
+def test_graph_is_recaptured_after_the_store_grows(gpu, tmp_path):
+    """A captured batch graph holds the photon store's addresses.  A device
+    trace larger than the store reallocates it; the next graph pass with the
+    same sources must not replay kernels into the freed buffers but capture
+    again, and give the eager pass's records."""
+    import ctypes
+    from marx_amd._lib import SoaView
+    t, dirs, counts = _batch(tmp_path, 24)
+    n = int(counts.sum())
+    t.trace_batch(dirs, counts, tag_start=3, exposure=2.0e4, use_graph=True)
+    first = _download(t, n)
+    view = SoaView()
+    assert t.lib.mxg_get_soa(ctypes.byref(view)) == 0
+    big = int(view.capacity) + 1
+    t.trace_device(big, tag_start=0)
+    assert t.lib.mxg_get_soa(ctypes.byref(view)) == 0
+    assert int(view.capacity) >= big
+    t.trace_batch(dirs, counts, tag_start=3, exposure=2.0e4, use_graph=True)
+    again = _download(t, n)
+    t.trace_batch(dirs, counts, tag_start=3, exposure=2.0e4, use_graph=False)
+    eager = _download(t, n)
+    assert_same_records(again, eager, "graph pass after the store grew")
+    assert_same_records(first, eager, "graph pass before the store grew")

@@ -7,12 +7,14 @@ Tolerances: the image is summed in FP32 on the GPU and in FP64 by the oracle
 are FP64 on both sides, so the gradient for the same d(loss)/d(image)
 agrees to 1e-6 relative (of the largest component); central differences
 of the FP64 loss of the GPU's per-photon state agree to 1e-4."""
+import ctypes
+
 import numpy as np
 import pytest
 import torch
 
 from marx_amd import Tracer
-from marx_amd._lib import ImageSpec
+from marx_amd._lib import ImageSpec, check
 from marx_amd.diff import NUM_PARAMS, DiffMirror, setup
 from oracle.oracle import Oracle
 
@@ -154,8 +156,29 @@ def test_adjoint_needs_the_latest_forward(setup_mirror):
     img = dm.image(params)
     with torch.no_grad():
         dm.image(torch.zeros(NUM_PARAMS, dtype=torch.float64))
-    with pytest.raises(RuntimeError, match="latest forward"):
+    with pytest.raises(RuntimeError, match="another forward pass ran since"):
         img.sum().backward()
+
+
+@pytest.mark.gpu
+def test_adjoint_refuses_another_mirrors_pass(setup_mirror):
+    """The device keeps one forward state for the process: a pass of a second
+    DiffMirror (here with a larger image) between a forward pass and its
+    backward makes the library refuse the backward instead of returning the
+    other pass's gradient or reading past the smaller gradient image."""
+    t, dm = setup_mirror
+    other = DiffMirror(t, photons=dm.photons, npix=2 * dm.npix, pixel_size=dm.spec.pixel_size)
+    params = torch.zeros(NUM_PARAMS, dtype=torch.float64, requires_grad=True)
+    img = dm.image(params)
+    with torch.no_grad():
+        other.image(torch.zeros(NUM_PARAMS, dtype=torch.float64))
+    with pytest.raises(RuntimeError, match="another forward pass ran since"):
+        img.sum().backward()
+    # the latest pass is the other mirror's, of another size
+    g = torch.zeros((dm.npix, dm.npix), dtype=torch.float32, device="cuda")
+    with pytest.raises(RuntimeError, match="pixels a side"):
+        check(dm.lib.mxg_diff_adjoint(dm.lib.mxg_diff_last_pass(), dm.npix, ctypes.c_void_p(g.data_ptr()),
+                                      (ctypes.c_double * NUM_PARAMS)()), "mxg_diff_adjoint")
 
 
 @pytest.mark.gpu

@@ -57,6 +57,34 @@ def test_device_writer_equals_the_aos_writer(gpu, tmp_path, workload):
 
 
 @pytest.mark.gpu
+def test_device_writer_equals_the_drop_in_route_letg(gpu, tmp_path):
+    """The LETG support-grating order columns: the device-resident trace and
+    the drop-in route (marx_create_photons, marx_mirror_reflect,
+    marx_grating_diffract, marx_detect, marx_write_photons) both write
+    ofine.dat and ocoarse1-3.dat (diffract.c:1098-1117 sets their history
+    bits), and the two routes write the same bytes in every file."""
+    n = 100_000
+    t = WORKLOADS["letg-hrc-s"](str(tmp_path), photons=n, exposure=1.0e4)
+    a, b = tmp_path / "device", tmp_path / "dropin"
+    a.mkdir()
+    b.mkdir()
+    t.trace_device(n, tag_start=0, start_time=0.0)
+    t.write_device(str(a))
+    p = Photons(n)
+    assert t.create_photons(p, n) == n
+    t.reflect(p)
+    t.diffract(p)
+    t.detect(p)
+    p.write(str(b))
+    got, ref = _read_dir(a), _read_dir(b)
+    for f in ("order.dat", "ofine.dat", "ocoarse1.dat", "ocoarse2.dat", "ocoarse3.dat"):
+        assert f in got and f in ref, f
+    assert sorted(got) == sorted(ref)
+    for f in ref:
+        assert got[f] == ref[f], f
+
+
+@pytest.mark.gpu
 def test_device_writer_after_a_batch(gpu, tmp_path):
     t, dirs, counts = batched_acis_i(str(tmp_path), num_sources=64, max_count=3000)
     n = int(counts.sum())
