@@ -157,6 +157,15 @@ int pf_set_value (Param_File_Type *pf, char *name, char *value);
  * is used. */
 int marx_set_data_directory (char *dir);
 
+/* The observation start, in years and in seconds from MJDref (1998.0), as
+ * marx.c's setup_tstart computes it from TStart before module set-up
+ * (marx/libsrc/marx.h:578, util.c:231-236).  Used by the exposure cut of
+ * photons with their own times and by the ACIS contamination's time
+ * interpolation, read at detector set-up. */
+int marx_set_time (double tyrs, double mjdsecs);
+extern double _Marx_TStart_MJDsecs;
+extern double _Marx_TStart_Yrs;
+
 /* ------------------------------------------------------------------------ */
 /* Hot-path entry points (marx/libsrc/marx.h:354-362), called from
  * marx/src/marx.c:240-273 (run) and :641-656 (init).  The init functions

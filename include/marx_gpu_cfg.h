@@ -144,6 +144,27 @@ typedef struct
 }
 mxg_rect;
 
+/* ACIS contamination of one chip (aciscontam.c:93-136, 218-465): the
+ * transmission exp(-sum_i mu_i(E) (tau0_i + tau1_i f(x, y))) over up to
+ * MXG_MAX_CONTAM_LAYERS layers, mu_i tabulated in energy (float arena),
+ * tau interpolated in time at set-up.  f is a table of (1024 / blocking)^2
+ * floats per layer (fxy_model 0, float arena at fxy_off[i]), the ACIS-I
+ * radial model about (x0, y0) (1) or the ACIS-S row profile (2).
+ * num_layers = 0: no contamination (transmission 1). */
+#define MXG_MAX_CONTAM_LAYERS 5
+#define MXG_CONTAM_TABLE    0
+#define MXG_CONTAM_RADIAL   1
+#define MXG_CONTAM_ROWS     2
+typedef struct
+{
+   unsigned int num_layers, fxy_model, blocking, pad;
+   double x0, y0;
+   double tau0[MXG_MAX_CONTAM_LAYERS], tau1[MXG_MAX_CONTAM_LAYERS];
+   unsigned int mu_n[MXG_MAX_CONTAM_LAYERS], mu_e_off[MXG_MAX_CONTAM_LAYERS];
+   unsigned int mu_off[MXG_MAX_CONTAM_LAYERS], fxy_off[MXG_MAX_CONTAM_LAYERS];
+}
+mxg_contam;
+
 typedef struct
 {
    int type;                     /* 0 NONE or MARX_DETECTOR_* */
@@ -181,6 +202,10 @@ typedef struct
     * channels and its Gaussians ([energy][gaussian][amp, centre, sigma]). */
    int fef_on, fef_max_gaussians;
    unsigned int fef_map_off, fef_desc_off;
+   /* ACIS contamination by chip slot (chip[k] <-> contam[k]); contam_on when
+    * any chip has layers */
+   int contam_on, contam_pad;
+   mxg_contam contam[MXG_MAX_CHIPS];
 }
 mxg_detector_cfg;
 

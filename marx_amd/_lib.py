@@ -11,7 +11,10 @@ import os
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG_DIR, "lib", "libmarx_gpu.so")
+# MARX_GPU_LIB names another build of the library (used by the oracle's
+# planted-mistake runs of the host set-up code); the default is the in-tree
+# build.
+LIB_PATH = os.environ.get("MARX_GPU_LIB") or os.path.join(PKG_DIR, "lib", "libmarx_gpu.so")
 DATA_DIR = os.path.join(PKG_DIR, "data")
 PAR_FILE = os.path.join(DATA_DIR, "marx.par")
 
@@ -156,6 +159,7 @@ _SIGNATURES = [
     ("pf_get_file", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint]),
     ("pf_set_value", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p]),
     ("marx_set_data_directory", ctypes.c_int, [ctypes.c_char_p]),
+    ("marx_set_time", ctypes.c_int, [ctypes.c_double, ctypes.c_double]),
     ("marx_mirror_init", ctypes.c_int, [ctypes.c_void_p]),
     ("marx_mirror_reflect", ctypes.c_int, [PPhotonList, ctypes.c_int]),
     ("marx_grating_init", ctypes.c_int, [ctypes.c_void_p]),

@@ -12,9 +12,15 @@
  *   MCP blur parameters           hrcblur.c:115-236
  *   HESF (Drake flat)             drake.c:170-260
  *
- * ACIS contamination and the FEF pulse-height response are not applied:
- * their calibration files are not part of the reference's data tree here
- * (the QE draw uses contamination = 1; pulse height and PI stay 0).
+ *   ACIS contamination            aciscontam.c:218-517 (ACIS<ccd>_CONTAM or
+ *                                 AXAF_CONTAM + CCD_ID; taus at TStart)
+ *   ACIS FEF                      acis_fef.c:194-298, 706-874
+ *
+ * The CALDB contamination and FEF files are missing from the reference's
+ * data tree (marx/data/caldb/.MISSING_LARGE_BLOBS); ACISContamFile and
+ * ACISFEFFile name other files.  Without a contamination file the QE draw
+ * uses a transmission of 1 and a message says so (the reference refuses to
+ * initialise ACIS then, unless DetIdeal is set).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -456,6 +462,235 @@ fail:
    return -1;
 }
 
+/* ---- ACIS contamination (aciscontam.c) ------------------------------------------ */
+
+static int match_contam (mxg_fits_table *t, void *client)
+{
+   int ccd = ((Ccd_Match *) client)->ccd;
+   char name[32], buf[32];
+   snprintf (name, sizeof (name), "ACIS%d_CONTAM", ccd);
+   if (mxg_fits_keyword_is (t, "EXTNAME", name))
+     return 1;
+   if (!mxg_fits_keyword_is (t, "EXTNAME", "AXAF_CONTAM"))
+     return 0;
+   if (-1 == mxg_fits_keyword (t, "CCD_ID", buf, sizeof (buf)))
+     return 0;
+   return atoi (buf) == ccd;
+}
+
+/* tau0 / tau1 of one layer at the observation start, linear in time and
+ * extrapolated past the ends (aciscontam.c:236-269) */
+static void contam_taus (const double *times, const float *tau0, const float *tau1, unsigned int n,
+                         double *t0_out, double *t1_out)
+{
+   double t = _Marx_TStart_MJDsecs, w;
+   unsigned int n0, n1 = 0;
+   if (n == 1)
+     {
+        *t0_out = tau0[0];
+        *t1_out = tau1[0];
+        return;
+     }
+   /* the first node above t (JDMbinary_search_d) */
+   while ((n1 < n) && (times[n1] <= t)) n1++;
+   if (n1 == n) n1--;
+   if (n1 == 0) n1 = 1;
+   n0 = n1 - 1;
+   w = (t - times[n0]) / (times[n1] - times[n0]);
+   *t0_out = tau0[n0] + (tau0[n1] - tau0[n0]) * w;
+   *t1_out = tau1[n0] + (tau1[n1] - tau1[n0]) * w;
+}
+
+static int contam_fail (mxg_fits_table *t, const char *file, unsigned int row, int ccd, const char *why)
+{
+   mxg_error ("%s: error in row %u of %s for CCD=%d", why, row + 1, file, ccd);
+   mxg_fits_close (t);
+   return -1;
+}
+
+/* One chip's layers from the contamination file (aciscontam.c:328-465). */
+static int read_chip_contam (const char *file, int ccd, mxg_contam *c)
+{
+   mxg_fits_table *t;
+   Ccd_Match m;
+   unsigned int nrows, layer, rep_e, rep_mu, rep_t, rep_t0, rep_t1, rep_f = 0;
+   int has_fxy;
+   char ty, buf[32];
+
+   memset (c, 0, sizeof (*c));
+   m.ccd = ccd;
+   if (NULL == (t = mxg_fits_find_table (file, match_contam, &m)))
+     {
+        mxg_error ("Unable to open acis contam file %s for CCD %d", file, ccd);
+        return -1;
+     }
+   has_fxy = (0 == mxg_fits_column (t, "fxy", &rep_f, &ty));
+   if ((-1 == mxg_fits_column (t, "component", NULL, &ty)) || (-1 == mxg_fits_column (t, "n_energy", NULL, &ty))
+       || (-1 == mxg_fits_column (t, "energy", &rep_e, &ty)) || (-1 == mxg_fits_column (t, "mu", &rep_mu, &ty))
+       || (-1 == mxg_fits_column (t, "n_time", NULL, &ty)) || (-1 == mxg_fits_column (t, "time", &rep_t, &ty))
+       || (-1 == mxg_fits_column (t, "tau0", &rep_t0, &ty)) || (-1 == mxg_fits_column (t, "tau1", &rep_t1, &ty)))
+     {
+        mxg_error ("Error processing ACIS contam table %s for CCD=%d", file, ccd);
+        mxg_fits_close (t);
+        return -1;
+     }
+   nrows = mxg_fits_num_rows (t);
+   if (nrows > MXG_MAX_CONTAM_LAYERS)
+     {
+        mxg_error ("The number of layers in %s for CCD=%d is larger than supported", file, ccd);
+        mxg_fits_close (t);
+        return -1;
+     }
+   if (has_fxy)
+     {
+        int b = 1;
+        if (0 == mxg_fits_keyword (t, "FXYBLK", buf, sizeof (buf)))
+          {
+             b = atoi (buf);
+             if ((b <= 0) || (b * (1024 / b) != 1024))
+               {
+                  mxg_error ("Invalid fxyblk value");
+                  mxg_fits_close (t);
+                  return -1;
+               }
+          }
+        c->blocking = (unsigned int) b;
+        c->fxy_model = MXG_CONTAM_TABLE;
+        if (rep_f != (1024u / c->blocking) * (1024u / c->blocking))
+          {
+             mxg_error ("The FXY column is expected to have %u * %u values", 1024 / c->blocking,
+                        1024 / c->blocking);
+             mxg_fits_close (t);
+             return -1;
+          }
+     }
+   else if (ccd <= 3)
+     {
+        /* aciscontam.c:141-172: radial about the array's centre */
+        c->fxy_model = MXG_CONTAM_RADIAL;
+        c->x0 = ((ccd == 1) || (ccd == 2)) ? 0.5 : 1024.5;
+        c->y0 = 1024.5;
+     }
+   else
+     c->fxy_model = MXG_CONTAM_ROWS;
+
+   for (layer = 0; layer < nrows; layer++)
+     {
+        double comp, ne, nt, *times;
+        float *tau0, *tau1, *e, *mu;
+        unsigned int j, n_time, n_energy;
+
+        if ((-1 == mxg_fits_get_double (t, "component", layer, 0, &comp))
+            || (-1 == mxg_fits_get_double (t, "n_energy", layer, 0, &ne))
+            || (-1 == mxg_fits_get_double (t, "n_time", layer, 0, &nt)))
+          return contam_fail (t, file, layer, ccd, "unreadable");
+        if (comp != 0.0)
+          return contam_fail (t, file, layer, ccd, "Expecting COMPONENT=0");
+        n_time = (unsigned int) nt;
+        n_energy = (unsigned int) ne;
+        if ((n_time <= 1) || (n_energy <= 1))
+          return contam_fail (t, file, layer, ccd, "Expecting n_energy or n_time to be greater than 1");
+        if ((n_time > rep_t) || (n_time > rep_t0) || (n_time > rep_t1))
+          return contam_fail (t, file, layer, ccd, "n_time value is larger than data arrays");
+        if ((n_energy > rep_e) || (n_energy > rep_mu))
+          return contam_fail (t, file, layer, ccd, "n_energy value is larger than data arrays");
+
+        times = (double *) malloc (n_time * sizeof (double));
+        tau0 = (float *) malloc (n_time * sizeof (float));
+        tau1 = (float *) malloc (n_time * sizeof (float));
+        c->mu_n[layer] = n_energy;
+        c->mu_e_off[layer] = mxg_farena_reserve (n_energy);
+        c->mu_off[layer] = mxg_farena_reserve (n_energy);
+        if ((times == NULL) || (tau0 == NULL) || (tau1 == NULL)
+            || (c->mu_e_off[layer] == (unsigned int) -1) || (c->mu_off[layer] == (unsigned int) -1))
+          {
+             free (times); free (tau0); free (tau1);
+             return contam_fail (t, file, layer, ccd, "out of memory");
+          }
+        e = mxg_farena_ptr (c->mu_e_off[layer]);
+        mu = mxg_farena_ptr (c->mu_off[layer]);
+        for (j = 0; j < n_time; j++)
+          if ((-1 == mxg_fits_get_double (t, "time", layer, j, times + j))
+              || (-1 == mxg_fits_get_float (t, "tau0", layer, j, tau0 + j))
+              || (-1 == mxg_fits_get_float (t, "tau1", layer, j, tau1 + j))
+              || ((j > 0) && !(times[j] > times[j - 1])))
+            {
+               free (times); free (tau0); free (tau1);
+               return contam_fail (t, file, layer, ccd, "Expecting the time grid to be monotonically increasing");
+            }
+        for (j = 0; j < n_energy; j++)
+          if ((-1 == mxg_fits_get_float (t, "energy", layer, j, e + j))
+              || (-1 == mxg_fits_get_float (t, "mu", layer, j, mu + j))
+              || ((j > 0) && !(e[j] > e[j - 1])))
+            {
+               free (times); free (tau0); free (tau1);
+               return contam_fail (t, file, layer, ccd, "Expecting the ENERGY grid to be monotonically increasing");
+            }
+        contam_taus (times, tau0, tau1, n_time, c->tau0 + layer, c->tau1 + layer);
+        free (times); free (tau0); free (tau1);
+        if (has_fxy)
+          {
+             float *f;
+             c->fxy_off[layer] = mxg_farena_reserve (rep_f);
+             if (c->fxy_off[layer] == (unsigned int) -1)
+               return contam_fail (t, file, layer, ccd, "out of memory");
+             f = mxg_farena_ptr (c->fxy_off[layer]);
+             for (j = 0; j < rep_f; j++)
+               if (-1 == mxg_fits_get_float (t, "fxy", layer, j, f + j))
+                 return contam_fail (t, file, layer, ccd, "unreadable FXY");
+          }
+     }
+   c->num_layers = nrows;
+   mxg_fits_close (t);
+   return 0;
+}
+
+/* The contamination of every chip (acis-s.c:486-487, acis-i.c:349): the
+ * file is ACISContamFile when set (an error when it cannot be read, as in
+ * aciscontam.c:311-316), else the CALDB ACISCONTAM file.  The CALDB file
+ * is missing from the reference's data: without it no contamination is
+ * applied (said once). */
+static int read_acis_contam (Param_File_Type *pf)
+{
+   mxg_detector_cfg *d = &Mxg_Cfg.det;
+   char buf[1024], *file;
+   FILE *fp;
+   int k, explicit_file;
+
+   d->contam_on = 0;
+   memset (d->contam, 0, sizeof (d->contam));
+   buf[0] = 0;
+   explicit_file = (0 == pf_get_string (pf, "ACISContamFile", buf, sizeof (buf))) && (buf[0] != 0);
+   file = explicit_file ? strdup (buf) : mxg_caldb_file ("ACISCONTAM");
+   if ((file == NULL) || (NULL == (fp = fopen (file, "rb"))))
+     {
+        if (explicit_file)
+          {
+             mxg_error ("Unable to open acis contam file %s", buf);
+             free (file);
+             return -1;
+          }
+        static int said = 0;
+        if (!said)
+          mxg_message (0, "ACIS contamination file not available: no contamination is applied "
+                          "(the reference needs the file unless DetIdeal is set)\n");
+        said = 1;
+        free (file);
+        return 0;
+     }
+   fclose (fp);
+   mxg_message (2, "\t%s\n", file);
+   for (k = 0; k < d->num_chips; k++)
+     if (-1 == read_chip_contam (file, d->chip[k].id, d->contam + k))
+       {
+          free (file);
+          return -1;
+       }
+   free (file);
+   d->contam_on = 1;
+   return 0;
+}
+
 static int acis_init (Param_File_Type *pf, int is_s)
 {
    mxg_detector_cfg *d = &Mxg_Cfg.det;
@@ -495,6 +730,8 @@ static int acis_init (Param_File_Type *pf, int is_s)
           if (-1 == read_acis_qe (g))
             return -1;
      }
+   if ((d->ideal == 0) && (-1 == read_acis_contam (pf)))
+     return -1;
    return read_acis_fef (pf, is_s ? 4 : 0, is_s ? 9 : 3);
 }
 

@@ -27,11 +27,20 @@ struct _Marx_Source_Type
    double ray_last_time;
 };
 
-/* The observation start in MJD seconds, as the reference's driver sets it
- * (marx.c:285-330 -> util.c:55).  It enters the exposure cut of sources
- * whose photons carry their own times (source.c:295-309).  A driver that
- * defines it supersedes this definition at link time. */
+/* The observation start in MJD seconds and in years, as the reference's
+ * driver sets them through marx_set_time (marx.c:285-373 -> util.c:231-236).
+ * The start enters the exposure cut of sources whose photons carry their own
+ * times (source.c:295-309) and the ACIS contamination's time interpolation
+ * (aciscontam.c:236-269). */
 double _Marx_TStart_MJDsecs = 0.0;
+double _Marx_TStart_Yrs = 0.0;
+
+int marx_set_time (double tyrs, double mjdsecs)
+{
+   _Marx_TStart_Yrs = tyrs;
+   _Marx_TStart_MJDsecs = mjdsecs;
+   return 0;
+}
 
 static char Ray_File[1024];
 

@@ -106,6 +106,7 @@ char *mxg_caldb_file (const char *object)
           {"LETGEFF", "caldb/letgD1996-11-01greffpr001N0007.fits"},
           {"ACISSUBPIX", "caldb/acisD1999-07-22subpixN0001_marx.fits"},
           {"ACISFEF", "caldb/acisfef.fits"},
+          {"ACISCONTAM", "caldb/acisD1999-08-13contamN0015_marx.fits"},
           {NULL, NULL}
      };
    unsigned int i;

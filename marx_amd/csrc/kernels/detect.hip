@@ -303,6 +303,50 @@ __device__ __noinline__ bool apply_fef (const mxg_detector_cfg &d, const float *
 // read, so that the common case keeps the registers of the kernel without
 // it); W: waves per SIMD to leave room for (0 unbounded: ACIS and HRC both
 // measured slower bounded)
+// ACIS contamination: the transmission exp(-sum_i mu_i(E) (tau0_i + tau1_i
+// f(x, y))) at the chip pixel (cx, cy) (aciscontam.c:93-136), f the chip's
+// table or one of the two analytic models (aciscontam.c:141-185).  Out of
+// line: only runs with a contamination file.
+__device__ __noinline__ double contam_transmission (const mxg_contam &c, const float *__restrict__ fa,
+                                                    double energy, double cx, double cy)
+{
+   const float ef = (float) energy;
+   double v = 0.0;
+   if (c.fxy_model == MXG_CONTAM_TABLE)
+     {
+        if ((cx < 0) || (cx >= 1024) || (cy < 0) || (cy >= 1024))
+          return 0.0;
+        cx /= c.blocking;
+        cy /= c.blocking;
+        const unsigned int ofs = (1024u / c.blocking) * (unsigned int) cy + (unsigned int) cx;
+        for (unsigned int i = 0; i < c.num_layers; i++)
+          {
+             double mu = interp_f (ef, fa + c.mu_e_off[i], fa + c.mu_off[i], c.mu_n[i]);
+             double fxy = fa[c.fxy_off[i] + ofs];
+             v += mu * (c.tau0[i] + c.tau1[i] * fxy);
+          }
+        return exp (-v);
+     }
+   double fxy;
+   if (c.fxy_model == MXG_CONTAM_RADIAL)
+     {
+        double dx = cx - c.x0, dy = cy - c.y0;
+        double r = (8.0 / 1024.0) * sqrt (dx * dx + dy * dy);
+        r /= 8.07;
+        fxy = 1.29 * r * r;
+     }
+   else if (cy <= 512.0)
+     fxy = pow (fabs ((cy - 512.0) / (64.0 - 512.0)), 5.5);
+   else
+     fxy = pow (fabs ((cy - 512.0) / (964.0 - 512.0)), 4.5);
+   for (unsigned int i = 0; i < c.num_layers; i++)
+     {
+        double mu = interp_f (ef, fa + c.mu_e_off[i], fa + c.mu_off[i], c.mu_n[i]);
+        v += mu * (c.tau0[i] + c.tau1[i] * fxy);
+     }
+   return exp (-v);
+}
+
 template <bool FEF, int W>
 __global__ __launch_bounds__ (kBlock, W > 0 ? W : 1)
 void k_detect (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
@@ -389,10 +433,14 @@ void k_detect (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
                             if (d.ideal == 0)
                               {
                                  double r = uniform (st);
-                                 double qe = 1.0;
+                                 double qe = 1.0, qe_contam = 1.0;
                                  if (g.qe_n != 0)
                                    qe = interp_f ((float) energy, fa + g.qe_e_off, fa + g.qe_off, g.qe_n);
-                                 if (r >= qe * 1.0 * 1.0)
+                                 if (d.contam_on && d.contam[k].num_layers)
+                                   qe_contam = contam_transmission (d.contam[k], fa, energy, ypix, zpix);
+                                 // the filter table is empty on the CALDB QE path
+                                 // (acis-s.c:303-304): filter transmission 1
+                                 if (r >= qe * 1.0 * qe_contam)
                                    {
                                       flags |= PHOTON_UNDETECTED;
                                       detected = false;

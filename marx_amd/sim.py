@@ -124,6 +124,21 @@ class Photons:
             pass
 
 
+def set_time(lib, tstart):
+    """marx_set_time from a TStart value as marx.c:285-373 reads it: years
+    below 2100 (seconds from MJDref = 1998.0 of 365.25-day years), else
+    seconds from MJDref; refused below 1999."""
+    secs_per_year = 365.25 * 24.0 * 3600.0
+    if tstart < 1999:
+        raise ValueError("A tstart value less than 1999 is not permitted")
+    if tstart < 2100:
+        yrs, secs = tstart, (tstart - 1998.0) * secs_per_year
+    else:
+        yrs, secs = 1998.0 + tstart / secs_per_year, tstart
+    check(lib.marx_set_time(yrs, secs), "marx_set_time")
+    return yrs, secs
+
+
 class Tracer:
     """Initialises the path's modules from a parameter file and runs them."""
 
@@ -131,6 +146,7 @@ class Tracer:
         self.lib = _lib.load()
         self.par = par if par is not None else ParamFile(**overrides)
         self.lib.mxg_set_seed(int(seed))
+        set_time(self.lib, self.par.get_double("TStart"))
         self.mirror = check(self.lib.marx_mirror_init(self.par.handle), "marx_mirror_init")
         self.grating = check(self.lib.marx_grating_init(self.par.handle), "marx_grating_init")
         self.detector = check(self.lib.marx_detector_init(self.par.handle), "marx_detector_init")

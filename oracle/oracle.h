@@ -135,6 +135,11 @@ int orc_create_batch (const orc_ctx *c, Marx_Photon_Type *pt, unsigned int nsrc,
                       const double *dirs, const unsigned int *counts, unsigned int tag_start,
                       double start_time, double exposure);
 int orc_mirror_reflect (const orc_ctx *c, Marx_Photon_Type *pt);
+/* Test tap of orc_mirror_reflect: taps[18 i + 6 k + (0..5)] receives the ray
+ * (x, then p) of the photon at attribute index i < n as it reaches strut set
+ * k (0 pre-, 1 cap, 2 post-collimator), whether or not struts are on.
+ * NULL switches the tap off. */
+void orc_set_strut_taps (double *taps, unsigned int n);
 int orc_grating_diffract (const orc_ctx *c, Marx_Photon_Type *pt);
 int orc_detect (const orc_ctx *c, Marx_Photon_Type *pt);
 

@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# MARX_ORACLE_LIB names another build of the oracle (a planted-mistake
+# build of oracle/mutants.py); the default is the in-tree build.
+LIB_PATH = os.environ.get("MARX_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 
 class OrcCtx(ctypes.Structure):
@@ -95,6 +97,8 @@ def load():
                                      f32p, u32p, u32p, i32p, i32p, f64p, f64p, f64p, f64p,
                                      ctypes.POINTER(ctypes.c_short)]
     lib.orc_event_coords.restype = ctypes.c_int
+    lib.orc_set_strut_taps.argtypes = [f64p, ctypes.c_uint]
+    lib.orc_set_strut_taps.restype = None
     lib.orc_subpix.argtypes = [f32p, ctypes.c_uint, ctypes.c_float, f32p, f32p]
     lib.orc_subpix.restype = ctypes.c_int
     _lib = lib
@@ -194,6 +198,18 @@ class Oracle:
 
     def reflect(self, photons):
         self._run(self.lib.orc_mirror_reflect, photons)
+
+    def reflect_with_strut_taps(self, photons):
+        """orc_mirror_reflect, returning the ray (x, p) of each photon
+        (attribute order) as it reached the pre-, cap and post-collimator
+        strut sets: array (capacity, 3, 6), NaN where it did not reach one."""
+        taps = np.full((photons.capacity, 3, 6), np.nan)
+        self.lib.orc_set_strut_taps(_arr(taps, ctypes.c_double), photons.capacity)
+        try:
+            self._run(self.lib.orc_mirror_reflect, photons)
+        finally:
+            self.lib.orc_set_strut_taps(None, 0)
+        return taps
 
     def dither(self, photons):
         """The dither of a replayed ray file applied to photons in place."""

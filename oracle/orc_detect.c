@@ -5,14 +5,17 @@
  *   marx/libsrc/trans.c:40-88         detector transform and its inverse
  *   marx/libsrc/acis-s.c:138-248      ACIS-S detect, QE draw
  *   marx/libsrc/acis-i.c:56-167       ACIS-I detect, frame-transfer streak
+ *   marx/libsrc/aciscontam.c:93-185   ACIS contamination transmission
  *   marx/libsrc/hrc-s.c:136-312       HRC-S detect, UV/ion-shield regions
  *   marx/libsrc/hrc-i.c:66-185        HRC-I detect, HRC pulse height
  *   marx/libsrc/hrcblur.c:239-297     MCP position blur
  *   marx/libsrc/hrc_s_geom.c:344-394, hrc_i_geom.c:146-156  pixel mapping
  *   marx/libsrc/drake.c:268-369       HESF pass before HRC-S detection
  *   marx/libsrc/reflect.c:79-91       marx_interp_reflectivity
- * ACIS contamination is taken as 1 and no FEF pulse height is applied
- * (their calibration files are absent from the reference's data tree).
+ *   marx/libsrc/acis_fef.c:380-579    ACIS FEF pulse height and PI
+ * Contamination and FEF apply when the set-up read their files (the CALDB
+ * ones are absent from the reference's data tree; tests write synthetic
+ * files).
  */
 #include <stdlib.h>
 #include <math.h>
@@ -285,6 +288,55 @@ static int apply_fef (const orc_ctx *c, const mxg_detector_cfg *d, int ccd, floa
    return 0;
 }
 
+/* aciscontam.c:141-185: the analytic spatial profiles */
+static double fxy_acis_i (double x, double y, double x_0, double y_0)
+{
+   double dx = x - x_0;
+   double dy = y - y_0;
+   double r = (8.0 / 1024.0) * sqrt (dx * dx + dy * dy);
+   r /= 8.07;
+   return 1.29 * r * r;
+}
+
+static double fxy_acis_s (double x, double y)
+{
+   double y_0 = 512.0;
+   (void) x;
+   if (y <= 512.0)
+     return pow (fabs ((y - y_0) / (64.0 - y_0)), 5.5);
+   return pow (fabs ((y - y_0) / (964.0 - y_0)), 4.5);
+}
+
+/* aciscontam.c:93-136: the transmission of chip slot k's contamination at
+ * chip pixel (cx, cy) */
+static double compute_contamination (const orc_ctx *c, const mxg_contam *ct, double en, double cx,
+                                     double cy)
+{
+   double v = 0.0, fxy = 0.0;
+   unsigned int i, ofs = 0;
+   if (ct->fxy_model == MXG_CONTAM_TABLE)
+     {
+        if ((cx < 0) || (cx >= 1024) || (cy < 0) || (cy >= 1024))
+          return 0.0;
+        cx /= ct->blocking;
+        cy /= ct->blocking;
+        ofs = (1024 / ct->blocking) * (unsigned int) cy + (unsigned int) cx;
+     }
+   else if (ct->fxy_model == MXG_CONTAM_RADIAL)
+     fxy = fxy_acis_i (cx, cy, ct->x0, ct->y0);
+   else
+     fxy = fxy_acis_s (cx, cy);
+   for (i = 0; i < ct->num_layers; i++)
+     {
+        double mu = orc_interpolate_f ((float) en, c->fa + ct->mu_e_off[i], c->fa + ct->mu_off[i],
+                                       ct->mu_n[i]);
+        if (ct->fxy_model == MXG_CONTAM_TABLE)
+          fxy = c->fa[ct->fxy_off[i] + ofs];
+        v += mu * (ct->tau0[i] + ct->tau1[i] * fxy);
+     }
+   return exp (-v);
+}
+
 static void acis_photon (const orc_ctx *c, Marx_Photon_Attr_Type *at, double tstart, orc_stream *st)
 {
    const mxg_detector_cfg *d = &c->cfg->det;
@@ -310,6 +362,9 @@ static void acis_photon (const orc_ctx *c, Marx_Photon_Attr_Type *at, double tst
              double qe = 1.0, qe_filter = 1.0, qe_contam = 1.0;
              if (g->qe_n != 0)
                qe = orc_interpolate_f ((float) at->energy, c->fa + g->qe_e_off, c->fa + g->qe_off, g->qe_n);
+             if (d->contam_on && d->contam[g - d->chip].num_layers)
+               qe_contam = compute_contamination (c, d->contam + (g - d->chip), at->energy, at->y_pixel,
+                                                  at->z_pixel);
              if (r >= qe * qe_filter * qe_contam)
                {
                   at->flags |= PHOTON_UNDETECTED;

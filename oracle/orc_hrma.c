@@ -48,6 +48,32 @@ int orc_strut_shadow (const mxg_hrma_cfg *h, orc_vec xv, orc_vec pv, const mxg_s
    return 0;
 }
 
+/* Test tap: when set, the ray (x, p) that reaches each strut set (0 pre-,
+ * 1 cap, 2 post-collimator) is recorded per photon, whether or not struts
+ * are on, at taps[(18 * attribute index) + 6 * set + (x, p)]; sets a photon
+ * does not reach stay as the caller filled them. */
+static double *Strut_Taps = NULL;
+static unsigned int Strut_Taps_N = 0;
+static const Marx_Photon_Attr_Type *Strut_Taps_Base = NULL;
+
+void orc_set_strut_taps (double *taps, unsigned int n)
+{
+   Strut_Taps = taps;
+   Strut_Taps_N = n;
+}
+
+static void tap_strut_ray (const Marx_Photon_Attr_Type *at, unsigned int set)
+{
+   size_t i;
+   double *t;
+   if ((Strut_Taps == NULL) || (Strut_Taps_Base == NULL)) return;
+   i = (size_t) (at - Strut_Taps_Base);
+   if (i >= Strut_Taps_N) return;
+   t = Strut_Taps + 18 * i + 6 * set;
+   t[0] = at->x.x; t[1] = at->x.y; t[2] = at->x.z;
+   t[3] = at->p.x; t[4] = at->p.y; t[5] = at->p.z;
+}
+
 static int intersects_struts (const mxg_hrma_cfg *h, Marx_Photon_Attr_Type *at,
                               const mxg_strut *s, unsigned int num)
 {
@@ -89,6 +115,7 @@ static int project_photon_to_hrma (const mxg_hrma_cfg *h, Marx_Photon_Attr_Type 
                   at->p = orc_v_ax1_bx2 (1.0, at->x, source_distance, at->p);
                   orc_v_normalize (&at->p);
                }
+             tap_strut_ray (at, 0);
              if (h->use_struts && (1 == intersects_struts (h, at, h->pre_struts, 2)))
                return -1;
              return 0;
@@ -227,6 +254,7 @@ int orc_mirror_reflect (const orc_ctx *c, Marx_Photon_Type *pt)
 
    if (NULL == (streams = orc_streams_alloc (pt->n_photons)))
      return -1;
+   Strut_Taps_Base = attrs;
 
    orc_prune_photons (pt);
    n = pt->num_sorted;
@@ -298,6 +326,7 @@ int orc_mirror_reflect (const orc_ctx *c, Marx_Photon_Type *pt)
         at->x = orc_m3_vmul (s->bwd[0], at->x);
         at->x = orc_v_diff (at->x, orc_v (s->to_osac[0][0], s->to_osac[0][1], s->to_osac[0][2]));
 
+        tap_strut_ray (at, 1);
         if (h->use_struts && (1 == intersects_struts (h, at, h->cap_struts, 2)))
           continue;
 
@@ -316,11 +345,13 @@ int orc_mirror_reflect (const orc_ctx *c, Marx_Photon_Type *pt)
         at->x = orc_m3_vmul (s->bwd[1], at->x);
         at->x = orc_v_diff (at->x, orc_v (s->to_osac[1][0], s->to_osac[1][1], s->to_osac[1][2]));
 
+        tap_strut_ray (at, 2);
         if (h->use_struts)
           (void) intersects_struts (h, at, h->post_struts, 2);
      }
 
    ORC_PHASE (0);
+   Strut_Taps_Base = NULL;
    free (streams);
    return 0;
 }

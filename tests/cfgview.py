@@ -65,6 +65,12 @@ class Rect(ctypes.Structure):
     _fields_ = [("a", D * 3), ("e1", D * 3), ("e2", D * 3), ("normal", D * 3), ("len1", D), ("len2", D)]
 
 
+class Contam(ctypes.Structure):
+    _fields_ = [("num_layers", U), ("fxy_model", U), ("blocking", U), ("pad", U), ("x0", D), ("y0", D),
+                ("tau0", D * 5), ("tau1", D * 5), ("mu_n", U * 5), ("mu_e_off", U * 5), ("mu_off", U * 5),
+                ("fxy_off", U * 5)]
+
+
 class DetectorCfg(ctypes.Structure):
     _fields_ = [("type", I), ("ideal", I), ("extend", I), ("num_chips", I), ("matrix", D * 9),
                 ("offset", D * 3), ("focal_length", D), ("chip", Chip * MAX_CHIPS),
@@ -82,7 +88,7 @@ class DetectorCfg(ctypes.Structure):
                 ("drake_c_n", U), ("drake_c_e_off", U), ("drake_c_beta_off", U), ("drake_c_delta_off", U),
                 ("drake_cr_n", U), ("drake_cr_e_off", U), ("drake_cr_beta_off", U),
                 ("drake_cr_delta_off", U), ("fef_on", I), ("fef_max_gaussians", I), ("fef_map_off", U),
-                ("fef_desc_off", U)]
+                ("fef_desc_off", U), ("contam_on", I), ("contam_pad", I), ("contam", Contam * MAX_CHIPS)]
 
 
 class SourceCfg(ctypes.Structure):
@@ -107,7 +113,7 @@ from marx_amd._lib import DiffCfg, DiffShell, EventCfg, EventColumns, ImageSpec 
 
 STRUCTS = {"mxg_event_cfg": EventCfg, "mxg_event_columns": EventColumns, "mxg_diff_shell": DiffShell, "mxg_diff_cfg": DiffCfg, "mxg_image_spec": ImageSpec,"mxg_wfold": Wfold, "mxg_shell_cfg": ShellCfg, "mxg_strut": Strut, "mxg_hrma_cfg": HrmaCfg,
            "mxg_grating": Grating, "mxg_sectors": Sectors, "mxg_grating_cfg": GratingCfg,
-           "mxg_chip": Chip, "mxg_qe_table": QeTable, "mxg_rect": Rect,
+           "mxg_chip": Chip, "mxg_qe_table": QeTable, "mxg_contam": Contam, "mxg_rect": Rect,
            "mxg_detector_cfg": DetectorCfg, "mxg_source_cfg": SourceCfg, "mxg_config": Config}
 
 

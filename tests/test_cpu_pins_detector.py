@@ -1,0 +1,279 @@
+"""Pins of the detector set-up and of the oracle's detector pass from the
+CALDB files and the reference's documents, not from the library's set-up
+code (CPU only):
+
+- ACIS chip corners (marx/libsrc/caldb.c:256-305, 404-491): each chip's
+  LL, LR and UL corners are the GEOM file's INSTGEOM (INSTRUME = ACIS) row
+  plus the AIMPTS file's AS1 / AI2 aimpoint plus the GEOM file's ACIS
+  origin, decoded here with a FITS reader of the test's own (tests/fitstab);
+  pixel sizes are the chip lengths over 1024 (acis_geom.c:54-67);
+- the on-axis image lands where marx/doc/aimpoint.txt:42-67 puts the HRMA
+  optical axis: S3 CHIPX 220.7 CHIPY 531.8, I3 984.4 / 994.8, HRC-I
+  7529.9 / 7745.0, HRC-S 2201.0 / 8976.5 (a chip moved by 0.5 mm misses by
+  20 ACIS or 78 HRC pixels);
+- the frame-transfer streak (acis-i.c:60-92): streaked photons keep their
+  column, take a row uniform on [1, 1023), lie on the chip plane at the new
+  pixel and travel from the mirror node (Marx_Focal_Length, 0, 0);
+- HRC QE (hrc-i.c:87-116, hrc-s.c:136-234): the detected fraction of
+  photons of one energy reaching an MCP is the MCP QE times the UV/ion
+  shield transmission of their shield region, both interpolated from the
+  data files by the test's own bdat reader; on HRC-S each region of the
+  shield drawing (hrc-s.c:112-134) is aimed at separately;
+- HRC PHA (hrc-i.c:66-85): (short) (f(E) (1 + 0.424661 g)), f = 141.582
+  sqrt(E) below 0.5 keV, 107.299 E^0.1 below 2 keV, else 115, clipped at 0
+  (chi-square over the integer bins).
+"""
+import os
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from marx_amd import Photons, Tracer, on_axis_photons
+from marx_amd._lib import ATTR_DTYPE, DATA_DIR
+from oracle.oracle import Oracle
+from cfgview import ConfigView
+from fitstab import read_tables
+from test_cpu_support_gratings import _bdat
+
+CALDB = os.path.join(DATA_DIR, "caldb")
+STREAKED, MISSED = 0x200, 0x08
+
+
+def _table(path, extname, instrume=None):
+    for h, c in read_tables(path):
+        if h.get("EXTNAME") == extname and (instrume is None or h.get("INSTRUME") == instrume):
+            return c
+    raise KeyError(extname)
+
+
+@pytest.mark.parametrize("det,aimpt", [("ACIS-S", "AS1"), ("ACIS-I", "AI2")])
+def test_acis_chip_corners_are_the_caldb_geometry(det, aimpt):
+    geom = os.path.join(CALDB, "telD1999-07-23geomN0006.fits")
+    aim = os.path.join(CALDB, "telD1999-07-23aimptsN0002.fits")
+    chips = _table(geom, "INSTGEOM", "ACIS")
+    inst = _table(geom, "INSTRUMENTS")
+    pts = _table(aim, "AIMPOINTS")
+    origin = inst["ORIGIN"][list(inst["INSTRUMENT"]).index("ACIS")]
+    ofs = pts["AIMPOINT"][list(pts["AIMPOINT_NAME"]).index(aimpt)] + origin
+    t = Tracer(GratingType="NONE", DetectorType=det)
+    d = ConfigView(t).cfg.det
+    ids = list(range(0, 4) if det == "ACIS-I" else range(4, 10))
+    assert [d.chip[k].id for k in range(d.num_chips)] == ids
+    for k in range(d.num_chips):
+        g = d.chip[k]
+        row = list(chips["CHIP_ID"]).index(g.id)
+        ll, lr, ul = (chips[c][row] + ofs for c in ("LL", "LR", "UL"))
+        x_ll, xhat, yhat = np.array(g.x_ll), np.array(g.xhat), np.array(g.yhat)
+        np.testing.assert_allclose(x_ll, ll, atol=1e-9, rtol=0)
+        np.testing.assert_allclose(x_ll + g.xlen * xhat, lr, atol=1e-9, rtol=0)
+        np.testing.assert_allclose(x_ll + g.ylen * yhat, ul, atol=1e-9, rtol=0)
+        assert g.x_pixel_size == pytest.approx(np.linalg.norm(lr - ll) / 1024.0, rel=1e-12)
+        assert g.y_pixel_size == pytest.approx(np.linalg.norm(ul - ll) / 1024.0, rel=1e-12)
+
+
+# marx/doc/aimpoint.txt:42-67: the HRMA optical axis intercepts
+@pytest.mark.parametrize("det,chip,cx,cy,tol", [("ACIS-S", 7, 220.7, 531.8, 1.5), ("ACIS-I", 3, 984.4, 994.8, 1.5),
+                                                ("HRC-I", 0, 7529.9, 7745.0, 4.0),
+                                                ("HRC-S", 2, 2201.0, 8976.5, 4.0)])
+def test_on_axis_image_lands_on_the_documented_aimpoint(det, chip, cx, cy, tol):
+    par = dict(GratingType="NONE", DetectorType=det, DitherModel="NONE", HRMA_Use_WFold="no",
+               HRMA_Use_Blur="no")
+    if det.startswith("HRC"):
+        par["HRC-HESF"] = "no"
+    t = Tracer(seed=3, **par)
+    n = 20000
+    ph = Photons(n)
+    ph.load(on_axis_photons(n, 1.49))
+    o = Oracle(t)
+    o.reflect(ph)
+    o.detect(ph)
+    a = ph.attrs[:n]
+    ok = (a["flags"] & 0xFF) == 0
+    assert ok.sum() > 1000
+    assert set(np.unique(a["ccd_num"][ok])) == {chip}
+    # chip pixels count from 0 at the chip's edge; CHIPX / CHIPY from 0.5
+    # (ACIS) -- within the tolerance either way
+    my, mz = np.median(a["y_pixel"][ok]), np.median(a["z_pixel"][ok])
+    assert abs(my - cx) < tol and abs(mz - cy) < tol, (my, mz)
+
+
+def test_streaked_photons_keep_their_column():
+    base = dict(GratingType="NONE", DetectorType="ACIS-I", DitherModel="NONE", DetIdeal="yes")
+    n = 60000
+    rec = on_axis_photons(n, 1.49, time_span=4.0e4)
+    runs = []
+    for ftt in (0.041, 0.0):
+        t = Tracer(seed=29, ACIS_Frame_Transfer_Time=ftt, **base)
+        ph = Photons(n)
+        ph.load(rec)
+        o = Oracle(t)
+        o.reflect(ph)
+        o.detect(ph)
+        # the configuration is the library's current one: copy it now
+        d = ConfigView(t).cfg.det
+        geo = {d.chip[k].id: (np.array(d.chip[k].x_ll), np.array(d.chip[k].xhat), np.array(d.chip[k].yhat),
+                              d.chip[k].x_pixel_size, d.chip[k].y_pixel_size, d.chip[k].xpixel_offset,
+                              d.chip[k].ypixel_offset) for k in range(d.num_chips)}
+        runs.append((ph.attrs[:n].copy(), geo, np.array(d.offset), d.frame_time))
+    (a, chip, offset, frame), (b, _, _, _) = runs
+    st = (a["flags"] & STREAKED) != 0
+    ok = ((a["flags"] & 0xFF) == 0) & ((b["flags"] & 0xFF) == 0)
+    assert not (st & ~ok).any()
+    assert st.sum() > 100
+    # outside the streak both runs are the same photon
+    same = ok & ~st
+    np.testing.assert_array_equal(a["z_pixel"][same], b["z_pixel"][same])
+    np.testing.assert_array_equal(a["y_pixel"][st], b["y_pixel"][st])
+    np.testing.assert_array_equal(a["ccd_num"][st], b["ccd_num"][st])
+    z = a["z_pixel"][st].astype(np.float64)
+    assert stats.kstest(z, stats.uniform(1.0, 1022.0).cdf).pvalue > 1e-3
+    # the new position is the chip pixel on the chip plane, the direction
+    # points away from the mirror node
+    for i in np.nonzero(st)[0][:500]:
+        x_ll, xhat, yhat, xps, yps, xoff, yoff = chip[int(a["ccd_num"][i])]
+        x = (x_ll + (float(a["y_pixel"][i]) - xoff) * xps * xhat + (float(a["z_pixel"][i]) - yoff) * yps * yhat)
+        x = x + offset
+        np.testing.assert_allclose(a["x"][i], x, atol=1e-6)
+        v = a["x"][i] - np.array([10065.5, 0.0, 0.0])
+        np.testing.assert_allclose(a["p"][i], v / np.linalg.norm(v), atol=1e-12)
+    # the streaked share of uniform arrival times: transfer / frame
+    assert frame == pytest.approx(3.2 + 0.041)
+    share = 0.041 / frame
+    m = ok.sum()
+    assert abs(st.sum() - m * share) < 5 * np.sqrt(m * share * (1 - share))
+
+
+def _rays_to(targets, energy, x0=10070.0, tag0=0):
+    """Records travelling from (x0, 0, 0) to the points (0, y, z)."""
+    n = len(targets)
+    rec = np.zeros(n, dtype=ATTR_DTYPE)
+    rec["energy"] = energy
+    rec["x"][:, 0] = x0
+    v = np.stack([-x0 * np.ones(n), targets[:, 0], targets[:, 1]], axis=1)
+    rec["p"] = v / np.linalg.norm(v, axis=1)[:, None]
+    rec["tag"] = np.arange(tag0, tag0 + n)
+    rec["arrival_time"] = np.linspace(0.0, 1.0e3, n)
+    return rec
+
+
+def _detect(t, rec):
+    p = Photons(len(rec))
+    p.load(rec)
+    p.pt.history = 0x1 | 0x2 | 0x4 | 0x8 | 0x10 | 0x20000    # energy, time, x, p, tag, mirror
+    Oracle(t).detect(p)
+    a = p.active
+    return a[np.argsort(a["tag"])]
+
+
+def _qe(name, energy):
+    tab = _bdat(os.path.join(DATA_DIR, "hrc", name))
+    return float(np.interp(np.float32(energy), tab[:, 0], tab[:, 1]))
+
+
+def _binomial_ok(k, n, p):
+    return abs(k - n * p) < 5.0 * np.sqrt(n * p * (1 - p))
+
+
+def test_hrc_i_detects_mcp_qe_times_uvis():
+    e = 0.2
+    t = Tracer(seed=31, GratingType="NONE", DetectorType="HRC-I", DitherModel="NONE")
+    n = 40000
+    rng = np.random.default_rng(1)
+    a = _detect(t, _rays_to(rng.uniform(-3, 3, (n, 2)), e))
+    hit = (a["flags"] & MISSED) == 0
+    det = (a["flags"] & 0xFF) == 0
+    want = _qe("mcp_qe_i.v2.2.dat", e) * _qe("uvis_i.v4.dat", e)
+    assert hit.sum() > 0.99 * n
+    assert _binomial_ok(det.sum(), hit.sum(), want), (det.sum() / hit.sum(), want)
+    assert not _binomial_ok(det.sum(), hit.sum(), _qe("mcp_qe_i.v2.2.dat", e))   # without the UVIS draw
+
+
+def _shield():
+    vals = {}
+    for line in open(os.path.join(DATA_DIR, "hrc", "hrc_s_geom.txt")):
+        f = line.split()
+        if f and not f[0].startswith("#") and f[0].startswith("Ion_Shield_"):
+            vals[f[0][len("Ion_Shield_"):]] = float(f[1])
+    return vals
+
+
+def test_hrc_s_detects_mcp_qe_times_the_region_transmission():
+    """The shield drawing of hrc-s.c:112-134, with the distances of
+    hrc_s_geom.txt, about the centre of the middle MCP: region 0 within L
+    (R on +y) of the centre and above T beyond that up to SL (SR), region 1
+    below T there, regions 2 (above T) and 3 (below T) past SL + GapL
+    (SR + GapR).  Photons are aimed at points well inside each region."""
+    e = 0.2
+    s = _shield()
+    t = Tracer(seed=37, GratingType="NONE", DetectorType="HRC-S", DitherModel="NONE", **{"HRC-HESF": "no"})
+    d = ConfigView(t).cfg.det
+    mid = [d.chip[k] for k in range(d.num_chips) if d.chip[k].id == 2][0]
+    ll = np.array(mid.x_ll)
+    ur = ll + mid.xlen * np.array(mid.xhat) + mid.ylen * np.array(mid.yhat)
+    cy, cz = 0.5 * (ll + ur)[1:]
+    T = s["T"]
+    cases = [  # (y, z) from the centre, region, UVIS file
+        ((-0.5 * s["L"], T - 3.0), 0, "uvis_s1.v4.dat"),
+        ((0.5 * s["R"], T + 3.0), 0, "uvis_s1.v4.dat"),
+        ((-0.5 * (s["L"] + s["SL"]), T + 3.0), 0, "uvis_s1.v4.dat"),
+        ((-0.5 * (s["L"] + s["SL"]), T - 3.0), 1, "uvis_s2.v4.dat"),
+        ((0.5 * (s["R"] + s["SR"]), T - 3.0), 1, "uvis_s2.v4.dat"),
+        ((-(s["SL"] + s["GapL"] + 30.0), T + 3.0), 2, "uvis_s56.v4.dat"),
+        ((s["SR"] + s["GapR"] + 30.0, T + 3.0), 2, "uvis_s56.v4.dat"),
+        ((-(s["SL"] + s["GapL"] + 30.0), T - 3.0), 3, "uvis_s34.v4.dat"),
+        ((s["SR"] + s["GapR"] + 30.0, T - 3.0), 3, "uvis_s34.v4.dat"),
+    ]
+    n = 20000
+    rng = np.random.default_rng(2)
+    mcp = _qe("mcp_qe_s.v3.dat", e)
+    for k, ((y, z), region, uvis) in enumerate(cases):
+        tg = np.stack([cy + y + rng.uniform(-0.5, 0.5, n), cz + z + rng.uniform(-0.5, 0.5, n)], axis=1)
+        a = _detect(t, _rays_to(tg, e, tag0=k * n))
+        hit = (a["flags"] & MISSED) == 0
+        det = (a["flags"] & 0xFF) == 0
+        assert hit.sum() > 0.99 * n, (k, hit.sum())
+        assert set(np.unique(a["detector_region"][det])) == {region}, k
+        want = mcp * _qe(uvis, e)
+        assert _binomial_ok(det.sum(), hit.sum(), want), (k, det.sum() / hit.sum(), want)
+
+
+def _pha_law(energy):
+    if energy <= 0.5:
+        f = 141.582 * np.sqrt(energy)
+    elif energy < 2.0:
+        f = 107.299 * energy ** 0.1
+    else:
+        f = 115.0
+    return f, 0.424661 * f
+
+
+@pytest.mark.parametrize("energy", [0.3, 1.2, 4.0])
+def test_hrc_pha_follows_its_gain_law(energy):
+    t = Tracer(seed=41, GratingType="NONE", DetectorType="HRC-I", DitherModel="NONE", DetIdeal="yes")
+    n = 40000
+    rng = np.random.default_rng(5)
+    a = _detect(t, _rays_to(rng.uniform(-3, 3, (n, 2)), energy))
+    det = (a["flags"] & 0xFF) == 0
+    pha = a["pulse_height"][det].astype(int)
+    f, w = _pha_law(energy)
+    kmax = int(f + 6 * w) + 1
+    edges = np.arange(0, kmax + 1)
+    cdf = stats.norm.cdf((edges - f) / w)
+    prob = np.diff(cdf)
+    prob[0] += cdf[0]                       # values below 0 are clipped to 0
+    obs = np.bincount(np.minimum(pha, kmax - 1), minlength=kmax)
+    # pool into bins of >= 20 expected counts
+    m = det.sum()
+    exp = prob * m
+    groups, cur_o, cur_e = [], 0.0, 0.0
+    for o_, e_ in zip(obs, exp):
+        cur_o += o_
+        cur_e += e_
+        if cur_e >= 20:
+            groups.append((cur_o, cur_e))
+            cur_o = cur_e = 0.0
+    go, ge = np.array(groups).T
+    ge *= go.sum() / ge.sum()
+    chi2 = ((go - ge) ** 2 / ge).sum()
+    assert stats.chi2.sf(chi2, len(go) - 1) > 1e-4
