@@ -491,11 +491,23 @@ def measure_host_aos(args, tmp, torch):
     t = hetg_acis_s(tmp, photons=N, exposure=args.exposure)
     ph = Photons(N)
 
-    def step():
+    calls = {"create": 0.0, "reflect": 0.0, "diffract": 0.0, "detect": 0.0}
+
+    def step(timed=False):
+        # every call returns with the caller's records valid (each one
+        # synchronises), so host clocks between them time each call
+        c0 = time.perf_counter()
         made = t.create_photons(ph, N)
+        c1 = time.perf_counter()
         t.reflect(ph)
+        c2 = time.perf_counter()
         t.diffract(ph)
+        c3 = time.perf_counter()
         t.detect(ph)
+        c4 = time.perf_counter()
+        if timed:
+            for k, d in zip(calls, (c1 - c0, c2 - c1, c3 - c2, c4 - c3)):
+                calls[k] += d
         return made
 
     for _ in range(args.warmup):
@@ -504,11 +516,12 @@ def measure_host_aos(args, tmp, torch):
     t0 = time.perf_counter()
     made = 0
     for _ in range(K):
-        made += step()
+        made += step(timed=True)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / K
     made /= K
-    progress(f"{made:.0f} photons per chunk, {dt * 1e3:.1f} ms per chunk")
+    calls_ms = {k: v / K * 1e3 for k, v in calls.items()}
+    progress(f"{made:.0f} photons per chunk, {dt * 1e3:.1f} ms per chunk, calls {calls_ms}")
     return {
         "metric": "photons/s traced (HRMA+grating+detector) 1 GPU vs host CPU; gradient evals/s",
         "value": made / dt, "unit": "photons/s (host AoS in and out, PCIe included)", "n_gpus": 1,
@@ -516,6 +529,7 @@ def measure_host_aos(args, tmp, torch):
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": DESCRIPTION["hetg-acis-s"], "name": cfg, "photons_per_step": made,
                    "route": "marx_create_photons + marx_mirror_reflect + marx_grating_diffract + marx_detect"},
+        "calls_ms": calls_ms,
     }
 
 

@@ -68,13 +68,28 @@ static int sync_config (void)
 
 /* ---- photon lists (photon.c) ------------------------------------------------ */
 
+/* Zeroed host memory for a photon list's arrays: pinned when a GPU is
+ * there, so that the stages' copies run at the link's DMA rate straight
+ * into the caller's records, else plain. */
+static void *list_alloc (size_t n, size_t size)
+{
+   void *p = mxgd_host_alloc (n * size);
+   return (p != NULL) ? p : calloc (n, size);
+}
+
+static void list_free (void *p)
+{
+   if ((p != NULL) && !mxgd_host_free (p))
+     free (p);
+}
+
 Marx_Photon_Type *marx_alloc_photon_type (unsigned int num)
 {
    Marx_Photon_Type *pt = (Marx_Photon_Type *) calloc (1, sizeof (Marx_Photon_Type));
    if (pt == NULL) return NULL;
-   pt->attributes = (Marx_Photon_Attr_Type *) calloc (num ? num : 1, sizeof (Marx_Photon_Attr_Type));
-   pt->sorted_energies = (double *) calloc (num ? num : 1, sizeof (double));
-   pt->sorted_index = (unsigned int *) calloc (num ? num : 1, sizeof (unsigned int));
+   pt->attributes = (Marx_Photon_Attr_Type *) list_alloc (num ? num : 1, sizeof (Marx_Photon_Attr_Type));
+   pt->sorted_energies = (double *) list_alloc (num ? num : 1, sizeof (double));
+   pt->sorted_index = (unsigned int *) list_alloc (num ? num : 1, sizeof (unsigned int));
    if ((pt->attributes == NULL) || (pt->sorted_energies == NULL) || (pt->sorted_index == NULL))
      {
         marx_dealloc_photon_type (pt);
@@ -88,9 +103,9 @@ int marx_dealloc_photon_type (Marx_Photon_Type *pt)
 {
    if (pt == NULL) return -1;
    if (Resident == pt) Resident = NULL;
-   free (pt->attributes);
-   free (pt->sorted_index);
-   free (pt->sorted_energies);
+   list_free (pt->attributes);
+   list_free (pt->sorted_index);
+   list_free (pt->sorted_energies);
    free (pt);
    return 0;
 }
@@ -133,14 +148,20 @@ static int make_resident (Marx_Photon_Type *pt)
    return 0;
 }
 
+/* The records back into the caller's list, and the pruned sorted_index:
+ * the last stage's output list on the device is the input list (the
+ * sorted_index uploaded) less the photons it absorbed, in the same order,
+ * which is what marx_prune_photons would rebuild (photon.c:40-63). */
 static int copy_back (Marx_Photon_Type *pt)
 {
-   if (-1 == mxgd_download_aos (pt->attributes, pt->n_photons))
+   unsigned int n = 0;
+   if ((-1 == mxgd_download_aos (pt->attributes, pt->n_photons))
+       || (-1 == mxgd_download_active (pt->sorted_index, pt->sorted_energies, pt->max_n_photons, &n)))
      {
         mxg_error ("%s", mxgd_last_error ());
         return -1;
      }
-   marx_prune_photons (pt);
+   pt->num_sorted = n;
    return 0;
 }
 
@@ -554,18 +575,19 @@ int marx_create_photons (Marx_Source_Type *st, Marx_Photon_Type *pt, unsigned in
         mxg_error ("%s", mxgd_last_error ());
         return -1;
      }
-   if ((n > 0) && (-1 == mxgd_download_aos (pt->attributes, n)))
+   /* the records, and their energy order sorted on the device */
+   if ((n > 0) && ((-1 == mxgd_download_aos (pt->attributes, n))
+                   || (-1 == mxgd_sort_store_by_energy (n, pt->sorted_index, pt->sorted_energies))))
      {
         mxg_error ("%s", mxgd_last_error ());
         return -1;
      }
    Resident = pt;
    pt->n_photons = n;
+   pt->num_sorted = n;
    pt->history |= (MARX_ENERGY_OK | MARX_TIME_OK | MARX_X_VECTOR_OK | MARX_P_VECTOR_OK | MARX_TAG_OK);
    pt->tag_start += n;
    Resident_Tag_Start = pt->tag_start;
-   if (-1 == sort_by_energy (pt))
-     return -1;
    if (n == 0) pt->total_time = 0.0;
    else pt->total_time = (s->time_mode == MXG_TIME_UNIFORM) ? s->uniform_tmax : last_t;
    *num_collected = n;
@@ -966,7 +988,12 @@ int mxg_download_photons (Marx_Photon_Type *pt)
    pt->n_photons = n;
    pt->tag_start = Last_Tag_Start + n;
    pt->history = Last_History;
-   if (-1 == sort_by_energy (pt)) return -1;
+   if ((n > 0) && (-1 == mxgd_sort_store_by_energy (n, pt->sorted_index, pt->sorted_energies)))
+     {
+        mxg_error ("%s", mxgd_last_error ());
+        return -1;
+     }
+   pt->num_sorted = n;
    marx_prune_photons (pt);
    return 0;
 }

@@ -155,6 +155,17 @@ int mxgd_active_list (unsigned int *idx, unsigned int n);
 
 int mxgd_synchronize (void);
 
+/* Pinned host memory for photon lists (NULL without a GPU: use malloc);
+ * mxgd_host_free returns 1 when p was such an allocation (and frees it). */
+void *mxgd_host_alloc (size_t bytes);
+int mxgd_host_free (void *p);
+/* The store's first n photons in energy order with the index as tie-break:
+ * their indices and energies into host arrays.  Synchronises. */
+int mxgd_sort_store_by_energy (unsigned int n, unsigned int *index, double *energies);
+/* The active list (the last stage's survivors, in input order) and its
+ * energies into host arrays of cap entries; *n its length.  Synchronises. */
+int mxgd_download_active (unsigned int *index, double *energies, unsigned int cap, unsigned int *n);
+
 #ifdef __cplusplus
 }
 #endif

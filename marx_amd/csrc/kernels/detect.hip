@@ -347,15 +347,18 @@ __device__ __noinline__ double contam_transmission (const mxg_contam &c, const f
    return exp (-v);
 }
 
-template <bool FEF, int W>
+// One instance per detector family (KIND: MXG_DET_ACIS, MXG_DET_HRC_S,
+// MXG_DET_HRC_I), so that the ACIS pass does not hold the registers of the
+// HRC and HESF code; FEF: the ACIS instance with the FEF step.
+template <bool FEF, int KIND, int W>
 __global__ __launch_bounds__ (kBlock, W > 0 ? W : 1)
 void k_detect (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
                const double *__restrict__ da, Soa s, ListIO io, double tstart,
                unsigned long long seed, uint32_t source_id)
 {
    const mxg_detector_cfg &d = cfg->det;
-   const bool is_acis = (d.type == MARX_DETECTOR_ACIS_S) || (d.type == MARX_DETECTOR_ACIS_I);
-   const bool is_hrc_s = (d.type == MARX_DETECTOR_HRC_S);
+   constexpr bool is_acis = (KIND == MXG_DET_ACIS);
+   constexpr bool is_hrc_s = (KIND == MXG_DET_HRC_S);
    __shared__ uint32_t seg_wc[2 * (kBlock / 64)];
    const BlockRange range = block_range (*io.n_in, blockIdx.x, gridDim.x);
    SegWriter survivors (io, range, seg_wc);
@@ -423,7 +426,7 @@ void k_detect (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
                             flags |= PHOTON_MISSED_DETECTOR;
                             s.ccd[id] = -1;
                          }
-                       else if (is_acis)
+                       else if constexpr (is_acis)
                          {
                             const mxg_chip &g = d.chip[k];
                             s.ccd[id] = (int8_t) g.id;
@@ -593,15 +596,22 @@ void k_detect (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
 
 hipError_t launch_detect (const mxg_config *cfg, const float *fa, const double *da, Soa s,
                           ListIO io, uint32_t n_upper, double tstart, unsigned long long seed,
-                          uint32_t source_id, bool fef, hipStream_t st)
+                          uint32_t source_id, bool fef, int kind, hipStream_t st)
 {
    if (n_upper == 0) return hipSuccess;
-   if (fef)
-     hipLaunchKernelGGL ((k_detect<true, MXG_DETECT_WAVES>), dim3 (io.grid), dim3 (kBlock), 0, st, cfg, fa, da,
-                         s, io, tstart, seed, source_id);
+   const dim3 g (io.grid), b (kBlock);
+   if (kind == MXG_DET_HRC_S)
+     hipLaunchKernelGGL ((k_detect<false, MXG_DET_HRC_S, MXG_DETECT_WAVES>), g, b, 0, st, cfg, fa, da, s, io,
+                         tstart, seed, source_id);
+   else if (kind == MXG_DET_HRC_I)
+     hipLaunchKernelGGL ((k_detect<false, MXG_DET_HRC_I, MXG_DETECT_WAVES>), g, b, 0, st, cfg, fa, da, s, io,
+                         tstart, seed, source_id);
+   else if (fef)
+     hipLaunchKernelGGL ((k_detect<true, MXG_DET_ACIS, MXG_DETECT_WAVES>), g, b, 0, st, cfg, fa, da, s, io,
+                         tstart, seed, source_id);
    else
-     hipLaunchKernelGGL ((k_detect<false, MXG_DETECT_WAVES>), dim3 (io.grid), dim3 (kBlock), 0, st, cfg, fa, da,
-                         s, io, tstart, seed, source_id);
+     hipLaunchKernelGGL ((k_detect<false, MXG_DET_ACIS, MXG_ACIS_DETECT_WAVES>), g, b, 0, st, cfg, fa, da, s,
+                         io, tstart, seed, source_id);
    hipError_t e = hipGetLastError ();
    if (e == hipSuccess) e = launch_compact (io, st);
    return e;

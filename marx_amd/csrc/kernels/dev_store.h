@@ -174,6 +174,10 @@ hipError_t launch_pack_columns (const Soa &s, const uint32_t *idx, const uint32_
 #ifndef MXG_DETECT_WAVES
 #define MXG_DETECT_WAVES 0
 #endif
+// the ACIS instance of the detector kernel (without FEF): waves per SIMD
+#ifndef MXG_ACIS_DETECT_WAVES
+#define MXG_ACIS_DETECT_WAVES 0
+#endif
 
 // Device buffers and launch shape of the mirror stage's second kernel.
 struct HrmaScratch
@@ -197,10 +201,17 @@ hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const flo
 hipError_t launch_grating (const mxg_config *cfg, const float *fa, const double *da, Soa s,
                            ListIO io, uint32_t n_upper, unsigned long long seed,
                            uint32_t source_id, bool letg, hipStream_t st);
-// fef: run the variant with the ACIS FEF step (the configuration read a FEF file)
+// fef: run the variant with the ACIS FEF step (the configuration read a FEF
+// file); kind: the detector family (detector_kind of the configuration)
+enum { MXG_DET_ACIS = 0, MXG_DET_HRC_S = 1, MXG_DET_HRC_I = 2 };
+inline int detector_kind (const mxg_config &c)
+{
+   return (c.det.type == MARX_DETECTOR_HRC_S) ? MXG_DET_HRC_S
+     : (c.det.type == MARX_DETECTOR_HRC_I) ? MXG_DET_HRC_I : MXG_DET_ACIS;
+}
 hipError_t launch_detect (const mxg_config *cfg, const float *fa, const double *da, Soa s,
                           ListIO io, uint32_t n_upper, double tstart,
-                          unsigned long long seed, uint32_t source_id, bool fef, hipStream_t st);
+                          unsigned long long seed, uint32_t source_id, bool fef, int kind, hipStream_t st);
 
 hipError_t launch_dither_store (const mxg_config *cfg, Soa s, uint32_t n, double start_time,
                                 unsigned long long seed, hipStream_t st);
@@ -235,5 +246,15 @@ hipError_t launch_diff_forward (const mxg_config *cfg, const float *fa, const do
 hipError_t launch_diff_adjoint (uint32_t n, DiffOut in, const mxg_image_spec *spec, const float *g,
                                 double *partial, uint32_t grid, double *grad_dev, hipStream_t st,
                                 hipEvent_t *ev);
+
+// The energy order of the store's first n photons (sort.hip): stable
+// radix sort of (energy, index) pairs.  With tmp == nullptr only
+// *tmp_bytes (rocPRIM's temporary storage) is set.
+hipError_t sort_energies (const double *energy, uint32_t n, double *keys_in, uint32_t *vals_in,
+                          double *keys_out, uint32_t *vals_out, void *tmp, size_t *tmp_bytes,
+                          hipStream_t st);
+// out[i] = energy[idx[i]] for i < *n_dev (n_upper an upper bound of *n_dev)
+hipError_t gather_energies (const double *energy, const uint32_t *idx, const uint32_t *n_dev, uint32_t n_upper,
+                            double *out, hipStream_t st);
 
 } // namespace mxg

@@ -518,7 +518,7 @@ int mxgd_run_detect (double tstart, unsigned long long seed, int slot, float *ms
    if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
    if (begin_stage (slot)) return -1;
    HIPCHECK (launch_detect (d_cfg, d_fa, d_da, g_soa, list_io (MXGD_K_DETECT), g_upper, tstart,
-                            seed, 0, g_hcfg.det.fef_on != 0, g_stream));
+                            seed, 0, g_hcfg.det.fef_on != 0, detector_kind (g_hcfg), g_stream));
    return end_stage (slot, ms);
 }
 
@@ -930,6 +930,116 @@ void mxgd_graph_destroy (mxgd_graph *g)
    (void) hipGraphExecDestroy (g->exec);
    (void) hipGraphDestroy (g->graph);
    free (g);
+}
+
+} // extern "C"
+
+/* ---- host photon lists: pinned memory, energy order, active list ------------------- */
+
+namespace {
+void *d_sort = nullptr;              // keys and values in / out, then rocPRIM's storage
+size_t g_sort_cap = 0, g_sort_tmp = 0;
+double *d_gather = nullptr;          // energies of the active list
+size_t g_gather_cap = 0;
+// pinned host allocations made for photon lists
+void **g_pinned = nullptr;
+size_t g_pinned_n = 0, g_pinned_cap = 0;
+
+int sort_buffers (size_t n)
+{
+   if (n <= g_sort_cap) return 0;
+   size_t cap = g_sort_cap ? g_sort_cap : 1024;
+   while (cap < n) cap *= 2;
+   size_t tmp = 0;
+   HIPCHECK (sort_energies (nullptr, (uint32_t) cap, nullptr, nullptr, nullptr, nullptr, nullptr, &tmp, g_stream));
+   if (d_sort) { HIPCHECK (hipStreamSynchronize (g_stream)); (void) hipFree (d_sort); d_sort = nullptr; }
+   g_sort_cap = 0;
+   HIPCHECK (hipMalloc (&d_sort, 2 * align256 (cap * sizeof (double)) + 2 * align256 (cap * sizeof (uint32_t))
+                                 + align256 (tmp)));
+   g_sort_cap = cap;
+   g_sort_tmp = tmp;
+   return 0;
+}
+} // namespace
+
+extern "C" {
+
+void *mxgd_host_alloc (size_t bytes)
+{
+   int n = 0;
+   if (!g_inited && ((hipGetDeviceCount (&n) != hipSuccess) || (n == 0)))
+     return nullptr;                       // no GPU: the caller uses malloc
+   if (mxgd_init ()) return nullptr;
+   void *p = nullptr;
+   if (hipHostMalloc (&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess)
+     return nullptr;
+   memset (p, 0, bytes);
+   if (g_pinned_n == g_pinned_cap)
+     {
+        size_t cap = g_pinned_cap ? 2 * g_pinned_cap : 64;
+        void **q = (void **) realloc (g_pinned, cap * sizeof (void *));
+        if (q == nullptr) { (void) hipHostFree (p); return nullptr; }
+        g_pinned = q;
+        g_pinned_cap = cap;
+     }
+   g_pinned[g_pinned_n++] = p;
+   return p;
+}
+
+int mxgd_host_free (void *p)
+{
+   for (size_t k = 0; k < g_pinned_n; k++)
+     if (g_pinned[k] == p)
+       {
+          g_pinned[k] = g_pinned[--g_pinned_n];
+          (void) hipStreamSynchronize (g_stream);
+          (void) hipHostFree (p);
+          return 1;
+       }
+   return 0;
+}
+
+int mxgd_sort_store_by_energy (unsigned int n, unsigned int *index, double *energies)
+{
+   if (mxgd_init ()) return -1;
+   if (n > g_n) return fail ("sort of %u photons in a store of %u", n, g_n);
+   if (n == 0) return 0;
+   if (sort_buffers (n)) return -1;
+   char *b = (char *) d_sort;
+   double *k_in = (double *) b; b += align256 (g_sort_cap * sizeof (double));
+   double *k_out = (double *) b; b += align256 (g_sort_cap * sizeof (double));
+   uint32_t *v_in = (uint32_t *) b; b += align256 (g_sort_cap * sizeof (uint32_t));
+   uint32_t *v_out = (uint32_t *) b; b += align256 (g_sort_cap * sizeof (uint32_t));
+   size_t tmp = g_sort_tmp;
+   HIPCHECK (sort_energies (g_soa.energy, n, k_in, v_in, k_out, v_out, b, &tmp, g_stream));
+   HIPCHECK (hipMemcpyAsync (index, v_out, (size_t) n * sizeof (uint32_t), hipMemcpyDeviceToHost, g_stream));
+   HIPCHECK (hipMemcpyAsync (energies, k_out, (size_t) n * sizeof (double), hipMemcpyDeviceToHost, g_stream));
+   HIPCHECK (hipStreamSynchronize (g_stream));
+   return 0;
+}
+
+int mxgd_download_active (unsigned int *index, double *energies, unsigned int cap, unsigned int *n_out)
+{
+   if (mxgd_init ()) return -1;
+   uint32_t m = 0;
+   HIPCHECK (hipMemcpyAsync (&m, d_counts + g_cur, sizeof (uint32_t), hipMemcpyDeviceToHost, g_stream));
+   HIPCHECK (hipStreamSynchronize (g_stream));
+   if (m > cap) return fail ("active list of %u photons for a list of %u", m, cap);
+   *n_out = m;
+   if (m == 0) return 0;
+   if (m > g_gather_cap)
+     {
+        if (d_gather) (void) hipFree (d_gather);
+        d_gather = nullptr;
+        g_gather_cap = 0;
+        HIPCHECK (hipMalloc (&d_gather, (size_t) m * sizeof (double)));
+        g_gather_cap = m;
+     }
+   HIPCHECK (gather_energies (g_soa.energy, d_idx[g_cur], d_counts + g_cur, m, d_gather, g_stream));
+   HIPCHECK (hipMemcpyAsync (index, d_idx[g_cur], (size_t) m * sizeof (uint32_t), hipMemcpyDeviceToHost, g_stream));
+   HIPCHECK (hipMemcpyAsync (energies, d_gather, (size_t) m * sizeof (double), hipMemcpyDeviceToHost, g_stream));
+   HIPCHECK (hipStreamSynchronize (g_stream));
+   return 0;
 }
 
 } // extern "C"

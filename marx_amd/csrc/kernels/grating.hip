@@ -176,10 +176,10 @@ __device__ __forceinline__ int diffract_from (const mxg_grating &g, const mxg_se
    return diffract_photon (g, gs, da, theta, energy, x, p, order, st);
 }
 
-// The same kernel for both gratings, compiled for the occupancy each
-// measured fastest at (W waves per SIMD, 0 unbounded): the LETG pass, with
-// its support-grating draws, gains from 4 waves; the HETG pass loses
-template <int W>
+// One kernel per grating: the LETG instance adds the support-grating draws,
+// which the HETG instance leaves out (so it does not hold their registers).
+// W: the occupancy each measured fastest at (waves per SIMD, 0 unbounded).
+template <bool LETG, int W>
 __global__ __launch_bounds__ (kBlock, W > 0 ? W : 1)
 void k_grating (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
                 const double *__restrict__ da, Soa s, ListIO io,
@@ -225,7 +225,7 @@ void k_grating (const mxg_config *__restrict__ cfg, const float *__restrict__ fa
                          flags |= PHOTON_UNDIFFRACTED;
                        s.order[id] = order;
 
-                       if (gc.type == MARX_GRATING_LETG)
+                       if constexpr (LETG)
                          {
                             for (int pass = 0; pass < 4; pass++)
                               {
@@ -263,11 +263,11 @@ hipError_t launch_grating (const mxg_config *cfg, const float *fa, const double 
 {
    if (n_upper == 0) return hipSuccess;
    if (letg)
-     hipLaunchKernelGGL (k_grating<MXG_LETG_GRATING_WAVES>, dim3 (io.grid), dim3 (kBlock), 0, st, cfg, fa, da,
-                         s, io, seed, source_id);
+     hipLaunchKernelGGL ((k_grating<true, MXG_LETG_GRATING_WAVES>), dim3 (io.grid), dim3 (kBlock), 0, st, cfg,
+                         fa, da, s, io, seed, source_id);
    else
-     hipLaunchKernelGGL (k_grating<MXG_GRATING_WAVES>, dim3 (io.grid), dim3 (kBlock), 0, st, cfg, fa, da, s,
-                         io, seed, source_id);
+     hipLaunchKernelGGL ((k_grating<false, MXG_GRATING_WAVES>), dim3 (io.grid), dim3 (kBlock), 0, st, cfg, fa,
+                         da, s, io, seed, source_id);
    hipError_t e = hipGetLastError ();
    if (e == hipSuccess) e = launch_compact (io, st);
    return e;

@@ -82,6 +82,17 @@ MUTANTS = {
     "pha_width": ("oracle", "orc_detect.c", "energy = energy * (1.0 + 0.424661 * orc_gaussian (st));",
                   "energy = energy * (1.0 + 0.5 * orc_gaussian (st));",
                   [PD + "::test_hrc_pha_follows_its_gain_law"], "HRC PHA width 0.5 instead of 0.424661"),
+    "contam_off": ("oracle", "orc_detect.c", "               qe_contam = compute_contamination (c, d->contam",
+                   "               qe_contam = 1.0 + 0.0 * compute_contamination (c, d->contam",
+                   [PC + "::test_detected_fraction_is_qe_times_the_transmission"],
+                   "ACIS contamination not applied"),
+    "contam_rows": ("oracle", "orc_detect.c", "return pow (fabs ((y - y_0) / (964.0 - y_0)), 4.5);",
+                    "return pow (fabs ((y - y_0) / (964.0 - y_0)), 5.5);",
+                    [PC + "::test_detected_fraction_is_qe_times_the_transmission"],
+                    "ACIS-S row profile with the lower exponent above the middle row"),
+    "contam_time": ("library", "host/detector_init.c", "   double t = _Marx_TStart_MJDsecs, w;",
+                    "   double t = 0.0, w;", [PC + "::test_contam_tables_are_read"],
+                    "contamination taken at MJDref, not at the observation start"),
     "chip_shift": ("library", "host/detector_init.c", "   ofs = mxg_v_sum (aim, origin);\n",
                    "   ofs = mxg_v_sum (aim, origin);\n   ofs.y += 0.5;\n",
                    [PD + "::test_acis_chip_corners_are_the_caldb_geometry",

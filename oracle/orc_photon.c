@@ -48,12 +48,21 @@ int orc_sort_photons (Marx_Photon_Type *pt)
         idx[i] = i;
      }
    qsort_r (idx, n, sizeof (unsigned int), energy_cmp, e);
-   free (pt->sorted_index);
-   pt->sorted_index = idx;
+   /* a list made by marx_alloc_photon_type holds its index in the library's
+    * (pinned) host memory, which free() must not see: the order is copied
+    * into it */
+   if (pt->sorted_index == NULL)
+     {
+        pt->sorted_index = idx;
+        idx = NULL;
+     }
+   else
+     memcpy (pt->sorted_index, idx, n * sizeof (unsigned int));
+   free (idx);
    if (pt->sorted_energies == NULL)
      pt->sorted_energies = (double *) malloc ((pt->max_n_photons ? pt->max_n_photons : 1) * sizeof (double));
    for (i = 0; i < n; i++)
-     pt->sorted_energies[i] = e[idx[i]];
+     pt->sorted_energies[i] = e[pt->sorted_index[i]];
    pt->num_sorted = n;
    free (e);
    return 0;
