@@ -224,6 +224,9 @@ def measure_batched(args, tmp, torch):
     progress(f"{K} graph-launched steps: {elapsed / K * 1e3:.2f} ms per step")
 
     # per-kernel device times from one uncaptured pass of the same batch
+    # under the staged schedule (the persistent pass is one kernel), which
+    # the per-source loop below also uses
+    tracer.set_schedule(persistent_tile=0)
     tracer.trace_batch(dirs, counts, exposure=T, use_graph=False)
     sms = tracer.stage_ms()
     kms, kin = tracer.kernel_stats()
@@ -245,6 +248,8 @@ def measure_batched(args, tmp, torch):
     torch.cuda.synchronize()
     loop_s = time.perf_counter() - t0
     progress(f"per-source loop: {loop_s:.2f} s")
+    tracer.set_schedule(persistent_tile=sched["persistent_tile"],
+                        persistent_blocks_per_cu=sched["persistent_blocks_per_cu"])
 
     out = {
         "metric": "photons/s traced (HRMA+grating+detector) 1 GPU vs host CPU; gradient evals/s",
@@ -254,6 +259,7 @@ def measure_batched(args, tmp, torch):
         "config": {"workload": DESCRIPTION[cfg], "name": cfg, "sources": len(counts),
                    "photons_per_step": N, "exposure_s": T, "detected_per_step": detected / K},
         "schedule": sched, "autotuned_pass_ms": tuned_ms,
+        "pass": "persistent" if sched["persistent_tile"] else "staged",
         "per_source_loop": {"value": N / loop_s, "unit": "photons/s", "seconds": loop_s,
                             "sources": len(counts)},
         "batched_over_loop": value / (N / loop_s),

@@ -414,24 +414,33 @@ int mxg_trace_batch (unsigned int num_sources, const double *dirs, const unsigne
                      unsigned int tag_start, double start_time, double exposure, int use_graph,
                      unsigned int *num_detected, float *ms);
 
-/* Launch schedule of the stage kernels: blocks per CU of the source,
- * paraboloid, reflection, grating and detector kernels (1..32), and the
- * minimum waves per SIMD the reflection kernel is compiled for (2, 3, 4). */
+/* Launch schedule of the batched pass.  Staged (persistent_tile = 0): one
+ * kernel per stage, with blocks per CU of the source, paraboloid,
+ * reflection, grating and detector kernels (1..32) and the minimum waves per
+ * SIMD the reflection kernel is compiled for (2, 3, 4).  Persistent
+ * (persistent_tile = a multiple of 256 up to 4096): the whole pass as one
+ * launch of persistent_blocks_per_cu (1..8) blocks per CU that take tiles of
+ * persistent_tile photons from a work counter and run every stage on them
+ * (a FEF run takes the staged pass).  Both give the same photons. */
 typedef struct
 {
    unsigned int blocks_per_cu[5];
    unsigned int reflect_waves;
+   unsigned int persistent_tile;
+   unsigned int persistent_blocks_per_cu;
 }
 mxg_schedule;
 int mxg_set_schedule (const mxg_schedule *s);
 int mxg_get_schedule (mxg_schedule *s);
 
 /* Autotuner: times the batch of mxg_trace_batch (same arguments) under
- * candidate schedules -- blocks per CU in {2, 4, 8, 16} for every stage
- * kernel, then 2, 3 and 4 waves for the reflection kernel -- keeps for each
- * kernel the fastest (HIP-event time, best of `repeats` runs), installs the
- * result and returns it in *best with the graph-launched pass time of the
- * batch under it in *best_ms. */
+ * candidate schedules -- for the staged pass, blocks per CU in {2, 4, 8, 16}
+ * for every stage kernel, then 2, 3 and 4 waves for the reflection kernel,
+ * keeping for each kernel the fastest (HIP-event time, best of `repeats`
+ * runs); then the persistent pass with tiles of {256, 512, 1024, 2048}
+ * photons at {1, 2, 4} blocks per CU -- and installs the schedule whose
+ * graph-launched pass is fastest, returning it in *best and that pass time
+ * in *best_ms. */
 int mxg_autotune_batch (unsigned int num_sources, const double *dirs, const unsigned int *counts,
                         unsigned int tag_start, double start_time, double exposure,
                         unsigned int repeats, mxg_schedule *best, float *best_ms);

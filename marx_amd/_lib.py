@@ -80,14 +80,18 @@ class SoaView(ctypes.Structure):
 
 class Schedule(ctypes.Structure):
     """mxg_schedule: blocks per CU of the source, paraboloid, reflection,
-    grating and detector kernels, and the reflection kernel's waves per SIMD."""
-    _fields_ = [("blocks_per_cu", ctypes.c_uint * 5), ("reflect_waves", ctypes.c_uint)]
+    grating and detector kernels, the reflection kernel's waves per SIMD, and
+    the persistent pass's tile (0: the staged pass) and blocks per CU."""
+    _fields_ = [("blocks_per_cu", ctypes.c_uint * 5), ("reflect_waves", ctypes.c_uint),
+                ("persistent_tile", ctypes.c_uint), ("persistent_blocks_per_cu", ctypes.c_uint)]
 
     KERNELS = ("source", "enter", "reflect", "grating", "detect")
 
     def as_dict(self):
         d = {k: int(v) for k, v in zip(self.KERNELS, self.blocks_per_cu)}
         d["reflect_waves"] = int(self.reflect_waves)
+        d["persistent_tile"] = int(self.persistent_tile)
+        d["persistent_blocks_per_cu"] = int(self.persistent_blocks_per_cu)
         return d
 
 

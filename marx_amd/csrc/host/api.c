@@ -693,10 +693,16 @@ static Batch_Key Batch_Graph_Key;
 
 
 /* one pass over the uploaded batch (n photons): source, mirror, grating,
- * detector, in timing slots 0-3 (during graph capture the stages record no
- * events but still copy their survivor counts) */
+ * detector, in timing slots 0-3, or the persistent pass in slot 0 (during
+ * graph capture the stages record no events but still copy their survivor
+ * counts) */
 static int batch_pass (unsigned int n, double start_time, double exposure, int with_grating)
 {
+   mxgd_schedule sched;
+   mxgd_get_schedule (&sched);
+   /* the persistent pass traces point sources without the FEF step */
+   if (sched.persist_tile && !Mxg_Cfg.det.fef_on && (Mxg_Cfg.src.type == MXG_SOURCE_POINT))
+     return mxgd_run_batch_persistent (n, exposure, start_time, Seed, 0);
    if ((-1 == mxgd_run_batch_source (n, exposure, start_time, Seed, 0))
        || (-1 == mxgd_run_mirror (Mxg_Cfg.src.distance, Seed, 1, NULL))
        || (with_grating && (-1 == mxgd_run_grating (Seed, 2, NULL)))
@@ -842,6 +848,8 @@ int mxg_set_schedule (const mxg_schedule *s)
    if (s == NULL) return -1;
    memcpy (d.blocks_per_cu, s->blocks_per_cu, sizeof (d.blocks_per_cu));
    d.reflect_waves = s->reflect_waves;
+   d.persist_tile = s->persistent_tile;
+   d.persist_blocks_per_cu = s->persistent_blocks_per_cu;
    if (-1 == mxgd_set_schedule (&d))
      {
         mxg_error ("%s", mxgd_last_error ());
@@ -857,6 +865,8 @@ int mxg_get_schedule (mxg_schedule *s)
    mxgd_get_schedule (&d);
    memcpy (s->blocks_per_cu, d.blocks_per_cu, sizeof (d.blocks_per_cu));
    s->reflect_waves = d.reflect_waves;
+   s->persistent_tile = d.persist_tile;
+   s->persistent_blocks_per_cu = d.persist_blocks_per_cu;
    return 0;
 }
 
@@ -885,24 +895,45 @@ static int time_kernels (unsigned int num_sources, const double *dirs, const uns
    return 0;
 }
 
+/* best graph-launched pass time of the batch under the installed schedule
+ * (the first launch, which captures, is not counted) */
+static int graph_pass_ms (unsigned int num_sources, const double *dirs, const unsigned int *counts,
+                          unsigned int tag_start, double start_time, double exposure, unsigned int repeats,
+                          float *best_ms)
+{
+   float ms = 0.0f;
+   *best_ms = 1e30f;
+   for (unsigned int r = 0; r < repeats + 1; r++)
+     {
+        if (-1 == mxg_trace_batch (num_sources, dirs, counts, tag_start, start_time, exposure, 1, NULL, &ms))
+          return -1;
+        if ((r > 0) && (ms < *best_ms)) *best_ms = ms;
+     }
+   return 0;
+}
+
 int mxg_autotune_batch (unsigned int num_sources, const double *dirs, const unsigned int *counts,
                         unsigned int tag_start, double start_time, double exposure,
                         unsigned int repeats, mxg_schedule *best, float *best_ms)
 {
    static const unsigned int bpc[4] = {2, 4, 8, 16};
+   static const unsigned int tiles[4] = {256, 512, 1024, 2048};
+   static const unsigned int tile_bpc[3] = {1, 2, 4};
    mxg_schedule cur, sel;
-   float best_t[5], t[5];
+   float best_t[5], t[5], pass_ms, ms;
    int k, c;
 
    if (repeats == 0) repeats = 1;
    if (-1 == mxg_get_schedule (&cur))
      return -1;
    sel = cur;
+   sel.persistent_tile = 0;
    for (k = 0; k < 5; k++) best_t[k] = 1e30f;
-   /* blocks per CU, every kernel at once (their times are independent) */
+   /* the staged pass: blocks per CU, every kernel at once (their times are
+    * independent) */
    for (c = 0; c < 4; c++)
      {
-        mxg_schedule s = cur;
+        mxg_schedule s = sel;
         for (k = 0; k < 5; k++) s.blocks_per_cu[k] = bpc[c];
         if ((-1 == mxg_set_schedule (&s))
             || (-1 == time_kernels (num_sources, dirs, counts, tag_start, start_time, exposure,
@@ -931,21 +962,36 @@ int mxg_autotune_batch (unsigned int num_sources, const double *dirs, const unsi
              sel.reflect_waves = (unsigned int) c;
           }
      }
+   /* the staged pass against the persistent one, whole graph-launched
+    * passes (the persistent pass traces point sources without the FEF
+    * step) */
+   if ((-1 == mxg_set_schedule (&sel))
+       || (-1 == graph_pass_ms (num_sources, dirs, counts, tag_start, start_time, exposure, repeats, &pass_ms)))
+     goto fail;
+   if (!Mxg_Cfg.det.fef_on && (Mxg_Cfg.src.type == MXG_SOURCE_POINT))
+     {
+        mxg_schedule staged = sel;
+        for (c = 0; c < 4; c++)
+          for (k = 0; k < 3; k++)
+            {
+               mxg_schedule s = staged;
+               s.persistent_tile = tiles[c];
+               s.persistent_blocks_per_cu = tile_bpc[k];
+               if ((-1 == mxg_set_schedule (&s))
+                   || (-1 == graph_pass_ms (num_sources, dirs, counts, tag_start, start_time, exposure, repeats,
+                                            &ms)))
+                 goto fail;
+               if (ms < pass_ms)
+                 {
+                    pass_ms = ms;
+                    sel = s;
+                 }
+            }
+     }
    if (-1 == mxg_set_schedule (&sel))
      goto fail;
    if (best) *best = sel;
-   if (best_ms)
-     {
-        float ms = 0.0f;
-        *best_ms = 1e30f;
-        for (unsigned int r = 0; r < repeats + 1; r++)
-          {
-             if (-1 == mxg_trace_batch (num_sources, dirs, counts, tag_start, start_time, exposure, 1,
-                                        NULL, &ms))
-               goto fail;
-             if ((r > 0) && (ms < *best_ms)) *best_ms = ms;
-          }
-     }
+   if (best_ms) *best_ms = pass_ms;
    return 0;
 
 fail:

@@ -74,14 +74,24 @@ int mxgd_batch_set_tag_start (unsigned int tag_start);
  * sources: arrival times uniform over [0, exposure). */
 int mxgd_run_batch_source (unsigned int n, double exposure, double start_time,
                            unsigned long long seed, int slot);
+/* The whole batched pass (source, mirror, grating, detector) as one
+ * persistent launch under the schedule's tile (not with a FEF file): the
+ * same photons and final list as the staged pass; timing slot `slot` holds
+ * the pass, the survivor counts land in the stage slots. */
+int mxgd_run_batch_persistent (unsigned int n, double exposure, double start_time,
+                               unsigned long long seed, int slot);
 
 /* Launch schedule: blocks per CU of the stage kernels and the minimum waves
- * per SIMD of the reflection kernel (2, 3 or 4). */
+ * per SIMD of the reflection kernel (2, 3 or 4); persist_tile != 0: the
+ * batched pass as one persistent launch of persist_blocks_per_cu blocks per
+ * CU over tiles of persist_tile photons (a multiple of 256, at most 4096). */
 enum { MXGD_K_SOURCE, MXGD_K_ENTER, MXGD_K_REFLECT, MXGD_K_GRATING, MXGD_K_DETECT, MXGD_NUM_K };
 typedef struct
 {
    unsigned int blocks_per_cu[MXGD_NUM_K];
    unsigned int reflect_waves;
+   unsigned int persist_tile;
+   unsigned int persist_blocks_per_cu;
 }
 mxgd_schedule;
 int mxgd_set_schedule (const mxgd_schedule *s);

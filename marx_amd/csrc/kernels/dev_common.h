@@ -28,6 +28,10 @@ __device__ __forceinline__ vec3 v3 (double x, double y, double z) { vec3 a; a.x 
 
 // ---- Philox4x32-10 ----------------------------------------------------------
 
+// b0..b3: the unused words of the current block, next first (a draw takes
+// b0 and shifts the rest down, so no word is selected by a run-time index:
+// an indexed pick puts the whole stream in scratch memory); pos: the words
+// of the block already used.
 struct Stream
 {
    uint32_t k0, k1;
@@ -77,14 +81,10 @@ __device__ __forceinline__ double uniform (Stream &s)
         s.c2++;
         s.pos = 0;
      }
-   uint32_t u;
-   switch (s.pos)
-     {
-      case 0: u = s.b0; break;
-      case 1: u = s.b1; break;
-      case 2: u = s.b2; break;
-      default: u = s.b3; break;
-     }
+   const uint32_t u = s.b0;
+   s.b0 = s.b1;
+   s.b1 = s.b2;
+   s.b2 = s.b3;
    s.pos++;
    return (double) u * (1.0 / ((double) 0xFFFFFFFFu));
 }

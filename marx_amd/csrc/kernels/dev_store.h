@@ -222,12 +222,48 @@ hipError_t launch_source (const mxg_config *cfg, const float *fa, const double *
                           uint32_t *idx, hipStream_t st);
 
 // Batched point sources (see source.hip): n photons over nsrc sources with
-// prefix offsets off[nsrc + 1] and unit directions dir[3 nsrc].
+// prefix offsets off[nsrc + 1] and unit directions dir[3 nsrc]; point: the
+// configured shape is a point (the instance without the extended shapes).
 hipError_t launch_batch_source (const mxg_config *cfg, const double *da, Soa s, uint32_t n,
                                 const uint32_t *off, uint32_t nsrc, const double *dir,
                                 const uint32_t *tag_start, double exposure, double start_time,
                                 unsigned long long seed, uint32_t *idx, uint32_t *count,
-                                uint32_t grid, hipStream_t st);
+                                uint32_t grid, bool point, hipStream_t st);
+
+// The batched pass as one persistent launch (persist.hip): blocks take
+// tiles of `tile` photons (a multiple of kBlock, at most kMaxPersistTile)
+// from *work and run every stage on them; the detected photons of the batch
+// end in idx_out / *n_out in id order, as after the staged pass.
+// stage_count (4 entries) receives the photons into the reflection and out
+// of the mirror, grating and detector; seg (n entries), tile_count and
+// tile_off (ceil(n / tile) entries each) are scratch.  ev, when non-null:
+// two events around the persistent kernel.
+constexpr uint32_t kMaxPersistTile = 4096;
+#ifndef MXG_PERSIST_WAVES
+#define MXG_PERSIST_WAVES 4
+#endif
+struct PersistLaunch
+{
+   const mxg_config *cfg;        // device copy
+   const mxg_config *hcfg;       // host copy (LDS layout)
+   const float *fa;
+   const double *da;
+   Soa s;
+   uint32_t n;
+   const uint32_t *off;
+   uint32_t nsrc;
+   const double *dir;
+   const uint32_t *tag_start;
+   double exposure, start_time;
+   unsigned long long seed;
+   uint32_t *rng_pos, *work, *stage_count;
+   uint32_t tile, blocks;
+   uint32_t *seg, *tile_count, *tile_off, *idx_out, *n_out;
+   int grating;                  // 0 none, 1 HETG, 2 LETG
+   int kind;                     // detector family, -1 none
+   hipEvent_t *ev;
+};
+hipError_t launch_batch_persistent (const PersistLaunch &p, hipStream_t st);
 
 // The differentiable mirror (diff.hip).
 struct DiffOut

@@ -46,7 +46,10 @@ uint32_t *d_seg = nullptr;         // per-block survivor segments of the running
 uint32_t *d_seg_count = nullptr;   // kMaxSegments entries
 uint32_t *d_rng_pos = nullptr;     // the mirror's per-photon stream position
 uint32_t *d_counts = nullptr;      // [0],[1] list lengths, [2] source cut, [4] middle list,
-                                   // [8..15] per-stage survivor counts
+                                   // [8..15] per-stage survivor counts, [32] the persistent
+                                   // pass's tile counter, [36..39] its stage counts
+uint32_t *d_tile_cnt = nullptr;    // the persistent pass's per-tile counts and offsets
+size_t g_tile_cap = 0;             // entries of each
 int g_cur = 0;                     // list holding the active photons
 unsigned int g_upper = 0;          // host-side upper bound of the active count
 Marx_Photon_Attr_Type *d_aos = nullptr;
@@ -78,7 +81,7 @@ double *d_diff_partial = nullptr;  // per-block gradient partials
 double *d_diff_small = nullptr;    // summed weight, gradient
 mxg_diff_cfg *d_diff_cfg = nullptr;
 // launch schedule and graph capture
-mxgd_schedule g_sched = {{8, 8, 8, 8, 8}, MXG_REFLECT_WAVES_DEFAULT};
+mxgd_schedule g_sched = {{8, 8, 8, 8, 8}, MXG_REFLECT_WAVES_DEFAULT, 0, 2};
 int g_capturing = 0;
 uint64_t *d_tile_tot = nullptr;
 // bumped whenever a buffer that a captured launch sequence may hold is
@@ -182,7 +185,9 @@ int ensure_capacity (size_t n)
    if (d_idx_mid) { (void) hipFree (d_idx_mid); d_idx_mid = nullptr; }
    if (d_seg) { (void) hipFree (d_seg); d_seg = nullptr; }
    if (d_rng_pos) { (void) hipFree (d_rng_pos); d_rng_pos = nullptr; }
+   if (d_tile_cnt) { (void) hipFree (d_tile_cnt); d_tile_cnt = nullptr; }
    g_capacity = 0;
+   g_tile_cap = 0;
    HIPCHECK (hipMalloc (&mem, total));
    HIPCHECK (hipMalloc (&d_ticks, cap * sizeof (uint64_t)));
    HIPCHECK (hipMalloc (&d_tile_tot, (cap / 2048 + 2) * sizeof (uint64_t)));
@@ -191,6 +196,9 @@ int ensure_capacity (size_t n)
    HIPCHECK (hipMalloc (&d_idx_mid, cap * sizeof (uint32_t)));
    HIPCHECK (hipMalloc (&d_seg, cap * sizeof (uint32_t)));
    HIPCHECK (hipMalloc (&d_rng_pos, cap * sizeof (uint32_t)));
+   // tiles of at least kBlock photons
+   HIPCHECK (hipMalloc (&d_tile_cnt, 2 * (cap / kBlock + 1) * sizeof (uint32_t)));
+   g_tile_cap = cap / kBlock + 1;
    d_store = mem;
    char *p = (char *) mem;
    size_t k = 0;
@@ -841,10 +849,70 @@ int mxgd_run_batch_source (unsigned int n, double exposure, double start_time,
    uint32_t grid = stage_grid (n, MXGD_K_SOURCE);
    HIPCHECK (launch_batch_source (d_cfg, d_da, g_soa, n, d_batch_off, g_batch_nsrc, d_batch_dir,
                                   d_batch_tag, exposure, start_time, seed, d_idx[0], d_counts, grid,
-                                  g_stream));
+                                  g_hcfg.src.type == MXG_SOURCE_POINT, g_stream));
    if (timed)
      HIPCHECK (hipEventRecord (g_tev[slot][1], g_stream));
    g_cur = 0;
+   g_n = n;
+   g_upper = n;
+   return 0;
+}
+
+int mxgd_run_batch_persistent (unsigned int n, double exposure, double start_time,
+                               unsigned long long seed, int slot)
+{
+   if (!d_cfg) return fail ("configuration not on the device");
+   if ((n > g_capacity) || (g_batch_nsrc == 0)) return fail ("batch not prepared");
+   if (g_sched.persist_tile == 0) return fail ("the schedule has no persistent tile");
+   if (g_hcfg.det.fef_on) return fail ("the persistent pass has no FEF step");
+   if (g_hcfg.src.type != MXG_SOURCE_POINT) return fail ("the persistent pass traces point sources only");
+   const uint32_t tiles = (n + g_sched.persist_tile - 1) / g_sched.persist_tile;
+   if (tiles > g_tile_cap) return fail ("persistent pass of %u tiles exceeds %zu", tiles, g_tile_cap);
+   bool timed = (slot >= 0) && (slot < kSlots) && !g_capturing;
+   if (timed)
+     {
+        HIPCHECK (hipEventRecord (g_tev[slot][0], g_stream));
+        g_slot_used[slot] = 1;
+     }
+   PersistLaunch p;
+   p.cfg = d_cfg;
+   p.hcfg = &g_hcfg;
+   p.fa = d_fa;
+   p.da = d_da;
+   p.s = g_soa;
+   p.n = n;
+   p.off = d_batch_off;
+   p.nsrc = g_batch_nsrc;
+   p.dir = d_batch_dir;
+   p.tag_start = d_batch_tag;
+   p.exposure = exposure;
+   p.start_time = start_time;
+   p.seed = seed;
+   p.rng_pos = d_rng_pos;
+   p.work = d_counts + 32;
+   p.stage_count = d_counts + 36;
+   p.tile = g_sched.persist_tile;
+   p.blocks = (uint32_t) (g_num_cus > 0 ? g_num_cus : 256) * g_sched.persist_blocks_per_cu;
+   p.seg = d_seg;
+   p.tile_count = d_tile_cnt;
+   p.tile_off = d_tile_cnt + g_tile_cap;
+   p.idx_out = d_idx[1];
+   p.n_out = d_counts + 1;
+   p.grating = (g_hcfg.grating.type == MARX_GRATING_HETG) ? 1 : (g_hcfg.grating.type == MARX_GRATING_LETG) ? 2 : 0;
+   p.kind = (g_hcfg.det.type > 0) ? detector_kind (g_hcfg) : -1;
+   p.ev = nullptr;
+   HIPCHECK (launch_batch_persistent (p, g_stream));
+   if (timed)
+     HIPCHECK (hipEventRecord (g_tev[slot][1], g_stream));
+   // the per-stage counts where the staged pass leaves them: photons into
+   // the two mirror kernels, out of the mirror, grating and detector
+   hipLaunchKernelGGL (k_set_u32, dim3 (1), dim3 (1), 0, g_stream, d_counts + 8 + MXGD_SLOT_HRMA_ENTER, (uint32_t) n);
+   hipLaunchKernelGGL (k_copy_u32, dim3 (1), dim3 (1), 0, g_stream, d_counts + 8 + MXGD_SLOT_HRMA_REFLECT,
+                       d_counts + 36);
+   for (int k = 1; k <= 3; k++)
+     hipLaunchKernelGGL (k_copy_u32, dim3 (1), dim3 (1), 0, g_stream, d_counts + 8 + k, d_counts + 36 + k);
+   HIPCHECK (hipGetLastError ());
+   g_cur = 1;
    g_n = n;
    g_upper = n;
    return 0;
@@ -857,6 +925,11 @@ int mxgd_set_schedule (const mxgd_schedule *s)
        return fail ("blocks per CU must be 1..32");
    if ((s->reflect_waves < 2) || (s->reflect_waves > 4))
      return fail ("reflection kernel waves per SIMD must be 2, 3 or 4");
+   if ((s->persist_tile % kBlock) || (s->persist_tile > kMaxPersistTile))
+     return fail ("persistent tile must be 0 or a multiple of %d up to %u photons", kBlock,
+                  (unsigned int) kMaxPersistTile);
+   if (s->persist_tile && ((s->persist_blocks_per_cu < 1) || (s->persist_blocks_per_cu > 8)))
+     return fail ("persistent blocks per CU must be 1..8");
    g_sched = *s;
    return 0;
 }

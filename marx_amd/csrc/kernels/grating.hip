@@ -1,180 +1,8 @@
-// grating.hip -- HETG / LETG facet diffraction, one lane per photon.
-//
-// Per photon (the operations of marx/libsrc/diffract.c:974-1130):
-//   grating vignetting draw -> rotate about X into the grating frame ->
-//   Rowland-torus intersection by Newton on the quartic (:622-687) ->
-//   order draw against the cumulative efficiency table interpolated at the
-//   photon energy (the per-photon form of the shell-major
-//   JDMinterpolate_n_fvector pass, finterpo.c:134-193) -> facet basis,
-//   sector lookup, dtheta and dP/P blur, grating equation (:706-823) ->
-//   LETG: fine support grating once, coarse three times (:917-970, 1081-1122)
-//   -> rotate back.  Efficiency tables are energy-major so the order walk of
-//   one lane touches two contiguous rows.
-#include "dev_common.h"
-#include "dev_store.h"
+// grating.hip -- the HETG / LETG stage kernel, one lane per photon (the
+// per-photon work is in grating_stage.h).
+#include "grating_stage.h"
 
 namespace mxg {
-
-__device__ __forceinline__ vec3 rotate_x (vec3 a, double theta)
-{
-   double s, c;
-   sincos (theta, &s, &c);
-   double ay = a.y, az = a.z;
-   a.y = c * ay - s * az;
-   a.z = s * ay + c * az;
-   return a;
-}
-
-__device__ __forceinline__ int newtons_quartic (double a, double b, double c, double d, double t0,
-                                                double &tp)
-{
-   unsigned int max_it = 10;
-   double a2 = 2.0 * a, a3 = 3.0 * a, b2 = 2.0 * b, t;
-   while (true)
-     {
-        double t2 = t0 * t0;
-        double num = t2 * (3.0 * t2 + a2 * t0 + b) - d;
-        double den = t2 * (4.0 * t0 + a3) + b2 * t0 + c;
-        t = num / den;
-        if (fabs (t - t0) < 1.0e-4) break;
-        max_it--;
-        if (max_it == 0) return -1;
-        t0 = t;
-     }
-   tp = t;
-   return 0;
-}
-
-__device__ __forceinline__ int torus_intersect (vec3 &x0, vec3 p, double rowland)
-{
-   double t = -x0.x / p.x;
-   x0.x = 0.0;
-   x0.y = x0.y + p.y * t;
-   x0.z = x0.z + p.z * t;
-   double pxpz_len = p.x * p.x + p.z * p.z;
-   double x2 = x0.z * x0.z + x0.y * x0.y;
-   double r2 = rowland * rowland;
-   double pdotx = dot (p, x0);
-   double a = 4.0 * pdotx;
-   double b = 2.0 * x2 + a * pdotx - r2 * pxpz_len;
-   double c = a * x2 - 2.0 * r2 * p.z * x0.z;
-   double d = x2 * x2 - r2 * x0.z * x0.z;
-   double t0 = -rowland * sqrt (pxpz_len);
-   if (-1 == newtons_quartic (a, b, c, d, t0, t))
-     return -1;
-   x0 = ax1_bx2 (1.0, x0, t, p);
-   return 0;
-}
-
-// Order draw: first k with r <= cumulative efficiency(E); -1 when none.
-// The bracket is the one the reference's merge walk over sorted energies
-// reaches: the first j >= 1 with E[j] >= e, else the last interval.
-__device__ __forceinline__ int draw_order (const mxg_grating &g, const float *__restrict__ fa,
-                                           double e, double r)
-{
-   const float *E = fa + g.energies_off;
-   const unsigned int ne = g.num_energies, no = g.num_orders;
-   unsigned int lo = 1, hi = ne - 1;
-   while (lo < hi)
-     {
-        unsigned int mid = (lo + hi) >> 1;
-        if (e > (double) E[mid]) lo = mid + 1;
-        else hi = mid;
-     }
-   const unsigned int count = lo;
-   const double x_0 = E[count - 1], x_1 = E[count];
-   const double dx_10 = x_1 - x_0;
-   const float *y0 = fa + g.cum_eff_off + (count - 1) * no;
-   const float *y1 = y0 + no;
-   for (unsigned int k = 0; k < no; k++)
-     {
-        float v;
-        if (dx_10 == 0.0)
-          v = y0[k];
-        else
-          {
-             double a = y0[k], b = y1[k];
-             v = (float) (a + (b - a) * (e - x_0) / dx_10);
-          }
-        if (r <= (double) v)
-          return (int) k;
-     }
-   return -1;
-}
-
-__device__ __forceinline__ int diffract_photon (const mxg_grating &g, const mxg_sectors &gs,
-                                                const double *__restrict__ da, double theta,
-                                                double energy, vec3 x, vec3 &p, int order,
-                                                Stream &st)
-{
-   double n_lambda_over_d = (double) order * (2.0 * MXG_PI * MXG_HBAR_C) / g.period / energy;
-   vec3 n = unit (x);
-   n.x = -n.x; n.y = -n.y; n.z = -n.z;
-   vec3 l = v3 (n.z, 0.0, -n.x);
-   l = unit (l);
-   vec3 d = cross (n, l);
-   double dtheta, dp_over_p;
-
-   if (gs.n > 0)
-     {
-        const double *min_a = da + gs.off;
-        const double *max_a = min_a + gs.n;
-        double sector = atan2 (x.y, x.z);
-        if (sector < 0) sector = 2 * MXG_PI + sector;
-        unsigned int sn = search_d (sector, min_a, gs.n) - 1;
-        if (sn >= gs.n) return -1;
-        if ((max_a[sn] <= sector) || (min_a[sn] > sector))
-          return -1;
-        dtheta = min_a[2 * gs.n + sn] + min_a[3 * gs.n + sn] * gaussian (st);
-        dp_over_p = min_a[4 * gs.n + sn] + min_a[5 * gs.n + sn] * gaussian (st);
-     }
-   else
-     {
-        dtheta = g.theta_blur * gaussian (st);
-        dp_over_p = g.dp_over_p * gaussian (st);
-     }
-
-   theta -= dtheta;
-   if (theta != 0.0)
-     {
-        double s, c;
-        sincos (theta, &s, &c);
-        vec3 l_tmp = l, d_tmp = d;
-        l = ax1_bx2 (c, l_tmp, s, d_tmp);
-        d = ax1_bx2 (-s, l_tmp, c, d_tmp);
-     }
-
-   double p_d = n_lambda_over_d + dot (p, d);
-   double p_l = dot (p, l);
-   double p_n = 1.0 - p_l * p_l - p_d * p_d;
-   if (p_n < 0.0) return -1;
-   p_n = sqrt (p_n);
-   vec3 q = ax1_bx2 (p_d, d, p_n, n);
-   p = ax1_bx2 (1.0, q, p_l, l);
-   if (dp_over_p == 0) return 0;
-   double factor = n_lambda_over_d * dp_over_p;
-   vec3 dp = ax1_bx2 (-factor, d, factor * (p_d / p_n), n);
-   p.x += dp.x;
-   p.y += dp.y;
-   p.z += dp.z;
-   p = unit (p);
-   return 0;
-}
-
-// one order draw plus diffraction off grating g; returns 0 or -1
-__device__ __forceinline__ int diffract_from (const mxg_grating &g, const mxg_sectors &gs,
-                                              const float *__restrict__ fa,
-                                              const double *__restrict__ da, double theta,
-                                              double energy, double ef, vec3 x, vec3 &p,
-                                              int8_t &order_out, Stream &st)
-{
-   double r = uniform (st);
-   int k = draw_order (g, fa, ef, r);
-   if (k < 0) return -1;
-   int order = (int) fa[g.orders_off + k];
-   order_out = (int8_t) order;
-   return diffract_photon (g, gs, da, theta, energy, x, p, order, st);
-}
 
 // One kernel per grating: the LETG instance adds the support-grating draws,
 // which the HETG instance leaves out (so it does not hold their registers).
@@ -189,7 +17,6 @@ void k_grating (const mxg_config *__restrict__ cfg, const float *__restrict__ fa
    __shared__ uint32_t seg_wc[2 * (kBlock / 64)];
    const BlockRange range = block_range (*io.n_in, blockIdx.x, gridDim.x);
    SegWriter survivors (io, range, seg_wc);
-   const mxg_sectors no_sectors = {0u, 0u};
 
    for (uint32_t base = range.lo; base < range.hi; base += kBlock)
      {
@@ -199,58 +26,7 @@ void k_grating (const mxg_config *__restrict__ cfg, const float *__restrict__ fa
         if (i < range.hi)
           {
              id = io.idx_in[i];
-             uint32_t flags = s.flags[id];
-             if ((flags & BAD_PHOTON_MASK) == 0)
-               {
-                  Stream st;
-                  stream_init (st, seed, s.tag[id], MXG_STAGE_GRATING, source_id);
-                  const int shell = s.shell[id];
-                  const mxg_grating &g = gc.g[shell];
-                  if (uniform (st) > g.vig)
-                    s.flags[id] = flags | PHOTON_MIRROR_VBLOCKED;
-                  else
-                    {
-                       vec3 x = v3 (s.x0[id], s.x1[id], s.x2[id]);
-                       vec3 p = v3 (s.p0[id], s.p1[id], s.p2[id]);
-                       const double energy = s.energy[id];
-                       const double ef = (double) (float) energy;
-                       int8_t order = s.order[id];
-
-                       x = rotate_x (x, -g.dispersion_angle);
-                       p = rotate_x (p, -g.dispersion_angle);
-                       if (-1 == torus_intersect (x, p, gc.rowland[shell]))
-                         flags |= PHOTON_UNDIFFRACTED;
-                       if (-1 == diffract_from (g, gc.sectors[shell], fa, da, 0.0, energy, ef, x, p,
-                                                order, st))
-                         flags |= PHOTON_UNDIFFRACTED;
-                       s.order[id] = order;
-
-                       if constexpr (LETG)
-                         {
-                            for (int pass = 0; pass < 4; pass++)
-                              {
-                                 const bool fine = (pass == 0);
-                                 if (fine ? !gc.has_fine : !gc.has_coarse) continue;
-                                 if (flags & BAD_PHOTON_MASK) continue;
-                                 const double theta = (pass == 0) ? MXG_PI / 2.0
-                                   : (pass == 1) ? MXG_PI / 3.0
-                                   : (pass == 2) ? 2.0 * MXG_PI / 3.0 : 0.0;
-                                 int8_t so = s.sup[pass][id];
-                                 if (-1 == diffract_from (fine ? gc.fine : gc.coarse, no_sectors, fa, da,
-                                                          theta, energy, ef, x, p, so, st))
-                                   flags |= PHOTON_UNDIFFRACTED;
-                                 s.sup[pass][id] = so;
-                              }
-                         }
-
-                       x = rotate_x (x, g.dispersion_angle);
-                       p = rotate_x (p, g.dispersion_angle);
-                       s.x0[id] = x.x; s.x1[id] = x.y; s.x2[id] = x.z;
-                       s.p0[id] = p.x; s.p1[id] = p.y; s.p2[id] = p.z;
-                       s.flags[id] = flags;
-                       keep = (flags & BAD_PHOTON_MASK) == 0;
-                    }
-               }
+             keep = grating_one<LETG> (gc, fa, da, s, id, seed, source_id);
           }
         survivors.add (keep, id);
      }

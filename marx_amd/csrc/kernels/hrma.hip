@@ -1,123 +1,6 @@
-// hrma.hip -- HRMA Wolter-I reflection, one lane per photon.
+// hrma.hip -- the HRMA stage kernels, one lane per photon (the per-photon
+// work is in hrma_stage.h).
 //
-// Per photon (the operations of marx/libsrc/hrma.c:1161-1341):
-//   vignetting draw -> aperture sampling by cumulative shell area, radius and
-//   shutter-checked azimuth (hrma.c:984-1050) -> paraboloid: conic
-//   intersection (hrma.c:411-478), normal blur (:1054-1093), Fresnel
-//   reflectivity draw (reflect.c:39-77), specular reflection, WFold scatter
-//   (wfold.c:334-369) -> cap struts -> hyperboloid: the same -> post-collimator
-//   struts.  Geometry is FP64.  Survivors go to the next active list in input
-//   order (SegWriter + k_compact).
-#include "dev_common.h"
-#include "dev_store.h"
-
-namespace mxg {
-
-// The hit of a ray on the quadric x^2+y^2+z^2 = a x^2 + b x + c between
-// xmin <= x < xmax (the larger valid root in x).  On success x0 becomes the
-// hit; on a miss x0 is unchanged and -1 is returned.
-__device__ __forceinline__ int conic_intersection (double a, double b, double c, vec3 &x0, vec3 p,
-                                                   double xmin, double xmax)
-{
-   double t_plus = -x0.x / p.x;
-   double x_y = x0.y + t_plus * p.y;
-   double x_z = x0.z + t_plus * p.z;
-   double alpha = a * p.x * p.x - 1.0;
-   double beta = b * p.x - 2.0 * (p.y * x_y + p.z * x_z);
-   double gamma = c - x_z * x_z - x_y * x_y;
-   double t_minus;
-
-   if (alpha == 0.0)
-     {
-        if (beta == 0.0) return -1;
-        t_plus = t_minus = -gamma / beta;
-     }
-   else if (0 >= quadratic_root (alpha, beta, gamma, t_plus, t_minus))
-     return -1;
-
-   double x_plus = p.x * t_plus;
-   double x_minus = p.x * t_minus;
-   bool plus_ok = (x_plus >= xmin) && (x_plus < xmax);
-   bool minus_ok = (x_minus >= xmin) && (x_minus < xmax);
-   double t;
-   if (plus_ok)
-     t = (minus_ok && (x_minus > x_plus)) ? t_minus : t_plus;
-   else if (minus_ok)
-     t = t_minus;
-   else
-     return -1;
-
-   x0.x = p.x * t;
-   x0.y = x_y + p.y * t;
-   x0.z = x_z + p.z * t;
-   return 0;
-}
-
-// inward unit normal of the quadric at a point on it (hrma.c:470-476)
-__device__ __forceinline__ vec3 conic_normal (double a, double b, vec3 x)
-{
-   vec3 n;
-   n.x = (a - 1) * x.x + 0.5 * b;
-   n.y = -x.y;
-   n.z = -x.z;
-   normalize (n);
-   return n;
-}
-
-__device__ __forceinline__ void blur_normal (vec3 &n, double blur, Stream &st)
-{
-   double n_y = n.y, n_z = n.z;
-   double len = sqrt (n_y * n_y + n_z * n_z);
-   vec3 perp = v3 (0.0, n_z / len, -n_y / len);
-   double phi = (2.0 * MXG_PI) * uniform (st);
-   perp = rotate_unit_vector (perp, n, phi);
-   phi = blur * (1.0 / 3600.0 * MXG_PI / 180.0);
-   phi = phi * gaussian (st);
-   n = rotate_unit_vector (n, perp, phi);
-}
-
-// Reflection at a surface point x already on conic k of shell S: normal
-// blur, Fresnel draw, specular reflection, WFold scatter (hrma.c:488-545).
-// 0 reflected, -1 absorbed or scattered out.
-__device__ __forceinline__ int reflect_at (const mxg_hrma_cfg &h, const mxg_shell_cfg &S, int k,
-                                           vec3 x, vec3 &p, double energy, double beta,
-                                           double delta, double corr,
-                                           const float *__restrict__ fa,
-                                           const double *__restrict__ da,
-                                           const float *__restrict__ e_alphas, Stream &st)
-{
-   vec3 normal = conic_normal (S.conic_a[k], S.conic_b[k], x);
-   if (h.use_blur)
-     blur_normal (normal, S.blur[k], st);
-   double p_dot_n = dot (p, normal);
-   if (h.ideal == 0)
-     {
-        double r = uniform (st);
-        double rfl = reflectivity (fabs (p_dot_n), beta, delta);
-        if (r >= rfl * corr)
-          return -1;
-     }
-   p = ax1_bx2 (1.0, p, -2.0 * p_dot_n, normal);
-   if (h.use_wfold == 0)
-     return 0;
-   double r = uniform (st);
-   double dg = wfold_interp (S.wfold[k], fa, da, e_alphas, energy, -p_dot_n, r);
-   dg *= S.scat_factor[k];
-   if (dg > MXG_PI / 4) return -1;
-   if (uniform (st) < 0.5) dg = -dg;
-   p = rotate_unit_vector (p, cross (p, normal), dg);
-   return 0;
-}
-
-// Copies the mirror configuration (a multiple of 4 bytes) into LDS.
-__device__ __forceinline__ void stage_hrma_cfg (const mxg_config *__restrict__ cfg, unsigned char *lds)
-{
-   constexpr uint32_t cfg_words = sizeof (mxg_hrma_cfg) / 4;
-   const uint32_t *src = (const uint32_t *) &cfg->hrma;
-   uint32_t *dst = (uint32_t *) lds;
-   for (uint32_t w = threadIdx.x; w < cfg_words; w += blockDim.x) dst[w] = src[w];
-}
-
 // The mirror runs as two kernels so that the reflection work runs on full
 // wavefronts: about half of the photons sampled over the aperture annulus
 // miss the paraboloid, and in a single kernel their lanes would idle through
@@ -131,10 +14,12 @@ __device__ __forceinline__ void stage_hrma_cfg (const mxg_config *__restrict__ c
 //                   intersection and reflection, post-collimator struts, for
 //                   the middle list.
 //
-// The stream position is the Philox block counter c2 and the index of the
-// next unused word of the current block, packed as (c2 << 3) | pos.  No
-// Gaussian variate is pending at the split (none is drawn before it), and
-// the buffered block is recomputed from c2 - 1 on resumption.
+// Survivors go to the next active list in input order (SegWriter +
+// k_compact).
+#include "hrma_stage.h"
+
+namespace mxg {
+
 #if MXG_ENTER_WAVES > 0
 __global__ __launch_bounds__ (kBlock, MXG_ENTER_WAVES)
 #else
@@ -160,87 +45,12 @@ void k_hrma_enter (const mxg_config *__restrict__ cfg, Soa s, ListIO io,
         if (i < range.hi)
           {
              id = io.idx_in[i];
-             uint32_t flags = s.flags[id];
-             if ((flags & BAD_PHOTON_MASK) == 0)
-               {
-                  Stream st;
-                  stream_init (st, seed, s.tag[id], MXG_STAGE_MIRROR, source_id);
-                  if ((h.ideal == 0) && (uniform (st) > h.vignetting))
-                    {
-                       s.flags[id] = flags | PHOTON_MIRROR_VBLOCKED;
-                    }
-                  else
-                    {
-                       vec3 x, p = v3 (s.p0[id], s.p1[id], s.p2[id]);
-
-                       // aperture: shell by cumulative area, radius, open quadrant
-                       int si;
-                       for (;;)
-                         {
-                            double r = uniform (st);
-                            si = -1;
-                            for (int k = 0; k < MXG_NUM_SHELLS; k++)
-                              if (r < h.shell[k].area_fraction) { si = k; break; }
-                            if (si >= 0) break;
-                         }
-                       const mxg_shell_cfg &S = h.shell[si];
-                       double radius = S.min_radius + (S.max_radius - S.min_radius) * uniform (st);
-                       double theta;
-                       unsigned int quad;
-                       do
-                         {
-                            theta = uniform (st);
-                            quad = (unsigned int) (4.0 * theta);
-                         }
-                       while (0 == (S.shutter_bitmap & (1u << quad)));
-                       theta = (2.0 * MXG_PI) * (theta - 1.0 / 8.0);
-                       double sn, cs;
-                       sincos (theta, &sn, &cs);
-                       x.z = radius * cs;
-                       x.y = radius * sn;
-                       x.x = S.front_position;
-                       x.z -= S.to_osac[0][2];
-                       x.y -= S.to_osac[0][1];
-                       if (source_distance > 0.0)
-                         {
-                            p = ax1_bx2 (1.0, x, source_distance, p);
-                            normalize (p);
-                         }
-                       if (h.use_struts && in_strut_shadow (h, x, p, h.pre_struts))
-                         flags |= PHOTON_MIRROR_VBLOCKED;
-
-                       x = add (x, ld3 (S.to_osac[0]));
-                       x = m3_vmul (S.fwd[0], x);
-                       p = m3_vmul (S.fwd[0], p);
-
-                       if (-1 == conic_intersection (S.conic_a[0], S.conic_b[0], S.conic_c[0], x, p,
-                                                     S.conic_xmin[0], S.conic_xmax[0]))
-                         flags |= PHOTON_UNREFLECTED;
-                       else
-                         {
-                            keep = true;
-                            rng_pos[id] = (st.c2 << 3) | (uint32_t) st.pos;
-                         }
-                       s.x0[id] = x.x; s.x1[id] = x.y; s.x2[id] = x.z;
-                       s.p0[id] = p.x; s.p1[id] = p.y; s.p2[id] = p.z;
-                       s.shell[id] = (uint8_t) si;
-                       s.flags[id] = flags;
-                    }
-               }
+             keep = hrma_enter_one (h, s, id, rng_pos, source_distance, seed, source_id);
           }
         survivors.add (keep, id);
      }
    survivors.finish (io);
 }
-
-// Offsets (in floats) of the tables staged in LDS by k_hrma_reflect.
-struct HrmaLds
-{
-   uint32_t opt;                  // energies, betas, deltas: 3 * opt_n
-   uint32_t corr_e[MXG_NUM_SHELLS], corr_f[MXG_NUM_SHELLS];
-   uint32_t ea[MXG_NUM_SHELLS][2];  // e_alpha of each WFold table (float32 on disk: exact)
-   uint32_t floats;               // staged floats after the configuration copy
-};
 
 // The block stages the mirror configuration and its small, hot tables (the
 // binary-search targets of every photon) in LDS once, then loops over its
@@ -255,28 +65,7 @@ void k_hrma_reflect (const mxg_config *__restrict__ cfg, const float *__restrict
 {
    extern __shared__ __align__ (16) unsigned char lds[];
    float *ltab = (float *) (lds + sizeof (mxg_hrma_cfg));
-   stage_hrma_cfg (cfg, lds);
-   {
-      const mxg_hrma_cfg &gh = cfg->hrma;
-      for (uint32_t w = threadIdx.x; w < gh.opt_n; w += blockDim.x)
-        {
-           ltab[lay.opt + w] = fa[gh.opt_e_off + w];
-           ltab[lay.opt + gh.opt_n + w] = fa[gh.opt_beta_off + w];
-           ltab[lay.opt + 2 * gh.opt_n + w] = fa[gh.opt_delta_off + w];
-        }
-      for (int sh = 0; sh < MXG_NUM_SHELLS; sh++)
-        {
-           const mxg_shell_cfg &S = gh.shell[sh];
-           for (uint32_t w = threadIdx.x; w < S.corr_n; w += blockDim.x)
-             {
-                ltab[lay.corr_e[sh] + w] = fa[S.corr_e_off + w];
-                ltab[lay.corr_f[sh] + w] = fa[S.corr_f_off + w];
-             }
-           for (int k = 0; k < 2; k++)
-             for (uint32_t w = threadIdx.x; w < S.wfold[k].n_arrays; w += blockDim.x)
-               ltab[lay.ea[sh][k] + w] = (float) da[S.wfold[k].e_alphas_off + w];
-        }
-   }
+   stage_hrma_tables (cfg, fa, da, lay, lds, ltab);
    __syncthreads ();
 
    const mxg_hrma_cfg &h = *(const mxg_hrma_cfg *) lds;
@@ -295,70 +84,7 @@ void k_hrma_reflect (const mxg_config *__restrict__ cfg, const float *__restrict
         if (i < range.hi)
           {
              id = io.idx_in[i];
-             uint32_t flags = s.flags[id];
-             const int si = s.shell[id];
-             const mxg_shell_cfg &S = h.shell[si];
-             Stream st;
-             stream_init (st, seed, s.tag[id], MXG_STAGE_MIRROR, source_id);
-             {
-                uint32_t w = rng_pos[id];
-                st.c2 = w >> 3;
-                st.pos = (int) (w & 7u);
-                if (st.pos < 4)
-                  philox (st.c0, st.c1, st.c2 - 1u, st.c3, st.k0, st.k1, st.b0, st.b1, st.b2, st.b3);
-             }
-             vec3 x = v3 (s.x0[id], s.x1[id], s.x2[id]);
-             vec3 p = v3 (s.p0[id], s.p1[id], s.p2[id]);
-             double energy = s.energy[id];
-
-             double beta = 0.0, delta = 1.0, corr = 1.0;
-             if (h.opt_n != 0)
-               {
-                  float ef = (float) energy, fb, fd;
-                  interp_f2 (ef, opt_e, opt_b, opt_d, h.opt_n, fb, fd);
-                  beta = fb;
-                  delta = fd;
-                  if (h.use_scale)
-                    {
-                       corr = interp_f (ef, ltab + lay.corr_e[si], ltab + lay.corr_f[si], S.corr_n);
-                       corr = sqrt (corr);
-                    }
-               }
-
-             if (0 != reflect_at (h, S, 0, x, p, energy, beta, delta, corr, fa, da,
-                                  ltab + lay.ea[si][0], st))
-               flags |= PHOTON_UNREFLECTED;
-             else
-               {
-                  p = m3_vmul (S.bwd[0], p);
-                  x = m3_vmul (S.bwd[0], x);
-                  x = sub (x, ld3 (S.to_osac[0]));
-                  if (h.use_struts && in_strut_shadow (h, x, p, h.cap_struts))
-                    flags |= PHOTON_MIRROR_VBLOCKED;
-                  else
-                    {
-                       x = add (x, ld3 (S.to_osac[1]));
-                       p = m3_vmul (S.fwd[1], p);
-                       x = m3_vmul (S.fwd[1], x);
-                       if ((-1 == conic_intersection (S.conic_a[1], S.conic_b[1], S.conic_c[1], x, p,
-                                                      S.conic_xmin[1], S.conic_xmax[1]))
-                           || (0 != reflect_at (h, S, 1, x, p, energy, beta, delta, corr, fa, da,
-                                                ltab + lay.ea[si][1], st)))
-                         flags |= PHOTON_UNREFLECTED;
-                       else
-                         {
-                            p = m3_vmul (S.bwd[1], p);
-                            x = m3_vmul (S.bwd[1], x);
-                            x = sub (x, ld3 (S.to_osac[1]));
-                            if (h.use_struts && in_strut_shadow (h, x, p, h.post_struts))
-                              flags |= PHOTON_MIRROR_VBLOCKED;
-                         }
-                    }
-               }
-             s.x0[id] = x.x; s.x1[id] = x.y; s.x2[id] = x.z;
-             s.p0[id] = p.x; s.p1[id] = p.y; s.p2[id] = p.z;
-             s.flags[id] = flags;
-             keep = (flags & BAD_PHOTON_MASK) == 0;
+             keep = hrma_reflect_one (h, ltab, lay, opt_e, opt_b, opt_d, fa, da, s, id, rng_pos, seed, source_id);
           }
         survivors.add (keep, id);
      }

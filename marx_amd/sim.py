@@ -245,10 +245,12 @@ class Tracer:
         check(self.lib.mxg_get_schedule(ctypes.byref(s)), "mxg_get_schedule")
         return s.as_dict()
 
-    def set_schedule(self, blocks_per_cu=None, reflect_waves=None, **per_kernel):
+    def set_schedule(self, blocks_per_cu=None, reflect_waves=None, persistent_tile=None,
+                     persistent_blocks_per_cu=None, **per_kernel):
         """Sets the launch schedule: blocks_per_cu for every kernel, or per
-        kernel (source=, enter=, reflect=, grating=, detect=), and the
-        reflection kernel's waves per SIMD."""
+        kernel (source=, enter=, reflect=, grating=, detect=), the
+        reflection kernel's waves per SIMD, and the persistent pass's tile
+        (0: the staged pass) and blocks per CU."""
         s = Schedule()
         check(self.lib.mxg_get_schedule(ctypes.byref(s)), "mxg_get_schedule")
         for i, k in enumerate(Schedule.KERNELS):
@@ -258,6 +260,10 @@ class Tracer:
                 s.blocks_per_cu[i] = int(per_kernel[k])
         if reflect_waves is not None:
             s.reflect_waves = int(reflect_waves)
+        if persistent_tile is not None:
+            s.persistent_tile = int(persistent_tile)
+        if persistent_blocks_per_cu is not None:
+            s.persistent_blocks_per_cu = int(persistent_blocks_per_cu)
         check(self.lib.mxg_set_schedule(ctypes.byref(s)), "mxg_set_schedule")
 
     def download(self, photons):

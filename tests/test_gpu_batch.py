@@ -1,7 +1,8 @@
 """Batched point sources (mxg_trace_batch): one device pass over many sources
 with uneven photon counts, against the CPU oracle; the same photons as a
-per-source loop; independent of the launch schedule; HIP-graph replays equal
-to eager passes; the autotuner installs a valid schedule."""
+per-source loop; independent of the launch schedule, staged or persistent;
+HIP-graph replays equal to eager passes; the autotuner installs a valid
+schedule."""
 import numpy as np
 import pytest
 
@@ -139,10 +140,11 @@ def test_autotuner_installs_a_valid_schedule(gpu, tmp_path):
     assert sched == t.get_schedule()
     assert all(1 <= sched[k] <= 32 for k in ("source", "enter", "reflect", "grating", "detect"))
     assert sched["reflect_waves"] in (2, 3, 4)
+    assert sched["persistent_tile"] in (0, 256, 512, 1024, 2048)
     assert ms > 0
     t.trace_batch(dirs, counts, use_graph=True)
     assert_same_records(_download(t, n), ref, "autotuned graph pass")
-    t.set_schedule(blocks_per_cu=8, reflect_waves=4)
+    t.set_schedule(blocks_per_cu=8, reflect_waves=4, persistent_tile=0)
 
 
 @pytest.mark.gpu
@@ -169,3 +171,32 @@ def test_graph_is_recaptured_after_the_store_grows(gpu, tmp_path):
     eager = _download(t, n)
     assert_same_records(again, eager, "graph pass after the store grew")
     assert_same_records(first, eager, "graph pass before the store grew")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("grating,det,dither", [("NONE", "ACIS-I", "NONE"), ("HETG", "ACIS-S", "INTERNAL"),
+                                                ("LETG", "HRC-S", "INTERNAL"), ("NONE", "HRC-I", "NONE"),
+                                                ("HETG", "NONE", "NONE")])
+def test_persistent_pass_equals_the_staged_pass(gpu, tmp_path, grating, det, dither):
+    """The batched pass as one persistent launch (blocks taking tiles of
+    photons from a work counter, every stage on a tile back to back) gives
+    the staged pass's records, detected list and stage counts, for tiles of
+    256 to 4096 photons at 1 to 8 blocks per CU, eager and graph-launched."""
+    t, dirs, counts = _batch(tmp_path, 48, grating=grating, det=det, dither=dither)
+    n = int(counts.sum())
+    t.set_schedule(blocks_per_cu=8, reflect_waves=4, persistent_tile=0)
+    ndet, _ = t.trace_batch(dirs, counts, tag_start=7, exposure=2.0e4, use_graph=False)
+    ref = _download(t, n)
+    ref_counts = t.stage_counts()
+    assert ref_counts[3] == ndet
+    try:
+        for tile, bpc in ((256, 1), (1024, 2), (4096, 4), (512, 8)):
+            t.set_schedule(persistent_tile=tile, persistent_blocks_per_cu=bpc)
+            for graph in (False, True, True):
+                print(f"persistent {grating}/{det}: tile {tile} x {bpc}, graph {graph}", flush=True)
+                nd, _ = t.trace_batch(dirs, counts, tag_start=7, exposure=2.0e4, use_graph=graph)
+                assert nd == ndet
+                assert t.stage_counts() == ref_counts
+                assert_same_records(_download(t, n), ref, f"persistent tile {tile} x {bpc}, graph {graph}")
+    finally:
+        t.set_schedule(persistent_tile=0)

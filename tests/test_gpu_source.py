@@ -135,17 +135,25 @@ def test_full_chain_aos_and_device(gpu, tmp_path, workload):
 def test_user_source_parity(gpu, tmp_path, args, dither, exposure):
     """A USER plug-in (marx_create_photons over s-user.c): energies drawn on
     the device, rays from the plug-in on the host, then cut, tags, dither
-    and sort, against the oracle calling the same plug-in."""
+    and sort, against the oracle calling the same plug-in.  The plug-in's
+    own times count from 0 while the cut is at the observation start plus
+    the exposure (source.c:295-309): TStart = 2100 reads as 2100 s from
+    MJDref, within reach of the plug-in's clock, so the cut is exercised."""
+    import ctypes
     from tests.user_plugin import build
     path, plug = build(str(tmp_path))
     t = Tracer(seed=5, GratingType="NONE", DetectorType="NONE", SourceType="USER", SpectrumType="FLAT",
                MinEnergy=1.0, MaxEnergy=10.0, DitherModel=dither, UserSourceFile=path, UserSourceArgs=args,
-               SourceRA=ON_AXIS["SourceRA"] + 0.01, SourceDEC=ON_AXIS["SourceDEC"])
+               SourceRA=ON_AXIS["SourceRA"] + 0.01, SourceDEC=ON_AXIS["SourceDEC"], TStart=2100.0)
     t.init_source(SourceFlux=20.0)
+    tstart = ctypes.c_double.in_dll(t.lib, "_Marx_TStart_MJDsecs").value
+    assert tstart == 2100.0
     n = 20000
     pg, po = Photons(n), Photons(n)
     ng = t.create_photons(pg, n, exposure)
-    no = Oracle(t).create_user(po, n, plug, exposure_time=exposure)
+    no = Oracle(t).create_user(po, n, plug, exposure_time=exposure, tstart=tstart)
+    if "times" in args:
+        assert no < n                          # the cut ends the chunk
     assert ng == no > 50
     assert np.array_equal(pg.sorted_index(), po.sorted_index())
     a, b = pg.active, po.active
