@@ -551,6 +551,28 @@ typedef struct
 mxg_soa_view;
 int mxg_get_soa (mxg_soa_view *view);
 
+/* No counterpart in the reference: mxg_trace_device for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The photons of the chunk go to
+ * the caller's columns *store (device pointers, capacity >= num; n is
+ * ignored); every list, counter and scan buffer lives in a fixed device
+ * array (29 MB) that the library holds on each GPU it runs on until the
+ * process ends.  The configuration is the library's device copy: at least
+ * one allocating call (mxg_trace_device, marx_mirror_reflect, ...) must
+ * have run since the last init, so that its device buffers exist and are
+ * large enough; the entry point then copies a changed configuration into
+ * them, else it fails with a message.  Limits: num <=
+ * MXG_NOALLOC_MAX_PHOTONS (marx.par bounds dNumRays, the chunk size, by
+ * 1000000); a larger chunk fails with a message.  mxg_init must have run.
+ * The entry point calls no allocator; it synchronises the library's stream
+ * before returning.  All calls on one GPU share that array: calls that
+ * overlap on one GPU (other threads) race.  Same photons and records as
+ * mxg_trace_device with the same arguments. */
+#define MXG_NOALLOC_MAX_PHOTONS 1048576u
+int mxg_trace_device_noalloc (const mxg_soa_view *store, unsigned int num, unsigned int tag_start,
+                              double start_time, double exposure_time, unsigned int *num_collected,
+                              unsigned int *num_detected);
+
 /* Name of the device the library runs on, and its compute-unit count. */
 int mxg_device_info (char *name, unsigned int namelen, int *num_cus);
 

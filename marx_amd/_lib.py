@@ -176,6 +176,9 @@ _SIGNATURES = [
     ("mxg_get_seed", ctypes.c_ulonglong, []),
     ("mxg_set_lazy_sync", None, [ctypes.c_int]),
     ("mxg_get_soa", ctypes.c_int, [ctypes.POINTER(SoaView)]),
+    ("mxg_trace_device_noalloc", ctypes.c_int, [ctypes.POINTER(SoaView), ctypes.c_uint, ctypes.c_uint,
+                                                ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_uint),
+                                                ctypes.POINTER(ctypes.c_uint)]),
     ("marx_create_source", ctypes.c_void_p, [ctypes.c_void_p]),
     ("marx_open_source", ctypes.c_int, [ctypes.c_void_p]),
     ("marx_close_source", ctypes.c_int, [ctypes.c_void_p]),

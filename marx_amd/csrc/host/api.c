@@ -677,6 +677,38 @@ int mxg_trace_device (unsigned int num, unsigned int tag_start, double start_tim
    return 0;
 }
 
+/* The device-resident trace into caller-owned columns with no allocation
+ * (no counterpart in the reference). */
+int mxg_trace_device_noalloc (const mxg_soa_view *store, unsigned int num, unsigned int tag_start,
+                              double start_time, double exposure_time, unsigned int *num_collected,
+                              unsigned int *num_detected)
+{
+   if (store == NULL)
+     {
+        mxg_error ("mxg_trace_device_noalloc: no columns given");
+        return -1;
+     }
+   if (Mirror_Type != MARX_MIRROR_HRMA)
+     {
+        mxg_error ("Mirror not initialized.");
+        return -1;
+     }
+   if ((Mxg_Cfg.src.type == MXG_SOURCE_RAYFILE) || (Mxg_Cfg.src.type == MXG_SOURCE_USER))
+     {
+        mxg_error ("mxg_trace_device_noalloc creates its photons on the device; a RAYFILE or USER source "
+                   "makes its rays on the host, through marx_create_photons");
+        return -1;
+     }
+   if (-1 == mxgd_trace_noalloc (store, num, tag_start, start_time, exposure_time, Seed, &Mxg_Cfg,
+                                 mxg_farena_base (), mxg_farena_size (), mxg_darena_base (),
+                                 mxg_darena_size (), Mxg_Cfg_Generation, num_collected, num_detected))
+     {
+        mxg_error ("%s", mxgd_last_error ());
+        return -1;
+     }
+   return 0;
+}
+
 /* ---- batched point sources -------------------------------------------------- */
 
 typedef struct

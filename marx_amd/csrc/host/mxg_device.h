@@ -176,6 +176,16 @@ int mxgd_sort_store_by_energy (unsigned int n, unsigned int *index, double *ener
  * energies into host arrays of cap entries; *n its length.  Synchronises. */
 int mxgd_download_active (unsigned int *index, double *energies, unsigned int cap, unsigned int *n);
 
+/* The trace of mxg_trace_device into caller-owned columns v (capacity >=
+ * n), with every list and counter in the library's fixed device scratch and
+ * the configuration in the device buffers an earlier call allocated: no
+ * allocation.  n <= MXG_NOALLOC_MAX_PHOTONS.  Synchronises. */
+int mxgd_trace_noalloc (const mxg_soa_view *v, unsigned int n, unsigned int tag_start,
+                        double start_time, double exposure_time, unsigned long long seed,
+                        const mxg_config *cfg, const float *fa, unsigned int nf, const double *da,
+                        unsigned int nd, unsigned int generation, unsigned int *n_collected,
+                        unsigned int *n_detected);
+
 #ifdef __cplusplus
 }
 #endif

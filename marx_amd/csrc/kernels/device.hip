@@ -332,6 +332,54 @@ extern "C" {
 
 const char *mxgd_last_error (void) { return g_err; }
 
+/* ---- the allocation-free trace (mxg_trace_device_noalloc) --------------------------- */
+
+namespace {
+
+// Fixed-size device memory for the allocation-free trace: the per-photon
+// lists, counters and scan buffers of one chunk of at most kNoAllocPhotons
+// photons (29 MB).
+constexpr size_t kNoAllocPhotons = MXG_NOALLOC_MAX_PHOTONS;
+constexpr size_t na256 (size_t n) { return (n + 255) & ~(size_t) 255; }
+constexpr size_t kNaCounts = 0;
+constexpr size_t kNaSegCount = kNaCounts + na256 (64 * sizeof (uint32_t));
+constexpr size_t kNaIdx0 = kNaSegCount + na256 (kMaxSegments * sizeof (uint32_t));
+constexpr size_t kNaIdx1 = kNaIdx0 + na256 (kNoAllocPhotons * sizeof (uint32_t));
+constexpr size_t kNaMid = kNaIdx1 + na256 (kNoAllocPhotons * sizeof (uint32_t));
+constexpr size_t kNaSeg = kNaMid + na256 (kNoAllocPhotons * sizeof (uint32_t));
+constexpr size_t kNaRng = kNaSeg + na256 (kNoAllocPhotons * sizeof (uint32_t));
+constexpr size_t kNaTicks = kNaRng + na256 (kNoAllocPhotons * sizeof (uint32_t));
+constexpr size_t kNaTiles = kNaTicks + na256 (kNoAllocPhotons * sizeof (uint64_t));
+constexpr size_t kNaBytes = kNaTiles + na256 ((kNoAllocPhotons / 2048 + 2) * sizeof (uint64_t));
+
+constexpr int kMaxDevices = 16;
+unsigned char *g_scratch_ptr[kMaxDevices];      // device address of `scratch`, per device
+
+} // namespace
+
+// The scratch memory itself: one copy per GPU, held until the process ends.
+// Host code never uses this name as an address (it is a host-side stand-in):
+// g_scratch_ptr holds each device's address from hipGetSymbolAddress.
+static __device__ __align__ (256) unsigned char scratch[kNaBytes];
+
+namespace {
+
+// At set-up, once per device with that device current.
+int scratch_setup (int dev)
+{
+   if ((dev < 0) || (dev >= kMaxDevices))
+     return fail ("device %d out of the %d supported", dev, kMaxDevices);
+   if (g_scratch_ptr[dev] != nullptr) return 0;
+   void *ptr = nullptr;
+   hipError_t e = hipGetSymbolAddress (&ptr, HIP_SYMBOL (scratch));
+   if ((e != hipSuccess) || (ptr == nullptr))
+     return fail ("hipGetSymbolAddress of the trace scratch failed: %s", hipGetErrorString (e));
+   g_scratch_ptr[dev] = (unsigned char *) ptr;
+   return 0;
+}
+
+} // namespace
+
 int mxgd_init (void)
 {
    if (g_inited) return 0;
@@ -358,6 +406,7 @@ int mxgd_init (void)
    HIPCHECK (hipMalloc (&d_counts, 64 * sizeof (uint32_t)));
    HIPCHECK (hipMalloc (&d_seg_count, kMaxSegments * sizeof (uint32_t)));
    HIPCHECK (hipMemset (d_counts, 0, 64 * sizeof (uint32_t)));
+   if (scratch_setup (dev)) return -1;
    g_inited = 1;
    return 0;
 }
@@ -1112,6 +1161,134 @@ int mxgd_download_active (unsigned int *index, double *energies, unsigned int ca
    HIPCHECK (hipMemcpyAsync (index, d_idx[g_cur], (size_t) m * sizeof (uint32_t), hipMemcpyDeviceToHost, g_stream));
    HIPCHECK (hipMemcpyAsync (energies, d_gather, (size_t) m * sizeof (double), hipMemcpyDeviceToHost, g_stream));
    HIPCHECK (hipStreamSynchronize (g_stream));
+   return 0;
+}
+
+} // extern "C"
+
+/* ---- the allocation-free trace ------------------------------------------------------- */
+
+extern "C" {
+
+int mxgd_trace_noalloc (const mxg_soa_view *v, unsigned int n, unsigned int tag_start,
+                        double start_time, double exposure_time, unsigned long long seed,
+                        const mxg_config *cfg, const float *fa, unsigned int nf, const double *da,
+                        unsigned int nd, unsigned int generation, unsigned int *n_collected,
+                        unsigned int *n_detected)
+{
+   // mxg_init sets the library up (streams, the scratch's address); this
+   // entry point allocates nothing, so it does not do that itself
+   if (!g_inited) return fail ("mxg_init has not run: the allocation-free trace needs the library set up");
+   if (g_capturing) return fail ("the allocation-free trace cannot run during the library's graph capture");
+   if (n > kNoAllocPhotons)
+     return fail ("chunk of %u photons exceeds the %u photons of mxg_trace_device_noalloc's scratch", n,
+                  (unsigned int) kNoAllocPhotons);
+   if (v->capacity < n)
+     return fail ("caller's columns hold %u photons, %u asked for", v->capacity, n);
+   // the configuration: the library's device copy, brought up to date in
+   // place when the buffers an earlier call allocated hold it
+   if (!d_cfg || (g_cfg_gen != generation))
+     {
+        if (!d_cfg || (nf + 1 > d_fa_cap) || (nd + 1 > d_da_cap))
+          return fail ("the configuration's device buffers are missing or too small for this set-up: make "
+                       "one allocating call (mxg_trace_device, marx_mirror_reflect, ...) after the last "
+                       "init first");
+        HIPCHECK (hipMemcpyAsync (d_cfg, cfg, sizeof (mxg_config), hipMemcpyHostToDevice, g_stream));
+        if (nf) HIPCHECK (hipMemcpyAsync (d_fa, fa, nf * sizeof (float), hipMemcpyHostToDevice, g_stream));
+        if (nd) HIPCHECK (hipMemcpyAsync (d_da, da, nd * sizeof (double), hipMemcpyHostToDevice, g_stream));
+        HIPCHECK (hipStreamSynchronize (g_stream));
+        memcpy (&g_hcfg, cfg, sizeof (mxg_config));
+        g_cfg_gen = generation;
+     }
+   int dev = 0;
+   HIPCHECK (hipGetDevice (&dev));
+   if (scratch_setup (dev)) return -1;
+   unsigned char *base = g_scratch_ptr[dev];
+   uint32_t *counts = (uint32_t *) (base + kNaCounts);
+   uint32_t *seg_count = (uint32_t *) (base + kNaSegCount);
+   uint32_t *idx[2] = {(uint32_t *) (base + kNaIdx0), (uint32_t *) (base + kNaIdx1)};
+   uint32_t *mid = (uint32_t *) (base + kNaMid);
+   uint32_t *seg = (uint32_t *) (base + kNaSeg);
+   uint32_t *rng = (uint32_t *) (base + kNaRng);
+   uint64_t *ticks = (uint64_t *) (base + kNaTicks);
+   uint64_t *tiles = (uint64_t *) (base + kNaTiles);
+   const mxg_config *c_cfg = d_cfg;
+   const float *c_fa = d_fa;
+   const double *c_da = d_da;
+
+   Soa s;
+   s.energy = v->energy;
+   s.x0 = v->x[0]; s.x1 = v->x[1]; s.x2 = v->x[2];
+   s.p0 = v->p[0]; s.p1 = v->p[1]; s.p2 = v->p[2];
+   s.t = v->arrival_time;
+   s.flags = v->flags;
+   s.ypix = v->y_pixel; s.zpix = v->z_pixel; s.upix = v->u_pixel; s.vpix = v->v_pixel;
+   for (int k = 0; k < 6; k++) s.dith[k] = v->dither[k];
+   s.pi = v->pi;
+   s.pha = v->pulse_height;
+   s.shell = v->mirror_shell;
+   s.ccd = v->ccd_num; s.region = v->detector_region; s.order = v->order;
+   for (int k = 0; k < 4; k++) s.sup[k] = v->support_orders[k];
+   s.tag = v->tag;
+
+   // counters start at zero (one-thread kernels, not memsets): [0] and [1]
+   // the two lists, [2] the source's exposure cut, [4] the mirror's middle
+   // list, [6] the photons collected
+   for (int k = 0; k < 8; k++)
+     hipLaunchKernelGGL (k_set_u32, dim3 (1), dim3 (1), 0, g_stream, counts + k, 0u);
+   HIPCHECK (hipGetLastError ());
+   HIPCHECK (launch_source (c_cfg, c_fa, c_da, s, n, tag_start, exposure_time, start_time, seed, 0, ticks, tiles,
+                            counts + 2, counts + 0, idx[0], g_stream));
+   hipLaunchKernelGGL (k_copy_u32, dim3 (1), dim3 (1), 0, g_stream, counts + 6, counts + 0);
+   HIPCHECK (hipGetLastError ());
+
+   // stages: list idx[cur] (length counts[cur]) in, idx[1 - cur] out; grids
+   // sized by n, an upper bound of every list (the kernels read the
+   // lengths from device memory)
+   int cur = 0;
+   auto io = [&] (int k) {
+      ListIO l;
+      l.idx_in = idx[cur];
+      l.n_in = counts + cur;
+      l.idx_out = idx[1 - cur];
+      l.n_out = counts + (1 - cur);
+      l.seg = seg;
+      l.seg_count = seg_count;
+      uint32_t blocks = (n + kBlock - 1) / kBlock;
+      uint32_t cap = (uint32_t) (g_num_cus > 0 ? g_num_cus : 256) * g_sched.blocks_per_cu[k];
+      if (cap > kMaxSegments) cap = kMaxSegments;
+      l.grid = blocks < cap ? (blocks ? blocks : 1u) : cap;
+      return l;
+   };
+   auto clear_out = [&] (void) {
+      hipLaunchKernelGGL (k_set_u32, dim3 (1), dim3 (1), 0, g_stream, counts + (1 - cur), 0u);
+      return hipGetLastError ();
+   };
+   HIPCHECK (clear_out ());
+   HrmaScratch hs = {mid, counts + 4, rng, io (MXGD_K_REFLECT).grid, (int) g_sched.reflect_waves};
+   HIPCHECK (launch_hrma (c_cfg, cfg, c_fa, c_da, s, io (MXGD_K_ENTER), hs, n, cfg->src.distance, seed, 0,
+                          g_stream, nullptr));
+   cur = 1 - cur;
+   if ((cfg->grating.type == MARX_GRATING_HETG) || (cfg->grating.type == MARX_GRATING_LETG))
+     {
+        HIPCHECK (clear_out ());
+        HIPCHECK (launch_grating (c_cfg, c_fa, c_da, s, io (MXGD_K_GRATING), n, seed, 0,
+                                  cfg->grating.type == MARX_GRATING_LETG, g_stream));
+        cur = 1 - cur;
+     }
+   if (cfg->det.type > 0)
+     {
+        HIPCHECK (clear_out ());
+        HIPCHECK (launch_detect (c_cfg, c_fa, c_da, s, io (MXGD_K_DETECT), n, start_time, seed, 0,
+                                 cfg->det.fef_on != 0, detector_kind (*cfg), g_stream));
+        cur = 1 - cur;
+     }
+   uint32_t out[2] = {0, 0};
+   HIPCHECK (hipMemcpyAsync (out, counts + 6, sizeof (uint32_t), hipMemcpyDeviceToHost, g_stream));
+   HIPCHECK (hipMemcpyAsync (out + 1, counts + cur, sizeof (uint32_t), hipMemcpyDeviceToHost, g_stream));
+   HIPCHECK (hipStreamSynchronize (g_stream));
+   if (n_collected) *n_collected = out[0];
+   if (n_detected) *n_detected = out[1];
    return 0;
 }
 
