@@ -31,6 +31,26 @@
  *   num_theta_values, theta float-arena offset                           */
 #define MXG_FOLD_STRIDE 8
 
+/* Bucket index of a strictly increasing, positive search table: for
+ * positive values the IEEE bit pattern orders like the value (and spaces
+ * like its logarithm), so the keys (bits - base) >> shift split the table
+ * into nb buckets; float-arena entries off .. off + nb hold, for bucket j,
+ * the number of table entries whose key is below j (entry nb: all of
+ * them).  A search then looks at one bucket's entries.  ok = 0: the table
+ * is not strictly increasing and positive, search it whole.  Float keys
+ * for tables searched with float abscissae, double keys (the float values
+ * widened) for those searched with double ones. */
+typedef struct
+{
+   unsigned long long base;
+   unsigned int off, nb, shift, ok;
+}
+mxg_bucket;
+/* largest bucket counts: optical constants, correction factors, e_alpha */
+#define MXG_BUCKETS_OPT 256
+#define MXG_BUCKETS_CORR 64
+#define MXG_BUCKETS_EALPHA 32
+
 typedef struct
 {
    unsigned int n_arrays;      /* arrays sorted by e_alpha, duplicates removed */
@@ -58,6 +78,8 @@ typedef struct
    unsigned int corr_n, corr_e_off, corr_f_off;   /* float arena */
    unsigned int mirror_number;
    mxg_wfold wfold[2];
+   mxg_bucket corr_bucket;      /* of the correction table's energies (float keys) */
+   mxg_bucket ea_bucket[2];     /* of each WFold table's e_alpha values (double keys) */
 }
 mxg_shell_cfg;
 
@@ -79,6 +101,7 @@ typedef struct
    mxg_shell_cfg shell[MXG_NUM_SHELLS];
    mxg_strut pre_struts[2], cap_struts[2], post_struts[2];
    double strut_cos, strut_sin;    /* cos, sin of the 30 degree strut clocking */
+   mxg_bucket opt_bucket;          /* of the optical constants' energies (float keys) */
 }
 mxg_hrma_cfg;
 

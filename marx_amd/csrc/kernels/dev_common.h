@@ -213,6 +213,94 @@ __device__ __forceinline__ unsigned int search_w (double x, const T *__restrict_
    return n0;
 }
 
+// search_f / search_w of a table with its bucket index (mxg_bucket; cnt: its
+// counts, possibly in LDS): the same index, from one bucket's entries.  For
+// a strictly increasing table the bisection returns the first index whose
+// entry is >= x, except for x equal to the first entry, which it never
+// probes: 1 there.
+__device__ __forceinline__ unsigned int search_fb (float x, const float *__restrict__ xp, unsigned int n,
+                                                   const mxg_bucket &b, const float *__restrict__ cnt)
+{
+   if (!b.ok) return search_f (x, xp, n);
+   const float x0 = xp[0];
+   if (!(x > x0))
+     {
+        if (x == x0) return 1;
+        if (x < x0) return 0;
+        return search_f (x, xp, n);          // NaN
+     }
+   const uint64_t k = ((uint64_t) __float_as_uint (x) - b.base) >> b.shift;
+   const uint32_t j = (k < b.nb) ? (uint32_t) k : b.nb - 1;
+   unsigned int lo = (unsigned int) cnt[j], hi = (unsigned int) cnt[j + 1];
+   while (lo < hi)
+     {
+        unsigned int mid = (lo + hi) >> 1;
+        if (xp[mid] < x) lo = mid + 1;
+        else hi = mid;
+     }
+   return lo;
+}
+
+__device__ __forceinline__ unsigned int search_wb (double x, const float *__restrict__ xp, unsigned int n,
+                                                   const mxg_bucket &b, const float *__restrict__ cnt)
+{
+   if (!b.ok) return search_w (x, xp, n);
+   const double x0 = (double) xp[0];
+   if (!(x > x0))
+     {
+        if (x == x0) return 1;
+        if (x < x0) return 0;
+        return search_w (x, xp, n);
+     }
+   const uint64_t k = ((uint64_t) __double_as_longlong (x) - b.base) >> b.shift;
+   const uint32_t j = (k < b.nb) ? (uint32_t) k : b.nb - 1;
+   unsigned int lo = (unsigned int) cnt[j], hi = (unsigned int) cnt[j + 1];
+   while (lo < hi)
+     {
+        unsigned int mid = (lo + hi) >> 1;
+        if ((double) xp[mid] < x) lo = mid + 1;
+        else hi = mid;
+     }
+   return lo;
+}
+
+// interp_f / interp_f2 with a bucket-indexed search (the same values)
+__device__ __forceinline__ float interp_fb (float x, const float *__restrict__ xp, const float *__restrict__ yp,
+                                            unsigned int n, const mxg_bucket &bk, const float *__restrict__ cnt)
+{
+   if (n == 1) return yp[0];
+   unsigned int n1 = search_fb (x, xp, n, bk, cnt);
+   unsigned int n0 = n1 - 1;
+   if ((n1 < n) && (x == xp[n1])) return yp[n1];
+   if (n1 == n) { n1--; n0--; }
+   if (n1 == 0) n0 = 1;
+   double x0 = xp[n0], x1 = xp[n1];
+   if (x1 == x0) return yp[n1];
+   float y0 = yp[n0];
+   float dy = yp[n1] - y0;
+   return (float) ((double) y0 + (double) dy / (x1 - x0) * ((double) x - x0));
+}
+
+__device__ __forceinline__ void interp_f2b (float x, const float *__restrict__ xp, const float *__restrict__ ya,
+                                            const float *__restrict__ yb, unsigned int n, const mxg_bucket &bk,
+                                            const float *__restrict__ cnt, float &a, float &b)
+{
+   if (n == 1) { a = ya[0]; b = yb[0]; return; }
+   unsigned int n1 = search_fb (x, xp, n, bk, cnt);
+   unsigned int n0 = n1 - 1;
+   if ((n1 < n) && (x == xp[n1])) { a = ya[n1]; b = yb[n1]; return; }
+   if (n1 == n) { n1--; n0--; }
+   if (n1 == 0) n0 = 1;
+   double x0 = xp[n0], x1 = xp[n1];
+   if (x1 == x0) { a = ya[n1]; b = yb[n1]; return; }
+   float y0 = ya[n0];
+   float dy = ya[n1] - y0;
+   a = (float) ((double) y0 + (double) dy / (x1 - x0) * ((double) x - x0));
+   y0 = yb[n0];
+   dy = yb[n1] - y0;
+   b = (float) ((double) y0 + (double) dy / (x1 - x0) * ((double) x - x0));
+}
+
 __device__ __forceinline__ double interp_d (double x, const double *__restrict__ xp,
                                             const double *__restrict__ yp, unsigned int n)
 {
@@ -348,27 +436,6 @@ __device__ __forceinline__ cplx c_mul (cplx a, cplx b)
    cplx z; z.r = a.r * b.r - a.i * b.i; z.i = a.r * b.i + a.i * b.r; return z;
 }
 
-__device__ __forceinline__ cplx c_div (cplx z1, cplx z2)
-{
-   cplx z;
-   double r1 = z1.r, i1 = z1.i, r2 = z2.r, i2 = z2.i, ratio, denom;
-   if (fabs (r2) > fabs (i2))
-     {
-        ratio = i2 / r2;
-        denom = r2 + i2 * ratio;
-        z.r = (r1 + ratio * i1) / denom;
-        z.i = (i1 - r1 * ratio) / denom;
-     }
-   else
-     {
-        ratio = r2 / i2;
-        denom = r2 * ratio + i2;
-        z.r = (r1 * ratio + i1) / denom;
-        z.i = (i1 * ratio - r1) / denom;
-     }
-   return z;
-}
-
 __device__ __forceinline__ cplx c_a_bz (double a, double b, cplx z1)
 {
    cplx z; z.r = a + b * z1.r; z.i = b * z1.i; return z;
@@ -379,22 +446,12 @@ __device__ __forceinline__ cplx c_az1_bz2 (double a, cplx z1, double b, cplx z2)
    cplx z; z.r = a * z1.r + b * z2.r; z.i = a * z1.i + b * z2.i; return z;
 }
 
-__device__ __forceinline__ double c_abs (cplx z)
-{
-   double fr = fabs (z.r), fi = fabs (z.i), ratio;
-   if (fr > fi)
-     {
-        ratio = z.i / z.r;
-        return fr * sqrt (1.0 + ratio * ratio);
-     }
-   if (fi == 0.0) return 0.0;
-   ratio = z.r / z.i;
-   return fi * sqrt (1.0 + ratio * ratio);
-}
-
+// The principal square root (non-negative real part) of a, by the half-angle
+// formulas; |a| = sqrt (re^2 + im^2) (reflectivity's arguments are of order
+// one, far from overflow).
 __device__ __forceinline__ cplx c_sqrt (cplx a)
 {
-   double r = c_abs (a);
+   double r = sqrt (a.r * a.r + a.i * a.i);
    if (r == 0.0) return a;
    if (a.r >= 0.0)
      {
@@ -411,15 +468,24 @@ __device__ __forceinline__ cplx c_sqrt (cplx a)
    return a;
 }
 
+// Reflectivity of a surface of complex index n = 1 - delta + i beta for a
+// ray at angle theta to the normal: the mean of |E_perp|^2 and |E_par|^2,
+// E_perp = (cos - root) / (cos + root), E_par = (n^2 cos - root) /
+// (n^2 cos + root), root = sqrt (n^2 - sin^2) (reflect.c:39-77).  The
+// squared modulus of each ratio is taken as the ratio of the squared moduli,
+// which needs no complex division (the value agrees with the divided form to
+// rounding).
 __device__ __forceinline__ double reflectivity (double cos_theta, double beta, double delta)
 {
    cplx n; n.r = 1.0 - delta; n.i = beta;
    double sin_theta = sqrt (1.0 - cos_theta * cos_theta);
    cplx nsqr = c_mul (n, n);
    cplx root = c_sqrt (c_a_bz (-sin_theta * sin_theta, 1.0, nsqr));
-   cplx e_perp = c_div (c_a_bz (cos_theta, -1.0, root), c_a_bz (cos_theta, 1.0, root));
-   cplx e_par = c_div (c_az1_bz2 (cos_theta, nsqr, -1.0, root), c_az1_bz2 (cos_theta, nsqr, 1.0, root));
-   return 0.5 * (e_par.r * e_par.r + e_par.i * e_par.i + e_perp.r * e_perp.r + e_perp.i * e_perp.i);
+   cplx a1 = c_a_bz (cos_theta, -1.0, root), b1 = c_a_bz (cos_theta, 1.0, root);
+   cplx a2 = c_az1_bz2 (cos_theta, nsqr, -1.0, root), b2 = c_az1_bz2 (cos_theta, nsqr, 1.0, root);
+   double perp = (a1.r * a1.r + a1.i * a1.i) / (b1.r * b1.r + b1.i * b1.i);
+   double par = (a2.r * a2.r + a2.i * a2.i) / (b2.r * b2.r + b2.i * b2.i);
+   return 0.5 * (par + perp);
 }
 
 // ---- WFold scattering-angle tables -------------------------------------------------
@@ -441,17 +507,26 @@ __device__ __forceinline__ double fold_theta (const double *__restrict__ fa, con
 
 // e_alphas: the table's e_alpha values (float32 file values, exact in
 // float), possibly staged in LDS; fold arrays stay in the double arena
+// bk, cnt: the bucket index of a float e_alphas table (nullptr: search it
+// whole)
 template <class T>
 __device__ __forceinline__ double wfold_interp (const mxg_wfold &w, const float *__restrict__ fa,
                                                 const double *__restrict__ da,
                                                 const T *__restrict__ e_alphas,
-                                                double energy, double sin_alpha, double r)
+                                                double energy, double sin_alpha, double r,
+                                                const mxg_bucket *bk = nullptr,
+                                                const float *__restrict__ cnt = nullptr)
 {
    if (w.n_arrays == 0) return 0.0;
    const double *arrays = da + w.arrays_off;
    if (w.n_arrays == 1) return fold_theta (arrays, fa, r);
    double e_alpha = energy * sin_alpha;
-   unsigned int i = search_w (e_alpha, e_alphas, w.n_arrays);
+   unsigned int i;
+   if constexpr (sizeof (T) == sizeof (float))
+     i = (bk != nullptr) ? search_wb (e_alpha, e_alphas, w.n_arrays, *bk, cnt)
+                         : search_w (e_alpha, e_alphas, w.n_arrays);
+   else
+     i = search_w (e_alpha, e_alphas, w.n_arrays);
    if (i == w.n_arrays) i--;
    if (i == 0) i++;
    double theta_0 = fold_theta (arrays + (i - 1) * MXG_FOLD_STRIDE, fa, r);

@@ -69,9 +69,6 @@ void k_hrma_reflect (const mxg_config *__restrict__ cfg, const float *__restrict
    __syncthreads ();
 
    const mxg_hrma_cfg &h = *(const mxg_hrma_cfg *) lds;
-   const float *opt_e = ltab + lay.opt;
-   const float *opt_b = opt_e + h.opt_n;
-   const float *opt_d = opt_b + h.opt_n;
    __shared__ uint32_t seg_wc[2 * (kBlock / 64)];
    const BlockRange range = block_range (*io.n_in, blockIdx.x, gridDim.x);
    SegWriter survivors (io, range, seg_wc);
@@ -84,7 +81,7 @@ void k_hrma_reflect (const mxg_config *__restrict__ cfg, const float *__restrict
         if (i < range.hi)
           {
              id = io.idx_in[i];
-             keep = hrma_reflect_one (h, ltab, lay, opt_e, opt_b, opt_d, fa, da, s, id, rng_pos, seed, source_id);
+             keep = hrma_reflect_one (h, ltab, lay, fa, da, s, id, rng_pos, seed, source_id);
           }
         survivors.add (keep, id);
      }
@@ -98,20 +95,8 @@ hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const flo
 {
    if (n_upper == 0) return hipSuccess;
    const mxg_hrma_cfg &h = hcfg->hrma;
-   HrmaLds lay;
-   uint32_t f = 0;
-   lay.opt = f; f += 3 * h.opt_n;
-   for (int sh = 0; sh < MXG_NUM_SHELLS; sh++)
-     {
-        lay.corr_e[sh] = f; f += h.shell[sh].corr_n;
-        lay.corr_f[sh] = f; f += h.shell[sh].corr_n;
-        for (int k = 0; k < 2; k++)
-          {
-             lay.ea[sh][k] = f;
-             f += h.shell[sh].wfold[k].n_arrays;
-          }
-     }
-   lay.floats = f;
+   const HrmaLds lay = hrma_lds_layout (h);
+   const uint32_t f = lay.floats;
    size_t lds_bytes = sizeof (mxg_hrma_cfg) + (size_t) f * sizeof (float);
    if (lds_bytes > 64 * 1024)
      return hipErrorInvalidConfiguration;

@@ -15,7 +15,9 @@
 // and the index of the next unused word of the current block, packed as
 // (c2 << 3) | pos.  No Gaussian variate is pending at the split (none is
 // drawn before it), and the buffered block is recomputed from c2 - 1 on
-// resumption.
+// resumption.  (A second split at the hyperboloid hit, each part on the
+// photons still alive, measured 37 percent slower: the second kernel
+// repeats the stream resumption, the table lookups and the LDS staging.)
 #pragma once
 
 #include "dev_common.h"
@@ -74,6 +76,24 @@ __device__ __forceinline__ vec3 conic_normal (double a, double b, vec3 x)
    return n;
 }
 
+// The rotation of rotate_unit_vector, for the small angles of the normal
+// blur and the WFold scatter (arcseconds to arcminutes): below 2^-6 rad the
+// sine and cosine come from their Taylor polynomials through t^7 and t^8
+// (truncation below 1e-20 relative), else from sincos.
+__device__ __forceinline__ vec3 rotate_small (vec3 p, vec3 n, double t)
+{
+   double sn, cs;
+   if (fabs (t) < 0.015625)
+     {
+        const double t2 = t * t;
+        sn = t * (1.0 - t2 * (1.0 / 6.0 - t2 * (1.0 / 120.0 - t2 * (1.0 / 5040.0))));
+        cs = 1.0 - t2 * (0.5 - t2 * (1.0 / 24.0 - t2 * (1.0 / 720.0 - t2 * (1.0 / 40320.0))));
+     }
+   else
+     sincos (t, &sn, &cs);
+   return rotate_unit_vector1 (p, n, cs, sn);
+}
+
 __device__ __forceinline__ void blur_normal (vec3 &n, double blur, Stream &st)
 {
    double n_y = n.y, n_z = n.z;
@@ -83,7 +103,7 @@ __device__ __forceinline__ void blur_normal (vec3 &n, double blur, Stream &st)
    perp = rotate_unit_vector (perp, n, phi);
    phi = blur * (1.0 / 3600.0 * MXG_PI / 180.0);
    phi = phi * gaussian (st);
-   n = rotate_unit_vector (n, perp, phi);
+   n = rotate_small (n, perp, phi);
 }
 
 // Reflection at a surface point x already on conic k of shell S: normal
@@ -94,7 +114,8 @@ __device__ __forceinline__ int reflect_at (const mxg_hrma_cfg &h, const mxg_shel
                                            double delta, double corr,
                                            const float *__restrict__ fa,
                                            const double *__restrict__ da,
-                                           const float *__restrict__ e_alphas, Stream &st)
+                                           const float *__restrict__ e_alphas,
+                                           const float *__restrict__ ea_cnt, Stream &st)
 {
    vec3 normal = conic_normal (S.conic_a[k], S.conic_b[k], x);
    if (h.use_blur)
@@ -111,11 +132,11 @@ __device__ __forceinline__ int reflect_at (const mxg_hrma_cfg &h, const mxg_shel
    if (h.use_wfold == 0)
      return 0;
    double r = uniform (st);
-   double dg = wfold_interp (S.wfold[k], fa, da, e_alphas, energy, -p_dot_n, r);
+   double dg = wfold_interp (S.wfold[k], fa, da, e_alphas, energy, -p_dot_n, r, &S.ea_bucket[k], ea_cnt);
    dg *= S.scat_factor[k];
    if (dg > MXG_PI / 4) return -1;
    if (uniform (st) < 0.5) dg = -dg;
-   p = rotate_unit_vector (p, cross (p, normal), dg);
+   p = rotate_small (p, cross (p, normal), dg);
    return 0;
 }
 
@@ -134,8 +155,45 @@ struct HrmaLds
    uint32_t opt;                  // energies, betas, deltas: 3 * opt_n
    uint32_t corr_e[MXG_NUM_SHELLS], corr_f[MXG_NUM_SHELLS];
    uint32_t ea[MXG_NUM_SHELLS][2];  // e_alpha of each WFold table (float32 on disk: exact)
+   // the bucket counts of the searched tables (mxg_bucket)
+   uint32_t bk_opt, bk_corr[MXG_NUM_SHELLS], bk_ea[MXG_NUM_SHELLS][2];
    uint32_t floats;               // staged floats after the configuration copy
 };
+
+// counts of a bucket index, or none
+inline uint32_t bucket_floats (const mxg_bucket &b) { return b.ok ? b.nb + 1 : 0u; }
+
+// The LDS layout of the reflection tables for configuration h (host side).
+inline HrmaLds hrma_lds_layout (const mxg_hrma_cfg &h)
+{
+   HrmaLds lay;
+   uint32_t f = 0;
+   lay.opt = f; f += 3 * h.opt_n;
+   lay.bk_opt = f; f += bucket_floats (h.opt_bucket);
+   for (int sh = 0; sh < MXG_NUM_SHELLS; sh++)
+     {
+        const mxg_shell_cfg &S = h.shell[sh];
+        lay.corr_e[sh] = f; f += S.corr_n;
+        lay.corr_f[sh] = f; f += S.corr_n;
+        lay.bk_corr[sh] = f; f += bucket_floats (S.corr_bucket);
+        for (int k = 0; k < 2; k++)
+          {
+             lay.ea[sh][k] = f;
+             f += S.wfold[k].n_arrays;
+             lay.bk_ea[sh][k] = f;
+             f += bucket_floats (S.ea_bucket[k]);
+          }
+     }
+   lay.floats = f;
+   return lay;
+}
+
+// bucket counts of table b from the float arena into LDS at dst
+__device__ __forceinline__ void stage_bucket (const mxg_bucket &b, const float *__restrict__ fa, float *dst)
+{
+   if (!b.ok) return;
+   for (uint32_t w = threadIdx.x; w <= b.nb; w += blockDim.x) dst[w] = fa[b.off + w];
+}
 
 // Stages the mirror configuration and the reflection tables (the
 // binary-search targets of every photon) in LDS: the configuration copy at
@@ -153,6 +211,7 @@ __device__ __forceinline__ void stage_hrma_tables (const mxg_config *__restrict_
            ltab[lay.opt + gh.opt_n + w] = fa[gh.opt_beta_off + w];
            ltab[lay.opt + 2 * gh.opt_n + w] = fa[gh.opt_delta_off + w];
         }
+      stage_bucket (gh.opt_bucket, fa, ltab + lay.bk_opt);
       for (int sh = 0; sh < MXG_NUM_SHELLS; sh++)
         {
            const mxg_shell_cfg &S = gh.shell[sh];
@@ -161,9 +220,13 @@ __device__ __forceinline__ void stage_hrma_tables (const mxg_config *__restrict_
                 ltab[lay.corr_e[sh] + w] = fa[S.corr_e_off + w];
                 ltab[lay.corr_f[sh] + w] = fa[S.corr_f_off + w];
              }
+           stage_bucket (S.corr_bucket, fa, ltab + lay.bk_corr[sh]);
            for (int k = 0; k < 2; k++)
-             for (uint32_t w = threadIdx.x; w < S.wfold[k].n_arrays; w += blockDim.x)
-               ltab[lay.ea[sh][k] + w] = (float) da[S.wfold[k].e_alphas_off + w];
+             {
+                for (uint32_t w = threadIdx.x; w < S.wfold[k].n_arrays; w += blockDim.x)
+                  ltab[lay.ea[sh][k] + w] = (float) da[S.wfold[k].e_alphas_off + w];
+                stage_bucket (S.ea_bucket[k], fa, ltab + lay.bk_ea[sh][k]);
+             }
         }
    }
 }
@@ -246,12 +309,57 @@ __device__ __forceinline__ bool hrma_enter_one (const mxg_hrma_cfg &h, Soa s, ui
    return keep;
 }
 
+// The stream of photon id at the mirror's split, saved by hrma_enter_one.
+__device__ __forceinline__ void resume_stream (Stream &st, const uint32_t *__restrict__ rng_pos, uint32_t id)
+{
+   const uint32_t w = rng_pos[id];
+   st.c2 = w >> 3;
+   st.pos = (int) (w & 7u);
+   if (st.pos < 4)
+     {
+        // the block again, less the words already used
+        philox (st.c0, st.c1, st.c2 - 1u, st.c3, st.k0, st.k1, st.b0, st.b1, st.b2, st.b3);
+        for (int k = 0; k < st.pos; k++)
+          {
+             st.b0 = st.b1;
+             st.b1 = st.b2;
+             st.b2 = st.b3;
+          }
+     }
+}
+
+// The photon's optical constants (beta, delta) and the square root of its
+// shell's area correction factor, interpolated at its energy from the
+// tables in LDS (the same values at both surfaces).
+__device__ __forceinline__ void mirror_constants (const mxg_hrma_cfg &h, const mxg_shell_cfg &S, int si,
+                                                  const float *ltab, const HrmaLds &lay, double energy,
+                                                  double &beta, double &delta, double &corr)
+{
+   beta = 0.0;
+   delta = 1.0;
+   corr = 1.0;
+   if (h.opt_n != 0)
+     {
+        const float *opt_e = ltab + lay.opt;
+        float ef = (float) energy, fb, fd;
+        interp_f2b (ef, opt_e, opt_e + h.opt_n, opt_e + 2 * h.opt_n, h.opt_n, h.opt_bucket, ltab + lay.bk_opt,
+                    fb, fd);
+        beta = fb;
+        delta = fd;
+        if (h.use_scale)
+          {
+             corr = interp_fb (ef, ltab + lay.corr_e[si], ltab + lay.corr_f[si], S.corr_n, S.corr_bucket,
+                               ltab + lay.bk_corr[si]);
+             corr = sqrt (corr);
+          }
+     }
+}
+
 // k_hrma_reflect's work for photon id of the middle list: paraboloid
 // reflection, cap struts, hyperboloid, post-collimator struts.  ltab: the
-// tables staged in LDS at the offsets lay; opt_e/b/d: the optical constants
-// there.  True when the photon leaves the mirror.
+// tables staged in LDS at the offsets lay.  True when the photon leaves the
+// mirror.
 __device__ __forceinline__ bool hrma_reflect_one (const mxg_hrma_cfg &h, const float *ltab, const HrmaLds &lay,
-                                                  const float *opt_e, const float *opt_b, const float *opt_d,
                                                   const float *__restrict__ fa, const double *__restrict__ da,
                                                   Soa s, uint32_t id, const uint32_t *__restrict__ rng_pos,
                                                   unsigned long long seed, uint32_t source_id)
@@ -261,42 +369,15 @@ __device__ __forceinline__ bool hrma_reflect_one (const mxg_hrma_cfg &h, const f
    const mxg_shell_cfg &S = h.shell[si];
    Stream st;
    stream_init (st, seed, s.tag[id], MXG_STAGE_MIRROR, source_id);
-   {
-      uint32_t w = rng_pos[id];
-      st.c2 = w >> 3;
-      st.pos = (int) (w & 7u);
-      if (st.pos < 4)
-        {
-           // the block again, less the words already used
-           philox (st.c0, st.c1, st.c2 - 1u, st.c3, st.k0, st.k1, st.b0, st.b1, st.b2, st.b3);
-           for (int k = 0; k < st.pos; k++)
-             {
-                st.b0 = st.b1;
-                st.b1 = st.b2;
-                st.b2 = st.b3;
-             }
-        }
-   }
+   resume_stream (st, rng_pos, id);
    vec3 x = v3 (s.x0[id], s.x1[id], s.x2[id]);
    vec3 p = v3 (s.p0[id], s.p1[id], s.p2[id]);
-   double energy = s.energy[id];
-
-   double beta = 0.0, delta = 1.0, corr = 1.0;
-   if (h.opt_n != 0)
-     {
-        float ef = (float) energy, fb, fd;
-        interp_f2 (ef, opt_e, opt_b, opt_d, h.opt_n, fb, fd);
-        beta = fb;
-        delta = fd;
-        if (h.use_scale)
-          {
-             corr = interp_f (ef, ltab + lay.corr_e[si], ltab + lay.corr_f[si], S.corr_n);
-             corr = sqrt (corr);
-          }
-     }
+   const double energy = s.energy[id];
+   double beta, delta, corr;
+   mirror_constants (h, S, si, ltab, lay, energy, beta, delta, corr);
 
    if (0 != reflect_at (h, S, 0, x, p, energy, beta, delta, corr, fa, da,
-                        ltab + lay.ea[si][0], st))
+                        ltab + lay.ea[si][0], ltab + lay.bk_ea[si][0], st))
      flags |= PHOTON_UNREFLECTED;
    else
      {
@@ -313,7 +394,7 @@ __device__ __forceinline__ bool hrma_reflect_one (const mxg_hrma_cfg &h, const f
              if ((-1 == conic_intersection (S.conic_a[1], S.conic_b[1], S.conic_c[1], x, p,
                                             S.conic_xmin[1], S.conic_xmax[1]))
                  || (0 != reflect_at (h, S, 1, x, p, energy, beta, delta, corr, fa, da,
-                                      ltab + lay.ea[si][1], st)))
+                                      ltab + lay.ea[si][1], ltab + lay.bk_ea[si][1], st)))
                flags |= PHOTON_UNREFLECTED;
              else
                {

@@ -112,9 +112,6 @@ void k_batch_persistent (PersistArgs a)
    __syncthreads ();
 
    const mxg_hrma_cfg &h = *(const mxg_hrma_cfg *) lds;
-   const float *opt_e = ltab + a.lay.opt;
-   const float *opt_b = opt_e + h.opt_n;
-   const float *opt_d = opt_b + h.opt_n;
    const mxg_source_cfg &src = a.cfg->src;
    const uint32_t t0 = *a.tag_start;
    const double distance = src.distance;
@@ -157,8 +154,7 @@ void k_batch_persistent (PersistArgs a)
                 if (j < m)
                   {
                      id = list[j];
-                     keep = hrma_reflect_one (h, ltab, a.lay, opt_e, opt_b, opt_d, a.fa, a.da, a.s, id, a.rng_pos,
-                                              a.seed, 0u);
+                     keep = hrma_reflect_one (h, ltab, a.lay, a.fa, a.da, a.s, id, a.rng_pos, a.seed, 0u);
                   }
                 r.add (keep, id, list);
              }
@@ -305,19 +301,7 @@ hipError_t launch_batch_persistent (const PersistLaunch &p, hipStream_t st)
      return hipErrorInvalidValue;
    const mxg_hrma_cfg &h = p.hcfg->hrma;
    PersistArgs a;
-   uint32_t f = 0;
-   a.lay.opt = f; f += 3 * h.opt_n;
-   for (int sh = 0; sh < MXG_NUM_SHELLS; sh++)
-     {
-        a.lay.corr_e[sh] = f; f += h.shell[sh].corr_n;
-        a.lay.corr_f[sh] = f; f += h.shell[sh].corr_n;
-        for (int k = 0; k < 2; k++)
-          {
-             a.lay.ea[sh][k] = f;
-             f += h.shell[sh].wfold[k].n_arrays;
-          }
-     }
-   a.lay.floats = f;
+   a.lay = hrma_lds_layout (h);
    const size_t lds = persist_lds_bytes (a.lay, p.tile, &a.list_off);
    if (lds > 64 * 1024)
      return hipErrorInvalidConfiguration;

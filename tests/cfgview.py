@@ -9,6 +9,10 @@ D, U, I = ctypes.c_double, ctypes.c_uint, ctypes.c_int
 NUM_SHELLS, MAX_CHIPS, MAX_DRAKE = 4, 10, 8
 
 
+class Bucket(ctypes.Structure):
+    _fields_ = [("base", ctypes.c_ulonglong), ("off", U), ("nb", U), ("shift", U), ("ok", U)]
+
+
 class Wfold(ctypes.Structure):
     _fields_ = [("n_arrays", U), ("arrays_off", U), ("e_alphas_off", U), ("pad", U)]
 
@@ -19,7 +23,8 @@ class ShellCfg(ctypes.Structure):
                 ("fwd", (D * 9) * 2), ("bwd", (D * 9) * 2), ("blur", D * 2), ("scat_factor", D * 2), ("tilt_deg", (D * 2) * 2),
                 ("front_position", D), ("area_fraction", D), ("min_radius", D), ("max_radius", D),
                 ("shutter_bitmap", U), ("num_open_shutters", U), ("corr_n", U), ("corr_e_off", U),
-                ("corr_f_off", U), ("mirror_number", U), ("wfold", Wfold * 2)]
+                ("corr_f_off", U), ("mirror_number", U), ("wfold", Wfold * 2), ("corr_bucket", Bucket),
+                ("ea_bucket", Bucket * 2)]
 
 
 class Strut(ctypes.Structure):
@@ -31,7 +36,7 @@ class HrmaCfg(ctypes.Structure):
                 ("use_struts", I), ("vignetting", D), ("cap_position", D), ("geometric_area", D),
                 ("opt_n", U), ("opt_e_off", U), ("opt_beta_off", U), ("opt_delta_off", U),
                 ("shell", ShellCfg * NUM_SHELLS), ("pre_struts", Strut * 2), ("cap_struts", Strut * 2),
-                ("post_struts", Strut * 2), ("strut_cos", D), ("strut_sin", D)]
+                ("post_struts", Strut * 2), ("strut_cos", D), ("strut_sin", D), ("opt_bucket", Bucket)]
 
 
 class Grating(ctypes.Structure):
@@ -111,7 +116,7 @@ class Config(ctypes.Structure):
 
 from marx_amd._lib import DiffCfg, DiffShell, EventCfg, EventColumns, ImageSpec  # noqa: E402
 
-STRUCTS = {"mxg_event_cfg": EventCfg, "mxg_event_columns": EventColumns, "mxg_diff_shell": DiffShell, "mxg_diff_cfg": DiffCfg, "mxg_image_spec": ImageSpec,"mxg_wfold": Wfold, "mxg_shell_cfg": ShellCfg, "mxg_strut": Strut, "mxg_hrma_cfg": HrmaCfg,
+STRUCTS = {"mxg_bucket": Bucket, "mxg_event_cfg": EventCfg, "mxg_event_columns": EventColumns, "mxg_diff_shell": DiffShell, "mxg_diff_cfg": DiffCfg, "mxg_image_spec": ImageSpec,"mxg_wfold": Wfold, "mxg_shell_cfg": ShellCfg, "mxg_strut": Strut, "mxg_hrma_cfg": HrmaCfg,
            "mxg_grating": Grating, "mxg_sectors": Sectors, "mxg_grating_cfg": GratingCfg,
            "mxg_chip": Chip, "mxg_qe_table": QeTable, "mxg_contam": Contam, "mxg_rect": Rect,
            "mxg_detector_cfg": DetectorCfg, "mxg_source_cfg": SourceCfg, "mxg_config": Config}

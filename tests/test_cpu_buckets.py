@@ -1,0 +1,99 @@
+"""The bucket indices of the mirror's search tables (mxg_bucket, built at
+HRMA init): a search that starts from the bucket holding the value must
+return the plain bisection's index (dev_common.h's search_f / search_w, which
+restate JDMbinary_search_f, jdmath/src/binsrch.c) for every value -- each
+table entry, its neighbouring floats, midpoints, values below and above the
+table and random energies.  The bisection below is the textbook one; the
+bucket counts are checked against a direct count of the table's keys.
+(CPU only: the counts are the host's, the search a restatement of the
+device's.)"""
+import numpy as np
+import pytest
+
+from marx_amd import Tracer
+from cfgview import ConfigView
+
+
+def bisect(x, xp):
+    """The bisection: entry index n0 < n1 while n1 > n0 + 1, returning an
+    exact hit, else n1 when x >= xp[n0], else n0."""
+    n0, n1 = 0, len(xp)
+    while n1 > n0 + 1:
+        n2 = (n0 + n1) >> 1
+        v = xp[n2]
+        if v >= x:
+            if v == x:
+                return n2
+            n1 = n2
+        else:
+            n0 = n2
+    return n1 if x >= xp[n0] else n0
+
+
+def keys(v, double_keys):
+    if double_keys:
+        return np.asarray(v, np.float64).view(np.uint64).astype(object)
+    return np.asarray(v, np.float32).view(np.uint32).astype(object)
+
+
+def bucketed(x, xp, b, cnt, double_keys):
+    x0 = xp[0]
+    if not (x > x0):
+        if x == x0:
+            return 1
+        return 0
+    k = (int(keys([x], double_keys)[0]) - int(b.base)) >> int(b.shift)
+    j = k if k < b.nb else b.nb - 1
+    lo, hi = int(cnt[j]), int(cnt[j + 1])
+    while lo < hi:
+        mid = (lo + hi) >> 1
+        if xp[mid] < x:
+            lo = mid + 1
+        else:
+            hi = mid
+    return lo
+
+
+def _check_table(xp, b, cnt, double_keys, nbmax, rng):
+    dt = np.float64 if double_keys else np.float32
+    xp = np.asarray(xp, dt)
+    assert b.ok == 1 and 1 <= b.nb <= nbmax
+    k = keys(xp, double_keys)
+    assert int(b.base) == int(k[0])
+    # counts: entries whose key lies below each bucket's first key
+    want = [sum(1 for kk in k if kk < int(b.base) + (j << int(b.shift))) for j in range(b.nb)] + [len(xp)]
+    assert [int(c) for c in cnt] == want
+    xs = list(xp) + list(np.nextafter(xp, np.inf).astype(dt)) + list(np.nextafter(xp, -np.inf).astype(dt))
+    xs += list(((xp[1:].astype(np.float64) + xp[:-1]) / 2).astype(dt))
+    xs += [dt(xp[0] / 2), dt(xp[-1] * 2), dt(0.0), dt(-1.0)]
+    xs += list(rng.uniform(float(xp[0]), float(xp[-1]), 2000).astype(dt))
+    for x in xs:
+        assert bucketed(x, xp, b, cnt, double_keys) == bisect(x, xp), (x, double_keys)
+
+
+def test_mirror_search_tables_bucket_index():
+    t = Tracer(seed=1, GratingType="NONE", DetectorType="NONE")
+    cv = ConfigView(t)
+    h = cv.cfg.hrma
+    rng = np.random.default_rng(4)
+    assert h.opt_n > 1000
+    b = h.opt_bucket
+    _check_table(cv.f(h.opt_e_off, h.opt_n), b, cv.f(b.off, b.nb + 1), False, 256, rng)
+    for sh in range(4):
+        S = h.shell[sh]
+        b = S.corr_bucket
+        assert S.corr_n > 100
+        _check_table(cv.f(S.corr_e_off, S.corr_n), b, cv.f(b.off, b.nb + 1), False, 64, rng)
+        for k in range(2):
+            w, b = S.wfold[k], S.ea_bucket[k]
+            assert w.n_arrays > 100
+            ea = cv.d(w.e_alphas_off, w.n_arrays).astype(np.float32)
+            _check_table(ea, b, cv.f(b.off, b.nb + 1), True, 32, rng)
+
+
+def test_unsorted_or_short_tables_are_searched_whole():
+    """A table the bucket index cannot serve (not strictly increasing, or a
+    single entry, as an ideal mirror has none) is marked ok = 0."""
+    t = Tracer(seed=1, GratingType="NONE", DetectorType="NONE", HRMA_Ideal="yes")
+    h = ConfigView(t).cfg.hrma
+    assert h.opt_n == 0 and h.opt_bucket.ok == 0
