@@ -50,6 +50,7 @@ mxg_bucket;
 #define MXG_BUCKETS_OPT 256
 #define MXG_BUCKETS_CORR 64
 #define MXG_BUCKETS_EALPHA 32
+#define MXG_BUCKETS_GRATING 256
 
 typedef struct
 {
@@ -117,6 +118,8 @@ typedef struct
    double dp_over_p;
    double theta_blur;           /* radians */
    double vig;
+   double disp_cos, disp_sin;   /* of dispersion_angle (host libm), for the frame rotations */
+   mxg_bucket e_bucket;         /* of the energies (float keys) */
 }
 mxg_grating;
 

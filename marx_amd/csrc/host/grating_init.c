@@ -608,6 +608,22 @@ int mxg_grating_init (Param_File_Type *pf)
           }
      }
 
+   /* the rotation into and out of each grating's frame, the same for every
+    * photon of a shell: computed once (diffract.c:974-980 rotates by
+    * -/+ theta; sine is odd and cosine even, so these give both) */
+   for (i = 0; i < MXG_NUM_SHELLS; i++)
+     {
+        c->g[i].disp_cos = cos (c->g[i].dispersion_angle);
+        c->g[i].disp_sin = sin (c->g[i].dispersion_angle);
+     }
+   /* bucket indices of the energy grids searched per photon */
+   for (i = 0; i < MXG_NUM_SHELLS + 2; i++)
+     {
+        mxg_grating *gt = (i < MXG_NUM_SHELLS) ? &c->g[i] : ((i == MXG_NUM_SHELLS) ? &c->fine : &c->coarse);
+        if ((gt->num_energies >= 2)
+            && (-1 == mxg_bucket_of_farena (gt->energies_off, gt->num_energies, MXG_BUCKETS_GRATING, &gt->e_bucket)))
+          return -1;
+     }
    c->type = is_letg ? MARX_GRATING_LETG : MARX_GRATING_HETG;
    Mxg_Cfg_Generation++;
    return c->type;

@@ -105,88 +105,6 @@ static int read_bdat_into_arena (const char *relname, unsigned int ncols,
 
 /* Tilt matrices of one surface from its azimuth / elevation misalignment
  * in degrees (hrma.c:842-897). */
-/* The bucket index of a search table (mxg_bucket in marx_gpu_cfg.h) with at
- * most nbmax buckets: keys are the bits of the float values, or of the
- * values widened to double (double_keys).  A table that is not strictly
- * increasing and positive gets ok = 0 (searched whole). */
-static unsigned long long table_key (float v, int double_keys)
-{
-   if (double_keys)
-     {
-        double d = (double) v;
-        unsigned long long k;
-        memcpy (&k, &d, sizeof (k));
-        return k;
-     }
-   else
-     {
-        unsigned int k;
-        memcpy (&k, &v, sizeof (k));
-        return k;
-     }
-}
-
-static int build_bucket (const float *xp, unsigned int n, int double_keys, unsigned int nbmax, mxg_bucket *b)
-{
-   unsigned long long k0, range;
-   unsigned int i, j, shift, nb;
-   float *cnt;
-
-   memset (b, 0, sizeof (*b));
-   if ((n < 2) || (n > (1u << 24)) || !(xp[0] > 0.0f))
-     return 0;
-   for (i = 1; i < n; i++)
-     if (!(xp[i] > xp[i - 1]))
-       return 0;
-   k0 = table_key (xp[0], double_keys);
-   range = table_key (xp[n - 1], double_keys) - k0;
-   shift = 0;
-   while ((range >> shift) + 1 > nbmax)
-     shift++;
-   nb = (unsigned int) (range >> shift) + 1;
-   if (NULL == (cnt = (float *) malloc ((nb + 1) * sizeof (float))))
-     {
-        mxg_error ("out of memory");
-        return -1;
-     }
-   for (i = 0, j = 0; i < nb; i++)
-     {
-        unsigned long long boundary = k0 + ((unsigned long long) i << shift);
-        while ((j < n) && (table_key (xp[j], double_keys) < boundary))
-          j++;
-        cnt[i] = (float) j;
-     }
-   cnt[nb] = (float) n;
-   b->off = mxg_farena_push (cnt, nb + 1);
-   free (cnt);
-   if (b->off == (unsigned int) -1)
-     return -1;
-   b->base = k0;
-   b->nb = nb;
-   b->shift = shift;
-   b->ok = 1;
-   return 0;
-}
-
-/* bucket index of float-arena table (off, n): copied first, as pushing to
- * the arena may move it */
-static int bucket_of_farena (unsigned int off, unsigned int n, unsigned int nbmax, mxg_bucket *b)
-{
-   float *t;
-   int status;
-   memset (b, 0, sizeof (*b));
-   if (n < 2) return 0;
-   if (NULL == (t = (float *) malloc (n * sizeof (float))))
-     {
-        mxg_error ("out of memory");
-        return -1;
-     }
-   memcpy (t, mxg_farena_ptr (off), n * sizeof (float));
-   status = build_bucket (t, n, 0, nbmax, b);
-   free (t);
-   return status;
-}
-
 /* bucket index of a WFold table's e_alpha values (double arena), as the
  * kernels stage them: float values compared as doubles */
 static int bucket_of_e_alphas (const mxg_wfold *w, unsigned int nbmax, mxg_bucket *b)
@@ -205,7 +123,7 @@ static int bucket_of_e_alphas (const mxg_wfold *w, unsigned int nbmax, mxg_bucke
    da = mxg_darena_ptr (w->e_alphas_off);
    for (i = 0; i < w->n_arrays; i++)
      t[i] = (float) da[i];
-   status = build_bucket (t, w->n_arrays, 1, nbmax, b);
+   status = mxg_build_bucket (t, w->n_arrays, 1, nbmax, b);
    free (t);
    return status;
 }
@@ -401,13 +319,13 @@ int mxg_hrma_init (Param_File_Type *pf)
    }
 
    /* bucket indices of the tables the reflection kernel searches per photon */
-   if (h->opt_n && (-1 == bucket_of_farena (h->opt_e_off, h->opt_n, MXG_BUCKETS_OPT, &h->opt_bucket)))
+   if (h->opt_n && (-1 == mxg_bucket_of_farena (h->opt_e_off, h->opt_n, MXG_BUCKETS_OPT, &h->opt_bucket)))
      return -1;
    for (i = 0; i < MXG_NUM_SHELLS; i++)
      {
         mxg_shell_cfg *s = h->shell + i;
         unsigned int k;
-        if (s->corr_n && (-1 == bucket_of_farena (s->corr_e_off, s->corr_n, MXG_BUCKETS_CORR, &s->corr_bucket)))
+        if (s->corr_n && (-1 == mxg_bucket_of_farena (s->corr_e_off, s->corr_n, MXG_BUCKETS_CORR, &s->corr_bucket)))
           return -1;
         for (k = 0; k < 2; k++)
           if (-1 == bucket_of_e_alphas (&s->wfold[k], MXG_BUCKETS_EALPHA, &s->ea_bucket[k]))

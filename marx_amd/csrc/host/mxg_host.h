@@ -30,6 +30,13 @@ unsigned int mxg_darena_reserve (unsigned int n);
 float *mxg_farena_ptr (unsigned int off);
 double *mxg_darena_ptr (unsigned int off);
 unsigned int mxg_farena_size (void);
+/* The bucket index (mxg_bucket) of a search table of n floats, keys the
+ * float bits (double_keys = 0) or those of the values widened to double, at
+ * most nbmax buckets; its counts are pushed to the float arena.  A table
+ * that is not strictly increasing and positive gets ok = 0.  0 or -1. */
+int mxg_build_bucket (const float *xp, unsigned int n, int double_keys, unsigned int nbmax, mxg_bucket *b);
+/* the same for float-arena table (off, n), float keys */
+int mxg_bucket_of_farena (unsigned int off, unsigned int n, unsigned int nbmax, mxg_bucket *b);
 unsigned int mxg_darena_size (void);
 const float *mxg_farena_base (void);
 const double *mxg_darena_base (void);

@@ -323,3 +323,88 @@ JDMVector_Type mxg_m3_vmul (const double a[9], JDMVector_Type v)
                  a[3] * v.x + a[4] * v.y + a[5] * v.z,
                  a[6] * v.x + a[7] * v.y + a[8] * v.z);
 }
+
+/* ---- bucket indices of search tables ------------------------------------------ */
+
+/* The bucket index of a search table (mxg_bucket in marx_gpu_cfg.h) with at
+ * most nbmax buckets: keys are the bits of the float values, or of the
+ * values widened to double (double_keys).  A table that is not strictly
+ * increasing and positive gets ok = 0 (searched whole). */
+static unsigned long long table_key (float v, int double_keys)
+{
+   if (double_keys)
+     {
+        double d = (double) v;
+        unsigned long long k;
+        memcpy (&k, &d, sizeof (k));
+        return k;
+     }
+   else
+     {
+        unsigned int k;
+        memcpy (&k, &v, sizeof (k));
+        return k;
+     }
+}
+
+int mxg_build_bucket (const float *xp, unsigned int n, int double_keys, unsigned int nbmax, mxg_bucket *b)
+{
+   unsigned long long k0, range;
+   unsigned int i, j, shift, nb;
+   float *cnt;
+
+   memset (b, 0, sizeof (*b));
+   if ((n < 2) || (n > (1u << 24)) || !(xp[0] > 0.0f))
+     return 0;
+   for (i = 1; i < n; i++)
+     if (!(xp[i] > xp[i - 1]))
+       return 0;
+   k0 = table_key (xp[0], double_keys);
+   range = table_key (xp[n - 1], double_keys) - k0;
+   shift = 0;
+   while ((range >> shift) + 1 > nbmax)
+     shift++;
+   nb = (unsigned int) (range >> shift) + 1;
+   if (NULL == (cnt = (float *) malloc ((nb + 1) * sizeof (float))))
+     {
+        mxg_error ("out of memory");
+        return -1;
+     }
+   for (i = 0, j = 0; i < nb; i++)
+     {
+        unsigned long long boundary = k0 + ((unsigned long long) i << shift);
+        while ((j < n) && (table_key (xp[j], double_keys) < boundary))
+          j++;
+        cnt[i] = (float) j;
+     }
+   cnt[nb] = (float) n;
+   b->off = mxg_farena_push (cnt, nb + 1);
+   free (cnt);
+   if (b->off == (unsigned int) -1)
+     return -1;
+   b->base = k0;
+   b->nb = nb;
+   b->shift = shift;
+   b->ok = 1;
+   return 0;
+}
+
+/* bucket index of float-arena table (off, n): copied first, as pushing to
+ * the arena may move it */
+int mxg_bucket_of_farena (unsigned int off, unsigned int n, unsigned int nbmax, mxg_bucket *b)
+{
+   float *t;
+   int status;
+   memset (b, 0, sizeof (*b));
+   if (n < 2) return 0;
+   if (NULL == (t = (float *) malloc (n * sizeof (float))))
+     {
+        mxg_error ("out of memory");
+        return -1;
+     }
+   memcpy (t, mxg_farena_ptr (off), n * sizeof (float));
+   status = mxg_build_bucket (t, n, 0, nbmax, b);
+   free (t);
+   return status;
+}
+

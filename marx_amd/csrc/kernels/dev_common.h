@@ -411,6 +411,22 @@ __device__ __forceinline__ vec3 rotate_unit_vector1 (vec3 p, vec3 n, double c, d
    return u;
 }
 
+// sin and cos of t; below 2^-6 rad (the arcsecond-to-arcminute angles of
+// the mirror blur, WFold scatter and dither offsets) from their Taylor
+// polynomials through t^7 and t^8 (truncation below 1e-20 relative: the
+// result differs from sincos's by rounding only), else sincos.
+__device__ __forceinline__ void sincos_small (double t, double *sn, double *cs)
+{
+   if (fabs (t) < 0.015625)
+     {
+        const double t2 = t * t;
+        *sn = t * (1.0 - t2 * (1.0 / 6.0 - t2 * (1.0 / 120.0 - t2 * (1.0 / 5040.0))));
+        *cs = 1.0 - t2 * (0.5 - t2 * (1.0 / 24.0 - t2 * (1.0 / 720.0 - t2 * (1.0 / 40320.0))));
+     }
+   else
+     sincos (t, sn, cs);
+}
+
 __device__ __forceinline__ vec3 rotate_unit_vector (vec3 p, vec3 n, double theta)
 {
    double s, c;

@@ -148,8 +148,8 @@ hipError_t launch_pack_columns (const Soa &s, const uint32_t *idx, const uint32_
 // waves per SIMD that the compiler must leave room for (0: unbounded).
 // k_source at 4 (128 VGPRs instead of 132) measured 6 percent faster,
 // k_batch_source at 4 (from 175) 29 percent, the LETG grating pass at 4 (from
-// 218) 25 percent; the HETG grating and the ACIS detector kernels measured
-// slower at 3 and 4 (spills)
+// 218) 25 percent, the HETG grating pass at 3 (from 208) 8 percent; the
+// ACIS detector kernel needs 112 VGPRs (4 waves) unbounded
 #ifndef MXG_SOURCE_WAVES
 #define MXG_SOURCE_WAVES 4
 #endif
@@ -169,7 +169,7 @@ hipError_t launch_pack_columns (const Soa &s, const uint32_t *idx, const uint32_
 #define MXG_LETG_GRATING_WAVES 4
 #endif
 #ifndef MXG_GRATING_WAVES
-#define MXG_GRATING_WAVES 0
+#define MXG_GRATING_WAVES 3
 #endif
 #ifndef MXG_DETECT_WAVES
 #define MXG_DETECT_WAVES 0

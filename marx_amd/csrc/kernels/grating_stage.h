@@ -18,10 +18,9 @@
 
 namespace mxg {
 
-__device__ __forceinline__ vec3 rotate_x (vec3 a, double theta)
+// rotation about x by the angle of cosine c and sine s
+__device__ __forceinline__ vec3 rotate_x (vec3 a, double c, double s)
 {
-   double s, c;
-   sincos (theta, &s, &c);
    double ay = a.y, az = a.z;
    a.y = c * ay - s * az;
    a.z = s * ay + c * az;
@@ -78,6 +77,24 @@ __device__ __forceinline__ int draw_order (const mxg_grating &g, const float *__
    const float *E = fa + g.energies_off;
    const unsigned int ne = g.num_energies, no = g.num_orders;
    unsigned int lo = 1, hi = ne - 1;
+   const mxg_bucket &b = g.e_bucket;
+   const float ef = (float) e;         // e holds a float value
+   if (b.ok && (ef > E[0]))
+     {
+        // the first entry >= e of a strictly increasing table, from the
+        // bucket that holds e (mxg_bucket); the bisection below then starts
+        // on the answer clamped to [1, ne - 1]
+        const uint64_t k = ((uint64_t) __float_as_uint (ef) - b.base) >> b.shift;
+        const uint32_t j = (k < b.nb) ? (uint32_t) k : b.nb - 1;
+        unsigned int l = (unsigned int) fa[b.off + j], h = (unsigned int) fa[b.off + j + 1];
+        while (l < h)
+          {
+             unsigned int mid = (l + h) >> 1;
+             if (E[mid] < ef) l = mid + 1;
+             else h = mid;
+          }
+        lo = hi = (l < 1) ? 1u : ((l > ne - 1) ? ne - 1 : l);
+     }
    while (lo < hi)
      {
         unsigned int mid = (lo + hi) >> 1;
@@ -89,14 +106,25 @@ __device__ __forceinline__ int draw_order (const mxg_grating &g, const float *__
    const double dx_10 = x_1 - x_0;
    const float *y0 = fa + g.cum_eff_off + (count - 1) * no;
    const float *y1 = y0 + no;
+   // Inside its interval the interpolated value of order k lies between
+   // y0[k] and y1[k] (every step of it rounds within those bounds), so the
+   // division is needed only for an order whose bounds straddle r; outside
+   // (extrapolation) every order is interpolated.
+   const bool inside = (e >= x_0) && (e <= x_1);
    for (unsigned int k = 0; k < no; k++)
      {
+        const float ya = y0[k], yb = y1[k];
+        if (inside)
+          {
+             if (r > (double) fmaxf (ya, yb)) continue;
+             if (r <= (double) fminf (ya, yb)) return (int) k;
+          }
         float v;
         if (dx_10 == 0.0)
-          v = y0[k];
+          v = ya;
         else
           {
-             double a = y0[k], b = y1[k];
+             double a = ya, b = yb;
              v = (float) (a + (b - a) * (e - x_0) / dx_10);
           }
         if (r <= (double) v)
@@ -206,8 +234,8 @@ __device__ __forceinline__ bool grating_one (const mxg_grating_cfg &gc, const fl
              const double ef = (double) (float) energy;
              int8_t order = s.order[id];
 
-             x = rotate_x (x, -g.dispersion_angle);
-             p = rotate_x (p, -g.dispersion_angle);
+             x = rotate_x (x, g.disp_cos, -g.disp_sin);
+             p = rotate_x (p, g.disp_cos, -g.disp_sin);
              if (-1 == torus_intersect (x, p, gc.rowland[shell]))
                flags |= PHOTON_UNDIFFRACTED;
              if (-1 == diffract_from (g, gc.sectors[shell], fa, da, 0.0, energy, ef, x, p,
@@ -233,8 +261,8 @@ __device__ __forceinline__ bool grating_one (const mxg_grating_cfg &gc, const fl
                     }
                }
 
-             x = rotate_x (x, g.dispersion_angle);
-             p = rotate_x (p, g.dispersion_angle);
+             x = rotate_x (x, g.disp_cos, g.disp_sin);
+             p = rotate_x (p, g.disp_cos, g.disp_sin);
              s.x0[id] = x.x; s.x1[id] = x.y; s.x2[id] = x.z;
              s.p0[id] = p.x; s.p1[id] = p.y; s.p2[id] = p.z;
              s.flags[id] = flags;

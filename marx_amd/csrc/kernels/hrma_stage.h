@@ -77,20 +77,11 @@ __device__ __forceinline__ vec3 conic_normal (double a, double b, vec3 x)
 }
 
 // The rotation of rotate_unit_vector, for the small angles of the normal
-// blur and the WFold scatter (arcseconds to arcminutes): below 2^-6 rad the
-// sine and cosine come from their Taylor polynomials through t^7 and t^8
-// (truncation below 1e-20 relative), else from sincos.
+// blur and the WFold scatter (see sincos_small).
 __device__ __forceinline__ vec3 rotate_small (vec3 p, vec3 n, double t)
 {
    double sn, cs;
-   if (fabs (t) < 0.015625)
-     {
-        const double t2 = t * t;
-        sn = t * (1.0 - t2 * (1.0 / 6.0 - t2 * (1.0 / 120.0 - t2 * (1.0 / 5040.0))));
-        cs = 1.0 - t2 * (0.5 - t2 * (1.0 / 24.0 - t2 * (1.0 / 720.0 - t2 * (1.0 / 40320.0))));
-     }
-   else
-     sincos (t, &sn, &cs);
+   sincos_small (t, &sn, &cs);
    return rotate_unit_vector1 (p, n, cs, sn);
 }
 

@@ -129,10 +129,14 @@ __device__ __forceinline__ void apply_internal_dither (const mxg_source_cfg &src
    double ra = (double) fra + delta_ra, dec = (double) fdec + delta_dec, roll = froll;
 
    vec3 p = v3 (s.p0[i], s.p1[i], s.p2[i]);
-   p = rotate_unit_vector (p, v3 (1, 0, 0), -roll);
+   {
+      double sr, cr;
+      sincos_small (-roll, &sr, &cr);
+      p = rotate_unit_vector1 (p, v3 (1, 0, 0), cr, sr);
+   }
    double sin_ra, cos_ra, sin_dec, cos_dec;
-   sincos (ra, &sin_ra, &cos_ra);
-   sincos (dec, &sin_dec, &cos_dec);
+   sincos_small (ra, &sin_ra, &cos_ra);
+   sincos_small (dec, &sin_dec, &cos_dec);
    double cos_theta = cos_dec * cos_ra;
    vec3 nv = v3 (0, sin_dec, -cos_dec * sin_ra);
    double sin_theta = length (nv);

@@ -42,7 +42,8 @@ class HrmaCfg(ctypes.Structure):
 class Grating(ctypes.Structure):
     _fields_ = [("num_orders", U), ("num_energies", U), ("energies_off", U), ("cum_eff_off", U),
                 ("orders_off", U), ("pad", U), ("dispersion_angle", D), ("period", D),
-                ("dp_over_p", D), ("theta_blur", D), ("vig", D)]
+                ("dp_over_p", D), ("theta_blur", D), ("vig", D),
+                ("disp_cos", D), ("disp_sin", D), ("e_bucket", Bucket)]
 
 
 class Sectors(ctypes.Structure):

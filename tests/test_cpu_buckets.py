@@ -1,5 +1,7 @@
-"""The bucket indices of the mirror's search tables (mxg_bucket, built at
-HRMA init): a search that starts from the bucket holding the value must
+"""The bucket indices of the search tables (mxg_bucket, built at init: the
+mirror's optical constants, area corrections and WFold e_alpha values, the
+gratings' energy grids): a search that starts from the bucket holding the
+value must
 return the plain bisection's index (dev_common.h's search_f / search_w, which
 restate JDMbinary_search_f, jdmath/src/binsrch.c) for every value -- each
 table entry, its neighbouring floats, midpoints, values below and above the
@@ -97,3 +99,38 @@ def test_unsorted_or_short_tables_are_searched_whole():
     t = Tracer(seed=1, GratingType="NONE", DetectorType="NONE", HRMA_Ideal="yes")
     h = ConfigView(t).cfg.hrma
     assert h.opt_n == 0 and h.opt_bucket.ok == 0
+
+
+def lower_bound_1(e, E):
+    """draw_order's bracket search: the first j in [1, n - 1] with E[j] >= e,
+    else n - 1 (bisection)."""
+    lo, hi = 1, len(E) - 1
+    while lo < hi:
+        mid = (lo + hi) >> 1
+        if e > E[mid]:
+            lo = mid + 1
+        else:
+            hi = mid
+    return lo
+
+
+@pytest.mark.parametrize("grating", ["HETG", "LETG"])
+def test_grating_energy_bucket_index(grating):
+    t = Tracer(seed=1, GratingType=grating, DetectorType="NONE")
+    cv = ConfigView(t)
+    gc = cv.cfg.grating
+    rng = np.random.default_rng(6)
+    tables = [gc.g[k] for k in range(4)] + ([gc.fine, gc.coarse] if grating == "LETG" else [])
+    for g in tables:
+        E = cv.f(g.energies_off, g.num_energies)
+        b = g.e_bucket
+        cnt = cv.f(b.off, b.nb + 1)
+        _check_table(E, b, cnt, False, 256, rng)
+        xs = list(E) + list(np.nextafter(E, np.inf)) + list(rng.uniform(E[0] * 0.5, E[-1] * 1.5, 3000).astype(np.float32))
+        for e in xs:
+            e = np.float32(e)
+            want = lower_bound_1(e, E)
+            if e > E[0]:
+                lb = bucketed(e, E, b, cnt, False)
+                got = min(max(lb, 1), len(E) - 1)
+                assert got == want, (e, got, want)
