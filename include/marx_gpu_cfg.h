@@ -112,7 +112,7 @@ typedef struct
    unsigned int energies_off;   /* float arena: num_energies */
    unsigned int cum_eff_off;    /* float arena: [num_energies][num_orders] */
    unsigned int orders_off;     /* float arena: num_orders diffraction orders */
-   unsigned int pad;
+   unsigned int cum_sorted;     /* every energy's cumulative row is non-decreasing in order */
    double dispersion_angle;     /* radians */
    double period;               /* um */
    double dp_over_p;

@@ -616,13 +616,26 @@ int mxg_grating_init (Param_File_Type *pf)
         c->g[i].disp_cos = cos (c->g[i].dispersion_angle);
         c->g[i].disp_sin = sin (c->g[i].dispersion_angle);
      }
-   /* bucket indices of the energy grids searched per photon */
+   /* bucket indices of the energy grids searched per photon, and whether
+    * each energy's cumulative efficiencies rise with the order index (the
+    * order draw may then search the orders by bisection) */
    for (i = 0; i < MXG_NUM_SHELLS + 2; i++)
      {
         mxg_grating *gt = (i < MXG_NUM_SHELLS) ? &c->g[i] : ((i == MXG_NUM_SHELLS) ? &c->fine : &c->coarse);
+        unsigned int j, k;
+        const float *cum;
         if ((gt->num_energies >= 2)
             && (-1 == mxg_bucket_of_farena (gt->energies_off, gt->num_energies, MXG_BUCKETS_GRATING, &gt->e_bucket)))
           return -1;
+        gt->cum_sorted = (gt->num_orders > 0);
+        cum = mxg_farena_ptr (gt->cum_eff_off);
+        for (j = 0; gt->cum_sorted && (j < gt->num_energies); j++)
+          for (k = 1; k < gt->num_orders; k++)
+            if (!(cum[j * gt->num_orders + k] >= cum[j * gt->num_orders + k - 1]))
+              {
+                 gt->cum_sorted = 0;
+                 break;
+              }
      }
    c->type = is_letg ? MARX_GRATING_LETG : MARX_GRATING_HETG;
    Mxg_Cfg_Generation++;

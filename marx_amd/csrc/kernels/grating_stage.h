@@ -111,7 +111,22 @@ __device__ __forceinline__ int draw_order (const mxg_grating &g, const float *__
    // division is needed only for an order whose bounds straddle r; outside
    // (extrapolation) every order is interpolated.
    const bool inside = (e >= x_0) && (e <= x_1);
-   for (unsigned int k = 0; k < no; k++)
+   unsigned int k0 = 0;
+   if (inside && g.cum_sorted)
+     {
+        // rows that rise with the order: the bounds do too, so every order
+        // before the first whose upper bound reaches r fails, found by
+        // bisection
+        unsigned int l = 0, h = no;
+        while (l < h)
+          {
+             const unsigned int mid = (l + h) >> 1;
+             if (r > (double) fmaxf (y0[mid], y1[mid])) l = mid + 1;
+             else h = mid;
+          }
+        k0 = l;
+     }
+   for (unsigned int k = k0; k < no; k++)
      {
         const float ya = y0[k], yb = y1[k];
         if (inside)

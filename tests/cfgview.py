@@ -41,7 +41,7 @@ class HrmaCfg(ctypes.Structure):
 
 class Grating(ctypes.Structure):
     _fields_ = [("num_orders", U), ("num_energies", U), ("energies_off", U), ("cum_eff_off", U),
-                ("orders_off", U), ("pad", U), ("dispersion_angle", D), ("period", D),
+                ("orders_off", U), ("cum_sorted", U), ("dispersion_angle", D), ("period", D),
                 ("dp_over_p", D), ("theta_blur", D), ("vig", D),
                 ("disp_cos", D), ("disp_sin", D), ("e_bucket", Bucket)]
 

@@ -123,6 +123,11 @@ def test_grating_energy_bucket_index(grating):
     tables = [gc.g[k] for k in range(4)] + ([gc.fine, gc.coarse] if grating == "LETG" else [])
     for g in tables:
         E = cv.f(g.energies_off, g.num_energies)
+        # the order draw bisects the orders when every energy's cumulative
+        # efficiencies rise with the order index
+        cum = cv.f(g.cum_eff_off, g.num_energies * g.num_orders).reshape(g.num_energies, g.num_orders)
+        assert g.cum_sorted == int(bool(np.all(np.diff(cum, axis=1) >= 0)))
+        assert g.cum_sorted == 1
         b = g.e_bucket
         cnt = cv.f(b.off, b.nb + 1)
         _check_table(E, b, cnt, False, 256, rng)
