@@ -437,8 +437,8 @@ int mxg_get_schedule (mxg_schedule *s);
  * candidate schedules -- for the staged pass, blocks per CU in {2, 4, 8, 16}
  * for every stage kernel, then 2, 3 and 4 waves for the reflection kernel,
  * keeping for each kernel the fastest (HIP-event time, best of `repeats`
- * runs); then the persistent pass with tiles of {256, 512, 1024, 2048}
- * photons at {1, 2, 4} blocks per CU -- and installs the schedule whose
+ * runs); then the persistent pass with tiles of {512, 1024, 2048, 4096}
+ * photons at {2, 3, 4} blocks per CU -- and installs the schedule whose
  * graph-launched pass is fastest, returning it in *best and that pass time
  * in *best_ms. */
 int mxg_autotune_batch (unsigned int num_sources, const double *dirs, const unsigned int *counts,

@@ -949,8 +949,8 @@ int mxg_autotune_batch (unsigned int num_sources, const double *dirs, const unsi
                         unsigned int repeats, mxg_schedule *best, float *best_ms)
 {
    static const unsigned int bpc[4] = {2, 4, 8, 16};
-   static const unsigned int tiles[4] = {256, 512, 1024, 2048};
-   static const unsigned int tile_bpc[3] = {1, 2, 4};
+   static const unsigned int tiles[4] = {512, 1024, 2048, 4096};
+   static const unsigned int tile_bpc[3] = {2, 3, 4};
    mxg_schedule cur, sel;
    float best_t[5], t[5], pass_ms, ms;
    int k, c;

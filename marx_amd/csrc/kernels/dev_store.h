@@ -146,10 +146,11 @@ hipError_t launch_pack_columns (const Soa &s, const uint32_t *idx, const uint32_
 // (128 VGPRs, a few spills) measured fastest on the HETG workload.
 #define MXG_REFLECT_WAVES_DEFAULT 4
 // waves per SIMD that the compiler must leave room for (0: unbounded).
-// k_source at 4 (128 VGPRs instead of 132) measured 6 percent faster,
-// k_batch_source at 4 (from 175) 29 percent, the LETG grating pass at 4 (from
-// 218) 25 percent, the HETG grating pass at 3 (from 208) 8 percent; the
-// ACIS detector kernel needs 112 VGPRs (4 waves) unbounded
+// k_source needs fewer VGPRs than 4 waves allow (56 for a point source),
+// k_batch_source at 4 (from 175) measured 29 percent faster, the LETG
+// grating pass at 4 (from 218) 25 percent, the HETG grating pass at 3 (from
+// 208) 8 percent; the ACIS detector kernel needs 112 VGPRs (4 waves)
+// unbounded
 #ifndef MXG_SOURCE_WAVES
 #define MXG_SOURCE_WAVES 4
 #endif
@@ -197,6 +198,12 @@ hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const flo
                         const double *da, Soa s, ListIO io, HrmaScratch hs, uint32_t n_upper,
                         double source_distance, unsigned long long seed, uint32_t source_id,
                         hipStream_t st, hipEvent_t *ev);
+// k_hrma_reflect alone, from io.idx_in (photons that hit the paraboloid,
+// stream positions in rng_pos) to io's survivor segments (io.grid blocks;
+// the caller compacts them); waves: the instance launched (2, 3 or 4)
+hipError_t launch_hrma_reflect (const mxg_config *cfg, const mxg_config *hcfg, const float *fa,
+                                const double *da, Soa s, ListIO io, const uint32_t *rng_pos, int waves,
+                                unsigned long long seed, uint32_t source_id, hipStream_t st);
 // letg: the instance compiled for the LETG pass
 hipError_t launch_grating (const mxg_config *cfg, const float *fa, const double *da, Soa s,
                            ListIO io, uint32_t n_upper, unsigned long long seed,
@@ -215,11 +222,15 @@ hipError_t launch_detect (const mxg_config *cfg, const float *fa, const double *
 
 hipError_t launch_dither_store (const mxg_config *cfg, Soa s, uint32_t n, double start_time,
                                 unsigned long long seed, hipStream_t st);
+// Photon creation (source.hip): photons 0..n-1 of the store from tag
+// tag_start, the exposure cut in *first_over, the active list idx and its
+// length *count; ticks (n entries) and tile_tot (ceil(n / 2048) entries)
+// are scratch; point: the configured shape is a point (the instance without
+// the extended shapes).
 hipError_t launch_source (const mxg_config *cfg, const float *fa, const double *da, Soa s, uint32_t n,
                           uint32_t tag_start, double exposure_time, double start_time,
-                          unsigned long long seed, uint32_t source_id, uint64_t *ticks,
-                          uint64_t *tile_tot, uint32_t *first_over, uint32_t *count,
-                          uint32_t *idx, hipStream_t st);
+                          unsigned long long seed, uint32_t source_id, uint64_t *ticks, uint64_t *tile_tot,
+                          uint32_t *first_over, uint32_t *count, uint32_t *idx, bool point, hipStream_t st);
 
 // Batched point sources (see source.hip): n photons over nsrc sources with
 // prefix offsets off[nsrc + 1] and unit directions dir[3 nsrc]; point: the
@@ -239,8 +250,11 @@ hipError_t launch_batch_source (const mxg_config *cfg, const double *da, Soa s, 
 // tile_off (ceil(n / tile) entries each) are scratch.  ev, when non-null:
 // two events around the persistent kernel.
 constexpr uint32_t kMaxPersistTile = 4096;
+// waves per SIMD of the persistent kernel: 3 (168 VGPRs, 250-660 B of
+// spills per lane) measured 7 percent faster than 4 (128 VGPRs) and than 2
+// (no spills) on the batched ACIS-I pass
 #ifndef MXG_PERSIST_WAVES
-#define MXG_PERSIST_WAVES 4
+#define MXG_PERSIST_WAVES 3
 #endif
 struct PersistLaunch
 {

@@ -636,7 +636,7 @@ int mxgd_run_source (unsigned int n, unsigned int tag_start, double start_time,
      }
    HIPCHECK (launch_source (d_cfg, d_fa, d_da, g_soa, n, tag_start, exposure_time, start_time, seed,
                             source_id, d_ticks, d_tile_tot, d_counts + 2, d_counts, d_idx[0],
-                            g_stream));
+                            g_hcfg.src.type == MXG_SOURCE_POINT, g_stream));
    if (slot >= 0 && slot < kSlots)
      HIPCHECK (hipEventRecord (g_tev[slot][1], g_stream));
    if (ms)
@@ -1238,7 +1238,7 @@ int mxgd_trace_noalloc (const mxg_soa_view *v, unsigned int n, unsigned int tag_
      hipLaunchKernelGGL (k_set_u32, dim3 (1), dim3 (1), 0, g_stream, counts + k, 0u);
    HIPCHECK (hipGetLastError ());
    HIPCHECK (launch_source (c_cfg, c_fa, c_da, s, n, tag_start, exposure_time, start_time, seed, 0, ticks, tiles,
-                            counts + 2, counts + 0, idx[0], g_stream));
+                            counts + 2, counts + 0, idx[0], cfg->src.type == MXG_SOURCE_POINT, g_stream));
    hipLaunchKernelGGL (k_copy_u32, dim3 (1), dim3 (1), 0, g_stream, counts + 6, counts + 0);
    HIPCHECK (hipGetLastError ());
 

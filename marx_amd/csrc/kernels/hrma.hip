@@ -64,11 +64,12 @@ void k_hrma_reflect (const mxg_config *__restrict__ cfg, const float *__restrict
                      uint32_t source_id, HrmaLds lay)
 {
    extern __shared__ __align__ (16) unsigned char lds[];
-   float *ltab = (float *) (lds + sizeof (mxg_hrma_cfg));
+   float *ltab = (float *) (lds + kHrmaTabOff);
    stage_hrma_tables (cfg, fa, da, lay, lds, ltab);
    __syncthreads ();
 
    const mxg_hrma_cfg &h = *(const mxg_hrma_cfg *) lds;
+   const HrmaLds &L = *(const HrmaLds *) (lds + kHrmaLayOff);
    __shared__ uint32_t seg_wc[2 * (kBlock / 64)];
    const BlockRange range = block_range (*io.n_in, blockIdx.x, gridDim.x);
    SegWriter survivors (io, range, seg_wc);
@@ -81,11 +82,28 @@ void k_hrma_reflect (const mxg_config *__restrict__ cfg, const float *__restrict
         if (i < range.hi)
           {
              id = io.idx_in[i];
-             keep = hrma_reflect_one (h, ltab, lay, fa, da, s, id, rng_pos, seed, source_id);
+             keep = hrma_reflect_one (h, ltab, L, fa, da, s, id, rng_pos, seed, source_id);
           }
         survivors.add (keep, id);
      }
    survivors.finish (io);
+}
+
+hipError_t launch_hrma_reflect (const mxg_config *cfg, const mxg_config *hcfg, const float *fa,
+                                const double *da, Soa s, ListIO io, const uint32_t *rng_pos, int waves,
+                                unsigned long long seed, uint32_t source_id, hipStream_t st)
+{
+   const HrmaLds lay = hrma_lds_layout (hcfg->hrma);
+   size_t lds_bytes = kHrmaTabOff + (size_t) lay.floats * sizeof (float);
+   if (lds_bytes > 64 * 1024)
+     return hipErrorInvalidConfiguration;
+   void (*reflect) (const mxg_config *, const float *, const double *, Soa, ListIO,
+                    const uint32_t *, unsigned long long, uint32_t, HrmaLds)
+     = (waves <= 2) ? k_hrma_reflect<2>
+       : (waves == 3) ? k_hrma_reflect<3> : k_hrma_reflect<MXG_REFLECT_WAVES_TOP>;
+   hipLaunchKernelGGL (reflect, dim3 (io.grid), dim3 (kBlock), lds_bytes, st, cfg, fa, da, s, io, rng_pos,
+                       seed, source_id, lay);
+   return hipGetLastError ();
 }
 
 hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const float *fa,
@@ -94,11 +112,8 @@ hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const flo
                         hipStream_t st, hipEvent_t *ev)
 {
    if (n_upper == 0) return hipSuccess;
-   const mxg_hrma_cfg &h = hcfg->hrma;
-   const HrmaLds lay = hrma_lds_layout (h);
-   const uint32_t f = lay.floats;
-   size_t lds_bytes = sizeof (mxg_hrma_cfg) + (size_t) f * sizeof (float);
-   if (lds_bytes > 64 * 1024)
+   const HrmaLds lay = hrma_lds_layout (hcfg->hrma);
+   if (kHrmaTabOff + (size_t) lay.floats * sizeof (float) > 64 * 1024)
      return hipErrorInvalidConfiguration;
 
    // io.idx_in -> k_hrma_enter -> middle list -> k_hrma_reflect -> io.idx_out
@@ -108,10 +123,6 @@ hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const flo
    b.idx_in = hs.idx_mid;
    b.n_in = hs.n_mid;
    b.grid = hs.reflect_grid;
-   void (*reflect) (const mxg_config *, const float *, const double *, Soa, ListIO,
-                    const uint32_t *, unsigned long long, uint32_t, HrmaLds)
-     = (hs.reflect_waves <= 2) ? k_hrma_reflect<2>
-       : (hs.reflect_waves == 3) ? k_hrma_reflect<3> : k_hrma_reflect<MXG_REFLECT_WAVES_TOP>;
    hipError_t e = hipSuccess;
    if (ev) e = hipEventRecord (ev[0], st);
    if (e != hipSuccess) return e;
@@ -121,10 +132,8 @@ hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const flo
    if ((e == hipSuccess) && ev) e = hipEventRecord (ev[1], st);
    if (e == hipSuccess) e = launch_compact (a, st);
    if ((e == hipSuccess) && ev) e = hipEventRecord (ev[2], st);
-   if (e != hipSuccess) return e;
-   hipLaunchKernelGGL (reflect, dim3 (b.grid), dim3 (kBlock), lds_bytes, st, cfg, fa, da, s,
-                       b, hs.rng_pos, seed, source_id, lay);
-   e = hipGetLastError ();
+   if (e == hipSuccess)
+     e = launch_hrma_reflect (cfg, hcfg, fa, da, s, b, hs.rng_pos, hs.reflect_waves, seed, source_id, st);
    if ((e == hipSuccess) && ev) e = hipEventRecord (ev[3], st);
    if (e == hipSuccess) e = launch_compact (b, st);
    return e;

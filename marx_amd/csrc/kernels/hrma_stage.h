@@ -151,6 +151,13 @@ struct HrmaLds
    uint32_t floats;               // staged floats after the configuration copy
 };
 
+// Dynamic LDS of the reflection: the configuration copy, a copy of the
+// layout at kHrmaLayOff (the reflection reads a table's offset by the
+// photon's shell from LDS, instead of holding every shell's offsets in
+// scalar registers and selecting per lane), the tables from kHrmaTabOff.
+constexpr size_t kHrmaLayOff = (sizeof (mxg_hrma_cfg) + 15) & ~(size_t) 15;
+constexpr size_t kHrmaTabOff = (kHrmaLayOff + sizeof (HrmaLds) + 15) & ~(size_t) 15;
+
 // counts of a bucket index, or none
 inline uint32_t bucket_floats (const mxg_bucket &b) { return b.ok ? b.nb + 1 : 0u; }
 
@@ -188,12 +195,14 @@ __device__ __forceinline__ void stage_bucket (const mxg_bucket &b, const float *
 
 // Stages the mirror configuration and the reflection tables (the
 // binary-search targets of every photon) in LDS: the configuration copy at
-// lds, the tables at ltab + the offsets of lay.  The caller synchronises.
+// lds, lay at lds + kHrmaLayOff, the tables at ltab (lds + kHrmaTabOff) +
+// the offsets of lay.  The caller synchronises.
 __device__ __forceinline__ void stage_hrma_tables (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
                                                    const double *__restrict__ da, const HrmaLds &lay,
                                                    unsigned char *lds, float *ltab)
 {
    stage_hrma_cfg (cfg, lds);
+   if (threadIdx.x == 0) *(HrmaLds *) (lds + kHrmaLayOff) = lay;
    {
       const mxg_hrma_cfg &gh = cfg->hrma;
       for (uint32_t w = threadIdx.x; w < gh.opt_n; w += blockDim.x)
@@ -222,9 +231,75 @@ __device__ __forceinline__ void stage_hrma_tables (const mxg_config *__restrict_
    }
 }
 
-// k_hrma_enter's work for photon id: the source-to-paraboloid part of the
-// mirror.  True when the photon hit the paraboloid (its stream position is
-// saved in rng_pos[id] for hrma_reflect_one).
+// The source-to-paraboloid part of the mirror for an unabsorbed photon with
+// tag `tag` and direction p: vignetting, aperture, precollimator struts,
+// frame change and the paraboloid intersection.  False when vignetting
+// blocks the photon (nothing else is drawn); else true, with its position
+// x, direction p, shell si and the mirror's marks added to flags; *hit: it
+// hit the paraboloid, its random-stream position in *rng.
+__device__ __forceinline__ bool hrma_enter_core (const mxg_hrma_cfg &h, uint32_t tag, double source_distance,
+                                                 unsigned long long seed, uint32_t source_id, vec3 &x,
+                                                 vec3 &p, int &si, uint32_t &flags, bool &hit, uint32_t &rng)
+{
+   Stream st;
+   stream_init (st, seed, tag, MXG_STAGE_MIRROR, source_id);
+   hit = false;
+   if ((h.ideal == 0) && (uniform (st) > h.vignetting))
+     return false;
+
+   // aperture: shell by cumulative area, radius, open quadrant
+   for (;;)
+     {
+        double r = uniform (st);
+        si = -1;
+        for (int k = 0; k < MXG_NUM_SHELLS; k++)
+          if (r < h.shell[k].area_fraction) { si = k; break; }
+        if (si >= 0) break;
+     }
+   const mxg_shell_cfg &S = h.shell[si];
+   double radius = S.min_radius + (S.max_radius - S.min_radius) * uniform (st);
+   double theta;
+   unsigned int quad;
+   do
+     {
+        theta = uniform (st);
+        quad = (unsigned int) (4.0 * theta);
+     }
+   while (0 == (S.shutter_bitmap & (1u << quad)));
+   theta = (2.0 * MXG_PI) * (theta - 1.0 / 8.0);
+   double sn, cs;
+   sincos (theta, &sn, &cs);
+   x.z = radius * cs;
+   x.y = radius * sn;
+   x.x = S.front_position;
+   x.z -= S.to_osac[0][2];
+   x.y -= S.to_osac[0][1];
+   if (source_distance > 0.0)
+     {
+        p = ax1_bx2 (1.0, x, source_distance, p);
+        normalize (p);
+     }
+   if (h.use_struts && in_strut_shadow (h, x, p, h.pre_struts))
+     flags |= PHOTON_MIRROR_VBLOCKED;
+
+   x = add (x, ld3 (S.to_osac[0]));
+   x = m3_vmul (S.fwd[0], x);
+   p = m3_vmul (S.fwd[0], p);
+
+   if (-1 == conic_intersection (S.conic_a[0], S.conic_b[0], S.conic_c[0], x, p,
+                                 S.conic_xmin[0], S.conic_xmax[0]))
+     flags |= PHOTON_UNREFLECTED;
+   else
+     {
+        hit = true;
+        rng = (st.c2 << 3) | (uint32_t) st.pos;
+     }
+   return true;
+}
+
+// k_hrma_enter's work for photon id of the store (hrma_enter_core).  True
+// when the photon hit the paraboloid (its stream position is saved in
+// rng_pos[id] for hrma_reflect_one).
 __device__ __forceinline__ bool hrma_enter_one (const mxg_hrma_cfg &h, Soa s, uint32_t id,
                                                 uint32_t *__restrict__ rng_pos, double source_distance,
                                                 unsigned long long seed, uint32_t source_id)
@@ -233,64 +308,14 @@ __device__ __forceinline__ bool hrma_enter_one (const mxg_hrma_cfg &h, Soa s, ui
    uint32_t flags = s.flags[id];
    if ((flags & BAD_PHOTON_MASK) == 0)
      {
-        Stream st;
-        stream_init (st, seed, s.tag[id], MXG_STAGE_MIRROR, source_id);
-        if ((h.ideal == 0) && (uniform (st) > h.vignetting))
-          {
-             s.flags[id] = flags | PHOTON_MIRROR_VBLOCKED;
-          }
+        vec3 x, p = v3 (s.p0[id], s.p1[id], s.p2[id]);
+        int si;
+        uint32_t rng = 0;
+        if (!hrma_enter_core (h, s.tag[id], source_distance, seed, source_id, x, p, si, flags, keep, rng))
+          s.flags[id] = flags | PHOTON_MIRROR_VBLOCKED;
         else
           {
-             vec3 x, p = v3 (s.p0[id], s.p1[id], s.p2[id]);
-
-             // aperture: shell by cumulative area, radius, open quadrant
-             int si;
-             for (;;)
-               {
-                  double r = uniform (st);
-                  si = -1;
-                  for (int k = 0; k < MXG_NUM_SHELLS; k++)
-                    if (r < h.shell[k].area_fraction) { si = k; break; }
-                  if (si >= 0) break;
-               }
-             const mxg_shell_cfg &S = h.shell[si];
-             double radius = S.min_radius + (S.max_radius - S.min_radius) * uniform (st);
-             double theta;
-             unsigned int quad;
-             do
-               {
-                  theta = uniform (st);
-                  quad = (unsigned int) (4.0 * theta);
-               }
-             while (0 == (S.shutter_bitmap & (1u << quad)));
-             theta = (2.0 * MXG_PI) * (theta - 1.0 / 8.0);
-             double sn, cs;
-             sincos (theta, &sn, &cs);
-             x.z = radius * cs;
-             x.y = radius * sn;
-             x.x = S.front_position;
-             x.z -= S.to_osac[0][2];
-             x.y -= S.to_osac[0][1];
-             if (source_distance > 0.0)
-               {
-                  p = ax1_bx2 (1.0, x, source_distance, p);
-                  normalize (p);
-               }
-             if (h.use_struts && in_strut_shadow (h, x, p, h.pre_struts))
-               flags |= PHOTON_MIRROR_VBLOCKED;
-
-             x = add (x, ld3 (S.to_osac[0]));
-             x = m3_vmul (S.fwd[0], x);
-             p = m3_vmul (S.fwd[0], p);
-
-             if (-1 == conic_intersection (S.conic_a[0], S.conic_b[0], S.conic_c[0], x, p,
-                                           S.conic_xmin[0], S.conic_xmax[0]))
-               flags |= PHOTON_UNREFLECTED;
-             else
-               {
-                  keep = true;
-                  rng_pos[id] = (st.c2 << 3) | (uint32_t) st.pos;
-               }
+             if (keep) rng_pos[id] = rng;
              s.x0[id] = x.x; s.x1[id] = x.y; s.x2[id] = x.z;
              s.p0[id] = p.x; s.p1[id] = p.y; s.p2[id] = p.z;
              s.shell[id] = (uint8_t) si;

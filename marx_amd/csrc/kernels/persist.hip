@@ -104,7 +104,7 @@ __global__ __launch_bounds__ (kBlock, MXG_PERSIST_WAVES)
 void k_batch_persistent (PersistArgs a)
 {
    extern __shared__ __align__ (16) unsigned char lds[];
-   float *ltab = (float *) (lds + sizeof (mxg_hrma_cfg));
+   float *ltab = (float *) (lds + kHrmaTabOff);
    uint32_t *list = (uint32_t *) (lds + a.list_off);
    __shared__ uint32_t wc[2 * (kBlock / 64)];
    __shared__ uint32_t sh_tile;
@@ -112,6 +112,7 @@ void k_batch_persistent (PersistArgs a)
    __syncthreads ();
 
    const mxg_hrma_cfg &h = *(const mxg_hrma_cfg *) lds;
+   const HrmaLds &L = *(const HrmaLds *) (lds + kHrmaLayOff);
    const mxg_source_cfg &src = a.cfg->src;
    const uint32_t t0 = *a.tag_start;
    const double distance = src.distance;
@@ -154,7 +155,7 @@ void k_batch_persistent (PersistArgs a)
                 if (j < m)
                   {
                      id = list[j];
-                     keep = hrma_reflect_one (h, ltab, a.lay, a.fa, a.da, a.s, id, a.rng_pos, a.seed, 0u);
+                     keep = hrma_reflect_one (h, ltab, L, a.fa, a.da, a.s, id, a.rng_pos, a.seed, 0u);
                   }
                 r.add (keep, id, list);
              }
@@ -267,7 +268,7 @@ void k_tile_gather (const uint32_t *__restrict__ seg, const uint32_t *__restrict
 // Dynamic LDS of the persistent kernel for a tile of `tile` photons.
 size_t persist_lds_bytes (const HrmaLds &lay, uint32_t tile, uint32_t *list_off)
 {
-   size_t off = sizeof (mxg_hrma_cfg) + (size_t) lay.floats * sizeof (float);
+   size_t off = kHrmaTabOff + (size_t) lay.floats * sizeof (float);
    off = (off + 15) & ~(size_t) 15;
    if (list_off) *list_off = (uint32_t) off;
    return off + (size_t) tile * sizeof (uint32_t);

@@ -13,7 +13,11 @@
 // The arrival time is a running sum of the gaps, which the reference adds up
 // serially in FP64.  Here each gap is quantised to 2^-32 of the mean gap and
 // summed as 64-bit integers -- exact and associative -- in a tiled scan, so
-// every photon's time is the same whatever the summation order.
+// every photon's time is the same whatever the summation order.  A first
+// pass (k_source_scan) draws the gaps only and scans them per tile; one
+// block scans the tile sums; the second pass (k_source, one photon per
+// thread) draws every photon again and writes each column once, its time
+// and dithered direction included.
 #include "source_stage.h"
 
 namespace mxg {
@@ -26,21 +30,52 @@ __device__ __forceinline__ uint64_t gap_ticks (double e)
    return (uint64_t) (e * 4294967296.0 + 0.5);
 }
 
-#if MXG_SOURCE_WAVES > 0
-__global__ __launch_bounds__ (kBlock, MXG_SOURCE_WAVES)
-#else
+// The source stream of photon `tag` (source.c:268-384): its energy, its
+// direction and, for Poisson arrivals, the gap to the photon before in
+// 2^-32 units of the mean gap (else its uniform arrival time t).  Both
+// passes over the photons draw through this, so they see the same values.
+// POINT: compiled for a point source only (its direction is the source's,
+// with no draw).
+template <bool POINT>
+__device__ __forceinline__ void source_draws (const mxg_source_cfg &src, const float *__restrict__ fa,
+                                              const double *__restrict__ da, uint32_t tag,
+                                              unsigned long long seed, uint32_t source_id, bool poisson,
+                                              double &energy, vec3 &p, uint64_t &gap, double &t)
+{
+   Stream st;
+   stream_init (st, seed, tag, MXG_STAGE_SOURCE, source_id);
+   energy = draw_energy (src, da, st);
+   if (POINT)
+     p = ld3 (src.p);
+   else
+     p = extended_direction (src, fa, ld3 (src.p), ld3 (src.p_normal), ld3 (src.line_normal), src.line_alpha, st);
+   gap = 0;
+   t = 0.0;
+   if (poisson)
+     gap = gap_ticks (expn (st));
+   else
+     t = src.uniform_tmax * uniform (st);
+}
+
+// Poisson arrivals, first pass: each photon's gap (the source stream's
+// draws up to it, no photon stored), the inclusive scan of the gaps within
+// its tile of kTile photons in ticks[i], the tile's total in tile_tot.
+template <bool POINT>
 __global__ __launch_bounds__ (kBlock)
-#endif
-void k_source (const mxg_config *__restrict__ cfg, const float *__restrict__ fa, const double *__restrict__ da, Soa s,
-               uint32_t n, uint32_t tag_start, unsigned long long seed, uint32_t source_id,
-               uint64_t *__restrict__ ticks, uint64_t *__restrict__ tile_tot)
+void k_source_scan (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
+                    const double *__restrict__ da, uint32_t n, uint32_t tag_start,
+                    unsigned long long seed, uint32_t source_id, uint64_t *__restrict__ ticks,
+                    uint64_t *__restrict__ tile_tot)
 {
    const mxg_source_cfg &src = cfg->src;
    __shared__ uint64_t sk[kTile];
    __shared__ uint64_t warp_tot[kBlock / 64];
+   if (src.time_mode != MXG_TIME_POISSON)
+     {
+        if (threadIdx.x == 0) tile_tot[blockIdx.x] = 0;
+        return;
+     }
    const uint32_t base = blockIdx.x * kTile;
-   const bool poisson = (src.time_mode == MXG_TIME_POISSON);
-
    for (int j = 0; j < kPer; j++)
      {
         const uint32_t li = j * kBlock + threadIdx.x;
@@ -48,24 +83,12 @@ void k_source (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
         uint64_t k = 0;
         if (i < n)
           {
-             const uint32_t tag = tag_start + i;
-             Stream st;
-             stream_init (st, seed, tag, MXG_STAGE_SOURCE, source_id);
-             double energy = draw_energy (src, da, st);
-             vec3 p = extended_direction (src, fa, ld3 (src.p), ld3 (src.p_normal), ld3 (src.line_normal),
-                                          src.line_alpha, st);
-             double t = 0.0;
-             if (poisson)
-               k = gap_ticks (expn (st));
-             else
-               t = src.uniform_tmax * uniform (st);
-
-             // k_source_finish writes Poisson times and the INTERNAL dither
-             init_photon (s, i, energy, p, t, tag, !poisson, src.dither_mode != MXG_DITHER_INTERNAL);
+             double energy, t;
+             vec3 p;
+             source_draws<POINT> (src, fa, da, tag_start + i, seed, source_id, true, energy, p, k, t);
           }
         sk[li] = k;
      }
-   if (!poisson) return;
    __syncthreads ();
 
    // inclusive scan of the tile: 8 serial per thread, then across threads
@@ -101,68 +124,107 @@ void k_source (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
      tile_tot[blockIdx.x] = sk[kTile - 1];
 }
 
-// exclusive scan of the tile totals (one block, sequential chunks of 256)
-__global__ __launch_bounds__ (kBlock)
+// exclusive scan of the tile totals in place: one block, each thread
+// summing a run of consecutive totals
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__ (kScanThreads)
 void k_scan_tiles (uint64_t *__restrict__ tile_tot, uint32_t ntiles)
 {
-   __shared__ uint64_t buf[kBlock];
-   uint64_t carry = 0;
-   for (uint32_t c0 = 0; c0 < ntiles; c0 += kBlock)
+   __shared__ uint64_t wave_tot[kScanThreads / 64];
+   const uint32_t per = (ntiles + kScanThreads - 1) / kScanThreads;
+   const uint32_t lo = threadIdx.x * per;
+   const uint32_t hi = (lo + per < ntiles) ? lo + per : ntiles;
+   uint64_t run = 0;
+   for (uint32_t i = lo; i < hi; i++) run += tile_tot[i];
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   uint64_t incl = run;
+   for (int off = 1; off < 64; off <<= 1)
      {
-        const uint32_t i = c0 + threadIdx.x;
-        uint64_t v = (i < ntiles) ? tile_tot[i] : 0;
-        buf[threadIdx.x] = v;
-        __syncthreads ();
-        for (int off = 1; off < kBlock; off <<= 1)
-          {
-             uint64_t add = (threadIdx.x >= (unsigned) off) ? buf[threadIdx.x - off] : 0;
-             __syncthreads ();
-             buf[threadIdx.x] += add;
-             __syncthreads ();
-          }
-        const uint64_t incl = buf[threadIdx.x];
-        if (i < ntiles) tile_tot[i] = carry + incl - v;
-        __syncthreads ();
-        carry += buf[kBlock - 1];
-        __syncthreads ();
+        uint64_t v = __shfl_up (incl, off);
+        if (lane >= off) incl += v;
+     }
+   if (lane == 63) wave_tot[wave] = incl;
+   __syncthreads ();
+   uint64_t acc = incl - run;
+   for (int w = 0; w < wave; w++) acc += wave_tot[w];
+   for (uint32_t i = lo; i < hi; i++)
+     {
+        const uint64_t v = tile_tot[i];
+        tile_tot[i] = acc;
+        acc += v;
      }
 }
 
-// arrival times, first photon at or past the exposure, dither
-__global__ __launch_bounds__ (kBlock)
-void k_source_finish (const mxg_config *__restrict__ cfg, Soa s, uint32_t n,
-                      const uint64_t *__restrict__ ticks, const uint64_t *__restrict__ tile_off,
-                      double exposure_time, double start_time, unsigned long long seed,
-                      uint32_t source_id, uint32_t *__restrict__ first_over)
+// A photon as the source makes it: energy, time, direction (dithered when
+// configured, with the offsets d) and whether it is active.
+struct NewPhoton
 {
-   const mxg_source_cfg &src = cfg->src;
-   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-   if (i >= n) return;
-   double t;
-   if (src.time_mode == MXG_TIME_POISSON)
-     {
-        const uint32_t tile = i / kTile;
-        const uint64_t S = ticks[i] + tile_off[tile];
-        const double mt = src.mean_time;
-        t = mt * ((double) S * (1.0 / 4294967296.0));
-        s.t[i] = t;
-        if ((exposure_time > 0.0) && (exposure_time <= t))
-          {
-             bool first = (i == 0);
-             if (!first)
-               {
-                  const uint64_t Sp = ((i % kTile) == 0) ? tile_off[tile] : ticks[i - 1] + tile_off[tile];
-                  const double tp = mt * ((double) Sp * (1.0 / 4294967296.0));
-                  first = !(exposure_time <= tp);
-               }
-             if (first) *first_over = i;
-          }
-     }
-   else
-     t = s.t[i];
+   double energy, t;
+   vec3 p;
+   float d[3];
+   bool dithered, active;
+};
 
-   if (src.dither_mode == MXG_DITHER_INTERNAL)
-     apply_internal_dither (src, s, i, t, start_time, seed, source_id);
+// Photon i: its draws; with Poisson arrivals its time from the first pass's
+// scan and its tile's offset tile_off, whether it is active (the first
+// photon, or its predecessor arrives before the exposure) and, for the first
+// photon at or past the exposure, *first_over (unique: the times do not
+// decrease); the INTERNAL dither.  POINT: as in source_draws.
+template <bool POINT>
+__device__ __forceinline__ NewPhoton source_photon (const mxg_source_cfg &src, const float *__restrict__ fa,
+                                                    const double *__restrict__ da, uint32_t i, uint32_t tag,
+                                                    double exposure_time, double start_time,
+                                                    unsigned long long seed, uint32_t source_id,
+                                                    const uint64_t *__restrict__ ticks,
+                                                    const uint64_t *__restrict__ tile_off,
+                                                    uint32_t *__restrict__ first_over)
+{
+   NewPhoton ph;
+   const bool poisson = (src.time_mode == MXG_TIME_POISSON);
+   uint64_t k;
+   source_draws<POINT> (src, fa, da, tag, seed, source_id, poisson, ph.energy, ph.p, k, ph.t);
+   ph.active = true;
+   if (poisson)
+     {
+        const uint64_t toff = tile_off[i / kTile];
+        const double mt = src.mean_time;
+        ph.t = mt * ((double) (ticks[i] + toff) * (1.0 / 4294967296.0));
+        if ((exposure_time > 0.0) && (i > 0))
+          {
+             const uint64_t Sp = ((i % kTile) == 0) ? toff : ticks[i - 1] + toff;
+             const double tp = mt * ((double) Sp * (1.0 / 4294967296.0));
+             ph.active = !(exposure_time <= tp);
+          }
+        if (ph.active && (exposure_time > 0.0) && (exposure_time <= ph.t)) *first_over = i;
+     }
+   ph.dithered = (src.dither_mode == MXG_DITHER_INTERNAL);
+   if (ph.dithered)
+     ph.p = internal_dither (src, ph.p, tag, ph.t, start_time, seed, source_id, ph.d);
+   return ph;
+}
+
+// Photon i, one per thread (source_photon): every column written once, and
+// idx[i] = i for an active photon.
+template <bool POINT>
+#if MXG_SOURCE_WAVES > 0
+__global__ __launch_bounds__ (kBlock, MXG_SOURCE_WAVES)
+#else
+__global__ __launch_bounds__ (kBlock)
+#endif
+void k_source (const mxg_config *__restrict__ cfg, const float *__restrict__ fa, const double *__restrict__ da, Soa s,
+               uint32_t n, uint32_t tag_start, double exposure_time, double start_time,
+               unsigned long long seed, uint32_t source_id, const uint64_t *__restrict__ ticks,
+               const uint64_t *__restrict__ tile_off, uint32_t *__restrict__ first_over,
+               uint32_t *__restrict__ idx)
+{
+   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+   if (i >= n) return;
+   const uint32_t tag = tag_start + i;
+   const NewPhoton ph = source_photon<POINT> (cfg->src, fa, da, i, tag, exposure_time, start_time, seed,
+                                              source_id, ticks, tile_off, first_over);
+   init_photon (s, i, ph.energy, ph.p, ph.t, tag, true, true, !ph.dithered);
+   if (ph.dithered) store_dith (s, i, ph.d);
+   if (ph.active) idx[i] = i;
 }
 
 // the configured dither applied to photons 0..n-1 of the store, each at its
@@ -186,24 +248,20 @@ hipError_t launch_dither_store (const mxg_config *cfg, Soa s, uint32_t n, double
    return hipGetLastError ();
 }
 
-__global__ void k_source_count (uint32_t n, const uint32_t *__restrict__ first_over,
-                                uint32_t *__restrict__ count, uint32_t *__restrict__ idx)
+// the length of the active list: up to and including the first photon at
+// or past the exposure, else all
+__global__ void k_source_count (uint32_t n, const uint32_t *__restrict__ first_over, uint32_t *__restrict__ count)
 {
-   // the active list of a fresh chunk is the identity
-   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-   uint32_t f = *first_over;
-   uint32_t m = (f < n) ? f + 1 : n;
-   if (i < m) idx[i] = i;
-   if (i == 0) *count = m;
+   const uint32_t f = *first_over;
+   *count = (f < n) ? f + 1 : n;
 }
 
 __global__ void k_reset (uint32_t *p, uint32_t v) { *p = v; }
 
 hipError_t launch_source (const mxg_config *cfg, const float *fa, const double *da, Soa s, uint32_t n,
                           uint32_t tag_start, double exposure_time, double start_time,
-                          unsigned long long seed, uint32_t source_id, uint64_t *ticks,
-                          uint64_t *tile_tot, uint32_t *first_over, uint32_t *count,
-                          uint32_t *idx, hipStream_t st)
+                          unsigned long long seed, uint32_t source_id, uint64_t *ticks, uint64_t *tile_tot,
+                          uint32_t *first_over, uint32_t *count, uint32_t *idx, bool point, hipStream_t st)
 {
    if (n == 0)
      {
@@ -211,15 +269,22 @@ hipError_t launch_source (const mxg_config *cfg, const float *fa, const double *
         return hipGetLastError ();
      }
    const uint32_t ntiles = (n + kTile - 1) / kTile;
+   const uint32_t blocks = (n + kBlock - 1) / kBlock;
    hipLaunchKernelGGL (k_reset, dim3 (1), dim3 (1), 0, st, first_over, 0xFFFFFFFFu);
-   hipLaunchKernelGGL (k_source, dim3 (ntiles), dim3 (kBlock), 0, st, cfg, fa, da, s, n, tag_start,
-                       seed, source_id, ticks, tile_tot);
-   hipLaunchKernelGGL (k_scan_tiles, dim3 (1), dim3 (kBlock), 0, st, tile_tot, ntiles);
-   hipLaunchKernelGGL (k_source_finish, dim3 ((n + kBlock - 1) / kBlock), dim3 (kBlock), 0, st,
-                       cfg, s, n, ticks, tile_tot, exposure_time, start_time, seed, source_id,
-                       first_over);
-   hipLaunchKernelGGL (k_source_count, dim3 ((n + kBlock - 1) / kBlock), dim3 (kBlock), 0, st, n,
-                       first_over, count, idx);
+   if (point)
+     hipLaunchKernelGGL (k_source_scan<true>, dim3 (ntiles), dim3 (kBlock), 0, st, cfg, fa, da, n, tag_start,
+                         seed, source_id, ticks, tile_tot);
+   else
+     hipLaunchKernelGGL (k_source_scan<false>, dim3 (ntiles), dim3 (kBlock), 0, st, cfg, fa, da, n, tag_start,
+                         seed, source_id, ticks, tile_tot);
+   hipLaunchKernelGGL (k_scan_tiles, dim3 (1), dim3 (kScanThreads), 0, st, tile_tot, ntiles);
+   if (point)
+     hipLaunchKernelGGL (k_source<true>, dim3 (blocks), dim3 (kBlock), 0, st, cfg, fa, da, s, n, tag_start,
+                         exposure_time, start_time, seed, source_id, ticks, tile_tot, first_over, idx);
+   else
+     hipLaunchKernelGGL (k_source<false>, dim3 (blocks), dim3 (kBlock), 0, st, cfg, fa, da, s, n, tag_start,
+                         exposure_time, start_time, seed, source_id, ticks, tile_tot, first_over, idx);
+   hipLaunchKernelGGL (k_source_count, dim3 (1), dim3 (1), 0, st, n, first_over, count);
    return hipGetLastError ();
 }
 

@@ -8,13 +8,18 @@
 namespace mxg {
 
 // every column of photon i set as a fresh photon of the given energy,
-// direction, time and tag; the time and the dither state are left to a
-// later pass that writes them (with_t, with_dith false)
+// direction, time and tag, with no dither offsets; a column that a later
+// step of the same kernel writes is left out (with_t, with_p, with_dith
+// false)
 __device__ __forceinline__ void init_photon (Soa s, uint32_t i, double energy, vec3 p, double t,
-                                             uint32_t tag, bool with_t = true, bool with_dith = true)
+                                             uint32_t tag, bool with_t = true, bool with_p = true,
+                                             bool with_dith = true)
 {
    s.energy[i] = energy;
-   s.p0[i] = p.x; s.p1[i] = p.y; s.p2[i] = p.z;
+   if (with_p)
+     {
+        s.p0[i] = p.x; s.p1[i] = p.y; s.p2[i] = p.z;
+     }
    s.x0[i] = 0.0; s.x1[i] = 0.0; s.x2[i] = 0.0;
    if (with_t) s.t[i] = t;
    s.flags[i] = 0u;
@@ -27,6 +32,14 @@ __device__ __forceinline__ void init_photon (Soa s, uint32_t i, double energy, v
    s.ccd[i] = 0; s.region[i] = 0; s.order[i] = 0;
    for (int q = 0; q < 4; q++) s.sup[q][i] = 0;
    s.tag[i] = tag;
+}
+
+// the dither state of photon i: the three pointing offsets d (ra, dec,
+// roll), the other three columns zero
+__device__ __forceinline__ void store_dith (Soa s, uint32_t i, const float d[3])
+{
+   s.dith[0][i] = d[0]; s.dith[1][i] = d[1]; s.dith[2][i] = d[2];
+   s.dith[3][i] = 0.0f; s.dith[4][i] = 0.0f; s.dith[5][i] = 0.0f;
 }
 
 // The direction of a photon of the configured source shape about the source
@@ -108,27 +121,26 @@ __device__ __forceinline__ double rotation_axis (vec3 a, vec3 b, vec3 &c)
    return theta;
 }
 
-// INTERNAL dither of photon i (dither.c:167-182, 551-581, 609-658): the
-// Lissajous pointing offset at its arrival time (stored as float, as the
-// reference stores the dither state), Gaussian aspect blur, and the rotation
-// of its direction.
-__device__ __forceinline__ void apply_internal_dither (const mxg_source_cfg &src, Soa s, uint32_t i,
-                                                       double t, double start_time,
-                                                       unsigned long long seed, uint32_t source_id)
+// INTERNAL dither (dither.c:167-182, 551-581, 609-658) of the photon with
+// tag `tag`, direction p and arrival time t: the Lissajous pointing offset
+// at that time, returned in d as float (as the reference stores the dither
+// state), Gaussian aspect blur, and the rotation of the direction, which is
+// returned.
+__device__ __forceinline__ vec3 internal_dither (const mxg_source_cfg &src, vec3 p, uint32_t tag, double t,
+                                                 double start_time, unsigned long long seed,
+                                                 uint32_t source_id, float d[3])
 {
    Stream st;
-   stream_init (st, seed, s.tag[i], MXG_STAGE_DITHER, source_id);
+   stream_init (st, seed, tag, MXG_STAGE_DITHER, source_id);
    double tt = (2.0 * MXG_PI) * (start_time + t);
    float fra = (float) (src.ra_amp * sin (tt / src.ra_period + src.ra_phase));
    float fdec = (float) (src.dec_amp * sin (tt / src.dec_period + src.dec_phase));
    float froll = (float) (src.nominal_roll + src.roll_amp * sin (tt / src.roll_period + src.roll_phase));
-   s.dith[0][i] = fra; s.dith[1][i] = fdec; s.dith[2][i] = froll;
-   s.dith[3][i] = 0.0f; s.dith[4][i] = 0.0f; s.dith[5][i] = 0.0f;
+   d[0] = fra; d[1] = fdec; d[2] = froll;
    double delta_ra = src.aspect_blur * gaussian (st);
    double delta_dec = src.aspect_blur * gaussian (st);
    double ra = (double) fra + delta_ra, dec = (double) fdec + delta_dec, roll = froll;
 
-   vec3 p = v3 (s.p0[i], s.p1[i], s.p2[i]);
    {
       double sr, cr;
       sincos_small (-roll, &sr, &cr);
@@ -147,6 +159,17 @@ __device__ __forceinline__ void apply_internal_dither (const mxg_source_cfg &src
         nv.z /= sin_theta;
         p = rotate_unit_vector1 (p, nv, cos_theta, sin_theta);
      }
+   return p;
+}
+
+// internal_dither applied in place to photon i of the store at time t
+__device__ __forceinline__ void apply_internal_dither (const mxg_source_cfg &src, Soa s, uint32_t i,
+                                                       double t, double start_time,
+                                                       unsigned long long seed, uint32_t source_id)
+{
+   float d[3];
+   vec3 p = internal_dither (src, v3 (s.p0[i], s.p1[i], s.p2[i]), s.tag[i], t, start_time, seed, source_id, d);
+   store_dith (s, i, d);
    s.p0[i] = p.x; s.p1[i] = p.y; s.p2[i] = p.z;
 }
 
@@ -200,9 +223,15 @@ __device__ __forceinline__ void batch_source_one (const mxg_source_cfg &src, con
         p = extended_direction (src, nullptr, p, nrm, ln, la, st);
      }
    double t = exposure * uniform (st);
-   init_photon (s, i, energy, p, t, tag);
    if (src.dither_mode == MXG_DITHER_INTERNAL)
-     apply_internal_dither (src, s, i, t, start_time, seed, 0u);
+     {
+        float d[3];
+        p = internal_dither (src, p, tag, t, start_time, seed, 0u, d);
+        init_photon (s, i, energy, p, t, tag, true, true, false);
+        store_dith (s, i, d);
+     }
+   else
+     init_photon (s, i, energy, p, t, tag);
 }
 
 } // namespace mxg
