@@ -283,7 +283,7 @@ int marx_grating_diffract (Marx_Photon_Type *pt, int verbose)
    pt->history |= grating_history ();
    if ((-1 == sync_config ()) || (-1 == make_resident (pt)))
      return -1;
-   if (-1 == mxgd_run_grating (Seed, -1, NULL))
+   if (-1 == mxgd_run_grating (Seed, -1, NULL, 0))
      {
         mxg_error ("%s", mxgd_last_error ());
         return -1;
@@ -332,7 +332,7 @@ int marx_detect (Marx_Photon_Type *pt, int verbose)
    pt->history |= detector_history ();
    if ((-1 == sync_config ()) || (-1 == make_resident (pt)))
      return -1;
-   if (-1 == mxgd_run_detect (pt->start_time, Seed, -1, NULL))
+   if (-1 == mxgd_run_detect (pt->start_time, Seed, -1, NULL, 0))
      {
         mxg_error ("%s", mxgd_last_error ());
         return -1;
@@ -619,6 +619,8 @@ int mxg_trace_device (unsigned int num, unsigned int tag_start, double start_tim
    double last_t;
    unsigned long hist = MARX_ENERGY_OK | MARX_TIME_OK | MARX_X_VECTOR_OK | MARX_P_VECTOR_OK | MARX_TAG_OK;
    int with_grating = (Grating_Type == MARX_GRATING_HETG) || (Grating_Type == MARX_GRATING_LETG);
+   /* the grating hands its survivors to the detector densely */
+   int relay = with_grating && (Detector_Type > 0);
 
    if (-1 == sync_config ())
      return -1;
@@ -637,8 +639,8 @@ int mxg_trace_device (unsigned int num, unsigned int tag_start, double start_tim
     * 4 and 5 the two mirror kernels alone */
    if ((-1 == mxgd_run_source (num, tag_start, start_time, exposure_time, Seed, 0, 0, &n, &last_t, NULL))
        || (-1 == mxgd_run_mirror (Mxg_Cfg.src.distance, Seed, 1, NULL))
-       || (with_grating && (-1 == mxgd_run_grating (Seed, 2, NULL)))
-       || ((Detector_Type > 0) && (-1 == mxgd_run_detect (start_time, Seed, 3, NULL)))
+       || (with_grating && (-1 == mxgd_run_grating (Seed, 2, NULL, relay)))
+       || ((Detector_Type > 0) && (-1 == mxgd_run_detect (start_time, Seed, 3, NULL, relay)))
        || (-1 == mxgd_read_timings (ms, counts, 6)))
      {
         mxg_error ("%s", mxgd_last_error ());
@@ -730,6 +732,8 @@ static Batch_Key Batch_Graph_Key;
  * counts) */
 static int batch_pass (unsigned int n, double start_time, double exposure, int with_grating)
 {
+   /* the grating hands its survivors to the detector densely */
+   int relay = with_grating && (Detector_Type > 0);
    mxgd_schedule sched;
    mxgd_get_schedule (&sched);
    /* the persistent pass traces point sources without the FEF step */
@@ -737,8 +741,8 @@ static int batch_pass (unsigned int n, double start_time, double exposure, int w
      return mxgd_run_batch_persistent (n, exposure, start_time, Seed, 0);
    if ((-1 == mxgd_run_batch_source (n, exposure, start_time, Seed, 0))
        || (-1 == mxgd_run_mirror (Mxg_Cfg.src.distance, Seed, 1, NULL))
-       || (with_grating && (-1 == mxgd_run_grating (Seed, 2, NULL)))
-       || ((Detector_Type > 0) && (-1 == mxgd_run_detect (start_time, Seed, 3, NULL))))
+       || (with_grating && (-1 == mxgd_run_grating (Seed, 2, NULL, relay)))
+       || ((Detector_Type > 0) && (-1 == mxgd_run_detect (start_time, Seed, 3, NULL, relay))))
      return -1;
    return 0;
 }

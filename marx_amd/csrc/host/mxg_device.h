@@ -43,8 +43,11 @@ int mxgd_run_mirror (double source_distance, unsigned long long seed, int slot, 
  * kernels alone. */
 #define MXGD_SLOT_HRMA_ENTER 4
 #define MXGD_SLOT_HRMA_REFLECT 5
-int mxgd_run_grating (unsigned long long seed, int slot, float *ms);
-int mxgd_run_detect (double tstart, unsigned long long seed, int slot, float *ms);
+/* relay: the grating also hands its survivors densely to the detector
+ * stage, which must then run next with relay set (and with no other stage,
+ * upload or list change in between); the detector reads them from there. */
+int mxgd_run_grating (unsigned long long seed, int slot, float *ms, int relay);
+int mxgd_run_detect (double tstart, unsigned long long seed, int slot, float *ms, int relay);
 /* Synchronises and returns the times (ms) of the stages recorded in slots
  * 0..nslots-1 and, for stage slots, the survivor count each one left. */
 int mxgd_read_timings (float *ms, unsigned int *counts, int nslots);

@@ -353,23 +353,25 @@ static __device__ __noinline__ double contam_transmission (const mxg_contam &c, 
 
 // k_detect's work for photon id (KIND: MXG_DET_ACIS, MXG_DET_HRC_S,
 // MXG_DET_HRC_I; FEF: with the ACIS FEF step).  True when the photon is
-// detected.
-template <bool FEF, int KIND>
+// detected.  RELAY: the photon's incoming x, p, energy, tag and flags are
+// read from row `slot` of *relay instead of the store.
+template <bool FEF, int KIND, bool RELAY = false>
 __device__ __forceinline__ bool detect_one (const mxg_detector_cfg &d, const float *__restrict__ fa,
                                             const double *__restrict__ da, Soa s, uint32_t id, double tstart,
-                                            unsigned long long seed, uint32_t source_id)
+                                            unsigned long long seed, uint32_t source_id,
+                                            const Relay *relay = nullptr, uint32_t slot = 0)
 {
    constexpr bool is_acis = (KIND == MXG_DET_ACIS);
    constexpr bool is_hrc_s = (KIND == MXG_DET_HRC_S);
    bool keep = false;
-   uint32_t flags = s.flags[id];
+   uint32_t flags = RELAY ? relay->flags[slot] : s.flags[id];
    if ((flags & BAD_PHOTON_MASK) == 0)
      {
         Stream st;
-        stream_init (st, seed, s.tag[id], MXG_STAGE_DETECT, source_id);
-        vec3 x = v3 (s.x0[id], s.x1[id], s.x2[id]);
-        vec3 p = v3 (s.p0[id], s.p1[id], s.p2[id]);
-        const double energy = s.energy[id];
+        stream_init (st, seed, RELAY ? relay->tag[slot] : s.tag[id], MXG_STAGE_DETECT, source_id);
+        vec3 x = RELAY ? v3 (relay->x0[slot], relay->x1[slot], relay->x2[slot]) : v3 (s.x0[id], s.x1[id], s.x2[id]);
+        vec3 p = RELAY ? v3 (relay->p0[slot], relay->p1[slot], relay->p2[slot]) : v3 (s.p0[id], s.p1[id], s.p2[id]);
+        const double energy = RELAY ? relay->energy[slot] : s.energy[id];
         bool detect = true;
 
         if (is_hrc_s && d.use_drake)

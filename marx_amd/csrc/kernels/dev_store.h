@@ -49,6 +49,20 @@ struct ListIO
 constexpr int kBlock = 256;
 constexpr uint32_t kMaxSegments = 8192;
 
+// A stage's survivors handed densely to the detector: the columns the
+// detector reads of a photon (x, p, energy, tag, flags), written by the
+// grating kernel at the survivor's segment slot (its position in seg), and
+// pos, written by k_compact: the slot of each entry of the compacted list.
+// After the grating about one photon id in sixteen is alive, so reading
+// these columns at the ids costs a cache line per column per photon; the
+// relay's rows are contiguous.
+struct Relay
+{
+   double *x0, *x1, *x2, *p0, *p1, *p2, *energy;
+   uint32_t *tag, *flags;
+   uint32_t *pos;
+};
+
 
 struct BlockRange { uint32_t lo, hi; };
 
@@ -73,13 +87,15 @@ struct SegWriter
 {
    uint32_t *out;
    uint32_t *wc;
+   uint32_t lo;
    uint32_t count;
    uint32_t parity;
 
    __device__ __forceinline__ SegWriter (const ListIO &io, const BlockRange &r, uint32_t *lds_wc)
-      : out (io.seg + r.lo), wc (lds_wc), count (0), parity (0) {}
+      : out (io.seg + r.lo), wc (lds_wc), lo (r.lo), count (0), parity (0) {}
 
-   __device__ __forceinline__ void add (bool keep, uint32_t id)
+   // returns the slot (position in seg) of a kept id
+   __device__ __forceinline__ uint32_t add (bool keep, uint32_t id)
    {
       constexpr int kWaves = kBlock / 64;
       unsigned long long mask = __ballot (keep);
@@ -95,10 +111,11 @@ struct SegWriter
            before += (k < wave) ? c : 0u;
            total += c;
         }
-      if (keep)
-        out[count + before + (uint32_t) __popcll (mask & ((1ull << lane) - 1ull))] = id;
+      const uint32_t q = count + before + (uint32_t) __popcll (mask & ((1ull << lane) - 1ull));
+      if (keep) out[q] = id;
       count += total;
       parity ^= 1u;
+      return lo + q;
    }
 
    __device__ __forceinline__ void finish (const ListIO &io)
@@ -107,8 +124,9 @@ struct SegWriter
    }
 };
 
-// Launches k_compact over the segments a stage kernel of io.grid blocks wrote.
-hipError_t launch_compact (const ListIO &io, hipStream_t st);
+// Launches k_compact over the segments a stage kernel of io.grid blocks
+// wrote; pos_out, when non-null, receives the slot of each list entry.
+hipError_t launch_compact (const ListIO &io, hipStream_t st, uint32_t *pos_out = nullptr);
 
 // Output files (output.hip): the photons not flagged bad, in order, then the
 // requested columns of them packed big-endian, column k at out + offset[k].
@@ -204,10 +222,11 @@ hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const flo
 hipError_t launch_hrma_reflect (const mxg_config *cfg, const mxg_config *hcfg, const float *fa,
                                 const double *da, Soa s, ListIO io, const uint32_t *rng_pos, int waves,
                                 unsigned long long seed, uint32_t source_id, hipStream_t st);
-// letg: the instance compiled for the LETG pass
+// letg: the instance compiled for the LETG pass; relay, when non-null:
+// the survivors also go to it (for launch_detect)
 hipError_t launch_grating (const mxg_config *cfg, const float *fa, const double *da, Soa s,
                            ListIO io, uint32_t n_upper, unsigned long long seed,
-                           uint32_t source_id, bool letg, hipStream_t st);
+                           uint32_t source_id, bool letg, const Relay *relay, hipStream_t st);
 // fef: run the variant with the ACIS FEF step (the configuration read a FEF
 // file); kind: the detector family (detector_kind of the configuration)
 enum { MXG_DET_ACIS = 0, MXG_DET_HRC_S = 1, MXG_DET_HRC_I = 2 };
@@ -216,9 +235,12 @@ inline int detector_kind (const mxg_config &c)
    return (c.det.type == MARX_DETECTOR_HRC_S) ? MXG_DET_HRC_S
      : (c.det.type == MARX_DETECTOR_HRC_I) ? MXG_DET_HRC_I : MXG_DET_ACIS;
 }
+// relay, when non-null: the input list is the one the grating stage left
+// with this relay, whose rows the kernel reads instead of the store's
 hipError_t launch_detect (const mxg_config *cfg, const float *fa, const double *da, Soa s,
                           ListIO io, uint32_t n_upper, double tstart,
-                          unsigned long long seed, uint32_t source_id, bool fef, int kind, hipStream_t st);
+                          unsigned long long seed, uint32_t source_id, bool fef, int kind,
+                          const Relay *relay, hipStream_t st);
 
 hipError_t launch_dither_store (const mxg_config *cfg, Soa s, uint32_t n, double start_time,
                                 unsigned long long seed, hipStream_t st);

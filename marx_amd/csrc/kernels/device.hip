@@ -55,6 +55,8 @@ unsigned int g_upper = 0;          // host-side upper bound of the active count
 Marx_Photon_Attr_Type *d_aos = nullptr;
 size_t g_aos_cap = 0;
 uint64_t *d_ticks = nullptr;       // arrival-time scan (fixed-point gaps)
+void *d_relay_mem = nullptr;       // the grating-to-detector relay (capacity rows)
+Relay g_relay = {};
 // batched point sources
 uint32_t *d_batch_off = nullptr;   // nsrc + 1 prefix offsets
 double *d_batch_dir = nullptr;     // 3 nsrc directions
@@ -186,6 +188,7 @@ int ensure_capacity (size_t n)
    if (d_seg) { (void) hipFree (d_seg); d_seg = nullptr; }
    if (d_rng_pos) { (void) hipFree (d_rng_pos); d_rng_pos = nullptr; }
    if (d_tile_cnt) { (void) hipFree (d_tile_cnt); d_tile_cnt = nullptr; }
+   if (d_relay_mem) { (void) hipFree (d_relay_mem); d_relay_mem = nullptr; }
    g_capacity = 0;
    g_tile_cap = 0;
    HIPCHECK (hipMalloc (&mem, total));
@@ -199,6 +202,18 @@ int ensure_capacity (size_t n)
    // tiles of at least kBlock photons
    HIPCHECK (hipMalloc (&d_tile_cnt, 2 * (cap / kBlock + 1) * sizeof (uint32_t)));
    g_tile_cap = cap / kBlock + 1;
+   {
+      // relay: seven double and three 32-bit columns
+      const size_t dcol = align256 (cap * sizeof (double)), ucol = align256 (cap * sizeof (uint32_t));
+      HIPCHECK (hipMalloc (&d_relay_mem, 7 * dcol + 3 * ucol));
+      char *q = (char *) d_relay_mem;
+      double **dc[7] = {&g_relay.x0, &g_relay.x1, &g_relay.x2, &g_relay.p0, &g_relay.p1, &g_relay.p2,
+                        &g_relay.energy};
+      for (int j = 0; j < 7; j++) { *dc[j] = (double *) q; q += dcol; }
+      g_relay.tag = (uint32_t *) q; q += ucol;
+      g_relay.flags = (uint32_t *) q; q += ucol;
+      g_relay.pos = (uint32_t *) q;
+   }
    d_store = mem;
    char *p = (char *) mem;
    size_t k = 0;
@@ -302,7 +317,7 @@ namespace mxg {
 __global__ __launch_bounds__ (kBlock)
 void k_compact (const uint32_t *__restrict__ seg, const uint32_t *__restrict__ seg_count,
                 const uint32_t *__restrict__ n_in, uint32_t *__restrict__ out,
-                uint32_t *__restrict__ n_out)
+                uint32_t *__restrict__ n_out, uint32_t *__restrict__ pos_out)
 {
    __shared__ uint32_t part[kBlock / 64];
    const uint32_t b = blockIdx.x;
@@ -316,13 +331,15 @@ void k_compact (const uint32_t *__restrict__ seg, const uint32_t *__restrict__ s
    const uint32_t cnt = seg_count[b];
    const BlockRange r = block_range (*n_in, b, gridDim.x);
    for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) out[prefix + k] = seg[r.lo + k];
+   if (pos_out)
+     for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) pos_out[prefix + k] = r.lo + k;
    if ((b == gridDim.x - 1) && (threadIdx.x == 0)) *n_out = prefix + cnt;
 }
 
-hipError_t launch_compact (const ListIO &io, hipStream_t st)
+hipError_t launch_compact (const ListIO &io, hipStream_t st, uint32_t *pos_out)
 {
    hipLaunchKernelGGL (k_compact, dim3 (io.grid), dim3 (kBlock), 0, st, io.seg, io.seg_count,
-                       io.n_in, io.idx_out, io.n_out);
+                       io.n_in, io.idx_out, io.n_out, pos_out);
    return hipGetLastError ();
 }
 
@@ -558,24 +575,25 @@ int mxgd_run_mirror (double source_distance, unsigned long long seed, int slot, 
    return end_stage (slot, ms);
 }
 
-int mxgd_run_grating (unsigned long long seed, int slot, float *ms)
+int mxgd_run_grating (unsigned long long seed, int slot, float *ms, int relay)
 {
    if (!d_cfg) return fail ("configuration not on the device");
    if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
    if (begin_stage (slot)) return -1;
    HIPCHECK (launch_grating (d_cfg, d_fa, d_da, g_soa, list_io (MXGD_K_GRATING), g_upper, seed, 0,
-                             g_hcfg.grating.type == MARX_GRATING_LETG,
+                             g_hcfg.grating.type == MARX_GRATING_LETG, relay ? &g_relay : nullptr,
                              g_stream));
    return end_stage (slot, ms);
 }
 
-int mxgd_run_detect (double tstart, unsigned long long seed, int slot, float *ms)
+int mxgd_run_detect (double tstart, unsigned long long seed, int slot, float *ms, int relay)
 {
    if (!d_cfg) return fail ("configuration not on the device");
    if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
    if (begin_stage (slot)) return -1;
    HIPCHECK (launch_detect (d_cfg, d_fa, d_da, g_soa, list_io (MXGD_K_DETECT), g_upper, tstart,
-                            seed, 0, g_hcfg.det.fef_on != 0, detector_kind (g_hcfg), g_stream));
+                            seed, 0, g_hcfg.det.fef_on != 0, detector_kind (g_hcfg),
+                            relay ? &g_relay : nullptr, g_stream));
    return end_stage (slot, ms);
 }
 
@@ -1273,14 +1291,14 @@ int mxgd_trace_noalloc (const mxg_soa_view *v, unsigned int n, unsigned int tag_
      {
         HIPCHECK (clear_out ());
         HIPCHECK (launch_grating (c_cfg, c_fa, c_da, s, io (MXGD_K_GRATING), n, seed, 0,
-                                  cfg->grating.type == MARX_GRATING_LETG, g_stream));
+                                  cfg->grating.type == MARX_GRATING_LETG, nullptr, g_stream));
         cur = 1 - cur;
      }
    if (cfg->det.type > 0)
      {
         HIPCHECK (clear_out ());
         HIPCHECK (launch_detect (c_cfg, c_fa, c_da, s, io (MXGD_K_DETECT), n, start_time, seed, 0,
-                                 cfg->det.fef_on != 0, detector_kind (*cfg), g_stream));
+                                 cfg->det.fef_on != 0, detector_kind (*cfg), nullptr, g_stream));
         cur = 1 - cur;
      }
    uint32_t out[2] = {0, 0};

@@ -222,13 +222,21 @@ __device__ __forceinline__ int diffract_from (const mxg_grating &g, const mxg_se
    return diffract_photon (g, gs, da, theta, energy, x, p, order, st);
 }
 
+// A photon leaving the grating stage: the columns the detector reads.
+struct GratingOut
+{
+   vec3 x, p;
+   double energy;
+   uint32_t tag, flags;
+};
+
 // k_grating's work for photon id: vignetting, the Rowland torus, the order
 // draw and diffraction, and for LETG the support gratings.  True when the
-// photon leaves the grating stage alive.
+// photon leaves the grating stage alive, its columns then also in o.
 template <bool LETG>
 __device__ __forceinline__ bool grating_one (const mxg_grating_cfg &gc, const float *__restrict__ fa,
                                              const double *__restrict__ da, Soa s, uint32_t id,
-                                             unsigned long long seed, uint32_t source_id)
+                                             unsigned long long seed, uint32_t source_id, GratingOut &o)
 {
    const mxg_sectors no_sectors = {0u, 0u};
    bool keep = false;
@@ -236,7 +244,8 @@ __device__ __forceinline__ bool grating_one (const mxg_grating_cfg &gc, const fl
    if ((flags & BAD_PHOTON_MASK) == 0)
      {
         Stream st;
-        stream_init (st, seed, s.tag[id], MXG_STAGE_GRATING, source_id);
+        const uint32_t tag = s.tag[id];
+        stream_init (st, seed, tag, MXG_STAGE_GRATING, source_id);
         const int shell = s.shell[id];
         const mxg_grating &g = gc.g[shell];
         if (uniform (st) > g.vig)
@@ -282,6 +291,11 @@ __device__ __forceinline__ bool grating_one (const mxg_grating_cfg &gc, const fl
              s.p0[id] = p.x; s.p1[id] = p.y; s.p2[id] = p.z;
              s.flags[id] = flags;
              keep = (flags & BAD_PHOTON_MASK) == 0;
+             o.x = x;
+             o.p = p;
+             o.energy = energy;
+             o.tag = tag;
+             o.flags = flags;
           }
      }
    return keep;

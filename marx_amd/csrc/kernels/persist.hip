@@ -175,7 +175,8 @@ void k_batch_persistent (PersistArgs a)
                   if (j < m)
                     {
                        id = list[j];
-                       keep = grating_one<G == 2> (a.cfg->grating, a.fa, a.da, a.s, id, a.seed, 0u);
+                       GratingOut o;
+                       keep = grating_one<G == 2> (a.cfg->grating, a.fa, a.da, a.s, id, a.seed, 0u, o);
                     }
                   r.add (keep, id, list);
                }
