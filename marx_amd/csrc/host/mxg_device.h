@@ -38,16 +38,16 @@ int mxgd_get_soa (mxg_soa_view *view);
 /* slot >= 0 records HIP events around the stage's kernels into timing slot
  * `slot` (read with mxgd_read_timings); ms, when non-NULL, synchronises and
  * returns the stage time at once. */
-int mxgd_run_mirror (double source_distance, unsigned long long seed, int slot, float *ms);
+/* relay_out (mirror, grating): the stage also hands its survivors densely
+ * to the next stage, which must run next with relay_in set (with no other
+ * stage, upload or list change in between) and reads them from there. */
+int mxgd_run_mirror (double source_distance, unsigned long long seed, int slot, float *ms, int relay_out);
 /* Timing slots that mxgd_run_mirror, given a slot, also fills with its two
  * kernels alone. */
 #define MXGD_SLOT_HRMA_ENTER 4
 #define MXGD_SLOT_HRMA_REFLECT 5
-/* relay: the grating also hands its survivors densely to the detector
- * stage, which must then run next with relay set (and with no other stage,
- * upload or list change in between); the detector reads them from there. */
-int mxgd_run_grating (unsigned long long seed, int slot, float *ms, int relay);
-int mxgd_run_detect (double tstart, unsigned long long seed, int slot, float *ms, int relay);
+int mxgd_run_grating (unsigned long long seed, int slot, float *ms, int relay_in, int relay_out);
+int mxgd_run_detect (double tstart, unsigned long long seed, int slot, float *ms, int relay_in);
 /* Synchronises and returns the times (ms) of the stages recorded in slots
  * 0..nslots-1 and, for stage slots, the survivor count each one left. */
 int mxgd_read_timings (float *ms, unsigned int *counts, int nslots);
@@ -60,6 +60,15 @@ int mxgd_read_timings (float *ms, unsigned int *counts, int nslots);
 int mxgd_run_source (unsigned int n, unsigned int tag_start, double start_time,
                      double exposure_time, unsigned long long seed, unsigned int source_id,
                      int slot, unsigned int *n_out, double *last_time, float *ms);
+
+/* As mxgd_run_source without reading anything back (no host round trip
+ * before the next stage): the number created goes to the count of timing
+ * slot `slot` (read with mxgd_read_timings), the stages that follow size
+ * their launches for n, and mxgd_set_created must then be given that
+ * number. */
+int mxgd_launch_source (unsigned int n, unsigned int tag_start, double start_time, double exposure_time,
+                        unsigned long long seed, unsigned int source_id, int slot);
+int mxgd_set_created (unsigned int m);
 
 /* ---- batched point sources ---- */
 

@@ -44,24 +44,46 @@ struct ListIO
    uint32_t *seg;          // per-block survivor segments, capacity entries
    uint32_t *seg_count;    // grid entries
    uint32_t grid;          // blocks of the stage kernel (<= kMaxSegments)
+   uint32_t *n_copy = nullptr;  // when non-null, k_compact also writes the length here
 };
 
 constexpr int kBlock = 256;
 constexpr uint32_t kMaxSegments = 8192;
 
-// A stage's survivors handed densely to the detector: the columns the
-// detector reads of a photon (x, p, energy, tag, flags), written by the
-// grating kernel at the survivor's segment slot (its position in seg), and
-// pos, written by k_compact: the slot of each entry of the compacted list.
-// After the grating about one photon id in sixteen is alive, so reading
-// these columns at the ids costs a cache line per column per photon; the
-// relay's rows are contiguous.
+// A stage's survivors handed densely to the next stage: the columns the
+// grating and the detector read of a photon (x, p, energy, tag, flags,
+// shell), written by the producing kernel (the reflection, the grating) at
+// the survivor's segment slot (its position in seg), and pos, written by
+// k_compact: the slot of each entry of the compacted list.  After the
+// mirror about one photon id in three is alive, after the grating one in
+// sixteen, so reading the double columns at the ids costs a cache line per
+// column for a few photons; the relay's rows are contiguous.
 struct Relay
 {
    double *x0, *x1, *x2, *p0, *p1, *p2, *energy;
    uint32_t *tag, *flags;
+   uint8_t *shell;
    uint32_t *pos;
 };
+
+// A photon leaving a stage: the columns of its relay row.
+struct PhotonOut
+{
+   double x0, x1, x2, p0, p1, p2, energy;
+   uint32_t tag, flags;
+   int shell;
+};
+
+// a survivor's row of a relay
+__device__ __forceinline__ void relay_put (const Relay &r, uint32_t q, const PhotonOut &o)
+{
+   r.x0[q] = o.x0; r.x1[q] = o.x1; r.x2[q] = o.x2;
+   r.p0[q] = o.p0; r.p1[q] = o.p1; r.p2[q] = o.p2;
+   r.energy[q] = o.energy;
+   r.tag[q] = o.tag;
+   r.flags[q] = o.flags;
+   r.shell[q] = (uint8_t) o.shell;
+}
 
 
 struct BlockRange { uint32_t lo, hi; };
@@ -206,27 +228,34 @@ struct HrmaScratch
    uint32_t *rng_pos;      // per photon: random-stream position at the split
    uint32_t reflect_grid;  // blocks of k_hrma_reflect
    int reflect_waves;      // 2, 3 or 4: the instance of k_hrma_reflect launched
+   uint32_t *n_mid_copy = nullptr;  // when non-null, also receives the middle list's length
 };
 
 // Stage launchers (defined in the stage .hip files).
 // cfg: device copy; hcfg: host copy of the same configuration (launch
 // geometry and LDS layout are computed from it).  ev, when non-null, holds
 // four events recorded around k_hrma_enter (0, 1) and k_hrma_reflect (2, 3).
+// relay_out, when non-null: the mirror's survivors also go to it (for the
+// grating or the detector)
 hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const float *fa,
                         const double *da, Soa s, ListIO io, HrmaScratch hs, uint32_t n_upper,
                         double source_distance, unsigned long long seed, uint32_t source_id,
-                        hipStream_t st, hipEvent_t *ev);
+                        const Relay *relay_out, hipStream_t st, hipEvent_t *ev);
 // k_hrma_reflect alone, from io.idx_in (photons that hit the paraboloid,
 // stream positions in rng_pos) to io's survivor segments (io.grid blocks;
 // the caller compacts them); waves: the instance launched (2, 3 or 4)
 hipError_t launch_hrma_reflect (const mxg_config *cfg, const mxg_config *hcfg, const float *fa,
                                 const double *da, Soa s, ListIO io, const uint32_t *rng_pos, int waves,
-                                unsigned long long seed, uint32_t source_id, hipStream_t st);
-// letg: the instance compiled for the LETG pass; relay, when non-null:
-// the survivors also go to it (for launch_detect)
+                                unsigned long long seed, uint32_t source_id, const Relay *relay_out,
+                                hipStream_t st);
+// letg: the instance compiled for the LETG pass; relay_in, when non-null:
+// the input list is the mirror's with this relay, whose rows the kernel
+// reads instead of the store's; relay_out, when non-null: the survivors
+// also go to it (for launch_detect)
 hipError_t launch_grating (const mxg_config *cfg, const float *fa, const double *da, Soa s,
                            ListIO io, uint32_t n_upper, unsigned long long seed,
-                           uint32_t source_id, bool letg, const Relay *relay, hipStream_t st);
+                           uint32_t source_id, bool letg, const Relay *relay_in, const Relay *relay_out,
+                           hipStream_t st);
 // fef: run the variant with the ACIS FEF step (the configuration read a FEF
 // file); kind: the detector family (detector_kind of the configuration)
 enum { MXG_DET_ACIS = 0, MXG_DET_HRC_S = 1, MXG_DET_HRC_I = 2 };
@@ -235,7 +264,7 @@ inline int detector_kind (const mxg_config &c)
    return (c.det.type == MARX_DETECTOR_HRC_S) ? MXG_DET_HRC_S
      : (c.det.type == MARX_DETECTOR_HRC_I) ? MXG_DET_HRC_I : MXG_DET_ACIS;
 }
-// relay, when non-null: the input list is the one the grating stage left
+// relay, when non-null: the input list is the one the previous stage left
 // with this relay, whose rows the kernel reads instead of the store's
 hipError_t launch_detect (const mxg_config *cfg, const float *fa, const double *da, Soa s,
                           ListIO io, uint32_t n_upper, double tstart,
@@ -246,13 +275,14 @@ hipError_t launch_dither_store (const mxg_config *cfg, Soa s, uint32_t n, double
                                 unsigned long long seed, hipStream_t st);
 // Photon creation (source.hip): photons 0..n-1 of the store from tag
 // tag_start, the exposure cut in *first_over, the active list idx and its
-// length *count; ticks (n entries) and tile_tot (ceil(n / 2048) entries)
-// are scratch; point: the configured shape is a point (the instance without
-// the extended shapes).
+// length *count (and *count_copy when non-null); ticks (n entries) and
+// tile_tot (ceil(n / 2048) entries) are scratch; point: the configured
+// shape is a point (the instance without the extended shapes).
 hipError_t launch_source (const mxg_config *cfg, const float *fa, const double *da, Soa s, uint32_t n,
                           uint32_t tag_start, double exposure_time, double start_time,
                           unsigned long long seed, uint32_t source_id, uint64_t *ticks, uint64_t *tile_tot,
-                          uint32_t *first_over, uint32_t *count, uint32_t *idx, bool point, hipStream_t st);
+                          uint32_t *first_over, uint32_t *count, uint32_t *count_copy, uint32_t *idx, bool point,
+                          hipStream_t st);
 
 // Batched point sources (see source.hip): n photons over nsrc sources with
 // prefix offsets off[nsrc + 1] and unit directions dir[3 nsrc]; point: the

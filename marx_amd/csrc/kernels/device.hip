@@ -55,8 +55,9 @@ unsigned int g_upper = 0;          // host-side upper bound of the active count
 Marx_Photon_Attr_Type *d_aos = nullptr;
 size_t g_aos_cap = 0;
 uint64_t *d_ticks = nullptr;       // arrival-time scan (fixed-point gaps)
-void *d_relay_mem = nullptr;       // the grating-to-detector relay (capacity rows)
-Relay g_relay = {};
+void *d_relay_mem = nullptr;       // two stage-to-stage relays (capacity rows each)
+Relay g_relay[2] = {};
+int g_relay_have = -1;             // the relay the last stage wrote for the current list, or -1
 // batched point sources
 uint32_t *d_batch_off = nullptr;   // nsrc + 1 prefix offsets
 double *d_batch_dir = nullptr;     // 3 nsrc directions
@@ -203,17 +204,24 @@ int ensure_capacity (size_t n)
    HIPCHECK (hipMalloc (&d_tile_cnt, 2 * (cap / kBlock + 1) * sizeof (uint32_t)));
    g_tile_cap = cap / kBlock + 1;
    {
-      // relay: seven double and three 32-bit columns
+      // each relay: seven double, three 32-bit and one byte column
       const size_t dcol = align256 (cap * sizeof (double)), ucol = align256 (cap * sizeof (uint32_t));
-      HIPCHECK (hipMalloc (&d_relay_mem, 7 * dcol + 3 * ucol));
-      char *q = (char *) d_relay_mem;
-      double **dc[7] = {&g_relay.x0, &g_relay.x1, &g_relay.x2, &g_relay.p0, &g_relay.p1, &g_relay.p2,
-                        &g_relay.energy};
-      for (int j = 0; j < 7; j++) { *dc[j] = (double *) q; q += dcol; }
-      g_relay.tag = (uint32_t *) q; q += ucol;
-      g_relay.flags = (uint32_t *) q; q += ucol;
-      g_relay.pos = (uint32_t *) q;
+      const size_t bcol = align256 (cap);
+      const size_t each = 7 * dcol + 3 * ucol + bcol;
+      HIPCHECK (hipMalloc (&d_relay_mem, 2 * each));
+      for (int k = 0; k < 2; k++)
+        {
+           Relay &r = g_relay[k];
+           char *q = (char *) d_relay_mem + k * each;
+           double **dc[7] = {&r.x0, &r.x1, &r.x2, &r.p0, &r.p1, &r.p2, &r.energy};
+           for (int j = 0; j < 7; j++) { *dc[j] = (double *) q; q += dcol; }
+           r.tag = (uint32_t *) q; q += ucol;
+           r.flags = (uint32_t *) q; q += ucol;
+           r.pos = (uint32_t *) q; q += ucol;
+           r.shell = (uint8_t *) q;
+        }
    }
+   g_relay_have = -1;
    d_store = mem;
    char *p = (char *) mem;
    size_t k = 0;
@@ -259,7 +267,15 @@ uint32_t stage_grid (uint32_t n_upper, int k)
    return blocks ? blocks : 1u;
 }
 
-ListIO list_io (int k)
+// count slot `slot` (d_counts[8 + slot]), or null for none
+uint32_t *slot_count (int slot)
+{
+   return ((slot >= 0) && (slot < kSlots)) ? d_counts + 8 + slot : nullptr;
+}
+
+// Stage k's lists: the active list in, the other one out; k_compact also
+// writes the survivors' number to count slot `slot` (when there is one).
+ListIO list_io (int k, int slot = -1)
 {
    ListIO io;
    io.idx_in = d_idx[g_cur];
@@ -269,6 +285,7 @@ ListIO list_io (int k)
    io.seg = d_seg;
    io.seg_count = d_seg_count;
    io.grid = stage_grid (g_upper, k);
+   io.n_copy = slot_count (slot);
    return io;
 }
 
@@ -276,8 +293,6 @@ __global__ void k_copy_u32 (uint32_t *dst, const uint32_t *src) { *dst = *src; }
 
 int begin_stage (int slot)
 {
-   hipLaunchKernelGGL (k_set_u32, dim3 (1), dim3 (1), 0, g_stream, d_counts + (1 - g_cur), 0u);
-   HIPCHECK (hipGetLastError ());
    if ((slot >= 0) && (slot < kSlots) && !g_capturing)
      {
         HIPCHECK (hipEventRecord (g_tev[slot][0], g_stream));
@@ -286,18 +301,22 @@ int begin_stage (int slot)
    return 0;
 }
 
-// count slot 8 + k receives the length of the survivor list of stage k
+// A stage whose kernels do not run (no photon can be active): its output
+// list and count slot are empty.  (A stage that runs gets both from
+// k_compact.)
+int skip_stage (const ListIO &io)
+{
+   hipLaunchKernelGGL (k_set_u32, dim3 (1), dim3 (1), 0, g_stream, io.n_out, 0u);
+   if (io.n_copy) hipLaunchKernelGGL (k_set_u32, dim3 (1), dim3 (1), 0, g_stream, io.n_copy, 0u);
+   HIPCHECK (hipGetLastError ());
+   return 0;
+}
+
 int end_stage (int slot, float *ms)
 {
    if ((slot >= 0) && (slot < kSlots) && !g_capturing)
      HIPCHECK (hipEventRecord (g_tev[slot][1], g_stream));
    g_cur = 1 - g_cur;
-   if ((slot >= 0) && (slot < kSlots))
-     {
-        hipLaunchKernelGGL (k_copy_u32, dim3 (1), dim3 (1), 0, g_stream, d_counts + 8 + slot,
-                            d_counts + g_cur);
-        HIPCHECK (hipGetLastError ());
-     }
    if (ms != nullptr)
      {
         HIPCHECK (hipEventRecord (g_ev1, g_stream));
@@ -317,7 +336,8 @@ namespace mxg {
 __global__ __launch_bounds__ (kBlock)
 void k_compact (const uint32_t *__restrict__ seg, const uint32_t *__restrict__ seg_count,
                 const uint32_t *__restrict__ n_in, uint32_t *__restrict__ out,
-                uint32_t *__restrict__ n_out, uint32_t *__restrict__ pos_out)
+                uint32_t *__restrict__ n_out, uint32_t *__restrict__ pos_out,
+                uint32_t *__restrict__ n_copy)
 {
    __shared__ uint32_t part[kBlock / 64];
    const uint32_t b = blockIdx.x;
@@ -333,13 +353,17 @@ void k_compact (const uint32_t *__restrict__ seg, const uint32_t *__restrict__ s
    for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) out[prefix + k] = seg[r.lo + k];
    if (pos_out)
      for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) pos_out[prefix + k] = r.lo + k;
-   if ((b == gridDim.x - 1) && (threadIdx.x == 0)) *n_out = prefix + cnt;
+   if ((b == gridDim.x - 1) && (threadIdx.x == 0))
+     {
+        *n_out = prefix + cnt;
+        if (n_copy) *n_copy = prefix + cnt;
+     }
 }
 
 hipError_t launch_compact (const ListIO &io, hipStream_t st, uint32_t *pos_out)
 {
    hipLaunchKernelGGL (k_compact, dim3 (io.grid), dim3 (kBlock), 0, st, io.seg, io.seg_count,
-                       io.n_in, io.idx_out, io.n_out, pos_out);
+                       io.n_in, io.idx_out, io.n_out, pos_out, io.n_copy);
    return hipGetLastError ();
 }
 
@@ -476,6 +500,7 @@ int mxgd_sync_config (const mxg_config *cfg, const float *fa, unsigned int nf,
 int mxgd_upload_aos (const Marx_Photon_Attr_Type *at, unsigned int n,
                      const unsigned int *active, unsigned int nactive)
 {
+   g_relay_have = -1;
    if (mxgd_init ()) return -1;
    if (nactive > n) return fail ("active list longer than the photon list");
    if (ensure_capacity (n ? n : 1) || ensure_aos (n ? n : 1)) return -1;
@@ -548,52 +573,102 @@ int mxgd_get_soa (mxg_soa_view *v)
    return 0;
 }
 
-int mxgd_run_mirror (double source_distance, unsigned long long seed, int slot, float *ms)
+// The relays of a stage: in, the one the previous stage left (fails when
+// there is none); out, the other one.  After the stage, the relay it wrote
+// (or none) is the current list's.
+int stage_relays (int relay_in, int relay_out, const Relay **in, const Relay **out, int *out_k)
+{
+   *in = nullptr;
+   *out = nullptr;
+   *out_k = -1;
+   int k_in = -1;
+   if (relay_in)
+     {
+        if (g_relay_have < 0) return fail ("no relay was left for this stage by the stage before");
+        k_in = g_relay_have;
+        *in = &g_relay[k_in];
+     }
+   if (relay_out)
+     {
+        *out_k = (k_in == 0) ? 1 : 0;
+        *out = &g_relay[*out_k];
+     }
+   return 0;
+}
+
+int mxgd_run_mirror (double source_distance, unsigned long long seed, int slot, float *ms, int relay_out)
 {
    if (!d_cfg) return fail ("configuration not on the device");
+   const Relay *rin, *rout;
+   int out_k;
+   if (stage_relays (0, relay_out, &rin, &rout, &out_k)) return -1;
    if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
    if (begin_stage (slot)) return -1;
+   bool timed = (slot >= 0) && (g_upper > 0);
    HrmaScratch hs = {d_idx_mid, d_counts + 4, d_rng_pos, stage_grid (g_upper, MXGD_K_REFLECT),
                      (int) g_sched.reflect_waves};
+   // photons entering each kernel: the active list, the middle list (the
+   // latter's length copied by k_compact)
+   if (timed) hs.n_mid_copy = slot_count (MXGD_SLOT_HRMA_REFLECT);
    hipEvent_t kev[4] = {g_tev[MXGD_SLOT_HRMA_ENTER][0], g_tev[MXGD_SLOT_HRMA_ENTER][1],
                         g_tev[MXGD_SLOT_HRMA_REFLECT][0], g_tev[MXGD_SLOT_HRMA_REFLECT][1]};
-   bool timed = (slot >= 0) && (g_upper > 0);
-   HIPCHECK (launch_hrma (d_cfg, &g_hcfg, d_fa, d_da, g_soa, list_io (MXGD_K_ENTER), hs, g_upper,
-                          source_distance, seed, 0, g_stream,
-                          (timed && !g_capturing) ? kev : nullptr));
+   const ListIO io = list_io (MXGD_K_ENTER, slot);
+   if (g_upper == 0)
+     {
+        if (skip_stage (io)) return -1;
+     }
+   else
+     HIPCHECK (launch_hrma (d_cfg, &g_hcfg, d_fa, d_da, g_soa, io, hs, g_upper, source_distance, seed, 0, rout,
+                            g_stream, (timed && !g_capturing) ? kev : nullptr));
+   g_relay_have = out_k;
    if (timed)
      {
-        // photons entering each kernel: the active list, the middle list
         if (!g_capturing)
           g_slot_used[MXGD_SLOT_HRMA_ENTER] = g_slot_used[MXGD_SLOT_HRMA_REFLECT] = 1;
         hipLaunchKernelGGL (k_copy_u32, dim3 (1), dim3 (1), 0, g_stream,
                             d_counts + 8 + MXGD_SLOT_HRMA_ENTER, d_counts + g_cur);
-        hipLaunchKernelGGL (k_copy_u32, dim3 (1), dim3 (1), 0, g_stream,
-                            d_counts + 8 + MXGD_SLOT_HRMA_REFLECT, hs.n_mid);
         HIPCHECK (hipGetLastError ());
      }
    return end_stage (slot, ms);
 }
 
-int mxgd_run_grating (unsigned long long seed, int slot, float *ms, int relay)
+int mxgd_run_grating (unsigned long long seed, int slot, float *ms, int relay_in, int relay_out)
 {
    if (!d_cfg) return fail ("configuration not on the device");
+   const Relay *rin, *rout;
+   int out_k;
+   if (stage_relays (relay_in, relay_out, &rin, &rout, &out_k)) return -1;
    if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
    if (begin_stage (slot)) return -1;
-   HIPCHECK (launch_grating (d_cfg, d_fa, d_da, g_soa, list_io (MXGD_K_GRATING), g_upper, seed, 0,
-                             g_hcfg.grating.type == MARX_GRATING_LETG, relay ? &g_relay : nullptr,
-                             g_stream));
+   const ListIO io = list_io (MXGD_K_GRATING, slot);
+   if (g_upper == 0)
+     {
+        if (skip_stage (io)) return -1;
+     }
+   else
+     HIPCHECK (launch_grating (d_cfg, d_fa, d_da, g_soa, io, g_upper, seed, 0,
+                               g_hcfg.grating.type == MARX_GRATING_LETG, rin, rout, g_stream));
+   g_relay_have = out_k;
    return end_stage (slot, ms);
 }
 
-int mxgd_run_detect (double tstart, unsigned long long seed, int slot, float *ms, int relay)
+int mxgd_run_detect (double tstart, unsigned long long seed, int slot, float *ms, int relay_in)
 {
    if (!d_cfg) return fail ("configuration not on the device");
+   const Relay *rin, *rout;
+   int out_k;
+   if (stage_relays (relay_in, 0, &rin, &rout, &out_k)) return -1;
    if (ms) HIPCHECK (hipEventRecord (g_ev0, g_stream));
    if (begin_stage (slot)) return -1;
-   HIPCHECK (launch_detect (d_cfg, d_fa, d_da, g_soa, list_io (MXGD_K_DETECT), g_upper, tstart,
-                            seed, 0, g_hcfg.det.fef_on != 0, detector_kind (g_hcfg),
-                            relay ? &g_relay : nullptr, g_stream));
+   const ListIO io = list_io (MXGD_K_DETECT, slot);
+   if (g_upper == 0)
+     {
+        if (skip_stage (io)) return -1;
+     }
+   else
+     HIPCHECK (launch_detect (d_cfg, d_fa, d_da, g_soa, io, g_upper, tstart, seed, 0, g_hcfg.det.fef_on != 0,
+                              detector_kind (g_hcfg), rin, g_stream));
+   g_relay_have = -1;
    return end_stage (slot, ms);
 }
 
@@ -642,6 +717,7 @@ int mxgd_run_source (unsigned int n, unsigned int tag_start, double start_time,
                      double exposure_time, unsigned long long seed, unsigned int source_id,
                      int slot, unsigned int *n_out, double *last_time, float *ms)
 {
+   g_relay_have = -1;
    if (!d_cfg) return fail ("configuration not on the device");
    if (n > (1u << 26)) return fail ("chunk of %u photons exceeds the 2^26 limit of one launch", n);
    if (ensure_capacity (n ? n : 1)) return -1;
@@ -653,7 +729,7 @@ int mxgd_run_source (unsigned int n, unsigned int tag_start, double start_time,
         g_slot_used[slot] = 1;
      }
    HIPCHECK (launch_source (d_cfg, d_fa, d_da, g_soa, n, tag_start, exposure_time, start_time, seed,
-                            source_id, d_ticks, d_tile_tot, d_counts + 2, d_counts, d_idx[0],
+                            source_id, d_ticks, d_tile_tot, d_counts + 2, d_counts, nullptr, d_idx[0],
                             g_hcfg.src.type == MXG_SOURCE_POINT, g_stream));
    if (slot >= 0 && slot < kSlots)
      HIPCHECK (hipEventRecord (g_tev[slot][1], g_stream));
@@ -676,6 +752,37 @@ int mxgd_run_source (unsigned int n, unsigned int tag_start, double start_time,
    g_upper = m;
    if (n_out) *n_out = m;
    if (last_time) *last_time = t;
+   return 0;
+}
+
+int mxgd_launch_source (unsigned int n, unsigned int tag_start, double start_time, double exposure_time,
+                        unsigned long long seed, unsigned int source_id, int slot)
+{
+   g_relay_have = -1;
+   if (!d_cfg) return fail ("configuration not on the device");
+   if (n > (1u << 26)) return fail ("chunk of %u photons exceeds the 2^26 limit of one launch", n);
+   if (ensure_capacity (n ? n : 1)) return -1;
+   g_cur = 0;
+   if ((slot >= 0) && (slot < kSlots) && !g_capturing)
+     {
+        HIPCHECK (hipEventRecord (g_tev[slot][0], g_stream));
+        g_slot_used[slot] = 1;
+     }
+   HIPCHECK (launch_source (d_cfg, d_fa, d_da, g_soa, n, tag_start, exposure_time, start_time, seed,
+                            source_id, d_ticks, d_tile_tot, d_counts + 2, d_counts, slot_count (slot), d_idx[0],
+                            g_hcfg.src.type == MXG_SOURCE_POINT, g_stream));
+   if ((slot >= 0) && (slot < kSlots) && !g_capturing)
+     HIPCHECK (hipEventRecord (g_tev[slot][1], g_stream));
+   // upper bounds until the created number is read back
+   g_n = n;
+   g_upper = n;
+   return 0;
+}
+
+int mxgd_set_created (unsigned int m)
+{
+   if (m > g_n) return fail ("%u photons created in a store of %u", m, g_n);
+   g_n = m;
    return 0;
 }
 
@@ -905,6 +1012,7 @@ int mxgd_batch_set_tag_start (unsigned int tag_start)
 int mxgd_run_batch_source (unsigned int n, double exposure, double start_time,
                            unsigned long long seed, int slot)
 {
+   g_relay_have = -1;
    if (!d_cfg) return fail ("configuration not on the device");
    if ((n > g_capacity) || (g_batch_nsrc == 0)) return fail ("batch not prepared");
    bool timed = (slot >= 0) && (slot < kSlots) && !g_capturing;
@@ -928,6 +1036,7 @@ int mxgd_run_batch_source (unsigned int n, double exposure, double start_time,
 int mxgd_run_batch_persistent (unsigned int n, double exposure, double start_time,
                                unsigned long long seed, int slot)
 {
+   g_relay_have = -1;
    if (!d_cfg) return fail ("configuration not on the device");
    if ((n > g_capacity) || (g_batch_nsrc == 0)) return fail ("batch not prepared");
    if (g_sched.persist_tile == 0) return fail ("the schedule has no persistent tile");
@@ -1256,7 +1365,7 @@ int mxgd_trace_noalloc (const mxg_soa_view *v, unsigned int n, unsigned int tag_
      hipLaunchKernelGGL (k_set_u32, dim3 (1), dim3 (1), 0, g_stream, counts + k, 0u);
    HIPCHECK (hipGetLastError ());
    HIPCHECK (launch_source (c_cfg, c_fa, c_da, s, n, tag_start, exposure_time, start_time, seed, 0, ticks, tiles,
-                            counts + 2, counts + 0, idx[0], cfg->src.type == MXG_SOURCE_POINT, g_stream));
+                            counts + 2, counts + 0, nullptr, idx[0], cfg->src.type == MXG_SOURCE_POINT, g_stream));
    hipLaunchKernelGGL (k_copy_u32, dim3 (1), dim3 (1), 0, g_stream, counts + 6, counts + 0);
    HIPCHECK (hipGetLastError ());
 
@@ -1285,13 +1394,13 @@ int mxgd_trace_noalloc (const mxg_soa_view *v, unsigned int n, unsigned int tag_
    HIPCHECK (clear_out ());
    HrmaScratch hs = {mid, counts + 4, rng, io (MXGD_K_REFLECT).grid, (int) g_sched.reflect_waves};
    HIPCHECK (launch_hrma (c_cfg, cfg, c_fa, c_da, s, io (MXGD_K_ENTER), hs, n, cfg->src.distance, seed, 0,
-                          g_stream, nullptr));
+                          nullptr, g_stream, nullptr));
    cur = 1 - cur;
    if ((cfg->grating.type == MARX_GRATING_HETG) || (cfg->grating.type == MARX_GRATING_LETG))
      {
         HIPCHECK (clear_out ());
         HIPCHECK (launch_grating (c_cfg, c_fa, c_da, s, io (MXGD_K_GRATING), n, seed, 0,
-                                  cfg->grating.type == MARX_GRATING_LETG, nullptr, g_stream));
+                                  cfg->grating.type == MARX_GRATING_LETG, nullptr, nullptr, g_stream));
         cur = 1 - cur;
      }
    if (cfg->det.type > 0)

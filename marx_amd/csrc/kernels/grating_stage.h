@@ -222,39 +222,36 @@ __device__ __forceinline__ int diffract_from (const mxg_grating &g, const mxg_se
    return diffract_photon (g, gs, da, theta, energy, x, p, order, st);
 }
 
-// A photon leaving the grating stage: the columns the detector reads.
-struct GratingOut
-{
-   vec3 x, p;
-   double energy;
-   uint32_t tag, flags;
-};
-
 // k_grating's work for photon id: vignetting, the Rowland torus, the order
 // draw and diffraction, and for LETG the support gratings.  True when the
 // photon leaves the grating stage alive, its columns then also in o.
-template <bool LETG>
+// RELAY: its incoming x, p, energy, tag, flags and shell are read from row
+// `slot` of *relay instead of the store.
+template <bool LETG, bool RELAY = false>
 __device__ __forceinline__ bool grating_one (const mxg_grating_cfg &gc, const float *__restrict__ fa,
                                              const double *__restrict__ da, Soa s, uint32_t id,
-                                             unsigned long long seed, uint32_t source_id, GratingOut &o)
+                                             unsigned long long seed, uint32_t source_id, PhotonOut &o,
+                                             const Relay *relay = nullptr, uint32_t slot = 0)
 {
    const mxg_sectors no_sectors = {0u, 0u};
    bool keep = false;
-   uint32_t flags = s.flags[id];
+   uint32_t flags = RELAY ? relay->flags[slot] : s.flags[id];
    if ((flags & BAD_PHOTON_MASK) == 0)
      {
         Stream st;
-        const uint32_t tag = s.tag[id];
+        const uint32_t tag = RELAY ? relay->tag[slot] : s.tag[id];
         stream_init (st, seed, tag, MXG_STAGE_GRATING, source_id);
-        const int shell = s.shell[id];
+        const int shell = RELAY ? (int) relay->shell[slot] : (int) s.shell[id];
         const mxg_grating &g = gc.g[shell];
         if (uniform (st) > g.vig)
           s.flags[id] = flags | PHOTON_MIRROR_VBLOCKED;
         else
           {
-             vec3 x = v3 (s.x0[id], s.x1[id], s.x2[id]);
-             vec3 p = v3 (s.p0[id], s.p1[id], s.p2[id]);
-             const double energy = s.energy[id];
+             vec3 x = RELAY ? v3 (relay->x0[slot], relay->x1[slot], relay->x2[slot])
+                            : v3 (s.x0[id], s.x1[id], s.x2[id]);
+             vec3 p = RELAY ? v3 (relay->p0[slot], relay->p1[slot], relay->p2[slot])
+                            : v3 (s.p0[id], s.p1[id], s.p2[id]);
+             const double energy = RELAY ? relay->energy[slot] : s.energy[id];
              const double ef = (double) (float) energy;
              int8_t order = s.order[id];
 
@@ -291,11 +288,12 @@ __device__ __forceinline__ bool grating_one (const mxg_grating_cfg &gc, const fl
              s.p0[id] = p.x; s.p1[id] = p.y; s.p2[id] = p.z;
              s.flags[id] = flags;
              keep = (flags & BAD_PHOTON_MASK) == 0;
-             o.x = x;
-             o.p = p;
+             o.x0 = x.x; o.x1 = x.y; o.x2 = x.z;
+             o.p0 = p.x; o.p1 = p.y; o.p2 = p.z;
              o.energy = energy;
              o.tag = tag;
              o.flags = flags;
+             o.shell = shell;
           }
      }
    return keep;

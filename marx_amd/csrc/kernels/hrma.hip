@@ -55,13 +55,14 @@ void k_hrma_enter (const mxg_config *__restrict__ cfg, Soa s, ListIO io,
 // The block stages the mirror configuration and its small, hot tables (the
 // binary-search targets of every photon) in LDS once, then loops over its
 // range of the middle list.
-// WAVES: minimum waves per SIMD the register allocation must allow.
-template <int WAVES>
+// WAVES: minimum waves per SIMD the register allocation must allow; RELAY:
+// the survivors' columns also go to relay at their segment slots.
+template <int WAVES, bool RELAY>
 __global__ __launch_bounds__ (kBlock, WAVES)
 void k_hrma_reflect (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
                      const double *__restrict__ da, Soa s, ListIO io,
                      const uint32_t *__restrict__ rng_pos, unsigned long long seed,
-                     uint32_t source_id, HrmaLds lay)
+                     uint32_t source_id, HrmaLds lay, Relay relay)
 {
    extern __shared__ __align__ (16) unsigned char lds[];
    float *ltab = (float *) (lds + kHrmaTabOff);
@@ -79,37 +80,44 @@ void k_hrma_reflect (const mxg_config *__restrict__ cfg, const float *__restrict
         uint32_t i = base + threadIdx.x;
         bool keep = false;
         uint32_t id = 0;
+        PhotonOut o;
         if (i < range.hi)
           {
              id = io.idx_in[i];
-             keep = hrma_reflect_one (h, ltab, L, fa, da, s, id, rng_pos, seed, source_id);
+             keep = hrma_reflect_one (h, ltab, L, fa, da, s, id, rng_pos, seed, source_id, o);
           }
-        survivors.add (keep, id);
+        const uint32_t q = survivors.add (keep, id);
+        if (RELAY && keep) relay_put (relay, q, o);
      }
    survivors.finish (io);
 }
 
 hipError_t launch_hrma_reflect (const mxg_config *cfg, const mxg_config *hcfg, const float *fa,
                                 const double *da, Soa s, ListIO io, const uint32_t *rng_pos, int waves,
-                                unsigned long long seed, uint32_t source_id, hipStream_t st)
+                                unsigned long long seed, uint32_t source_id, const Relay *relay_out,
+                                hipStream_t st)
 {
    const HrmaLds lay = hrma_lds_layout (hcfg->hrma);
    size_t lds_bytes = kHrmaTabOff + (size_t) lay.floats * sizeof (float);
    if (lds_bytes > 64 * 1024)
      return hipErrorInvalidConfiguration;
-   void (*reflect) (const mxg_config *, const float *, const double *, Soa, ListIO,
-                    const uint32_t *, unsigned long long, uint32_t, HrmaLds)
-     = (waves <= 2) ? k_hrma_reflect<2>
-       : (waves == 3) ? k_hrma_reflect<3> : k_hrma_reflect<MXG_REFLECT_WAVES_TOP>;
+   typedef void (*ReflectFn) (const mxg_config *, const float *, const double *, Soa, ListIO,
+                              const uint32_t *, unsigned long long, uint32_t, HrmaLds, Relay);
+   const bool r = relay_out != nullptr;
+   const ReflectFn reflect
+     = (waves <= 2) ? (r ? k_hrma_reflect<2, true> : k_hrma_reflect<2, false>)
+       : (waves == 3) ? (r ? k_hrma_reflect<3, true> : k_hrma_reflect<3, false>)
+       : (r ? k_hrma_reflect<MXG_REFLECT_WAVES_TOP, true> : k_hrma_reflect<MXG_REFLECT_WAVES_TOP, false>);
+   const Relay rel = r ? *relay_out : Relay {};
    hipLaunchKernelGGL (reflect, dim3 (io.grid), dim3 (kBlock), lds_bytes, st, cfg, fa, da, s, io, rng_pos,
-                       seed, source_id, lay);
+                       seed, source_id, lay, rel);
    return hipGetLastError ();
 }
 
 hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const float *fa,
                         const double *da, Soa s, ListIO io, HrmaScratch hs, uint32_t n_upper,
                         double source_distance, unsigned long long seed, uint32_t source_id,
-                        hipStream_t st, hipEvent_t *ev)
+                        const Relay *relay_out, hipStream_t st, hipEvent_t *ev)
 {
    if (n_upper == 0) return hipSuccess;
    const HrmaLds lay = hrma_lds_layout (hcfg->hrma);
@@ -120,6 +128,7 @@ hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const flo
    ListIO a = io, b = io;
    a.idx_out = hs.idx_mid;
    a.n_out = hs.n_mid;
+   a.n_copy = hs.n_mid_copy;
    b.idx_in = hs.idx_mid;
    b.n_in = hs.n_mid;
    b.grid = hs.reflect_grid;
@@ -133,9 +142,10 @@ hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const flo
    if (e == hipSuccess) e = launch_compact (a, st);
    if ((e == hipSuccess) && ev) e = hipEventRecord (ev[2], st);
    if (e == hipSuccess)
-     e = launch_hrma_reflect (cfg, hcfg, fa, da, s, b, hs.rng_pos, hs.reflect_waves, seed, source_id, st);
+     e = launch_hrma_reflect (cfg, hcfg, fa, da, s, b, hs.rng_pos, hs.reflect_waves, seed, source_id, relay_out,
+                              st);
    if ((e == hipSuccess) && ev) e = hipEventRecord (ev[3], st);
-   if (e == hipSuccess) e = launch_compact (b, st);
+   if (e == hipSuccess) e = launch_compact (b, st, relay_out ? relay_out->pos : nullptr);
    return e;
 }
 

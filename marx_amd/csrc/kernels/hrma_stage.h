@@ -374,17 +374,18 @@ __device__ __forceinline__ void mirror_constants (const mxg_hrma_cfg &h, const m
 // k_hrma_reflect's work for photon id of the middle list: paraboloid
 // reflection, cap struts, hyperboloid, post-collimator struts.  ltab: the
 // tables staged in LDS at the offsets lay.  True when the photon leaves the
-// mirror.
+// mirror, its columns then also in o.
 __device__ __forceinline__ bool hrma_reflect_one (const mxg_hrma_cfg &h, const float *ltab, const HrmaLds &lay,
                                                   const float *__restrict__ fa, const double *__restrict__ da,
                                                   Soa s, uint32_t id, const uint32_t *__restrict__ rng_pos,
-                                                  unsigned long long seed, uint32_t source_id)
+                                                  unsigned long long seed, uint32_t source_id, PhotonOut &o)
 {
    uint32_t flags = s.flags[id];
    const int si = s.shell[id];
    const mxg_shell_cfg &S = h.shell[si];
    Stream st;
-   stream_init (st, seed, s.tag[id], MXG_STAGE_MIRROR, source_id);
+   const uint32_t tag = s.tag[id];
+   stream_init (st, seed, tag, MXG_STAGE_MIRROR, source_id);
    resume_stream (st, rng_pos, id);
    vec3 x = v3 (s.x0[id], s.x1[id], s.x2[id]);
    vec3 p = v3 (s.p0[id], s.p1[id], s.p2[id]);
@@ -425,6 +426,12 @@ __device__ __forceinline__ bool hrma_reflect_one (const mxg_hrma_cfg &h, const f
    s.x0[id] = x.x; s.x1[id] = x.y; s.x2[id] = x.z;
    s.p0[id] = p.x; s.p1[id] = p.y; s.p2[id] = p.z;
    s.flags[id] = flags;
+   o.x0 = x.x; o.x1 = x.y; o.x2 = x.z;
+   o.p0 = p.x; o.p1 = p.y; o.p2 = p.z;
+   o.energy = energy;
+   o.tag = tag;
+   o.flags = flags;
+   o.shell = si;
    return (flags & BAD_PHOTON_MASK) == 0;
 }
 

@@ -155,7 +155,8 @@ void k_batch_persistent (PersistArgs a)
                 if (j < m)
                   {
                      id = list[j];
-                     keep = hrma_reflect_one (h, ltab, L, a.fa, a.da, a.s, id, a.rng_pos, a.seed, 0u);
+                     PhotonOut o;
+                     keep = hrma_reflect_one (h, ltab, L, a.fa, a.da, a.s, id, a.rng_pos, a.seed, 0u, o);
                   }
                 r.add (keep, id, list);
              }
@@ -175,7 +176,7 @@ void k_batch_persistent (PersistArgs a)
                   if (j < m)
                     {
                        id = list[j];
-                       GratingOut o;
+                       PhotonOut o;
                        keep = grating_one<G == 2> (a.cfg->grating, a.fa, a.da, a.s, id, a.seed, 0u, o);
                     }
                   r.add (keep, id, list);

@@ -249,11 +249,14 @@ hipError_t launch_dither_store (const mxg_config *cfg, Soa s, uint32_t n, double
 }
 
 // the length of the active list: up to and including the first photon at
-// or past the exposure, else all
-__global__ void k_source_count (uint32_t n, const uint32_t *__restrict__ first_over, uint32_t *__restrict__ count)
+// or past the exposure, else all (also to *count_copy when non-null)
+__global__ void k_source_count (uint32_t n, const uint32_t *__restrict__ first_over, uint32_t *__restrict__ count,
+                                uint32_t *__restrict__ count_copy)
 {
    const uint32_t f = *first_over;
-   *count = (f < n) ? f + 1 : n;
+   const uint32_t m = (f < n) ? f + 1 : n;
+   *count = m;
+   if (count_copy) *count_copy = m;
 }
 
 __global__ void k_reset (uint32_t *p, uint32_t v) { *p = v; }
@@ -261,11 +264,13 @@ __global__ void k_reset (uint32_t *p, uint32_t v) { *p = v; }
 hipError_t launch_source (const mxg_config *cfg, const float *fa, const double *da, Soa s, uint32_t n,
                           uint32_t tag_start, double exposure_time, double start_time,
                           unsigned long long seed, uint32_t source_id, uint64_t *ticks, uint64_t *tile_tot,
-                          uint32_t *first_over, uint32_t *count, uint32_t *idx, bool point, hipStream_t st)
+                          uint32_t *first_over, uint32_t *count, uint32_t *count_copy, uint32_t *idx, bool point,
+                          hipStream_t st)
 {
    if (n == 0)
      {
         hipLaunchKernelGGL (k_reset, dim3 (1), dim3 (1), 0, st, count, 0u);
+        if (count_copy) hipLaunchKernelGGL (k_reset, dim3 (1), dim3 (1), 0, st, count_copy, 0u);
         return hipGetLastError ();
      }
    const uint32_t ntiles = (n + kTile - 1) / kTile;
@@ -284,7 +289,7 @@ hipError_t launch_source (const mxg_config *cfg, const float *fa, const double *
    else
      hipLaunchKernelGGL (k_source<false>, dim3 (blocks), dim3 (kBlock), 0, st, cfg, fa, da, s, n, tag_start,
                          exposure_time, start_time, seed, source_id, ticks, tile_tot, first_over, idx);
-   hipLaunchKernelGGL (k_source_count, dim3 (1), dim3 (1), 0, st, n, first_over, count);
+   hipLaunchKernelGGL (k_source_count, dim3 (1), dim3 (1), 0, st, n, first_over, count, count_copy);
    return hipGetLastError ();
 }
 
