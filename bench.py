@@ -83,8 +83,11 @@ BYTES_PER_PHOTON = {
 # profiler runs): each entry names the profile it comes from, and the bench
 # line carries that name beside `traffic`.
 TRAFFIC_BYTES_PER_PHOTON = {
-    ("hetg-acis-s", "k_hrma_reflect"): (321.9, "profiles/pmc_hetg_acis_s_1e7.md"),
-    ("letg-hrc-s", "grating"): (876.9, "profiles/pmc_letg_diff.md"),
+    ("hetg-acis-s", "k_hrma_reflect"): (304.7, "profiles/pmc_hetg_acis_s_1e7.md"),
+    ("letg-hrc-s", "k_hrma_reflect"): (331.0, "profiles/pmc_letg_diff.md"),
+    ("letg-hrc-s", "grating"): (455.7, "profiles/pmc_letg_diff.md"),
+    # per listed photon
+    ("diff", "k_diff_forward"): (357.1, "profiles/pmc_letg_diff.md"),
 }
 
 
@@ -355,9 +358,10 @@ def measure_diff(args, torch):
     kernels["value_only_pass"] = {"ms": value_enter_ms + value_ms, "k_diff_forward<0>_ms": value_ms,
                                   "k_diff_enter_ms": value_enter_ms}
     fk = kernels["k_diff_forward"]
+    traffic, traffic_src = traffic_of(cfg, "k_diff_forward", listed)
     rf = {"bound": "valu", "pipe": VALU_PIPE,
           "kernel": "k_diff_forward<2>", "achieved": fk["tflops"], "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-          "frac": fk["tflops"] / PEAK_FP64_TFLOPS, "traffic": None,
+          "frac": fk["tflops"] / PEAK_FP64_TFLOPS, "traffic": traffic, "traffic_source": traffic_src,
           "flops_per_pass_photon": FLOPS_PER_PHOTON[cfg]["k_diff_forward"],
           "transcendentals_per_pass_photon": TRANSCENDENTALS_PER_PHOTON[cfg]["k_diff_forward"],
           "pass_photons": N, "photons_per_launch": listed, "launch_ms": fk["ms"],

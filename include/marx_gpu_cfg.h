@@ -46,6 +46,21 @@ typedef struct
    unsigned int off, nb, shift, ok;
 }
 mxg_bucket;
+/* Linear bucket index of a strictly increasing double table xp[0..n-1]
+ * whose entries spread evenly over their range (the cumulative spectrum,
+ * searched with a uniform variate): the key of x > x0 is
+ * min(nb - 1, (unsigned) ((x - x0) * scale)), monotone in x under IEEE
+ * rounding and computed alike on host and device; double-arena entries
+ * off .. off + nb hold, for bucket j, the number of table entries whose key
+ * is below j (entry nb: all of them).  ok = 0: search the table whole. */
+typedef struct
+{
+   double x0, scale;
+   unsigned int off, nb, ok, pad;
+}
+mxg_lbucket;
+/* largest linear bucket count (the cumulative spectrum) */
+#define MXG_BUCKETS_SPECTRUM 4096
 /* largest bucket counts: optical constants, correction factors, e_alpha */
 #define MXG_BUCKETS_OPT 256
 #define MXG_BUCKETS_CORR 64
@@ -275,6 +290,7 @@ typedef struct
    double line_alpha, line_normal[3];    /* LINE, IMAGE: the rotation of -x onto the source direction */
    unsigned int image_n, image_nx, image_ny, image_off;   /* IMAGE: cumulative pixel distribution */
    double image_xscale, image_yscale;    /* radians per pixel (float arena at image_off) */
+   mxg_lbucket file_bucket;              /* FILE: of the cumulative distribution */
 }
 mxg_source_cfg;
 

@@ -35,6 +35,9 @@ unsigned int mxg_farena_size (void);
  * most nbmax buckets; its counts are pushed to the float arena.  A table
  * that is not strictly increasing and positive gets ok = 0.  0 or -1. */
 int mxg_build_bucket (const float *xp, unsigned int n, int double_keys, unsigned int nbmax, mxg_bucket *b);
+/* the linear bucket index of a double table (counts in the double arena);
+ * b->ok = 0 when the table is not strictly increasing */
+int mxg_build_lbucket (const double *xp, unsigned int n, unsigned int nbmax, mxg_lbucket *b);
 /* the same for float-arena table (off, n), float keys */
 int mxg_bucket_of_farena (unsigned int off, unsigned int n, unsigned int nbmax, mxg_bucket *b);
 unsigned int mxg_darena_size (void);

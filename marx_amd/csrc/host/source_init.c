@@ -250,6 +250,12 @@ static int file_spectrum (Param_File_Type *pf, mxg_source_cfg *s)
    s->file_cum_off = mxg_darena_push (cum, n);
    s->total_flux = flux;
    s->spectrum = MXG_SPECTRUM_FILE;
+   /* the energy draw searches the cumulative table by bucket */
+   if (-1 == mxg_build_lbucket (cum, n, MXG_BUCKETS_SPECTRUM, &s->file_bucket))
+     {
+        free (cum); free (cols[0]); free (cols[1]);
+        return -1;
+     }
    free (cum); free (cols[0]); free (cols[1]);
    return 0;
 }

@@ -347,6 +347,56 @@ static unsigned long long table_key (float v, int double_keys)
      }
 }
 
+/* the linear bucket key of x >= b->x0 (see mxg_lbucket) */
+static unsigned int lbucket_key (const mxg_lbucket *b, double x)
+{
+   double t = (x - b->x0) * b->scale;
+   return (t < (double) b->nb) ? (unsigned int) t : b->nb - 1;
+}
+
+int mxg_build_lbucket (const double *xp, unsigned int n, unsigned int nbmax, mxg_lbucket *b)
+{
+   unsigned int i, j, nb;
+   double range, *cnt;
+
+   memset (b, 0, sizeof (*b));
+   if ((n < 2) || (n > (1u << 24)))
+     return 0;
+   for (i = 1; i < n; i++)
+     if (!(xp[i] > xp[i - 1]))
+       return 0;
+   range = xp[n - 1] - xp[0];
+   if (!(range > 0.0) || !isfinite (range))
+     return 0;
+   nb = (n < nbmax) ? n : nbmax;
+   b->x0 = xp[0];
+   b->scale = (double) nb / range;
+   b->nb = nb;
+   if (!isfinite (b->scale))
+     {
+        memset (b, 0, sizeof (*b));
+        return 0;
+     }
+   if (NULL == (cnt = (double *) malloc ((nb + 1) * sizeof (double))))
+     {
+        mxg_error ("out of memory");
+        return -1;
+     }
+   for (j = 0, i = 0; j < nb; j++)
+     {
+        while ((i < n) && (lbucket_key (b, xp[i]) < j))
+          i++;
+        cnt[j] = (double) i;
+     }
+   cnt[nb] = (double) n;
+   b->off = mxg_darena_push (cnt, nb + 1);
+   free (cnt);
+   if (b->off == (unsigned int) -1)
+     return -1;
+   b->ok = 1;
+   return 0;
+}
+
 int mxg_build_bucket (const float *xp, unsigned int n, int double_keys, unsigned int nbmax, mxg_bucket *b)
 {
    unsigned long long k0, range;

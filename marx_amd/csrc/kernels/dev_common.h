@@ -301,11 +301,30 @@ __device__ __forceinline__ void interp_f2b (float x, const float *__restrict__ x
    b = (float) ((double) y0 + (double) dy / (x1 - x0) * ((double) x - x0));
 }
 
-__device__ __forceinline__ double interp_d (double x, const double *__restrict__ xp,
-                                            const double *__restrict__ yp, unsigned int n)
+// search_d over a strictly increasing table with the linear bucket index b
+// (mxg_lbucket; counts cnt[0..nb]): the same index.  For x > xp[0] both
+// are the first entry >= x; the key is monotone in x, so that entry lies
+// between the counts of x's bucket and of the next.
+__device__ __forceinline__ unsigned int search_dl (double x, const double *__restrict__ xp, unsigned int n,
+                                                   const mxg_lbucket &b, const double *__restrict__ cnt)
 {
-   if (n == 1) return yp[0];
-   unsigned int n1 = search_d (x, xp, n);
+   if (!(x > xp[0])) return (x == xp[0]) ? 1u : 0u;
+   const double t = (x - b.x0) * b.scale;
+   const unsigned int j = (t < (double) b.nb) ? (unsigned int) t : b.nb - 1;
+   unsigned int lo = (unsigned int) cnt[j], hi = (unsigned int) cnt[j + 1];
+   while (lo < hi)
+     {
+        const unsigned int mid = (lo + hi) >> 1;
+        if (xp[mid] < x) lo = mid + 1;
+        else hi = mid;
+     }
+   return lo;
+}
+
+// interp_d's interpolation once the search gave n1
+__device__ __forceinline__ double interp_d_at (double x, const double *__restrict__ xp,
+                                               const double *__restrict__ yp, unsigned int n, unsigned int n1)
+{
    unsigned int n0 = n1 - 1;
    if ((n1 < n) && (x == xp[n1])) return yp[n1];
    if (n1 == n) { n1--; n0--; }
@@ -313,6 +332,13 @@ __device__ __forceinline__ double interp_d (double x, const double *__restrict__
    double x0 = xp[n0], x1 = xp[n1];
    if (x1 == x0) return yp[n1];
    return yp[n0] + (yp[n1] - yp[n0]) / (x1 - x0) * (x - x0);
+}
+
+__device__ __forceinline__ double interp_d (double x, const double *__restrict__ xp,
+                                            const double *__restrict__ yp, unsigned int n)
+{
+   if (n == 1) return yp[0];
+   return interp_d_at (x, xp, yp, n, search_d (x, xp, n));
 }
 
 // numerically stable roots; returns 1 real, 0 complex, -1 if a == 0
@@ -589,7 +615,11 @@ __device__ __forceinline__ double draw_energy (const mxg_source_cfg &src, const 
         return emin + de * uniform (st);
      }
    double r = uniform (st);
-   return interp_d (r, da + src.file_cum_off, da + src.file_e_off, src.file_n);
+   const double *cum = da + src.file_cum_off;
+   if (src.file_bucket.ok)
+     return interp_d_at (r, cum, da + src.file_e_off, src.file_n,
+                         search_dl (r, cum, src.file_n, src.file_bucket, da + src.file_bucket.off));
+   return interp_d (r, cum, da + src.file_e_off, src.file_n);
 }
 
 } // namespace mxg

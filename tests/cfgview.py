@@ -13,6 +13,10 @@ class Bucket(ctypes.Structure):
     _fields_ = [("base", ctypes.c_ulonglong), ("off", U), ("nb", U), ("shift", U), ("ok", U)]
 
 
+class LBucket(ctypes.Structure):
+    _fields_ = [("x0", D), ("scale", D), ("off", U), ("nb", U), ("ok", U), ("pad", U)]
+
+
 class Wfold(ctypes.Structure):
     _fields_ = [("n_arrays", U), ("arrays_off", U), ("e_alphas_off", U), ("pad", U)]
 
@@ -108,7 +112,7 @@ class SourceCfg(ctypes.Structure):
                 ("nominal_roll", D), ("aspect_blur", D), ("gauss_sigma", D), ("beta_core", D), ("beta_xpon", D),
                 ("line_theta", D), ("line_cos_phi", D), ("line_sin_phi", D), ("line_alpha", D),
                 ("line_normal", D * 3), ("image_n", U), ("image_nx", U), ("image_ny", U),
-                ("image_off", U), ("image_xscale", D), ("image_yscale", D)]
+                ("image_off", U), ("image_xscale", D), ("image_yscale", D), ("file_bucket", LBucket)]
 
 
 class Config(ctypes.Structure):
@@ -117,7 +121,7 @@ class Config(ctypes.Structure):
 
 from marx_amd._lib import DiffCfg, DiffShell, EventCfg, EventColumns, ImageSpec  # noqa: E402
 
-STRUCTS = {"mxg_bucket": Bucket, "mxg_event_cfg": EventCfg, "mxg_event_columns": EventColumns, "mxg_diff_shell": DiffShell, "mxg_diff_cfg": DiffCfg, "mxg_image_spec": ImageSpec,"mxg_wfold": Wfold, "mxg_shell_cfg": ShellCfg, "mxg_strut": Strut, "mxg_hrma_cfg": HrmaCfg,
+STRUCTS = {"mxg_bucket": Bucket, "mxg_lbucket": LBucket, "mxg_event_cfg": EventCfg, "mxg_event_columns": EventColumns, "mxg_diff_shell": DiffShell, "mxg_diff_cfg": DiffCfg, "mxg_image_spec": ImageSpec,"mxg_wfold": Wfold, "mxg_shell_cfg": ShellCfg, "mxg_strut": Strut, "mxg_hrma_cfg": HrmaCfg,
            "mxg_grating": Grating, "mxg_sectors": Sectors, "mxg_grating_cfg": GratingCfg,
            "mxg_chip": Chip, "mxg_qe_table": QeTable, "mxg_contam": Contam, "mxg_rect": Rect,
            "mxg_detector_cfg": DetectorCfg, "mxg_source_cfg": SourceCfg, "mxg_config": Config}

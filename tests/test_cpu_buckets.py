@@ -139,3 +139,54 @@ def test_grating_energy_bucket_index(grating):
                 lb = bucketed(e, E, b, cnt, False)
                 got = min(max(lb, 1), len(E) - 1)
                 assert got == want, (e, got, want)
+
+
+def test_spectrum_linear_bucket_index(tmp_path):
+    """The FILE spectrum's cumulative distribution gets a linear bucket index
+    (mxg_lbucket): the counts are the entries whose key min(nb - 1,
+    int((x - x0) * scale)) is below each bucket, and the bucketed search
+    (dev_common.h's search_dl, restated) returns the plain bisection's index
+    for uniform variates, 0 and 1, every entry and its neighbours."""
+    from marx_amd.spectra import write_power_law
+    spec = write_power_law(str(tmp_path / "pl.dat"), gamma=2.0, emin=0.4, emax=8.0)
+    t = Tracer(seed=1, source=True, GratingType="NONE", DetectorType="NONE", SpectrumType="FILE",
+               SpectrumFile=spec)
+    cv = ConfigView(t)
+    s = cv.cfg.src
+    b = s.file_bucket
+    cum = cv.d(s.file_cum_off, s.file_n)
+    assert s.file_n == 2000 and b.ok == 1 and b.nb == 2000
+    assert cum[0] == 0.0 and cum[-1] == 1.0
+    assert b.x0 == cum[0] and b.scale == b.nb / (cum[-1] - cum[0])
+
+    def key(x):
+        t_ = (np.float64(x) - np.float64(b.x0)) * np.float64(b.scale)
+        return int(t_) if t_ < b.nb else b.nb - 1
+
+    cnt = cv.d(b.off, b.nb + 1)
+    keys = [key(x) for x in cum]
+    assert all(k1 <= k2 for k1, k2 in zip(keys, keys[1:]))
+    assert [int(c) for c in cnt] == [sum(1 for k in keys if k < j) for j in range(b.nb)] + [len(cum)]
+
+    def search_dl(x):
+        if not (x > cum[0]):
+            return 1 if x == cum[0] else 0
+        j = key(x)
+        lo, hi = int(cnt[j]), int(cnt[j + 1])
+        while lo < hi:
+            mid = (lo + hi) >> 1
+            if cum[mid] < x:
+                lo = mid + 1
+            else:
+                hi = mid
+        return lo
+
+    rng = np.random.default_rng(9)
+    u = rng.integers(0, 2**32, 20000, dtype=np.uint64).astype(np.float64) / float(0xFFFFFFFF)
+    xs = list(cum) + list(np.nextafter(cum, np.inf)) + list(np.nextafter(cum, -np.inf)) + [0.0, 1.0] + list(u)
+    for x in xs:
+        assert search_dl(x) == bisect(x, cum), x
+    # a flat spectrum has none
+    t2 = Tracer(seed=1, source=True, GratingType="NONE", DetectorType="NONE", SpectrumType="FLAT",
+                MinEnergy=1.0, MaxEnergy=2.0)
+    assert ConfigView(t2).cfg.src.file_bucket.ok == 0
