@@ -732,7 +732,7 @@ def measure():
     if args.gpus > 1 or world > 1:
         print(f"bench.py runs on one GPU only (asked for {max(args.gpus, world)}): photons are "
               "independent, a multi-GPU run would only shard photon ids and add one final "
-              "reduction, and this build measures the one-GPU share.", file=sys.stderr)
+              "reduction, so the benchmark measures one GPU's share.", file=sys.stderr)
         sys.exit(2)
 
     import torch
