@@ -183,12 +183,17 @@ def test_graph_is_recaptured_after_the_store_grows(gpu, tmp_path):
 @pytest.mark.gpu
 @pytest.mark.parametrize("grating,det,dither", [("NONE", "ACIS-I", "NONE"), ("HETG", "ACIS-S", "INTERNAL"),
                                                 ("LETG", "HRC-S", "INTERNAL"), ("NONE", "HRC-I", "NONE"),
-                                                ("HETG", "NONE", "NONE")])
+                                                ("HETG", "NONE", "NONE"), ("LETG", "ACIS-S", "NONE"),
+                                                ("HETG", "HRC-I", "INTERNAL")])
 def test_persistent_pass_equals_the_staged_pass(gpu, tmp_path, grating, det, dither):
     """The batched pass as one persistent launch (blocks taking tiles of
     photons from a work counter, every stage on a tile back to back) gives
     the staged pass's records, detected list and stage counts, for tiles of
-    256 to 4096 photons at 1 to 8 blocks per CU, eager and graph-launched."""
+    256 to 4096 photons at 1 to 8 blocks per CU, eager and graph-launched.
+    The staged pass hands survivors on through the dense relays (grating to
+    detector; mirror to an LETG grating) while the persistent pass reads
+    the store, so the combinations also check every detector family's and
+    both gratings' relay instances against the store path."""
     t, dirs, counts = _batch(tmp_path, 48, grating=grating, det=det, dither=dither)
     n = int(counts.sum())
     t.set_schedule(blocks_per_cu=8, reflect_waves=4, persistent_tile=0)
