@@ -216,7 +216,7 @@ int marx_mirror_reflect (Marx_Photon_Type *pt, int verbose)
    pt->history |= MARX_MIRROR_SHELL_OK;
    if ((-1 == sync_config ()) || (-1 == make_resident (pt)))
      return -1;
-   if (-1 == mxgd_run_mirror (pt->source_distance, Seed, -1, NULL, 0))
+   if (-1 == mxgd_run_mirror (pt->source_distance, Seed, -1, NULL, 0, 0))
      {
         mxg_error ("%s", mxgd_last_error ());
         return -1;
@@ -640,8 +640,8 @@ int mxg_trace_device (unsigned int num, unsigned int tag_start, double start_tim
    /* timing slots: 0 source+dither (its count: the photons created, read
     * back with the timings, so no stage waits on the host), 1 mirror, 2
     * grating, 3 detector, 4 and 5 the two mirror kernels alone */
-   if ((-1 == mxgd_launch_source (num, tag_start, start_time, exposure_time, Seed, 0, 0))
-       || (-1 == mxgd_run_mirror (Mxg_Cfg.src.distance, Seed, 1, NULL, relay))
+   if ((-1 == mxgd_launch_source (num, tag_start, start_time, exposure_time, Seed, 0, 0, 1))
+       || (-1 == mxgd_run_mirror (Mxg_Cfg.src.distance, Seed, 1, NULL, relay, 1))
        || (with_grating && (-1 == mxgd_run_grating (Seed, 2, NULL, relay, Detector_Type > 0)))
        || ((Detector_Type > 0) && (-1 == mxgd_run_detect (start_time, Seed, 3, NULL, with_grating)))
        || (-1 == mxgd_read_timings (ms, counts, 6))
@@ -748,7 +748,7 @@ static int batch_pass (unsigned int n, double start_time, double exposure, int w
    if (sched.persist_tile && !Mxg_Cfg.det.fef_on && (Mxg_Cfg.src.type == MXG_SOURCE_POINT))
      return mxgd_run_batch_persistent (n, exposure, start_time, Seed, 0);
    if ((-1 == mxgd_run_batch_source (n, exposure, start_time, Seed, 0))
-       || (-1 == mxgd_run_mirror (Mxg_Cfg.src.distance, Seed, 1, NULL, relay))
+       || (-1 == mxgd_run_mirror (Mxg_Cfg.src.distance, Seed, 1, NULL, relay, 0))
        || (with_grating && (-1 == mxgd_run_grating (Seed, 2, NULL, relay, Detector_Type > 0)))
        || ((Detector_Type > 0) && (-1 == mxgd_run_detect (start_time, Seed, 3, NULL, with_grating))))
      return -1;

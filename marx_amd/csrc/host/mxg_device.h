@@ -41,7 +41,11 @@ int mxgd_get_soa (mxg_soa_view *view);
 /* relay_out (mirror, grating): the stage also hands its survivors densely
  * to the next stage, which must run next with relay_in set (with no other
  * stage, upload or list change in between) and reads them from there. */
-int mxgd_run_mirror (double source_distance, unsigned long long seed, int slot, float *ms, int relay_out);
+/* fresh: the active photons come straight from mxgd_launch_source with
+ * entry_follows set (the mirror's entry writes their position, shell and
+ * flags). */
+int mxgd_run_mirror (double source_distance, unsigned long long seed, int slot, float *ms, int relay_out,
+                     int fresh);
 /* Timing slots that mxgd_run_mirror, given a slot, also fills with its two
  * kernels alone. */
 #define MXGD_SLOT_HRMA_ENTER 4
@@ -65,9 +69,10 @@ int mxgd_run_source (unsigned int n, unsigned int tag_start, double start_time,
  * before the next stage): the number created goes to the count of timing
  * slot `slot` (read with mxgd_read_timings), the stages that follow size
  * their launches for n, and mxgd_set_created must then be given that
- * number. */
+ * number.  entry_follows: mxgd_run_mirror with fresh set runs next, and
+ * the source leaves the photons' position, shell and flags to it. */
 int mxgd_launch_source (unsigned int n, unsigned int tag_start, double start_time, double exposure_time,
-                        unsigned long long seed, unsigned int source_id, int slot);
+                        unsigned long long seed, unsigned int source_id, int slot, int entry_follows);
 int mxgd_set_created (unsigned int m);
 
 /* ---- batched point sources ---- */

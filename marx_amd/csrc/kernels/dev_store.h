@@ -237,10 +237,13 @@ struct HrmaScratch
 // four events recorded around k_hrma_enter (0, 1) and k_hrma_reflect (2, 3).
 // relay_out, when non-null: the mirror's survivors also go to it (for the
 // grating or the detector)
+// fresh: the input photons come straight from launch_source with
+// entry_follows (unread flags, all zero; the entry writes every photon's
+// position, shell and flags)
 hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const float *fa,
                         const double *da, Soa s, ListIO io, HrmaScratch hs, uint32_t n_upper,
                         double source_distance, unsigned long long seed, uint32_t source_id,
-                        const Relay *relay_out, hipStream_t st, hipEvent_t *ev);
+                        const Relay *relay_out, bool fresh, hipStream_t st, hipEvent_t *ev);
 // k_hrma_reflect alone, from io.idx_in (photons that hit the paraboloid,
 // stream positions in rng_pos) to io's survivor segments (io.grid blocks;
 // the caller compacts them); waves: the instance launched (2, 3 or 4)
@@ -277,12 +280,14 @@ hipError_t launch_dither_store (const mxg_config *cfg, Soa s, uint32_t n, double
 // tag_start, the exposure cut in *first_over, the active list idx and its
 // length *count (and *count_copy when non-null); ticks (n entries) and
 // tile_tot (ceil(n / 2048) entries) are scratch; point: the configured
-// shape is a point (the instance without the extended shapes).
+// shape is a point (the instance without the extended shapes);
+// entry_follows: launch_hrma with fresh set follows on the created photons
+// (the source leaves their position, shell and flags to it).
 hipError_t launch_source (const mxg_config *cfg, const float *fa, const double *da, Soa s, uint32_t n,
                           uint32_t tag_start, double exposure_time, double start_time,
                           unsigned long long seed, uint32_t source_id, uint64_t *ticks, uint64_t *tile_tot,
                           uint32_t *first_over, uint32_t *count, uint32_t *count_copy, uint32_t *idx, bool point,
-                          hipStream_t st);
+                          bool entry_follows, hipStream_t st);
 
 // Batched point sources (see source.hip): n photons over nsrc sources with
 // prefix offsets off[nsrc + 1] and unit directions dir[3 nsrc]; point: the

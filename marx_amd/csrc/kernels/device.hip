@@ -596,7 +596,8 @@ int stage_relays (int relay_in, int relay_out, const Relay **in, const Relay **o
    return 0;
 }
 
-int mxgd_run_mirror (double source_distance, unsigned long long seed, int slot, float *ms, int relay_out)
+int mxgd_run_mirror (double source_distance, unsigned long long seed, int slot, float *ms, int relay_out,
+                     int fresh)
 {
    if (!d_cfg) return fail ("configuration not on the device");
    const Relay *rin, *rout;
@@ -619,7 +620,7 @@ int mxgd_run_mirror (double source_distance, unsigned long long seed, int slot, 
      }
    else
      HIPCHECK (launch_hrma (d_cfg, &g_hcfg, d_fa, d_da, g_soa, io, hs, g_upper, source_distance, seed, 0, rout,
-                            g_stream, (timed && !g_capturing) ? kev : nullptr));
+                            fresh != 0, g_stream, (timed && !g_capturing) ? kev : nullptr));
    g_relay_have = out_k;
    if (timed)
      {
@@ -730,7 +731,7 @@ int mxgd_run_source (unsigned int n, unsigned int tag_start, double start_time,
      }
    HIPCHECK (launch_source (d_cfg, d_fa, d_da, g_soa, n, tag_start, exposure_time, start_time, seed,
                             source_id, d_ticks, d_tile_tot, d_counts + 2, d_counts, nullptr, d_idx[0],
-                            g_hcfg.src.type == MXG_SOURCE_POINT, g_stream));
+                            g_hcfg.src.type == MXG_SOURCE_POINT, false, g_stream));
    if (slot >= 0 && slot < kSlots)
      HIPCHECK (hipEventRecord (g_tev[slot][1], g_stream));
    if (ms)
@@ -756,7 +757,7 @@ int mxgd_run_source (unsigned int n, unsigned int tag_start, double start_time,
 }
 
 int mxgd_launch_source (unsigned int n, unsigned int tag_start, double start_time, double exposure_time,
-                        unsigned long long seed, unsigned int source_id, int slot)
+                        unsigned long long seed, unsigned int source_id, int slot, int entry_follows)
 {
    g_relay_have = -1;
    if (!d_cfg) return fail ("configuration not on the device");
@@ -770,7 +771,7 @@ int mxgd_launch_source (unsigned int n, unsigned int tag_start, double start_tim
      }
    HIPCHECK (launch_source (d_cfg, d_fa, d_da, g_soa, n, tag_start, exposure_time, start_time, seed,
                             source_id, d_ticks, d_tile_tot, d_counts + 2, d_counts, slot_count (slot), d_idx[0],
-                            g_hcfg.src.type == MXG_SOURCE_POINT, g_stream));
+                            g_hcfg.src.type == MXG_SOURCE_POINT, entry_follows != 0, g_stream));
    if ((slot >= 0) && (slot < kSlots) && !g_capturing)
      HIPCHECK (hipEventRecord (g_tev[slot][1], g_stream));
    // upper bounds until the created number is read back
@@ -1365,7 +1366,8 @@ int mxgd_trace_noalloc (const mxg_soa_view *v, unsigned int n, unsigned int tag_
      hipLaunchKernelGGL (k_set_u32, dim3 (1), dim3 (1), 0, g_stream, counts + k, 0u);
    HIPCHECK (hipGetLastError ());
    HIPCHECK (launch_source (c_cfg, c_fa, c_da, s, n, tag_start, exposure_time, start_time, seed, 0, ticks, tiles,
-                            counts + 2, counts + 0, nullptr, idx[0], cfg->src.type == MXG_SOURCE_POINT, g_stream));
+                            counts + 2, counts + 0, nullptr, idx[0], cfg->src.type == MXG_SOURCE_POINT, false,
+                            g_stream));
    hipLaunchKernelGGL (k_copy_u32, dim3 (1), dim3 (1), 0, g_stream, counts + 6, counts + 0);
    HIPCHECK (hipGetLastError ());
 
@@ -1394,7 +1396,7 @@ int mxgd_trace_noalloc (const mxg_soa_view *v, unsigned int n, unsigned int tag_
    HIPCHECK (clear_out ());
    HrmaScratch hs = {mid, counts + 4, rng, io (MXGD_K_REFLECT).grid, (int) g_sched.reflect_waves};
    HIPCHECK (launch_hrma (c_cfg, cfg, c_fa, c_da, s, io (MXGD_K_ENTER), hs, n, cfg->src.distance, seed, 0,
-                          nullptr, g_stream, nullptr));
+                          nullptr, false, g_stream, nullptr));
    cur = 1 - cur;
    if ((cfg->grating.type == MARX_GRATING_HETG) || (cfg->grating.type == MARX_GRATING_LETG))
      {

@@ -20,6 +20,8 @@
 
 namespace mxg {
 
+// FRESH: the photons come straight from the source (hrma_enter_one<true>)
+template <bool FRESH>
 #if MXG_ENTER_WAVES > 0
 __global__ __launch_bounds__ (kBlock, MXG_ENTER_WAVES)
 #else
@@ -45,7 +47,7 @@ void k_hrma_enter (const mxg_config *__restrict__ cfg, Soa s, ListIO io,
         if (i < range.hi)
           {
              id = io.idx_in[i];
-             keep = hrma_enter_one (h, s, id, rng_pos, source_distance, seed, source_id);
+             keep = hrma_enter_one<FRESH> (h, s, id, rng_pos, source_distance, seed, source_id);
           }
         survivors.add (keep, id);
      }
@@ -117,7 +119,7 @@ hipError_t launch_hrma_reflect (const mxg_config *cfg, const mxg_config *hcfg, c
 hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const float *fa,
                         const double *da, Soa s, ListIO io, HrmaScratch hs, uint32_t n_upper,
                         double source_distance, unsigned long long seed, uint32_t source_id,
-                        const Relay *relay_out, hipStream_t st, hipEvent_t *ev)
+                        const Relay *relay_out, bool fresh, hipStream_t st, hipEvent_t *ev)
 {
    if (n_upper == 0) return hipSuccess;
    const HrmaLds lay = hrma_lds_layout (hcfg->hrma);
@@ -135,8 +137,12 @@ hipError_t launch_hrma (const mxg_config *cfg, const mxg_config *hcfg, const flo
    hipError_t e = hipSuccess;
    if (ev) e = hipEventRecord (ev[0], st);
    if (e != hipSuccess) return e;
-   hipLaunchKernelGGL (k_hrma_enter, dim3 (a.grid), dim3 (kBlock), 0, st, cfg, s, a, hs.rng_pos,
-                       source_distance, seed, source_id);
+   if (fresh)
+     hipLaunchKernelGGL (k_hrma_enter<true>, dim3 (a.grid), dim3 (kBlock), 0, st, cfg, s, a, hs.rng_pos,
+                         source_distance, seed, source_id);
+   else
+     hipLaunchKernelGGL (k_hrma_enter<false>, dim3 (a.grid), dim3 (kBlock), 0, st, cfg, s, a, hs.rng_pos,
+                         source_distance, seed, source_id);
    e = hipGetLastError ();
    if ((e == hipSuccess) && ev) e = hipEventRecord (ev[1], st);
    if (e == hipSuccess) e = launch_compact (a, st);

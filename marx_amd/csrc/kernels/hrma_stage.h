@@ -299,20 +299,31 @@ __device__ __forceinline__ bool hrma_enter_core (const mxg_hrma_cfg &h, uint32_t
 
 // k_hrma_enter's work for photon id of the store (hrma_enter_core).  True
 // when the photon hit the paraboloid (its stream position is saved in
-// rng_pos[id] for hrma_reflect_one).
+// rng_pos[id] for hrma_reflect_one).  FRESH: the photon is as the source
+// made it without its position, shell and flags (flags zero, not read);
+// they are written here, the position and shell zero when vignetting
+// blocks it.
+template <bool FRESH = false>
 __device__ __forceinline__ bool hrma_enter_one (const mxg_hrma_cfg &h, Soa s, uint32_t id,
                                                 uint32_t *__restrict__ rng_pos, double source_distance,
                                                 unsigned long long seed, uint32_t source_id)
 {
    bool keep = false;
-   uint32_t flags = s.flags[id];
+   uint32_t flags = FRESH ? 0u : s.flags[id];
    if ((flags & BAD_PHOTON_MASK) == 0)
      {
         vec3 x, p = v3 (s.p0[id], s.p1[id], s.p2[id]);
         int si;
         uint32_t rng = 0;
         if (!hrma_enter_core (h, s.tag[id], source_distance, seed, source_id, x, p, si, flags, keep, rng))
-          s.flags[id] = flags | PHOTON_MIRROR_VBLOCKED;
+          {
+             s.flags[id] = flags | PHOTON_MIRROR_VBLOCKED;
+             if (FRESH)
+               {
+                  s.x0[id] = 0.0; s.x1[id] = 0.0; s.x2[id] = 0.0;
+                  s.shell[id] = 0;
+               }
+          }
         else
           {
              if (keep) rng_pos[id] = rng;

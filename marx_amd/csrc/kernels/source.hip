@@ -204,8 +204,11 @@ __device__ __forceinline__ NewPhoton source_photon (const mxg_source_cfg &src, c
 }
 
 // Photon i, one per thread (source_photon): every column written once, and
-// idx[i] = i for an active photon.
-template <bool POINT>
+// idx[i] = i for an active photon.  ENTRY: the mirror's entry follows with
+// the photons as fresh and writes their position, shell and flags, so
+// those are left out (a photon past the exposure cut keeps stale ones: it
+// is not among the photons created).
+template <bool POINT, bool ENTRY>
 #if MXG_SOURCE_WAVES > 0
 __global__ __launch_bounds__ (kBlock, MXG_SOURCE_WAVES)
 #else
@@ -222,7 +225,7 @@ void k_source (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
    const uint32_t tag = tag_start + i;
    const NewPhoton ph = source_photon<POINT> (cfg->src, fa, da, i, tag, exposure_time, start_time, seed,
                                               source_id, ticks, tile_off, first_over);
-   init_photon (s, i, ph.energy, ph.p, ph.t, tag, true, true, !ph.dithered);
+   init_photon (s, i, ph.energy, ph.p, ph.t, tag, true, true, !ph.dithered, !ENTRY);
    if (ph.dithered) store_dith (s, i, ph.d);
    if (ph.active) idx[i] = i;
 }
@@ -265,7 +268,7 @@ hipError_t launch_source (const mxg_config *cfg, const float *fa, const double *
                           uint32_t tag_start, double exposure_time, double start_time,
                           unsigned long long seed, uint32_t source_id, uint64_t *ticks, uint64_t *tile_tot,
                           uint32_t *first_over, uint32_t *count, uint32_t *count_copy, uint32_t *idx, bool point,
-                          hipStream_t st)
+                          bool entry_follows, hipStream_t st)
 {
    if (n == 0)
      {
@@ -283,12 +286,12 @@ hipError_t launch_source (const mxg_config *cfg, const float *fa, const double *
      hipLaunchKernelGGL (k_source_scan<false>, dim3 (ntiles), dim3 (kBlock), 0, st, cfg, fa, da, n, tag_start,
                          seed, source_id, ticks, tile_tot);
    hipLaunchKernelGGL (k_scan_tiles, dim3 (1), dim3 (kScanThreads), 0, st, tile_tot, ntiles);
-   if (point)
-     hipLaunchKernelGGL (k_source<true>, dim3 (blocks), dim3 (kBlock), 0, st, cfg, fa, da, s, n, tag_start,
-                         exposure_time, start_time, seed, source_id, ticks, tile_tot, first_over, idx);
-   else
-     hipLaunchKernelGGL (k_source<false>, dim3 (blocks), dim3 (kBlock), 0, st, cfg, fa, da, s, n, tag_start,
-                         exposure_time, start_time, seed, source_id, ticks, tile_tot, first_over, idx);
+   void (*kern) (const mxg_config *, const float *, const double *, Soa, uint32_t, uint32_t, double, double,
+                 unsigned long long, uint32_t, const uint64_t *, const uint64_t *, uint32_t *, uint32_t *)
+     = point ? (entry_follows ? k_source<true, true> : k_source<true, false>)
+             : (entry_follows ? k_source<false, true> : k_source<false, false>);
+   hipLaunchKernelGGL (kern, dim3 (blocks), dim3 (kBlock), 0, st, cfg, fa, da, s, n, tag_start, exposure_time,
+                       start_time, seed, source_id, ticks, tile_tot, first_over, idx);
    hipLaunchKernelGGL (k_source_count, dim3 (1), dim3 (1), 0, st, n, first_over, count, count_copy);
    return hipGetLastError ();
 }

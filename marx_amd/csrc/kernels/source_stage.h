@@ -9,26 +9,30 @@ namespace mxg {
 
 // every column of photon i set as a fresh photon of the given energy,
 // direction, time and tag, with no dither offsets; a column that a later
-// step of the same kernel writes is left out (with_t, with_p, with_dith
-// false)
+// step writes is left out (with_t, with_p, with_dith false; with_entry
+// false: the position, shell and flags, which the mirror's entry writes
+// for every photon it is given as fresh)
 __device__ __forceinline__ void init_photon (Soa s, uint32_t i, double energy, vec3 p, double t,
                                              uint32_t tag, bool with_t = true, bool with_p = true,
-                                             bool with_dith = true)
+                                             bool with_dith = true, bool with_entry = true)
 {
    s.energy[i] = energy;
    if (with_p)
      {
         s.p0[i] = p.x; s.p1[i] = p.y; s.p2[i] = p.z;
      }
-   s.x0[i] = 0.0; s.x1[i] = 0.0; s.x2[i] = 0.0;
+   if (with_entry)
+     {
+        s.x0[i] = 0.0; s.x1[i] = 0.0; s.x2[i] = 0.0;
+        s.flags[i] = 0u;
+        s.shell[i] = 0;
+     }
    if (with_t) s.t[i] = t;
-   s.flags[i] = 0u;
    s.ypix[i] = 0.0f; s.zpix[i] = 0.0f; s.upix[i] = 0.0f; s.vpix[i] = 0.0f;
    if (with_dith)
      for (int q = 0; q < 6; q++) s.dith[q][i] = 0.0f;
    s.pi[i] = 0.0f;
    s.pha[i] = 0;
-   s.shell[i] = 0;
    s.ccd[i] = 0; s.region[i] = 0; s.order[i] = 0;
    for (int q = 0; q < 4; q++) s.sup[q][i] = 0;
    s.tag[i] = tag;
