@@ -747,8 +747,8 @@ static int batch_pass (unsigned int n, double start_time, double exposure, int w
    /* the persistent pass traces point sources without the FEF step */
    if (sched.persist_tile && !Mxg_Cfg.det.fef_on && (Mxg_Cfg.src.type == MXG_SOURCE_POINT))
      return mxgd_run_batch_persistent (n, exposure, start_time, Seed, 0);
-   if ((-1 == mxgd_run_batch_source (n, exposure, start_time, Seed, 0))
-       || (-1 == mxgd_run_mirror (Mxg_Cfg.src.distance, Seed, 1, NULL, relay, 0))
+   if ((-1 == mxgd_run_batch_source (n, exposure, start_time, Seed, 0, 1))
+       || (-1 == mxgd_run_mirror (Mxg_Cfg.src.distance, Seed, 1, NULL, relay, 1))
        || (with_grating && (-1 == mxgd_run_grating (Seed, 2, NULL, relay, Detector_Type > 0)))
        || ((Detector_Type > 0) && (-1 == mxgd_run_detect (start_time, Seed, 3, NULL, with_grating))))
      return -1;

@@ -88,9 +88,10 @@ int mxgd_batch_prepare (unsigned int n);
  * device memory, so a captured launch sequence picks it up). */
 int mxgd_batch_set_tag_start (unsigned int tag_start);
 /* Launches the batch source kernel over the n photons of the uploaded
- * sources: arrival times uniform over [0, exposure). */
+ * sources: arrival times uniform over [0, exposure).  entry_follows: as
+ * for mxgd_launch_source (mxgd_run_mirror with fresh set runs next). */
 int mxgd_run_batch_source (unsigned int n, double exposure, double start_time,
-                           unsigned long long seed, int slot);
+                           unsigned long long seed, int slot, int entry_follows);
 /* The whole batched pass (source, mirror, grating, detector) as one
  * persistent launch under the schedule's tile (not with a FEF file): the
  * same photons and final list as the staged pass; timing slot `slot` holds

@@ -291,12 +291,14 @@ hipError_t launch_source (const mxg_config *cfg, const float *fa, const double *
 
 // Batched point sources (see source.hip): n photons over nsrc sources with
 // prefix offsets off[nsrc + 1] and unit directions dir[3 nsrc]; point: the
-// configured shape is a point (the instance without the extended shapes).
+// configured shape is a point (the instance without the extended shapes);
+// entry_follows: launch_hrma with fresh set follows (as for launch_source).
 hipError_t launch_batch_source (const mxg_config *cfg, const double *da, Soa s, uint32_t n,
                                 const uint32_t *off, uint32_t nsrc, const double *dir,
                                 const uint32_t *tag_start, double exposure, double start_time,
                                 unsigned long long seed, uint32_t *idx, uint32_t *count,
-                                uint32_t grid, bool point, hipStream_t st);
+                                uint32_t grid, bool point, bool entry_follows,
+                                hipStream_t st);
 
 // The batched pass as one persistent launch (persist.hip): blocks take
 // tiles of `tile` photons (a multiple of kBlock, at most kMaxPersistTile)

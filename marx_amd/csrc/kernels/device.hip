@@ -1011,7 +1011,7 @@ int mxgd_batch_set_tag_start (unsigned int tag_start)
 }
 
 int mxgd_run_batch_source (unsigned int n, double exposure, double start_time,
-                           unsigned long long seed, int slot)
+                           unsigned long long seed, int slot, int entry_follows)
 {
    g_relay_have = -1;
    if (!d_cfg) return fail ("configuration not on the device");
@@ -1025,7 +1025,7 @@ int mxgd_run_batch_source (unsigned int n, double exposure, double start_time,
    uint32_t grid = stage_grid (n, MXGD_K_SOURCE);
    HIPCHECK (launch_batch_source (d_cfg, d_da, g_soa, n, d_batch_off, g_batch_nsrc, d_batch_dir,
                                   d_batch_tag, exposure, start_time, seed, d_idx[0], d_counts, grid,
-                                  g_hcfg.src.type == MXG_SOURCE_POINT, g_stream));
+                                  g_hcfg.src.type == MXG_SOURCE_POINT, entry_follows != 0, g_stream));
    if (timed)
      HIPCHECK (hipEventRecord (g_tev[slot][1], g_stream));
    g_cur = 0;

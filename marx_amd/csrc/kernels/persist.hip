@@ -134,9 +134,9 @@ void k_batch_persistent (PersistArgs a)
              bool keep = false;
              if (i < hi)
                {
-                  batch_source_one<true> (src, a.da, a.s, i, a.off, a.nsrc, a.dir, t0, a.exposure,
-                                          a.start_time, a.seed);
-                  keep = hrma_enter_one (h, a.s, i, a.rng_pos, distance, a.seed, 0u);
+                  batch_source_one<true, true> (src, a.da, a.s, i, a.off, a.nsrc, a.dir, t0, a.exposure,
+                                                a.start_time, a.seed);
+                  keep = hrma_enter_one<true> (h, a.s, i, a.rng_pos, distance, a.seed, 0u);
                }
              c.add (keep, i, list);
           }

@@ -297,7 +297,7 @@ hipError_t launch_source (const mxg_config *cfg, const float *fa, const double *
 }
 
 // POINT: the instance for point sources (see batch_source_one)
-template <bool POINT>
+template <bool POINT, bool ENTRY>
 #if MXG_BATCH_SOURCE_WAVES > 0
 __global__ __launch_bounds__ (kBlock, MXG_BATCH_SOURCE_WAVES)
 #else
@@ -314,7 +314,7 @@ void k_batch_source (const mxg_config *__restrict__ cfg, const double *__restric
    const uint32_t stride = gridDim.x * blockDim.x;
    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
      {
-        batch_source_one<POINT> (src, da, s, i, off, nsrc, dir, t0, exposure, start_time, seed);
+        batch_source_one<POINT, ENTRY> (src, da, s, i, off, nsrc, dir, t0, exposure, start_time, seed);
         idx[i] = i;
      }
    if ((blockIdx.x == 0) && (threadIdx.x == 0)) *count = n;
@@ -324,15 +324,15 @@ hipError_t launch_batch_source (const mxg_config *cfg, const double *da, Soa s, 
                                 const uint32_t *off, uint32_t nsrc, const double *dir,
                                 const uint32_t *tag_start, double exposure, double start_time,
                                 unsigned long long seed, uint32_t *idx, uint32_t *count,
-                                uint32_t grid, bool point, hipStream_t st)
+                                uint32_t grid, bool point, bool entry_follows, hipStream_t st)
 {
    if (grid == 0) grid = 1;
-   if (point)
-     hipLaunchKernelGGL (k_batch_source<true>, dim3 (grid), dim3 (kBlock), 0, st, cfg, da, s, n, off, nsrc,
-                         dir, tag_start, exposure, start_time, seed, idx, count);
-   else
-     hipLaunchKernelGGL (k_batch_source<false>, dim3 (grid), dim3 (kBlock), 0, st, cfg, da, s, n, off, nsrc,
-                         dir, tag_start, exposure, start_time, seed, idx, count);
+   void (*kern) (const mxg_config *, const double *, Soa, uint32_t, const uint32_t *, uint32_t, const double *,
+                 const uint32_t *, double, double, unsigned long long, uint32_t *, uint32_t *)
+     = point ? (entry_follows ? k_batch_source<true, true> : k_batch_source<true, false>)
+             : (entry_follows ? k_batch_source<false, true> : k_batch_source<false, false>);
+   hipLaunchKernelGGL (kern, dim3 (grid), dim3 (kBlock), 0, st, cfg, da, s, n, off, nsrc, dir, tag_start,
+                       exposure, start_time, seed, idx, count);
    return hipGetLastError ();
 }
 

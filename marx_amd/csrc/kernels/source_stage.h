@@ -202,8 +202,10 @@ __device__ __forceinline__ uint32_t source_of (const uint32_t *__restrict__ off,
 
 // Photon i of the batch (tag t0 + i): k_batch_source's work for one photon.
 // POINT: compiled for point sources only (the extended shapes' code, and
-// its registers, left out).
-template <bool POINT>
+// its registers, left out); ENTRY: the mirror's entry follows with the
+// photon as fresh (see init_photon), so its position, shell and flags are
+// left to it.
+template <bool POINT, bool ENTRY = false>
 __device__ __forceinline__ void batch_source_one (const mxg_source_cfg &src, const double *__restrict__ da, Soa s,
                                                   uint32_t i, const uint32_t *__restrict__ off, uint32_t nsrc,
                                                   const double *__restrict__ dir, uint32_t t0, double exposure,
@@ -231,11 +233,11 @@ __device__ __forceinline__ void batch_source_one (const mxg_source_cfg &src, con
      {
         float d[3];
         p = internal_dither (src, p, tag, t, start_time, seed, 0u, d);
-        init_photon (s, i, energy, p, t, tag, true, true, false);
+        init_photon (s, i, energy, p, t, tag, true, true, false, !ENTRY);
         store_dith (s, i, d);
      }
    else
-     init_photon (s, i, energy, p, t, tag);
+     init_photon (s, i, energy, p, t, tag, true, true, true, !ENTRY);
 }
 
 } // namespace mxg
