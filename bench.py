@@ -60,9 +60,12 @@ TRANSCENDENTALS_PER_PHOTON = {
 # arrays columns it must read and write, plus the active-list index in and
 # the survivor index out.
 BYTES_PER_PHOTON = {
-    "source": 8 + 24 + 8 + 4 + 4 + 24 + 8,     # energy, p, t, flags, tag, dither, scan word
-    # in: index, flags, tag, p; out: x, p, shell, flags, stream position, index
-    "k_hrma_enter": (4 + 4 + 4 + 24) + (24 + 24 + 1 + 4 + 4 + 4),
+    # energy, p, t, tag, dither, scan word (the position, shell and flags
+    # are the mirror entry's to write)
+    "source": 8 + 24 + 8 + 4 + 24 + 8,
+    # in: index, tag, p (fresh photons: no flags read); out: x, p, shell,
+    # flags, stream position, index
+    "k_hrma_enter": (4 + 4 + 24) + (24 + 24 + 1 + 4 + 4 + 4),
     # in: index, flags, shell, tag, stream position, x, p, energy; out: x, p, flags, index
     "k_hrma_reflect": (4 + 4 + 1 + 4 + 4 + 24 + 24 + 8) + (24 + 24 + 4 + 4),
     "grating": (4 + 4 + 4 + 1 + 8 + 24 + 24 + 1) + (24 + 24 + 4 + 1 + 4),
