@@ -605,7 +605,8 @@ __device__ __forceinline__ bool in_strut_shadow (const mxg_hrma_cfg &h, vec3 x, 
    return false;
 }
 
-// energy of the FLAT or FILE spectrum (spectrum.c:138-147, 184-189)
+// energy of the FLAT or FILE spectrum (spectrum.c:138-147, 184-189); one
+// uniform either way
 __device__ __forceinline__ double draw_energy (const mxg_source_cfg &src, const double *__restrict__ da,
                                                Stream &st)
 {

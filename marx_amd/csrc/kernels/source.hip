@@ -35,8 +35,9 @@ __device__ __forceinline__ uint64_t gap_ticks (double e)
 // 2^-32 units of the mean gap (else its uniform arrival time t).  Both
 // passes over the photons draw through this, so they see the same values.
 // POINT: compiled for a point source only (its direction is the source's,
-// with no draw).
-template <bool POINT>
+// with no draw); GAP: only the gap is wanted (the energy draw, one uniform
+// for either spectrum, is skipped over without its table search).
+template <bool POINT, bool GAP = false>
 __device__ __forceinline__ void source_draws (const mxg_source_cfg &src, const float *__restrict__ fa,
                                               const double *__restrict__ da, uint32_t tag,
                                               unsigned long long seed, uint32_t source_id, bool poisson,
@@ -44,7 +45,13 @@ __device__ __forceinline__ void source_draws (const mxg_source_cfg &src, const f
 {
    Stream st;
    stream_init (st, seed, tag, MXG_STAGE_SOURCE, source_id);
-   energy = draw_energy (src, da, st);
+   if (GAP)
+     {
+        (void) uniform (st);
+        energy = 0.0;
+     }
+   else
+     energy = draw_energy (src, da, st);
    if (POINT)
      p = ld3 (src.p);
    else
@@ -85,7 +92,7 @@ void k_source_scan (const mxg_config *__restrict__ cfg, const float *__restrict_
           {
              double energy, t;
              vec3 p;
-             source_draws<POINT> (src, fa, da, tag_start + i, seed, source_id, true, energy, p, k, t);
+             source_draws<POINT, true> (src, fa, da, tag_start + i, seed, source_id, true, energy, p, k, t);
           }
         sk[li] = k;
      }
