@@ -48,6 +48,7 @@ uint32_t *d_rng_pos = nullptr;     // the mirror's per-photon stream position
 uint32_t *d_counts = nullptr;      // [0],[1] list lengths, [2] source cut, [4] middle list,
                                    // [8..15] per-stage survivor counts, [32] the persistent
                                    // pass's tile counter, [36..39] its stage counts
+uint32_t *h_counts = nullptr;      // pinned host copy of d_counts[8..15], read once per call
 uint32_t *d_tile_cnt = nullptr;    // the persistent pass's per-tile counts and offsets
 size_t g_tile_cap = 0;             // entries of each
 int g_cur = 0;                     // list holding the active photons
@@ -447,6 +448,7 @@ int mxgd_init (void)
    HIPCHECK (hipMalloc (&d_counts, 64 * sizeof (uint32_t)));
    HIPCHECK (hipMalloc (&d_seg_count, kMaxSegments * sizeof (uint32_t)));
    HIPCHECK (hipMemset (d_counts, 0, 64 * sizeof (uint32_t)));
+   HIPCHECK (hipHostMalloc ((void **) &h_counts, kSlots * sizeof (uint32_t), hipHostMallocDefault));
    if (scratch_setup (dev)) return -1;
    g_inited = 1;
    return 0;
@@ -693,13 +695,15 @@ int mxgd_active_list (unsigned int *idx, unsigned int n)
 
 int mxgd_read_timings (float *ms, unsigned int *counts, int nslots)
 {
+   // One wait for the stages and the counts: the copy into pinned memory is
+   // queued behind the last stage, so no second round trip follows the sync.
+   HIPCHECK (hipMemcpyAsync (h_counts, d_counts + 8, kSlots * sizeof (uint32_t),
+                             hipMemcpyDeviceToHost, g_stream));
    HIPCHECK (hipStreamSynchronize (g_stream));
-   uint32_t c[kSlots];
-   HIPCHECK (hipMemcpy (c, d_counts + 8, sizeof (c), hipMemcpyDeviceToHost));
    for (int k = 0; k < nslots && k < kSlots; k++)
      {
         ms[k] = 0.0f;
-        if (counts) counts[k] = c[k];
+        if (counts) counts[k] = h_counts[k];
         if (g_slot_used[k])
           HIPCHECK (hipEventElapsedTime (ms + k, g_tev[k][0], g_tev[k][1]));
         g_slot_used[k] = 0;
