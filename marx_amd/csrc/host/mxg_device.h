@@ -40,7 +40,9 @@ int mxgd_get_soa (mxg_soa_view *view);
  * returns the stage time at once. */
 /* relay_out (mirror, grating): the stage also hands its survivors densely
  * to the next stage, which must run next with relay_in set (with no other
- * stage, upload or list change in between) and reads them from there. */
+ * stage, upload or list change in between) and reads them from there.  The
+ * stage then leaves its survivors' x, p and flags in the store to the next
+ * one, which stores them for every photon it reads. */
 /* fresh: the active photons come straight from mxgd_launch_source with
  * entry_follows set (the mirror's entry writes their position, shell and
  * flags). */

@@ -235,8 +235,9 @@ struct HrmaScratch
 // cfg: device copy; hcfg: host copy of the same configuration (launch
 // geometry and LDS layout are computed from it).  ev, when non-null, holds
 // four events recorded around k_hrma_enter (0, 1) and k_hrma_reflect (2, 3).
-// relay_out, when non-null: the mirror's survivors also go to it (for the
-// grating or the detector)
+// relay_out, when non-null: the mirror's survivors go to it for the grating
+// or the detector, which store their x, p and flags (the reflection kernel
+// stores those of the photons it drops only)
 // fresh: the input photons come straight from launch_source with
 // entry_follows (unread flags, all zero; the entry writes every photon's
 // position, shell and flags)
@@ -253,8 +254,9 @@ hipError_t launch_hrma_reflect (const mxg_config *cfg, const mxg_config *hcfg, c
                                 hipStream_t st);
 // letg: the instance compiled for the LETG pass; relay_in, when non-null:
 // the input list is the mirror's with this relay, whose rows the kernel
-// reads instead of the store's; relay_out, when non-null: the survivors
-// also go to it (for launch_detect)
+// reads instead of the store's; relay_out, when non-null: the survivors go
+// to it for launch_detect, which stores their x, p and flags (the kernel
+// stores those of the photons it drops only)
 hipError_t launch_grating (const mxg_config *cfg, const float *fa, const double *da, Soa s,
                            ListIO io, uint32_t n_upper, unsigned long long seed,
                            uint32_t source_id, bool letg, const Relay *relay_in, const Relay *relay_out,

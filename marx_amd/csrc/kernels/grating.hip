@@ -30,8 +30,8 @@ void k_grating (const mxg_config *__restrict__ cfg, const float *__restrict__ fa
         if (i < range.hi)
           {
              id = io.idx_in[i];
-             keep = grating_one<LETG, RIN> (gc, fa, da, s, id, seed, source_id, o, &rin,
-                                            RIN ? rin.pos[i] : 0u);
+             keep = grating_one<LETG, RIN, ROUT> (gc, fa, da, s, id, seed, source_id, o, &rin,
+                                                  RIN ? rin.pos[i] : 0u);
           }
         const uint32_t q = survivors.add (keep, id);
         if (ROUT && keep) relay_put (rout, q, o);

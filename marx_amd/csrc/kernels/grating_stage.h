@@ -226,8 +226,11 @@ __device__ __forceinline__ int diffract_from (const mxg_grating &g, const mxg_se
 // draw and diffraction, and for LETG the support gratings.  True when the
 // photon leaves the grating stage alive, its columns then also in o.
 // RELAY: its incoming x, p, energy, tag, flags and shell are read from row
-// `slot` of *relay instead of the store.
-template <bool LETG, bool RELAY = false>
+// `slot` of *relay instead of the store, where the mirror left x, p and
+// flags for this stage to write (a vignetted photon's as they came).  HANDOFF: the photons it keeps go
+// on to the detector through o, and the detector stores their x, p and
+// flags; only the photons it drops have theirs stored here.
+template <bool LETG, bool RELAY = false, bool HANDOFF = false>
 __device__ __forceinline__ bool grating_one (const mxg_grating_cfg &gc, const float *__restrict__ fa,
                                              const double *__restrict__ da, Soa s, uint32_t id,
                                              unsigned long long seed, uint32_t source_id, PhotonOut &o,
@@ -244,7 +247,14 @@ __device__ __forceinline__ bool grating_one (const mxg_grating_cfg &gc, const fl
         const int shell = RELAY ? (int) relay->shell[slot] : (int) s.shell[id];
         const mxg_grating &g = gc.g[shell];
         if (uniform (st) > g.vig)
-          s.flags[id] = flags | PHOTON_MIRROR_VBLOCKED;
+          {
+             s.flags[id] = flags | PHOTON_MIRROR_VBLOCKED;
+             if (RELAY)
+               {
+                  s.x0[id] = relay->x0[slot]; s.x1[id] = relay->x1[slot]; s.x2[id] = relay->x2[slot];
+                  s.p0[id] = relay->p0[slot]; s.p1[id] = relay->p1[slot]; s.p2[id] = relay->p2[slot];
+               }
+          }
         else
           {
              vec3 x = RELAY ? v3 (relay->x0[slot], relay->x1[slot], relay->x2[slot])
@@ -284,10 +294,13 @@ __device__ __forceinline__ bool grating_one (const mxg_grating_cfg &gc, const fl
 
              x = rotate_x (x, g.disp_cos, g.disp_sin);
              p = rotate_x (p, g.disp_cos, g.disp_sin);
-             s.x0[id] = x.x; s.x1[id] = x.y; s.x2[id] = x.z;
-             s.p0[id] = p.x; s.p1[id] = p.y; s.p2[id] = p.z;
-             s.flags[id] = flags;
              keep = (flags & BAD_PHOTON_MASK) == 0;
+             if (!(HANDOFF && keep))
+               {
+                  s.x0[id] = x.x; s.x1[id] = x.y; s.x2[id] = x.z;
+                  s.p0[id] = p.x; s.p1[id] = p.y; s.p2[id] = p.z;
+                  s.flags[id] = flags;
+               }
              o.x0 = x.x; o.x1 = x.y; o.x2 = x.z;
              o.p0 = p.x; o.p1 = p.y; o.p2 = p.z;
              o.energy = energy;

@@ -58,7 +58,8 @@ void k_hrma_enter (const mxg_config *__restrict__ cfg, Soa s, ListIO io,
 // binary-search targets of every photon) in LDS once, then loops over its
 // range of the middle list.
 // WAVES: minimum waves per SIMD the register allocation must allow; RELAY:
-// the survivors' columns also go to relay at their segment slots.
+// the survivors' columns go to relay at their segment slots, for the
+// grating, which stores their x, p and flags.
 template <int WAVES, bool RELAY>
 __global__ __launch_bounds__ (kBlock, WAVES)
 void k_hrma_reflect (const mxg_config *__restrict__ cfg, const float *__restrict__ fa,
@@ -86,7 +87,7 @@ void k_hrma_reflect (const mxg_config *__restrict__ cfg, const float *__restrict
         if (i < range.hi)
           {
              id = io.idx_in[i];
-             keep = hrma_reflect_one (h, ltab, L, fa, da, s, id, rng_pos, seed, source_id, o);
+             keep = hrma_reflect_one<RELAY> (h, ltab, L, fa, da, s, id, rng_pos, seed, source_id, o);
           }
         const uint32_t q = survivors.add (keep, id);
         if (RELAY && keep) relay_put (relay, q, o);

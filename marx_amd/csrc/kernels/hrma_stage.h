@@ -385,7 +385,10 @@ __device__ __forceinline__ void mirror_constants (const mxg_hrma_cfg &h, const m
 // k_hrma_reflect's work for photon id of the middle list: paraboloid
 // reflection, cap struts, hyperboloid, post-collimator struts.  ltab: the
 // tables staged in LDS at the offsets lay.  True when the photon leaves the
-// mirror, its columns then also in o.
+// mirror, its columns then also in o.  HANDOFF: such photons go on to the
+// grating through o, and the grating stores their x, p and flags; only the
+// photons the mirror drops have theirs stored here.
+template <bool HANDOFF = false>
 __device__ __forceinline__ bool hrma_reflect_one (const mxg_hrma_cfg &h, const float *ltab, const HrmaLds &lay,
                                                   const float *__restrict__ fa, const double *__restrict__ da,
                                                   Soa s, uint32_t id, const uint32_t *__restrict__ rng_pos,
@@ -434,16 +437,20 @@ __device__ __forceinline__ bool hrma_reflect_one (const mxg_hrma_cfg &h, const f
                }
           }
      }
-   s.x0[id] = x.x; s.x1[id] = x.y; s.x2[id] = x.z;
-   s.p0[id] = p.x; s.p1[id] = p.y; s.p2[id] = p.z;
-   s.flags[id] = flags;
+   const bool keep = (flags & BAD_PHOTON_MASK) == 0;
+   if (!(HANDOFF && keep))
+     {
+        s.x0[id] = x.x; s.x1[id] = x.y; s.x2[id] = x.z;
+        s.p0[id] = p.x; s.p1[id] = p.y; s.p2[id] = p.z;
+        s.flags[id] = flags;
+     }
    o.x0 = x.x; o.x1 = x.y; o.x2 = x.z;
    o.p0 = p.x; o.p1 = p.y; o.p2 = p.z;
    o.energy = energy;
    o.tag = tag;
    o.flags = flags;
    o.shell = si;
-   return (flags & BAD_PHOTON_MASK) == 0;
+   return keep;
 }
 
 } // namespace mxg
