@@ -86,9 +86,9 @@ BYTES_PER_PHOTON = {
 # profiler runs): each entry names the profile it comes from, and the bench
 # line carries that name beside `traffic`.
 TRAFFIC_BYTES_PER_PHOTON = {
-    ("hetg-acis-s", "k_hrma_reflect"): (304.7, "profiles/pmc_hetg_acis_s_1e7.md"),
-    ("letg-hrc-s", "k_hrma_reflect"): (331.0, "profiles/pmc_letg_diff.md"),
-    ("letg-hrc-s", "grating"): (455.7, "profiles/pmc_letg_diff.md"),
+    ("hetg-acis-s", "k_hrma_reflect"): (304.5, "profiles/pmc_hetg_acis_s_1e7.md"),
+    ("letg-hrc-s", "k_hrma_reflect"): (318.7, "profiles/pmc_letg_diff.md"),
+    ("letg-hrc-s", "grating"): (394.3, "profiles/pmc_letg_diff.md"),
     # per listed photon
     ("diff", "k_diff_forward"): (357.1, "profiles/pmc_letg_diff.md"),
 }
