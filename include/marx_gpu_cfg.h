@@ -36,8 +36,11 @@
  * like its logarithm), so the keys (bits - base) >> shift split the table
  * into nb buckets; float-arena entries off .. off + nb hold, for bucket j,
  * the number of table entries whose key is below j (entry nb: all of
- * them).  A search then looks at one bucket's entries.  ok = 0: the table
- * is not strictly increasing and positive, search it whole.  Float keys
+ * them).  A search then looks at one bucket's entries.  ok = 1: the table
+ * is strictly increasing and positive; ok = 2: it repeats some entries (the
+ * HRC MCP and UV/ion shield tables do), and a value equal to a repeated
+ * entry is searched whole, since the bisection's pick among equal entries
+ * depends on its path; ok = 0: search it whole.  Float keys
  * for tables searched with float abscissae, double keys (the float values
  * widened) for those searched with double ones. */
 typedef struct
@@ -66,6 +69,7 @@ mxg_lbucket;
 #define MXG_BUCKETS_CORR 64
 #define MXG_BUCKETS_EALPHA 32
 #define MXG_BUCKETS_GRATING 256
+#define MXG_BUCKETS_QE 256
 
 typedef struct
 {
@@ -168,6 +172,7 @@ typedef struct
    double xpixel_offset, ypixel_offset;
    unsigned int qe_n, qe_e_off, qe_off;          /* float arena */
    unsigned int filter_n, filter_e_off, filter_off;
+   mxg_bucket qe_bucket;                          /* of the QE energies */
 }
 mxg_chip;
 
@@ -175,6 +180,7 @@ typedef struct
 {
    unsigned int n, e_off, eff_off;   /* float arena; n = 0 -> unit efficiency */
    unsigned int pad;
+   mxg_bucket bucket;                 /* of the energies */
 }
 mxg_qe_table;
 

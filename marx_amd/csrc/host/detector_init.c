@@ -227,7 +227,7 @@ static int read_acis_qe (mxg_chip *g)
           return -1;
        }
    mxg_fits_close (t);
-   return 0;
+   return mxg_bucket_of_farena (g->qe_e_off, n, MXG_BUCKETS_QE, &g->qe_bucket);
 }
 
 /* two-column bdat efficiency file; an empty name means unit efficiency */
@@ -235,6 +235,7 @@ static int read_qe_file (Param_File_Type *pf, const char *pname, int ideal, mxg_
 {
    char buf[1024], *file;
    float *cols[2];
+   int status;
    memset (q, 0, sizeof (*q));
    if (-1 == pf_get_file (pf, (char *) pname, buf, sizeof (buf)))
      return -1;
@@ -251,9 +252,16 @@ static int read_qe_file (Param_File_Type *pf, const char *pname, int ideal, mxg_
    free (file);
    q->e_off = mxg_farena_push (cols[0], q->n);
    q->eff_off = mxg_farena_push (cols[1], q->n);
-   free (cols[0]);
    free (cols[1]);
-   return ((q->e_off == (unsigned int) -1) || (q->eff_off == (unsigned int) -1)) ? -1 : 0;
+   if ((q->e_off == (unsigned int) -1) || (q->eff_off == (unsigned int) -1))
+     {
+        free (cols[0]);
+        return -1;
+     }
+   /* the bucket index of the energies: the MCP QE tables hold 2e4-4e4 rows */
+   status = mxg_build_bucket (cols[0], q->n, 0, MXG_BUCKETS_QE, &q->bucket);
+   free (cols[0]);
+   return status;
 }
 
 static int read_opt_consts (Param_File_Type *pf, const char *pname, unsigned int *n,

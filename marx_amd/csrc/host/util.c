@@ -402,13 +402,18 @@ int mxg_build_bucket (const float *xp, unsigned int n, int double_keys, unsigned
    unsigned long long k0, range;
    unsigned int i, j, shift, nb;
    float *cnt;
+   unsigned int ok = 1;
 
    memset (b, 0, sizeof (*b));
    if ((n < 2) || (n > (1u << 24)) || !(xp[0] > 0.0f))
      return 0;
    for (i = 1; i < n; i++)
      if (!(xp[i] > xp[i - 1]))
-       return 0;
+       {
+          if (!(xp[i] == xp[i - 1]))
+            return 0;
+          ok = 2;
+       }
    k0 = table_key (xp[0], double_keys);
    range = table_key (xp[n - 1], double_keys) - k0;
    shift = 0;
@@ -435,7 +440,7 @@ int mxg_build_bucket (const float *xp, unsigned int n, int double_keys, unsigned
    b->base = k0;
    b->nb = nb;
    b->shift = shift;
-   b->ok = 1;
+   b->ok = ok;
    return 0;
 }
 

@@ -90,7 +90,7 @@ __device__ __forceinline__ int find_chip (const mxg_detector_cfg &d, vec3 &x, ve
 
 __device__ __forceinline__ double qe_at (const mxg_qe_table &q, const float *__restrict__ fa, double energy)
 {
-   return interp_f ((float) energy, fa + q.e_off, fa + q.eff_off, q.n);
+   return interp_fb ((float) energy, fa + q.e_off, fa + q.eff_off, q.n, q.bucket, fa + q.bucket.off);
 }
 
 __device__ __forceinline__ int filter_region (const mxg_detector_cfg &d, double y, double z)
@@ -432,7 +432,8 @@ __device__ __forceinline__ bool detect_one (const mxg_detector_cfg &d, const flo
                        double r = uniform (st);
                        double qe = 1.0, qe_contam = 1.0;
                        if (g.qe_n != 0)
-                         qe = interp_f ((float) energy, fa + g.qe_e_off, fa + g.qe_off, g.qe_n);
+                         qe = interp_fb ((float) energy, fa + g.qe_e_off, fa + g.qe_off, g.qe_n, g.qe_bucket,
+                                         fa + g.qe_bucket.off);
                        if (d.contam_on && d.contam[k].num_layers)
                          qe_contam = contam_transmission (d.contam[k], fa, energy, ypix, zpix);
                        // the filter table is empty on the CALDB QE path

@@ -215,9 +215,10 @@ __device__ __forceinline__ unsigned int search_w (double x, const T *__restrict_
 
 // search_f / search_w of a table with its bucket index (mxg_bucket; cnt: its
 // counts, possibly in LDS): the same index, from one bucket's entries.  For
-// a strictly increasing table the bisection returns the first index whose
-// entry is >= x, except for x equal to the first entry, which it never
-// probes: 1 there.
+// a non-decreasing table the bisection returns the first index whose entry
+// is >= x, except for x equal to the first entry, which it never probes (1
+// there), and for x equal to a repeated entry, where its pick among the
+// equal entries depends on its path (the whole bisection runs there).
 __device__ __forceinline__ unsigned int search_fb (float x, const float *__restrict__ xp, unsigned int n,
                                                    const mxg_bucket &b, const float *__restrict__ cnt)
 {
@@ -238,6 +239,8 @@ __device__ __forceinline__ unsigned int search_fb (float x, const float *__restr
         if (xp[mid] < x) lo = mid + 1;
         else hi = mid;
      }
+   if ((b.ok == 2) && (lo + 1 < n) && (xp[lo] == x) && (xp[lo + 1] == x))
+     return search_f (x, xp, n);
    return lo;
 }
 
@@ -261,6 +264,8 @@ __device__ __forceinline__ unsigned int search_wb (double x, const float *__rest
         if ((double) xp[mid] < x) lo = mid + 1;
         else hi = mid;
      }
+   if ((b.ok == 2) && (lo + 1 < n) && ((double) xp[lo] == x) && ((double) xp[lo + 1] == x))
+     return search_w (x, xp, n);
    return lo;
 }
 

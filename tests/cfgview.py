@@ -64,11 +64,11 @@ class Chip(ctypes.Structure):
     _fields_ = [("id", I), ("pad", I), ("x_ll", D * 3), ("xhat", D * 3), ("yhat", D * 3),
                 ("normal", D * 3), ("xlen", D), ("ylen", D), ("x_pixel_size", D), ("y_pixel_size", D),
                 ("xpixel_offset", D), ("ypixel_offset", D), ("qe_n", U), ("qe_e_off", U), ("qe_off", U),
-                ("filter_n", U), ("filter_e_off", U), ("filter_off", U)]
+                ("filter_n", U), ("filter_e_off", U), ("filter_off", U), ("qe_bucket", Bucket)]
 
 
 class QeTable(ctypes.Structure):
-    _fields_ = [("n", U), ("e_off", U), ("eff_off", U), ("pad", U)]
+    _fields_ = [("n", U), ("e_off", U), ("eff_off", U), ("pad", U), ("bucket", Bucket)]
 
 
 class Rect(ctypes.Structure):

@@ -1,6 +1,6 @@
 """The bucket indices of the search tables (mxg_bucket, built at init: the
 mirror's optical constants, area corrections and WFold e_alpha values, the
-gratings' energy grids): a search that starts from the bucket holding the
+gratings' energy grids, the detectors' QE tables): a search that starts from the bucket holding the
 value must
 return the plain bisection's index (dev_common.h's search_f / search_w, which
 restate JDMbinary_search_f, jdmath/src/binsrch.c) for every value -- each
@@ -53,20 +53,30 @@ def bucketed(x, xp, b, cnt, double_keys):
             lo = mid + 1
         else:
             hi = mid
+    if b.ok == 2 and lo + 1 < len(xp) and xp[lo] == x and xp[lo + 1] == x:
+        return bisect(x, xp)
     return lo
 
 
-def _check_table(xp, b, cnt, double_keys, nbmax, rng):
+def _check_table(xp, b, cnt, double_keys, nbmax, rng, max_entries=4000):
+    """Bucket counts against a direct count of the keys, and the bucketed
+    search against the bisection for the entries (a random max_entries of
+    them on longer tables), their neighbours, midpoints, values outside the
+    table and random values inside it."""
     dt = np.float64 if double_keys else np.float32
     xp = np.asarray(xp, dt)
-    assert b.ok == 1 and 1 <= b.nb <= nbmax
+    repeats = bool(np.any(xp[1:] == xp[:-1]))
+    assert b.ok == (2 if repeats else 1) and 1 <= b.nb <= nbmax
     k = keys(xp, double_keys)
     assert int(b.base) == int(k[0])
     # counts: entries whose key lies below each bucket's first key
-    want = [sum(1 for kk in k if kk < int(b.base) + (j << int(b.shift))) for j in range(b.nb)] + [len(xp)]
+    ku = np.array([int(kk) for kk in k], dtype=np.uint64)
+    bounds = np.array([int(b.base) + (j << int(b.shift)) for j in range(b.nb)], dtype=np.uint64)
+    want = [int(np.count_nonzero(ku < x)) for x in bounds] + [len(xp)]
     assert [int(c) for c in cnt] == want
-    xs = list(xp) + list(np.nextafter(xp, np.inf).astype(dt)) + list(np.nextafter(xp, -np.inf).astype(dt))
-    xs += list(((xp[1:].astype(np.float64) + xp[:-1]) / 2).astype(dt))
+    ent = xp if len(xp) <= max_entries else xp[np.sort(rng.choice(len(xp) - 1, max_entries, replace=False))]
+    xs = list(ent) + list(np.nextafter(ent, np.inf).astype(dt)) + list(np.nextafter(ent, -np.inf).astype(dt))
+    xs += list(((xp[1:].astype(np.float64) + xp[:-1]) / 2).astype(dt))[:max_entries]
     xs += [dt(xp[0] / 2), dt(xp[-1] * 2), dt(0.0), dt(-1.0)]
     xs += list(rng.uniform(float(xp[0]), float(xp[-1]), 2000).astype(dt))
     for x in xs:
@@ -91,6 +101,24 @@ def test_mirror_search_tables_bucket_index():
             assert w.n_arrays > 100
             ea = cv.d(w.e_alphas_off, w.n_arrays).astype(np.float32)
             _check_table(ea, b, cv.f(b.off, b.nb + 1), True, 32, rng)
+
+
+@pytest.mark.parametrize("det", ["ACIS-S", "ACIS-I", "HRC-S", "HRC-I"])
+def test_detector_qe_bucket_index(det):
+    """The detector's QE tables: each ACIS chip's CALDB QE, the HRC MCP QE
+    (2e4-4e4 rows) and UV/ion shield transmissions."""
+    t = Tracer(seed=1, GratingType="NONE", DetectorType=det)
+    cv = ConfigView(t)
+    d = cv.cfg.det
+    rng = np.random.default_rng(8)
+    tables = []
+    if det.startswith("ACIS"):
+        tables = [(d.chip[k].qe_e_off, d.chip[k].qe_n, d.chip[k].qe_bucket) for k in range(d.num_chips)]
+    else:
+        tables = [(q.e_off, q.n, q.bucket) for q in list(d.mcp_qe) + list(d.filter_qe) if q.n > 0]
+    assert tables and all(n > 1000 for _, n, _ in tables[:1])
+    for off, n, b in tables:
+        _check_table(cv.f(off, n), b, cv.f(b.off, b.nb + 1), False, 256, rng)
 
 
 def test_unsorted_or_short_tables_are_searched_whole():
