@@ -269,6 +269,9 @@ __device__ __forceinline__ unsigned int search_wb (double x, const float *__rest
    return lo;
 }
 
+// counts of a bucket index (floats), or none
+inline __host__ __device__ uint32_t bucket_floats (const mxg_bucket &b) { return b.ok ? b.nb + 1 : 0u; }
+
 // interp_f / interp_f2 with a bucket-indexed search (the same values)
 __device__ __forceinline__ float interp_fb (float x, const float *__restrict__ xp, const float *__restrict__ yp,
                                             unsigned int n, const mxg_bucket &bk, const float *__restrict__ cnt)

@@ -252,12 +252,14 @@ hipError_t launch_hrma_reflect (const mxg_config *cfg, const mxg_config *hcfg, c
                                 const double *da, Soa s, ListIO io, const uint32_t *rng_pos, int waves,
                                 unsigned long long seed, uint32_t source_id, const Relay *relay_out,
                                 hipStream_t st);
-// letg: the instance compiled for the LETG pass; relay_in, when non-null:
+// hcfg: the host copy of cfg (for the layout of the tables the kernel
+// stages in LDS); letg: the instance compiled for the LETG pass; relay_in,
+// when non-null:
 // the input list is the mirror's with this relay, whose rows the kernel
 // reads instead of the store's; relay_out, when non-null: the survivors go
 // to it for launch_detect, which stores their x, p and flags (the kernel
 // stores those of the photons it drops only)
-hipError_t launch_grating (const mxg_config *cfg, const float *fa, const double *da, Soa s,
+hipError_t launch_grating (const mxg_config *cfg, const mxg_config *hcfg, const float *fa, const double *da, Soa s,
                            ListIO io, uint32_t n_upper, unsigned long long seed,
                            uint32_t source_id, bool letg, const Relay *relay_in, const Relay *relay_out,
                            hipStream_t st);

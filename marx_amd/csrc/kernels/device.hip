@@ -649,7 +649,7 @@ int mxgd_run_grating (unsigned long long seed, int slot, float *ms, int relay_in
         if (skip_stage (io)) return -1;
      }
    else
-     HIPCHECK (launch_grating (d_cfg, d_fa, d_da, g_soa, io, g_upper, seed, 0,
+     HIPCHECK (launch_grating (d_cfg, &g_hcfg, d_fa, d_da, g_soa, io, g_upper, seed, 0,
                                g_hcfg.grating.type == MARX_GRATING_LETG, rin, rout, g_stream));
    g_relay_have = out_k;
    return end_stage (slot, ms);
@@ -1405,7 +1405,7 @@ int mxgd_trace_noalloc (const mxg_soa_view *v, unsigned int n, unsigned int tag_
    if ((cfg->grating.type == MARX_GRATING_HETG) || (cfg->grating.type == MARX_GRATING_LETG))
      {
         HIPCHECK (clear_out ());
-        HIPCHECK (launch_grating (c_cfg, c_fa, c_da, s, io (MXGD_K_GRATING), n, seed, 0,
+        HIPCHECK (launch_grating (c_cfg, cfg, c_fa, c_da, s, io (MXGD_K_GRATING), n, seed, 0,
                                   cfg->grating.type == MARX_GRATING_LETG, nullptr, nullptr, g_stream));
         cur = 1 - cur;
      }

@@ -70,11 +70,11 @@ __device__ __forceinline__ int torus_intersect (vec3 &x0, vec3 p, double rowland
 
 // Order draw: first k with r <= cumulative efficiency(E); -1 when none.
 // The bracket is the one the reference's merge walk over sorted energies
-// reaches: the first j >= 1 with E[j] >= e, else the last interval.
+// reaches: the first j >= 1 with E[j] >= e, else the last interval.  E:
+// the grating's energies, bcnt: their bucket counts (LDS or the arena).
 __device__ __forceinline__ int draw_order (const mxg_grating &g, const float *__restrict__ fa,
-                                           double e, double r)
+                                           const float *E, const float *bcnt, double e, double r)
 {
-   const float *E = fa + g.energies_off;
    const unsigned int ne = g.num_energies, no = g.num_orders;
    unsigned int lo = 1, hi = ne - 1;
    const mxg_bucket &b = g.e_bucket;
@@ -86,7 +86,7 @@ __device__ __forceinline__ int draw_order (const mxg_grating &g, const float *__
         // on the answer clamped to [1, ne - 1]
         const uint64_t k = ((uint64_t) __float_as_uint (ef) - b.base) >> b.shift;
         const uint32_t j = (k < b.nb) ? (uint32_t) k : b.nb - 1;
-        unsigned int l = (unsigned int) fa[b.off + j], h = (unsigned int) fa[b.off + j + 1];
+        unsigned int l = (unsigned int) bcnt[j], h = (unsigned int) bcnt[j + 1];
         while (l < h)
           {
              unsigned int mid = (l + h) >> 1;
@@ -148,9 +148,10 @@ __device__ __forceinline__ int draw_order (const mxg_grating &g, const float *__
    return -1;
 }
 
-__device__ __forceinline__ int diffract_photon (const mxg_grating &g, const mxg_sectors &gs,
-                                                const double *__restrict__ da, double theta,
-                                                double energy, vec3 x, vec3 &p, int order,
+// nsec, sec: the shell's sector count and its six columns (LDS or the
+// arena), nsec = 0 without a sector table
+__device__ __forceinline__ int diffract_photon (const mxg_grating &g, unsigned int nsec, const double *sec,
+                                                double theta, double energy, vec3 x, vec3 &p, int order,
                                                 Stream &st)
 {
    double n_lambda_over_d = (double) order * (2.0 * MXG_PI * MXG_HBAR_C) / g.period / energy;
@@ -161,18 +162,18 @@ __device__ __forceinline__ int diffract_photon (const mxg_grating &g, const mxg_
    vec3 d = cross (n, l);
    double dtheta, dp_over_p;
 
-   if (gs.n > 0)
+   if (nsec > 0)
      {
-        const double *min_a = da + gs.off;
-        const double *max_a = min_a + gs.n;
+        const double *min_a = sec;
+        const double *max_a = min_a + nsec;
         double sector = atan2 (x.y, x.z);
         if (sector < 0) sector = 2 * MXG_PI + sector;
-        unsigned int sn = search_d (sector, min_a, gs.n) - 1;
-        if (sn >= gs.n) return -1;
+        unsigned int sn = search_d (sector, min_a, nsec) - 1;
+        if (sn >= nsec) return -1;
         if ((max_a[sn] <= sector) || (min_a[sn] > sector))
           return -1;
-        dtheta = min_a[2 * gs.n + sn] + min_a[3 * gs.n + sn] * gaussian (st);
-        dp_over_p = min_a[4 * gs.n + sn] + min_a[5 * gs.n + sn] * gaussian (st);
+        dtheta = min_a[2 * nsec + sn] + min_a[3 * nsec + sn] * gaussian (st);
+        dp_over_p = min_a[4 * nsec + sn] + min_a[5 * nsec + sn] * gaussian (st);
      }
    else
      {
@@ -207,19 +208,130 @@ __device__ __forceinline__ int diffract_photon (const mxg_grating &g, const mxg_
    return 0;
 }
 
+// The small tables of grating g (energies, their bucket counts, orders)
+// and its shell's sector columns, where the order draw and the diffraction
+// search them.
+struct GratingTabs
+{
+   const float *E, *bcnt, *orders;
+   const double *sec;
+   unsigned int nsec;
+};
+
 // one order draw plus diffraction off grating g; returns 0 or -1
-__device__ __forceinline__ int diffract_from (const mxg_grating &g, const mxg_sectors &gs,
-                                              const float *__restrict__ fa,
-                                              const double *__restrict__ da, double theta,
+__device__ __forceinline__ int diffract_from (const mxg_grating &g, const GratingTabs &t,
+                                              const float *__restrict__ fa, double theta,
                                               double energy, double ef, vec3 x, vec3 &p,
                                               int8_t &order_out, Stream &st)
 {
    double r = uniform (st);
-   int k = draw_order (g, fa, ef, r);
+   int k = draw_order (g, fa, t.E, t.bcnt, ef, r);
    if (k < 0) return -1;
-   int order = (int) fa[g.orders_off + k];
+   int order = (int) t.orders[k];
    order_out = (int8_t) order;
-   return diffract_photon (g, gs, da, theta, energy, x, p, order, st);
+   return diffract_photon (g, t.nsec, t.sec, theta, energy, x, p, order, st);
+}
+
+// The search tables of the grating stage staged in LDS by k_grating:
+// offsets in floats from the table area of gratings 0-3 (by shell), 4 (the
+// LETG fine support grating) and 5 (coarse), and of each shell's sector
+// columns (doubles, at even offsets).
+struct GratingLds
+{
+   uint32_t e[6], bk[6], ord[6];
+   uint32_t sec[MXG_NUM_SHELLS];
+   uint32_t floats;
+};
+
+// Dynamic LDS of k_grating: the layout at 0, the tables from kGratingTabOff.
+constexpr size_t kGratingTabOff = (sizeof (GratingLds) + 15) & ~(size_t) 15;
+
+__host__ __device__ inline const mxg_grating &grating_of (const mxg_grating_cfg &gc, int k)
+{
+   return (k < MXG_NUM_SHELLS) ? gc.g[k] : ((k == MXG_NUM_SHELLS) ? gc.fine : gc.coarse);
+}
+
+// The LDS layout of configuration gc's search tables (host side); letg:
+// with the support gratings.
+inline GratingLds grating_lds_layout (const mxg_grating_cfg &gc, bool letg)
+{
+   GratingLds lay = {};
+   uint32_t f = 0;
+   for (int k = 0; k < 6; k++)
+     {
+        if ((k >= MXG_NUM_SHELLS) && !(letg && ((k == MXG_NUM_SHELLS) ? gc.has_fine : gc.has_coarse)))
+          continue;
+        const mxg_grating &g = grating_of (gc, k);
+        lay.e[k] = f; f += g.num_energies;
+        lay.bk[k] = f; f += bucket_floats (g.e_bucket);
+        lay.ord[k] = f; f += g.num_orders;
+     }
+   for (int sh = 0; sh < MXG_NUM_SHELLS; sh++)
+     {
+        f = (f + 1) & ~1u;
+        lay.sec[sh] = f;
+        f += 2 * 6 * gc.sectors[sh].n;
+     }
+   lay.floats = f;
+   return lay;
+}
+
+// Stages the tables of lay in LDS (the layout at lds, the tables at ltab =
+// lds + kGratingTabOff); letg as for the layout.  The caller synchronises.
+__device__ __forceinline__ void stage_grating_tables (const mxg_grating_cfg &gc, const float *__restrict__ fa,
+                                                      const double *__restrict__ da, const GratingLds &lay,
+                                                      bool letg, unsigned char *lds, float *ltab)
+{
+   if (threadIdx.x == 0) *(GratingLds *) lds = lay;
+   for (int k = 0; k < 6; k++)
+     {
+        if ((k >= MXG_NUM_SHELLS) && !(letg && ((k == MXG_NUM_SHELLS) ? gc.has_fine : gc.has_coarse)))
+          continue;
+        const mxg_grating &g = grating_of (gc, k);
+        for (uint32_t w = threadIdx.x; w < g.num_energies; w += blockDim.x)
+          ltab[lay.e[k] + w] = fa[g.energies_off + w];
+        const uint32_t nb = bucket_floats (g.e_bucket);
+        for (uint32_t w = threadIdx.x; w < nb; w += blockDim.x)
+          ltab[lay.bk[k] + w] = fa[g.e_bucket.off + w];
+        for (uint32_t w = threadIdx.x; w < g.num_orders; w += blockDim.x)
+          ltab[lay.ord[k] + w] = fa[g.orders_off + w];
+     }
+   for (int sh = 0; sh < MXG_NUM_SHELLS; sh++)
+     {
+        double *dst = (double *) (ltab + lay.sec[sh]);
+        const uint32_t n = 6 * gc.sectors[sh].n;
+        for (uint32_t w = threadIdx.x; w < n; w += blockDim.x)
+          dst[w] = da[gc.sectors[sh].off + w];
+     }
+}
+
+// The tables of grating k (0-3 by shell, 4 fine, 5 coarse) and, for k < 4,
+// of shell k's sectors: from LDS (STAGED: layout L, tables ltab) or from the
+// arenas.
+template <bool STAGED>
+__device__ __forceinline__ GratingTabs grating_tabs (const mxg_grating_cfg &gc, int k, const float *__restrict__ fa,
+                                                     const double *__restrict__ da, const GratingLds *L,
+                                                     const float *ltab)
+{
+   GratingTabs t;
+   const mxg_grating &g = grating_of (gc, k);
+   const bool shell = k < MXG_NUM_SHELLS;
+   t.nsec = shell ? gc.sectors[k].n : 0u;
+   if constexpr (STAGED)
+     {
+        t.E = ltab + L->e[k];
+        t.bcnt = ltab + L->bk[k];
+        t.orders = ltab + L->ord[k];
+        t.sec = shell ? (const double *) (ltab + L->sec[k]) : nullptr;
+     }
+   else
+     {
+        t.E = fa + g.energies_off;
+        t.bcnt = fa + g.e_bucket.off;
+        t.orders = fa + g.orders_off;
+        t.sec = shell ? da + gc.sectors[k].off : nullptr;
+     }
+   return t;
 }
 
 // k_grating's work for photon id: vignetting, the Rowland torus, the order
@@ -227,16 +339,18 @@ __device__ __forceinline__ int diffract_from (const mxg_grating &g, const mxg_se
 // photon leaves the grating stage alive, its columns then also in o.
 // RELAY: its incoming x, p, energy, tag, flags and shell are read from row
 // `slot` of *relay instead of the store, where the mirror left x, p and
-// flags for this stage to write (a vignetted photon's as they came).  HANDOFF: the photons it keeps go
-// on to the detector through o, and the detector stores their x, p and
-// flags; only the photons it drops have theirs stored here.
-template <bool LETG, bool RELAY = false, bool HANDOFF = false>
+// flags for this stage to write (a vignetted photon's as they came).
+// HANDOFF: the photons it keeps go on to the detector through o, and the
+// detector stores their x, p and flags; only the photons it drops have
+// theirs stored here.  STAGED: the search tables are read from LDS (layout
+// L, tables ltab, as k_grating stages them), else from the arenas.
+template <bool LETG, bool RELAY = false, bool HANDOFF = false, bool STAGED = false>
 __device__ __forceinline__ bool grating_one (const mxg_grating_cfg &gc, const float *__restrict__ fa,
                                              const double *__restrict__ da, Soa s, uint32_t id,
                                              unsigned long long seed, uint32_t source_id, PhotonOut &o,
-                                             const Relay *relay = nullptr, uint32_t slot = 0)
+                                             const Relay *relay = nullptr, uint32_t slot = 0,
+                                             const GratingLds *L = nullptr, const float *ltab = nullptr)
 {
-   const mxg_sectors no_sectors = {0u, 0u};
    bool keep = false;
    uint32_t flags = RELAY ? relay->flags[slot] : s.flags[id];
    if ((flags & BAD_PHOTON_MASK) == 0)
@@ -269,8 +383,8 @@ __device__ __forceinline__ bool grating_one (const mxg_grating_cfg &gc, const fl
              p = rotate_x (p, g.disp_cos, -g.disp_sin);
              if (-1 == torus_intersect (x, p, gc.rowland[shell]))
                flags |= PHOTON_UNDIFFRACTED;
-             if (-1 == diffract_from (g, gc.sectors[shell], fa, da, 0.0, energy, ef, x, p,
-                                      order, st))
+             if (-1 == diffract_from (g, grating_tabs<STAGED> (gc, shell, fa, da, L, ltab), fa, 0.0, energy,
+                                      ef, x, p, order, st))
                flags |= PHOTON_UNDIFFRACTED;
              s.order[id] = order;
 
@@ -285,8 +399,10 @@ __device__ __forceinline__ bool grating_one (const mxg_grating_cfg &gc, const fl
                          : (pass == 1) ? MXG_PI / 3.0
                          : (pass == 2) ? 2.0 * MXG_PI / 3.0 : 0.0;
                        int8_t so = s.sup[pass][id];
-                       if (-1 == diffract_from (fine ? gc.fine : gc.coarse, no_sectors, fa, da,
-                                                theta, energy, ef, x, p, so, st))
+                       if (-1 == diffract_from (fine ? gc.fine : gc.coarse,
+                                                grating_tabs<STAGED> (gc, fine ? MXG_NUM_SHELLS : MXG_NUM_SHELLS + 1,
+                                                                      fa, da, L, ltab),
+                                                fa, theta, energy, ef, x, p, so, st))
                          flags |= PHOTON_UNDIFFRACTED;
                        s.sup[pass][id] = so;
                     }

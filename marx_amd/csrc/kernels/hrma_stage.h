@@ -158,9 +158,6 @@ struct HrmaLds
 constexpr size_t kHrmaLayOff = (sizeof (mxg_hrma_cfg) + 15) & ~(size_t) 15;
 constexpr size_t kHrmaTabOff = (kHrmaLayOff + sizeof (HrmaLds) + 15) & ~(size_t) 15;
 
-// counts of a bucket index, or none
-inline uint32_t bucket_floats (const mxg_bucket &b) { return b.ok ? b.nb + 1 : 0u; }
-
 // The LDS layout of the reflection tables for configuration h (host side).
 inline HrmaLds hrma_lds_layout (const mxg_hrma_cfg &h)
 {
