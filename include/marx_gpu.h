@@ -416,7 +416,9 @@ int mxg_trace_batch (unsigned int num_sources, const double *dirs, const unsigne
 
 /* Launch schedule of the batched pass.  Staged (persistent_tile = 0): one
  * kernel per stage, with blocks per CU of the source, paraboloid,
- * reflection, grating and detector kernels (1..32) and the minimum waves per
+ * reflection, grating and detector kernels (1..32; for the grating also 0,
+ * the default: one round of the blocks a CU holds at once, which its LDS
+ * tables and register budget set) and the minimum waves per
  * SIMD the reflection kernel is compiled for (2, 3, 4).  Persistent
  * (persistent_tile = a multiple of 256 up to 4096): the whole pass as one
  * launch of persistent_blocks_per_cu (1..8) blocks per CU that take tiles of

@@ -101,7 +101,8 @@ int mxgd_run_batch_source (unsigned int n, double exposure, double start_time,
 int mxgd_run_batch_persistent (unsigned int n, double exposure, double start_time,
                                unsigned long long seed, int slot);
 
-/* Launch schedule: blocks per CU of the stage kernels and the minimum waves
+/* Launch schedule: blocks per CU of the stage kernels (the grating's 0: one
+ * round of the blocks a CU holds at once) and the minimum waves
  * per SIMD of the reflection kernel (2, 3 or 4); persist_tile != 0: the
  * batched pass as one persistent launch of persist_blocks_per_cu blocks per
  * CU over tiles of persist_tile photons (a multiple of 256, at most 4096). */

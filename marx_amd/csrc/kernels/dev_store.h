@@ -252,6 +252,11 @@ hipError_t launch_hrma_reflect (const mxg_config *cfg, const mxg_config *hcfg, c
                                 const double *da, Soa s, ListIO io, const uint32_t *rng_pos, int waves,
                                 unsigned long long seed, uint32_t source_id, const Relay *relay_out,
                                 hipStream_t st);
+// The blocks of k_grating that one CU holds at once for configuration hcfg
+// (its waves per SIMD, fewer if its LDS tables do not fit that many), 8 if
+// the runtime cannot say.  A grid of whole rounds of them measured fastest
+// (HETG + ACIS-S: 3 blocks per CU 0.36 ms, 4 0.47 ms, 8 0.40 ms).
+uint32_t grating_blocks_per_cu (const mxg_config *hcfg);
 // hcfg: the host copy of cfg (for the layout of the tables the kernel
 // stages in LDS); letg: the instance compiled for the LETG pass; relay_in,
 // when non-null:

@@ -85,7 +85,8 @@ double *d_diff_partial = nullptr;  // per-block gradient partials
 double *d_diff_small = nullptr;    // summed weight, gradient
 mxg_diff_cfg *d_diff_cfg = nullptr;
 // launch schedule and graph capture
-mxgd_schedule g_sched = {{8, 8, 8, 8, 8}, MXG_REFLECT_WAVES_DEFAULT, 0, 2};
+mxgd_schedule g_sched = {{8, 8, 8, 0, 8}, MXG_REFLECT_WAVES_DEFAULT, 0, 2};
+uint32_t g_grating_bpc = 8;        // one round of k_grating blocks per CU (grating_blocks_per_cu)
 int g_capturing = 0;
 uint64_t *d_tile_tot = nullptr;
 // bumped whenever a buffer that a captured launch sequence may hold is
@@ -256,13 +257,21 @@ int ensure_aos (size_t n)
    return 0;
 }
 
+// The schedule's blocks per CU of stage kernel k; 0 (the grating's
+// default): one round of the blocks a CU holds at once.
+uint32_t sched_blocks_per_cu (int k)
+{
+   const uint32_t b = g_sched.blocks_per_cu[k];
+   return b ? b : g_grating_bpc;
+}
+
 // Blocks of stage kernel k over at most n_upper active photons: the
 // schedule's blocks per CU times the CUs (each block owning an equal range),
 // fewer when there are fewer photons than that.
 uint32_t stage_grid (uint32_t n_upper, int k)
 {
    uint32_t blocks = (n_upper + kBlock - 1) / kBlock;
-   uint32_t cap = (uint32_t) (g_num_cus > 0 ? g_num_cus : 256) * g_sched.blocks_per_cu[k];
+   uint32_t cap = (uint32_t) (g_num_cus > 0 ? g_num_cus : 256) * sched_blocks_per_cu (k);
    if (cap > kMaxSegments) cap = kMaxSegments;
    if (blocks > cap) blocks = cap;
    return blocks ? blocks : 1u;
@@ -493,6 +502,7 @@ int mxgd_sync_config (const mxg_config *cfg, const float *fa, unsigned int nf,
      }
    HIPCHECK (hipMemcpy (d_cfg, cfg, sizeof (mxg_config), hipMemcpyHostToDevice));
    memcpy (&g_hcfg, cfg, sizeof (mxg_config));
+   g_grating_bpc = grating_blocks_per_cu (&g_hcfg);
    if (nf) HIPCHECK (hipMemcpy (d_fa, fa, nf * sizeof (float), hipMemcpyHostToDevice));
    if (nd) HIPCHECK (hipMemcpy (d_da, da, nd * sizeof (double), hipMemcpyHostToDevice));
    g_cfg_gen = generation;
@@ -1102,8 +1112,9 @@ int mxgd_run_batch_persistent (unsigned int n, double exposure, double start_tim
 int mxgd_set_schedule (const mxgd_schedule *s)
 {
    for (int k = 0; k < MXGD_NUM_K; k++)
-     if ((s->blocks_per_cu[k] < 1) || (s->blocks_per_cu[k] > 32))
-       return fail ("blocks per CU must be 1..32");
+     if (((s->blocks_per_cu[k] < 1) && !((k == MXGD_K_GRATING) && (s->blocks_per_cu[k] == 0)))
+         || (s->blocks_per_cu[k] > 32))
+       return fail ("blocks per CU must be 1..32 (0 for the grating: one round of the blocks a CU holds)");
    if ((s->reflect_waves < 2) || (s->reflect_waves > 4))
      return fail ("reflection kernel waves per SIMD must be 2, 3 or 4");
    if ((s->persist_tile % kBlock) || (s->persist_tile > kMaxPersistTile))
@@ -1330,6 +1341,7 @@ int mxgd_trace_noalloc (const mxg_soa_view *v, unsigned int n, unsigned int tag_
         if (nd) HIPCHECK (hipMemcpyAsync (d_da, da, nd * sizeof (double), hipMemcpyHostToDevice, g_stream));
         HIPCHECK (hipStreamSynchronize (g_stream));
         memcpy (&g_hcfg, cfg, sizeof (mxg_config));
+        g_grating_bpc = grating_blocks_per_cu (&g_hcfg);
         g_cfg_gen = generation;
      }
    int dev = 0;
@@ -1388,7 +1400,7 @@ int mxgd_trace_noalloc (const mxg_soa_view *v, unsigned int n, unsigned int tag_
       l.seg = seg;
       l.seg_count = seg_count;
       uint32_t blocks = (n + kBlock - 1) / kBlock;
-      uint32_t cap = (uint32_t) (g_num_cus > 0 ? g_num_cus : 256) * g_sched.blocks_per_cu[k];
+      uint32_t cap = (uint32_t) (g_num_cus > 0 ? g_num_cus : 256) * sched_blocks_per_cu (k);
       if (cap > kMaxSegments) cap = kMaxSegments;
       l.grid = blocks < cap ? (blocks ? blocks : 1u) : cap;
       return l;

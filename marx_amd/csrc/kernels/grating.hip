@@ -67,6 +67,20 @@ static void launch_io (dim3 g, size_t lds_bytes, hipStream_t st, const mxg_confi
                          source_id, lay, a, b);
 }
 
+uint32_t grating_blocks_per_cu (const mxg_config *hcfg)
+{
+   const bool letg = hcfg->grating.type == MARX_GRATING_LETG;
+   const GratingLds lay = grating_lds_layout (hcfg->grating, letg);
+   const size_t lds_bytes = kGratingTabOff + (size_t) lay.floats * sizeof (float);
+   int n = 0;
+   const hipError_t e
+     = letg ? hipOccupancyMaxActiveBlocksPerMultiprocessor (&n, k_grating<true, MXG_LETG_GRATING_WAVES, false, false>,
+                                                             kBlock, lds_bytes)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor (&n, k_grating<false, MXG_GRATING_WAVES, false, false>,
+                                                             kBlock, lds_bytes);
+   return ((e == hipSuccess) && (n > 0)) ? (uint32_t) n : 8u;
+}
+
 hipError_t launch_grating (const mxg_config *cfg, const mxg_config *hcfg, const float *fa, const double *da,
                            Soa s, ListIO io, uint32_t n_upper, unsigned long long seed,
                            uint32_t source_id, bool letg, const Relay *relay_in, const Relay *relay_out,

@@ -248,7 +248,8 @@ class Tracer:
     def set_schedule(self, blocks_per_cu=None, reflect_waves=None, persistent_tile=None,
                      persistent_blocks_per_cu=None, **per_kernel):
         """Sets the launch schedule: blocks_per_cu for every kernel, or per
-        kernel (source=, enter=, reflect=, grating=, detect=), the
+        kernel (source=, enter=, reflect=, grating=, detect=; grating=0,
+        the default, is one round of the blocks a CU holds at once), the
         reflection kernel's waves per SIMD, and the persistent pass's tile
         (0: the staged pass) and blocks per CU."""
         s = Schedule()

@@ -152,3 +152,25 @@ def test_standard_parameter_file_opens():
     pf = marx_amd.ParamFile()
     assert pf.get_string("GratingType") in ("HETG", "LETG", "NONE")
     assert pf.get_double("FocalLength") == pytest.approx(10070.0)
+
+
+def test_launch_schedule_validation():
+    """mxg_set_schedule: blocks per CU 1..32 for every stage kernel, and 0
+    for the grating only (its default: one round of the blocks a CU holds
+    at once); a rejected schedule leaves the installed one unchanged."""
+    t = marx_amd.Tracer(seed=1, GratingType="NONE", DetectorType="NONE")
+    default = t.get_schedule()
+    assert default["grating"] == 0 and default["source"] >= 1
+    try:
+        t.set_schedule(grating=8)
+        assert t.get_schedule()["grating"] == 8
+        t.set_schedule(grating=0)
+        assert t.get_schedule()["grating"] == 0
+        for k in ("source", "enter", "reflect", "detect"):
+            with pytest.raises(RuntimeError):
+                t.set_schedule(**{k: 0})
+        with pytest.raises(RuntimeError):
+            t.set_schedule(grating=33)
+        assert t.get_schedule() == default
+    finally:
+        t.set_schedule(**{k: default[k] for k in ("source", "enter", "reflect", "grating", "detect")})

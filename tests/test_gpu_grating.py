@@ -25,12 +25,19 @@ def _run(tracer, rec):
 
 
 @pytest.mark.gpu
-def test_hetg_parity(gpu):
+@pytest.mark.parametrize("grid", [0, 8])
+def test_hetg_parity(gpu, grid):
+    """grid: the grating kernel's blocks per CU (0, the default: one round
+    of the blocks a CU holds at once)."""
     t = Tracer(seed=11, GratingType="HETG", DetectorType="NONE")
     rng = np.random.default_rng(5)
     rec = on_axis_photons(200000, rng.uniform(0.4, 8.0, 200000))
-    pg, po = _run(t, rec)
-    compare(pg.active, po.active, max_flips=2, label="hetg")
+    t.set_schedule(grating=grid)
+    try:
+        pg, po = _run(t, rec)
+    finally:
+        t.set_schedule(grating=0)
+    compare(pg.active, po.active, max_flips=2, label=f"hetg, grid {grid}")
     a = pg.active[(pg.active["flags"] & 0xFF) == 0]
     # orders come from the full -11..+11 table; 0 and +-1 dominate
     frac = {o: np.mean(a["order"] == o) for o in (-1, 0, 1)}
@@ -48,12 +55,17 @@ def test_hetg_no_sectors_computed_efficiencies(gpu):
 
 
 @pytest.mark.gpu
-def test_letg_parity(gpu):
+@pytest.mark.parametrize("grid", [0, 8])
+def test_letg_parity(gpu, grid):
     t = Tracer(seed=13, GratingType="LETG", DetectorType="NONE")
     rng = np.random.default_rng(7)
     rec = on_axis_photons(200000, rng.uniform(0.07, 10.0, 200000))
-    pg, po = _run(t, rec)
-    compare(pg.active, po.active, max_flips=2, label="letg")
+    t.set_schedule(grating=grid)
+    try:
+        pg, po = _run(t, rec)
+    finally:
+        t.set_schedule(grating=0)
+    compare(pg.active, po.active, max_flips=2, label=f"letg, grid {grid}")
     a = pg.active[(pg.active["flags"] & 0xFF) == 0]
     assert (a["support_orders"][:, 0] != 0).any()
 
