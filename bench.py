@@ -87,8 +87,8 @@ BYTES_PER_PHOTON = {
 # line carries that name beside `traffic`.
 TRAFFIC_BYTES_PER_PHOTON = {
     ("hetg-acis-s", "k_hrma_reflect"): (304.5, "profiles/pmc_hetg_acis_s_1e7.md"),
-    ("letg-hrc-s", "k_hrma_reflect"): (318.7, "profiles/pmc_letg_diff.md"),
-    ("letg-hrc-s", "grating"): (394.3, "profiles/pmc_letg_diff.md"),
+    ("letg-hrc-s", "k_hrma_reflect"): (318.6, "profiles/pmc_letg_diff.md"),
+    ("letg-hrc-s", "grating"): (333.9, "profiles/pmc_letg_diff.md"),
     # per listed photon
     ("diff", "k_diff_forward"): (357.1, "profiles/pmc_letg_diff.md"),
 }
