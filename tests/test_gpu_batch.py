@@ -12,6 +12,19 @@ from oracle.oracle import Oracle
 from tests.parity import compare
 
 ON_AXIS = dict(SourceRA=250.2134679741175, SourceDEC=-53.75743813458669)
+KERNELS = ("source", "enter", "reflect", "grating", "detect")
+
+
+@pytest.fixture(autouse=True)
+def keep_schedule(gpu):
+    """Every test here leaves the library's launch schedule as it found it
+    (the default, for the tests that follow)."""
+    import ctypes
+    from marx_amd._lib import Schedule
+    s = Schedule()
+    assert gpu.mxg_get_schedule(ctypes.byref(s)) == 0
+    yield
+    assert gpu.mxg_set_schedule(ctypes.byref(s)) == 0
 
 
 def _batch(tmp_path, nsrc, lo=100, hi=5000, seed=41, dither="NONE", grating="NONE", det="ACIS-I",
@@ -111,7 +124,6 @@ def test_batch_is_independent_of_the_schedule(gpu, tmp_path):
         t.set_schedule(blocks_per_cu=bpc, reflect_waves=waves)
         t.trace_batch(dirs, counts, tag_start=5, exposure=1.0e4, use_graph=False)
         outs.append(_download(t, n))
-    t.set_schedule(blocks_per_cu=8, reflect_waves=4)
     assert_same_records(outs[0], outs[1], "schedule (2, 2) vs (16, 4)")
     assert_same_records(outs[0], outs[2], "schedule (2, 2) vs (5, 3)")
 
@@ -144,7 +156,6 @@ def test_autotuner_installs_a_valid_schedule(gpu, tmp_path):
     assert ms > 0
     t.trace_batch(dirs, counts, use_graph=True)
     assert_same_records(_download(t, n), ref, "autotuned graph pass")
-    t.set_schedule(blocks_per_cu=8, reflect_waves=4, persistent_tile=0)
 
 
 @pytest.mark.gpu
