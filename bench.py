@@ -86,8 +86,8 @@ BYTES_PER_PHOTON = {
 # profiler runs): each entry names the profile it comes from, and the bench
 # line carries that name beside `traffic`.
 TRAFFIC_BYTES_PER_PHOTON = {
-    ("hetg-acis-s", "k_hrma_reflect"): (304.5, "profiles/pmc_hetg_acis_s_1e7.md"),
-    ("letg-hrc-s", "k_hrma_reflect"): (318.6, "profiles/pmc_letg_diff.md"),
+    ("hetg-acis-s", "k_hrma_reflect"): (312.9, "profiles/pmc_hetg_acis_s_1e7.md"),
+    ("letg-hrc-s", "k_hrma_reflect"): (321.2, "profiles/pmc_letg_diff.md"),
     ("letg-hrc-s", "grating"): (333.9, "profiles/pmc_letg_diff.md"),
     # per listed photon
     ("diff", "k_diff_forward"): (357.1, "profiles/pmc_letg_diff.md"),
