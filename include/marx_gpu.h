@@ -437,7 +437,8 @@ int mxg_get_schedule (mxg_schedule *s);
 
 /* Autotuner: times the batch of mxg_trace_batch (same arguments) under
  * candidate schedules -- for the staged pass, blocks per CU in {2, 4, 8, 16}
- * for every stage kernel, then 2, 3 and 4 waves for the reflection kernel,
+ * for every stage kernel and one round (0) for the grating, then 2, 3 and 4
+ * waves for the reflection kernel,
  * keeping for each kernel the fastest (HIP-event time, best of `repeats`
  * runs); then the persistent pass with tiles of {512, 1024, 2048, 4096}
  * photons at {2, 3, 4} blocks per CU -- and installs the schedule whose

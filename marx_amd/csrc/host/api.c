@@ -990,6 +990,21 @@ int mxg_autotune_batch (unsigned int num_sources, const double *dirs, const unsi
                sel.blocks_per_cu[k] = bpc[c];
             }
      }
+   /* the grating at one round of the blocks a CU holds at once (0), which
+    * none of the counts above need be (three for HETG) */
+   {
+      mxg_schedule s = sel;
+      s.blocks_per_cu[3] = 0;
+      if ((-1 == mxg_set_schedule (&s))
+          || (-1 == time_kernels (num_sources, dirs, counts, tag_start, start_time, exposure,
+                                  repeats, t)))
+        goto fail;
+      if (t[3] < best_t[3])
+        {
+           best_t[3] = t[3];
+           sel.blocks_per_cu[3] = 0;
+        }
+   }
    /* the reflection kernel's register budget */
    best_t[2] = 1e30f;
    for (c = 2; c <= 4; c++)
