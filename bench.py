@@ -699,6 +699,40 @@ def cpu_baseline_batch(tracer, dirs, counts, exposure, target_seconds=20.0):
                       f"task per source, {threads} threads), {dt:.1f} s"}
 
 
+GRID_CANDIDATES = {"enter": (4, 5, 8, 10, 15, 20), "reflect": (4, 8, 12, 16),
+                   "grating": (0, 2, 3, 4, 6, 8, 12), "detect": (1, 2, 4, 8)}
+
+
+def sweep_grids(args, tracer, step):
+    """Each stage kernel's time over candidate blocks per CU, one kernel at a
+    time with the others at the library's defaults (grating 0: one round of
+    the blocks a CU holds at once).  The mirror's kernels are timed alone
+    with HIP events, the grating and detector as their stages."""
+    default = tracer.get_schedule()
+    k = 0
+    rows = []
+    try:
+        for kern, values in GRID_CANDIDATES.items():
+            for v in values:
+                tracer.set_schedule(**{kern: v})
+                for _ in range(args.warmup):
+                    step(k)
+                    k += 1
+                ms = 0.0
+                for _ in range(args.steps):
+                    _, _, stage = step(k, timing=True)
+                    k += 1
+                    kms, _ = tracer.kernel_stats()
+                    ms += {"enter": kms[0], "reflect": kms[1], "grating": stage[3], "detect": stage[4]}[kern]
+                rows.append({"kernel": kern, "blocks_per_cu": v, "ms": round(ms / args.steps, 4)})
+                print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+                tracer.set_schedule(**{kern: default[kern]})
+    finally:
+        tracer.set_schedule(**{kk: default[kk] for kk in ("source", "enter", "reflect", "grating", "detect")})
+    return {"sweep": "blocks per CU of each stage kernel", "config": args.config, "steps": args.steps,
+            "warmup": args.warmup, "default": default, "rows": rows}
+
+
 def run_quietly(fn):
     """Runs fn with file descriptor 1 sent to stderr, so that the library's
     progress messages do not mix with the one JSON line on stdout."""
@@ -726,6 +760,9 @@ def measure():
     ap.add_argument("--photons", type=int, default=0, help="photons per step (default: the config's)")
     ap.add_argument("--exposure", type=float, default=50_000.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--sweep-grids", action="store_true",
+                    help="instead of the bench line: each stage kernel's HIP-event time over candidate "
+                         "blocks per CU (the others at their defaults), one JSON line per candidate")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the steps, copy a MIB-MiB device buffer three times (the known-size "
                          "reference that HBM-traffic counters are scaled by)")
@@ -766,6 +803,8 @@ def measure():
     def step(k, timing=False):
         return tracer.trace_device(N, tag_start=(k * N) % (1 << 31), start_time=k * T, timing=timing)
 
+    if args.sweep_grids:
+        return sweep_grids(args, tracer, step)
     for k in range(args.warmup):
         step(k)
     torch.cuda.synchronize()

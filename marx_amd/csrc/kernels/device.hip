@@ -86,10 +86,10 @@ double *d_diff_small = nullptr;    // summed weight, gradient
 mxg_diff_cfg *d_diff_cfg = nullptr;
 // launch schedule and graph capture
 // Default grids: whole rounds of the blocks a CU holds at once, where that
-// measured fastest (HETG + ACIS-S): the entry (five blocks a CU) at two
-// rounds, 0.241 ms against 0.264 at 8 blocks per CU; the reflection (two to
-// four) at 12 blocks, 0.668 against 0.677; the grating one round (0).
-mxgd_schedule g_sched = {{8, 10, 12, 0, 8}, MXG_REFLECT_WAVES_DEFAULT, 0, 2};
+// measured fastest (HETG + ACIS-S): the entry (five blocks a CU) at three
+// rounds, 0.240 ms against 0.264 at 8 blocks per CU; the reflection (two to
+// four) at 12 blocks, 0.668 against 0.676; the grating one round (0).
+mxgd_schedule g_sched = {{8, 15, 12, 0, 8}, MXG_REFLECT_WAVES_DEFAULT, 0, 2};
 uint32_t g_grating_bpc = 8;        // one round of k_grating blocks per CU (grating_blocks_per_cu)
 int g_capturing = 0;
 uint64_t *d_tile_tot = nullptr;
